@@ -1,0 +1,514 @@
+// C ABI of one device-resident CA domain (include/amca3d.h).
+//
+// The reference keeps the grid as raw host arrays owned by
+// CellularAutomataManager (CellularAutomataManager.cpp:303-326) and one heap
+// Grain per solid cell.  Here the grid and the per-cell envelopes live in HBM
+// for the lifetime of the handle, a step is a fixed sequence of six kernel
+// launches that reads every scalar it needs from device memory, and the host
+// only reads the counters back between batches of steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/amca3d.h"
+#include "plain_step.h"
+
+namespace {
+
+thread_local std::string g_last_error;
+
+amca_status fail(amca_status st, const std::string& msg) {
+  g_last_error = msg;
+  return st;
+}
+
+#define HIP_TRY(expr)                                                                   \
+  do {                                                                                  \
+    hipError_t e_ = (expr);                                                             \
+    if (e_ != hipSuccess)                                                               \
+      return fail(AMCA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
+  } while (0)
+
+enum KernelId { K_NUCLEATE = 0, K_CLAIM0, K_EMIT0, K_CLAIM1, K_EMIT1, K_TAIL, K_COUNT };
+const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim_wave0", "k_resolve_emit_wave0",
+                                     "k_claim_wave1", "k_resolve_emit_wave1", "k_tail"};
+
+}  // namespace
+
+struct amca_domain {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  amca_domain_desc desc{};
+  amca::DevParams P{};
+  amca::DevBuffers B{};
+  int site_cap = 0;
+  int orient_pushed = 0;
+  int nuc_blocks = 1;
+  int list_blocks = 1;
+  int emit_blocks = 1;
+  bool timing = false;
+  // event pairs recorded around timed launches, resolved when times are read
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, size_t>> ev_pending;  // (kernel id, index of first event)
+  size_t ev_used = 0;
+  double kernel_ms[K_COUNT] = {};
+  int64_t kernel_launches[K_COUNT] = {};
+  std::vector<void*> allocs;
+
+  template <class T>
+  amca_status alloc(T** p, size_t count) {
+    void* q = nullptr;
+    size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+      return fail(AMCA_ERR_HIP, std::string("hipMalloc(") + std::to_string(bytes) +
+                                    "): " + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return AMCA_OK;
+  }
+
+  ~amca_domain() {
+    for (void* p : allocs) hipFree(p);
+    for (auto& e : ev_pool)
+      if (e) (void)hipEventDestroy(e);
+    if (stream) hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+#define AMCA_TRY(expr)                  \
+  do {                                  \
+    amca_status s_ = (expr);            \
+    if (s_ != AMCA_OK) return s_;       \
+  } while (0)
+
+amca_status read_counters(amca_domain* d, amca::DevCounters* c) {
+  HIP_TRY(hipMemcpyAsync(c, d->B.ctr, sizeof(*c), hipMemcpyDeviceToHost, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+void fill_counters(const amca_domain* d, const amca::DevCounters& c, amca_counters* out) {
+  if (!out) return;
+  out->step = c.step;
+  out->time = c.t;
+  out->dt = c.dt;
+  out->n_liquid = d->P.N - c.n_solid;
+  out->n_grains = c.n_grains;
+  out->n_orient = c.n_orient;
+  out->n_active = c.nA_in;
+  out->n_sites = c.n_sites;
+  out->done = (c.done || !(c.t < d->desc.final_time)) ? 1 : 0;
+  out->max_wave = c.max_wave;
+}
+
+amca_status resolve_events(amca_domain* d) {
+  if (d->ev_pending.empty()) return AMCA_OK;
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  for (auto& p : d->ev_pending) {
+    float ms = 0.f;
+    HIP_TRY(hipEventElapsedTime(&ms, d->ev_pool[p.second], d->ev_pool[p.second + 1]));
+    d->kernel_ms[p.first] += ms;
+    d->kernel_launches[p.first] += 1;
+  }
+  d->ev_pending.clear();
+  d->ev_used = 0;
+  return AMCA_OK;
+}
+
+template <class F>
+amca_status timed(amca_domain* d, int k, F&& launch) {
+  if (!d->timing) {
+    HIP_TRY(launch());
+    return AMCA_OK;
+  }
+  if (d->ev_used + 2 > d->ev_pool.size()) {
+    if (d->ev_pool.size() >= 1u << 16) AMCA_TRY(resolve_events(d));
+    while (d->ev_used + 2 > d->ev_pool.size()) {
+      hipEvent_t e;
+      HIP_TRY(hipEventCreate(&e));
+      d->ev_pool.push_back(e);
+    }
+  }
+  const size_t i = d->ev_used;
+  d->ev_used += 2;
+  HIP_TRY(hipEventRecord(d->ev_pool[i], d->stream));
+  HIP_TRY(launch());
+  HIP_TRY(hipEventRecord(d->ev_pool[i + 1], d->stream));
+  d->ev_pending.emplace_back(k, i);
+  return AMCA_OK;
+}
+
+// The fixed launch sequence of one step.
+amca_status enqueue_step(amca_domain* d) {
+  const amca::DevParams& P = d->P;
+  const amca::DevBuffers& B = d->B;
+  hipStream_t s = d->stream;
+  AMCA_TRY(timed(d, K_NUCLEATE, [&] { return amca::launch_nucleate(P, B, d->nuc_blocks, s); }));
+  AMCA_TRY(timed(d, K_CLAIM0, [&] { return amca::launch_claim(P, B, 0, d->list_blocks, s); }));
+  AMCA_TRY(timed(d, K_EMIT0, [&] { return amca::launch_resolve_emit(P, B, 0, d->emit_blocks, s); }));
+  AMCA_TRY(timed(d, K_CLAIM1, [&] { return amca::launch_claim(P, B, 1, d->list_blocks, s); }));
+  AMCA_TRY(timed(d, K_EMIT1, [&] { return amca::launch_resolve_emit(P, B, 1, d->emit_blocks, s); }));
+  AMCA_TRY(timed(d, K_TAIL, [&] { return amca::launch_tail(P, B, s); }));
+  return AMCA_OK;
+}
+
+amca_status check_err(amca_domain* d, const amca::DevCounters& c) {
+  if (c.err)
+    return fail(AMCA_ERR_CAPACITY,
+                "orientation table exhausted: push more rows with amca_domain_push_orientations");
+  return AMCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* amca_last_error(void) { return g_last_error.c_str(); }
+const char* amca_version(void) { return "amca3d-mi355x 0.1 (gfx950)"; }
+
+amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capacity,
+                               amca_domain** out) {
+  if (!desc || !out) return fail(AMCA_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1)
+    return fail(AMCA_ERR_INVALID, "domain needs at least one cell per axis");
+  const long long nxG = desc->nx + 2LL, nyG = desc->ny + 2LL, nzG = desc->nz + 2LL;
+  const long long NG = nxG * nyG * nzG;
+  if (NG > (long long)INT_MAX)
+    return fail(AMCA_ERR_INVALID, "ghosted grid exceeds 2^31-1 cells (int32 cell index)");
+  if (!(desc->h > 0)) return fail(AMCA_ERR_INVALID, "cell size must be positive");
+  if (orient_capacity < 0) return fail(AMCA_ERR_INVALID, "negative orientation capacity");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(AMCA_ERR_NO_DEVICE, "no HIP device visible");
+
+  auto* d = new amca_domain();
+  HIP_TRY(hipGetDevice(&d->device));
+  HIP_TRY(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  d->desc = *desc;
+  amca::DevParams& P = d->P;
+  P.nx = desc->nx; P.ny = desc->ny; P.nz = desc->nz;
+  P.nxG = (int)nxG; P.nyG = (int)nyG; P.nzG = (int)nzG;
+  P.nxyG = (int)(nxG * nyG);
+  P.NG = (int)NG;
+  P.N = (long long)desc->nx * desc->ny * desc->nz;
+  P.h = desc->h; P.x0 = desc->x0; P.y0 = desc->y0; P.z0 = desc->z0;
+  P.T0 = desc->T0; P.Tm = desc->Tm; P.Tdot = desc->Tdot;
+  P.a1 = desc->a1; P.a2 = desc->a2; P.a3 = desc->a3;
+  P.trunc = desc->trunc;
+  P.tsf = desc->time_step_factor;
+  P.max_dt = desc->max_dt;
+  P.final_time = desc->final_time;
+  P.sqrt2f = (double)sqrtf(2.0f);
+  P.orient_cap = orient_capacity;
+  {
+    int s = 0;
+    for (int dk = -1; dk <= 1; dk++)
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++) {
+          if (!dk && !dj && !di) continue;
+          P.off[s++] = dk * P.nxyG + dj * P.nxG + di;
+        }
+  }
+  amca::DevBuffers& B = d->B;
+  const size_t N = (size_t)P.N;
+  amca_status st;
+#define ALLOC(ptr, n)                 \
+  st = d->alloc(&(ptr), (n));         \
+  if (st != AMCA_OK) {                \
+    delete d;                         \
+    return st;                        \
+  }
+  ALLOC(B.gid, (size_t)NG);
+  ALLOC(B.oid, (size_t)NG);
+  ALLOC(B.claim, (size_t)NG);
+  ALLOC(B.env, (size_t)NG);
+  ALLOC(B.listA[0], N);
+  ALLOC(B.listA[1], N);
+  ALLOC(B.listW[0], N);
+  ALLOC(B.listW[1], N);
+  ALLOC(B.masks, N);
+  ALLOC(B.res, N);
+  ALLOC(B.l2g, 9 * (size_t)std::max(orient_capacity, 1));
+  ALLOC(B.lb_flag, (size_t)amca::kLbSites * amca::kMaxBlocks);
+  ALLOC(B.lb_agg, (size_t)amca::kLbSites * amca::kMaxBlocks);
+  ALLOC(B.lb_pref, (size_t)amca::kLbSites * amca::kMaxBlocks);
+  ALLOC(B.ctr, 1);
+  ALLOC(B.site_cell[0], 1);
+  ALLOC(B.site_cell[1], 1);
+  ALLOC(B.site_dt[0], 1);
+  ALLOC(B.site_dt[1], 1);
+#undef ALLOC
+  B.log = nullptr;
+  d->site_cap = 1;
+  HIP_TRY(hipMemsetAsync(B.lb_flag, 0, sizeof(unsigned) * amca::kLbSites * amca::kMaxBlocks, d->stream));
+  amca::DevCounters c{};
+  c.t = 0.0;
+  HIP_TRY(hipMemcpyAsync(B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream));
+  HIP_TRY(amca::launch_init_grid(P, B, d->stream));
+  // Grid sizes: enough blocks to fill 256 CUs several times over.
+  int cus = 256;
+  hipDeviceProp_t prop;
+  if (hipGetDeviceProperties(&prop, d->device) == hipSuccess && prop.multiProcessorCount > 0)
+    cus = prop.multiProcessorCount;
+  long long want = (P.N + amca::kBlock - 1) / amca::kBlock;
+  d->list_blocks = (int)std::max(1LL, std::min<long long>(want, 8LL * cus));
+  d->emit_blocks = (int)std::max(1LL, std::min<long long>(want, amca::kMaxBlocks));
+  d->nuc_blocks = 1;
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  *out = d;
+  return AMCA_OK;
+}
+
+amca_status amca_domain_destroy(amca_domain* d) {
+  if (!d) return AMCA_OK;
+  hipStreamSynchronize(d->stream);
+  delete d;
+  return AMCA_OK;
+}
+
+amca_status amca_domain_set_sites(amca_domain* d, int32_t n, const int32_t* cell,
+                                  const double* dT_crit) {
+  if (!d || n < 0 || (n > 0 && (!cell || !dT_crit))) return fail(AMCA_ERR_INVALID, "bad sites");
+  for (int i = 0; i < n; i++) {
+    if (cell[i] < 0 || cell[i] >= d->P.NG) return fail(AMCA_ERR_INVALID, "site cell out of range");
+    if (i > 0 && cell[i] <= cell[i - 1])
+      return fail(AMCA_ERR_INVALID, "sites must be strictly ascending by cell");
+  }
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  int cap = std::max(n, 1);
+  if (cap > d->site_cap) {
+    for (int b = 0; b < 2; b++) {
+      for (auto& p : d->allocs) {
+        if (p == d->B.site_cell[b] || p == d->B.site_dt[b]) {
+          hipFree(p);
+          p = nullptr;
+        }
+      }
+    }
+    d->allocs.erase(std::remove(d->allocs.begin(), d->allocs.end(), nullptr), d->allocs.end());
+    for (int b = 0; b < 2; b++) {
+      AMCA_TRY(d->alloc(&d->B.site_cell[b], (size_t)cap));
+      AMCA_TRY(d->alloc(&d->B.site_dt[b], (size_t)cap));
+    }
+    d->site_cap = cap;
+  }
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  if (n > 0) {
+    HIP_TRY(hipMemcpyAsync(d->B.site_cell[c.site_sel], cell, sizeof(int32_t) * n,
+                           hipMemcpyHostToDevice, d->stream));
+    HIP_TRY(hipMemcpyAsync(d->B.site_dt[c.site_sel], dT_crit, sizeof(double) * n,
+                           hipMemcpyHostToDevice, d->stream));
+  }
+  c.n_sites = n;
+  HIP_TRY(hipMemcpyAsync(d->B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream));
+  long long want = ((long long)n + amca::kBlock - 1) / amca::kBlock;
+  d->nuc_blocks = (int)std::max(1LL, std::min<long long>(want, amca::kMaxBlocks));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+amca_status amca_domain_push_orientations(amca_domain* d, int32_t n, const double* euler3,
+                                          const double* l2g9) {
+  (void)euler3;
+  if (!d || n < 0 || (n > 0 && !l2g9)) return fail(AMCA_ERR_INVALID, "bad orientation rows");
+  if (d->orient_pushed + n > d->P.orient_cap)
+    return fail(AMCA_ERR_CAPACITY, "orientation table capacity exceeded");
+  if (n > 0)
+    HIP_TRY(hipMemcpyAsync(d->B.l2g + 9 * (size_t)d->orient_pushed, l2g9, sizeof(double) * 9 * n,
+                           hipMemcpyHostToDevice, d->stream));
+  d->orient_pushed += n;
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+amca_status amca_domain_step(amca_domain* d, amca_counters* out) {
+  if (!d) return fail(AMCA_ERR_INVALID, "null domain");
+  AMCA_TRY(enqueue_step(d));
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  fill_counters(d, c, out);
+  return check_err(d, c);
+}
+
+amca_status amca_domain_run(amca_domain* d, int64_t max_steps, amca_counters* out) {
+  if (!d) return fail(AMCA_ERR_INVALID, "null domain");
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  const long long start = c.step;
+  long long batch = 8;
+  while (true) {
+    bool finished = c.done || !(c.t < d->desc.final_time);
+    if (finished || c.err) break;
+    long long taken = c.step - start;
+    if (max_steps >= 0 && taken >= max_steps) break;
+    long long k = batch;
+    if (max_steps >= 0) k = std::min<long long>(k, max_steps - taken);
+    for (long long q = 0; q < k; q++) AMCA_TRY(enqueue_step(d));
+    AMCA_TRY(read_counters(d, &c));
+    batch = std::min<long long>(batch * 2, 64);
+  }
+  fill_counters(d, c, out);
+  return check_err(d, c);
+}
+
+amca_status amca_domain_run_fixed(amca_domain* d, int64_t n_steps) {
+  if (!d || n_steps < 0) return fail(AMCA_ERR_INVALID, "bad arguments");
+  for (int64_t q = 0; q < n_steps; q++) AMCA_TRY(enqueue_step(d));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+amca_status amca_domain_counters(amca_domain* d, amca_counters* out) {
+  if (!d || !out) return fail(AMCA_ERR_INVALID, "null argument");
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  fill_counters(d, c, out);
+  return check_err(d, c);
+}
+
+amca_status amca_domain_enable_log(amca_domain* d, int32_t capacity) {
+  if (!d || capacity < 0) return fail(AMCA_ERR_INVALID, "bad arguments");
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  if (d->B.log) {
+    for (auto& p : d->allocs)
+      if (p == d->B.log) {
+        hipFree(p);
+        p = nullptr;
+      }
+    d->allocs.erase(std::remove(d->allocs.begin(), d->allocs.end(), nullptr), d->allocs.end());
+    d->B.log = nullptr;
+  }
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  c.log_cap = capacity;
+  if (capacity > 0) AMCA_TRY(d->alloc(&d->B.log, (size_t)capacity));
+  HIP_TRY(hipMemcpyAsync(d->B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+amca_status amca_domain_read_log(amca_domain* d, int32_t cap, amca_step_record* rows,
+                                 int32_t* n_rows) {
+  if (!d || !n_rows) return fail(AMCA_ERR_INVALID, "null argument");
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  long long n = std::min<long long>(c.step, c.log_cap);
+  n = std::min<long long>(n, cap);
+  *n_rows = (int32_t)n;
+  if (n > 0 && rows && d->B.log) {
+    std::vector<amca::StepLogRow> tmp((size_t)n);
+    HIP_TRY(hipMemcpy(tmp.data(), d->B.log, sizeof(amca::StepLogRow) * n, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < n; i++) {
+      rows[i].time_before = tmp[i].time_before;
+      rows[i].dt = tmp[i].dt;
+      rows[i].n_nucleated = tmp[i].n_nucleated;
+      rows[i].n_active_before = tmp[i].n_active_before;
+      rows[i].n_captured = tmp[i].n_captured;
+      rows[i].n_active_after = tmp[i].n_active_after;
+      rows[i].n_liquid = tmp[i].n_liquid;
+    }
+  }
+  return AMCA_OK;
+}
+
+amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain, double* env,
+                                 int32_t site_cap, int32_t* site_cell, double* site_dt,
+                                 int32_t* n_sites) {
+  if (!d) return fail(AMCA_ERR_INVALID, "null domain");
+  const size_t NG = (size_t)d->P.NG;
+  HIP_TRY(hipStreamSynchronize(d->stream));
+  if (orient) HIP_TRY(hipMemcpy(orient, d->B.oid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
+  if (grain) HIP_TRY(hipMemcpy(grain, d->B.gid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
+  if (env) HIP_TRY(hipMemcpy(env, d->B.env, sizeof(double) * 4 * NG, hipMemcpyDeviceToHost));
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  if (n_sites) *n_sites = c.n_sites;
+  int m = std::min(c.n_sites, site_cap);
+  if (m > 0 && site_cell)
+    HIP_TRY(hipMemcpy(site_cell, d->B.site_cell[c.site_sel], sizeof(int32_t) * m, hipMemcpyDeviceToHost));
+  if (m > 0 && site_dt)
+    HIP_TRY(hipMemcpy(site_dt, d->B.site_dt[c.site_sel], sizeof(double) * m, hipMemcpyDeviceToHost));
+  return AMCA_OK;
+}
+
+amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
+                                      void** stream) {
+  if (!d) return fail(AMCA_ERR_INVALID, "null domain");
+  if (orient) *orient = d->B.oid;
+  if (grain) *grain = d->B.gid;
+  if (env) *env = d->B.env;
+  if (stream) *stream = d->stream;
+  return AMCA_OK;
+}
+
+amca_status amca_domain_enable_timing(amca_domain* d, int32_t on) {
+  if (!d) return fail(AMCA_ERR_INVALID, "null domain");
+  AMCA_TRY(resolve_events(d));
+  d->timing = on != 0;
+  for (int k = 0; k < K_COUNT; k++) {
+    d->kernel_ms[k] = 0.0;
+    d->kernel_launches[k] = 0;
+  }
+  return AMCA_OK;
+}
+
+int32_t amca_kernel_count(void) { return K_COUNT; }
+
+const char* amca_kernel_name(int32_t i) {
+  if (i < 0 || i >= K_COUNT) return "";
+  return kKernelNames[i];
+}
+
+amca_status amca_domain_kernel_times(amca_domain* d, double* ms, int64_t* launches) {
+  if (!d) return fail(AMCA_ERR_INVALID, "null domain");
+  AMCA_TRY(resolve_events(d));
+  for (int k = 0; k < K_COUNT; k++) {
+    if (ms) ms[k] = d->kernel_ms[k];
+    if (launches) launches[k] = d->kernel_launches[k];
+  }
+  return AMCA_OK;
+}
+
+amca_status amca_envelope_selftest(int32_t n, const double* env4, const double* l2g9,
+                                   const double* cell3, double h, double trunc,
+                                   int32_t* inside, double* child4) {
+  if (n < 0 || (n > 0 && (!env4 || !l2g9 || !cell3 || !inside || !child4)))
+    return fail(AMCA_ERR_INVALID, "bad arguments");
+  if (n == 0) return AMCA_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return fail(AMCA_ERR_NO_DEVICE, "no HIP device visible");
+  double *de = nullptr, *dr = nullptr, *dc = nullptr, *dout = nullptr;
+  int* din = nullptr;
+  HIP_TRY(hipMalloc(&de, sizeof(double) * 4 * n));
+  HIP_TRY(hipMalloc(&dr, sizeof(double) * 9 * n));
+  HIP_TRY(hipMalloc(&dc, sizeof(double) * 3 * n));
+  HIP_TRY(hipMalloc(&dout, sizeof(double) * 4 * n));
+  HIP_TRY(hipMalloc(&din, sizeof(int) * n));
+  HIP_TRY(hipMemcpy(de, env4, sizeof(double) * 4 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dr, l2g9, sizeof(double) * 9 * n, hipMemcpyHostToDevice));
+  HIP_TRY(hipMemcpy(dc, cell3, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+  HIP_TRY(amca::launch_envelope_selftest(n, de, dr, dc, h, trunc, (double)sqrtf(2.0f), din, dout, 0));
+  HIP_TRY(hipDeviceSynchronize());
+  HIP_TRY(hipMemcpy(inside, din, sizeof(int) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(child4, dout, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
+  hipFree(de);
+  hipFree(dr);
+  hipFree(dc);
+  hipFree(dout);
+  hipFree(din);
+  return AMCA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,135 @@
+// Realm-level C ABI (amca_sim_*): the Simulation / CellularAutomataSolver
+// layer of the reference (Simulation.cpp:36-93, CellularAutomataSolver.cpp:40-202)
+// over the device domain.
+#include <cstring>
+#include <memory>
+#include <stdexcept>
+#include <string>
+
+#include "../../../include/amca3d.h"
+#include "realm.h"
+
+struct amca_sim {
+  amca_host::SimulationConfig cfg;
+  std::unique_ptr<amca_host::PlainRealm> plain;
+  bool initialized = false;
+  bool prepared = false;
+};
+
+namespace {
+thread_local std::string g_host_error;
+
+amca_status host_fail(amca_status s, const std::string& m) {
+  g_host_error = m;
+  return s;
+}
+
+template <class F>
+amca_status guarded(F&& f) {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    return host_fail(AMCA_ERR_IO, e.what());
+  }
+}
+
+amca_status create_from_doc(const amca_host::yaml::Node& doc, amca_sim** out) {
+  auto s = std::make_unique<amca_sim>();
+  s->cfg = amca_host::load_simulation(doc);
+  if (s->cfg.realms.empty()) return host_fail(AMCA_ERR_INVALID, "no cellular automata realm");
+  const auto& r = s->cfg.realms[0];
+  if (s->cfg.finite_element_method || r.type != "cellular_automata")
+    return host_fail(AMCA_ERR_INVALID,
+                     "realm type '" + r.type + "' with these solvers is not available in this build");
+  s->plain = std::make_unique<amca_host::PlainRealm>(r, s->cfg.ti);
+  *out = s.release();
+  return AMCA_OK;
+}
+}  // namespace
+
+extern "C" {
+
+const char* amca_sim_last_error(void) { return g_host_error.c_str(); }
+
+amca_status amca_sim_create(const char* yaml_text, amca_sim** out) {
+  if (!yaml_text || !out) return host_fail(AMCA_ERR_INVALID, "null argument");
+  return guarded([&] { return create_from_doc(amca_host::yaml::parse(yaml_text), out); });
+}
+
+amca_status amca_sim_create_from_file(const char* path, amca_sim** out) {
+  if (!path || !out) return host_fail(AMCA_ERR_INVALID, "null argument");
+  return guarded([&] { return create_from_doc(amca_host::yaml::parse_file(path), out); });
+}
+
+amca_status amca_sim_initialize(amca_sim* s) {
+  if (!s) return host_fail(AMCA_ERR_INVALID, "null simulation");
+  return guarded([&] {
+    s->plain->initialize();
+    s->initialized = true;
+    s->prepared = true;
+    return AMCA_OK;
+  });
+}
+
+amca_status amca_sim_prepare(amca_sim* s) {
+  if (!s) return host_fail(AMCA_ERR_INVALID, "null simulation");
+  return guarded([&] {
+    s->plain->prepare();
+    s->prepared = true;
+    return AMCA_OK;
+  });
+}
+
+amca_status amca_sim_run(amca_sim* s, int64_t max_steps, amca_counters* out) {
+  if (!s || !s->initialized) return host_fail(AMCA_ERR_INVALID, "simulation not initialized");
+  return guarded([&] {
+    amca_counters c = s->plain->run(max_steps);
+    if (out) *out = c;
+    return AMCA_OK;
+  });
+}
+
+amca_domain* amca_sim_domain(amca_sim* s) { return s && s->plain ? s->plain->domain() : nullptr; }
+
+amca_status amca_sim_get_info(amca_sim* s, amca_sim_info* info) {
+  if (!s || !info) return host_fail(AMCA_ERR_INVALID, "null argument");
+  std::memset(info, 0, sizeof(*info));
+  const auto& r = s->cfg.realms[0];
+  info->nx = r.ncell[0];
+  info->ny = r.ncell[1];
+  info->nz = r.ncell[2];
+  info->h = r.h;
+  info->termination_time = s->cfg.ti.termination_time;
+  info->time_step = s->cfg.ti.time_step;
+  if (s->plain && s->prepared) {
+    info->n_sites = (int32_t)s->plain->initial_sites().size();
+    info->n_surface_sites = s->plain->n_surface_sites();
+    info->n_bulk_sites = s->plain->n_bulk_sites();
+    info->n_site_draws = s->plain->site_draws();
+  }
+  return AMCA_OK;
+}
+
+amca_status amca_sim_initial_sites(amca_sim* s, int32_t cap, int32_t* cell, double* dT_crit,
+                                   double* euler3, int32_t* n) {
+  if (!s || !s->prepared || !n) return host_fail(AMCA_ERR_INVALID, "simulation not prepared");
+  const auto& sc = s->plain->initial_sites();
+  const auto& sd = s->plain->initial_site_dt();
+  const auto& eu = s->plain->euler_rows();
+  *n = (int32_t)sc.size();
+  int m = std::min<int>(cap, (int)sc.size());
+  for (int i = 0; i < m; i++) {
+    if (cell) cell[i] = sc[i];
+    if (dT_crit) dT_crit[i] = sd[i];
+    if (euler3)
+      for (int q = 0; q < 3; q++) euler3[3 * i + q] = eu[3 * (size_t)i + q];
+  }
+  return AMCA_OK;
+}
+
+amca_status amca_sim_destroy(amca_sim* s) {
+  delete s;
+  return AMCA_OK;
+}
+
+}  // extern "C"

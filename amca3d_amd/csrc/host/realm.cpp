@@ -1,0 +1,276 @@
+#include "realm.h"
+
+#include <cmath>
+#include <stdexcept>
+#include <unordered_map>
+
+namespace amca_host {
+
+using namespace yaml;
+
+namespace {
+
+void load_nucleation(const Node& realm, RealmConfig& c) {
+  // CellularAutomataManager_3D.cpp:193-236
+  const Node* rules = realm.FindValue("nucleation_rules");
+  if (!rules) return;
+  for (size_t r = 0; r < rules->size(); r++) {
+    const Node& rule = (*rules)[r];
+    if (const Node* s = rule.FindValue("surface")) {
+      std::string type = "Gaussian";
+      get_required(*s, "type", type);
+      get_required(*s, "site_density", c.surface.density);
+      get_required(*s, "mean", c.surface.mean);
+      get_required(*s, "standard_deviation", c.surface.sd);
+      // extension: which faces carry surface sites (default: the reference's six)
+      std::string faces = "all";
+      get_if_present(*s, "faces", faces, faces);
+      if (faces == "all") c.surface_faces = SurfaceFaces::kReferenceSix;
+      else if (faces == "z_min") c.surface_faces = SurfaceFaces::kZMin;
+      else throw std::runtime_error("parser error: nucleation_rules-surface-faces must be all or z_min");
+    }
+    if (const Node* b = rule.FindValue("bulk")) {
+      std::string type = "Gaussian";
+      get_if_present(*b, "type", type, type);
+      get_required(*b, "site_density", c.bulk.density);
+      get_required(*b, "mean", c.bulk.mean);
+      get_required(*b, "standard_deviation", c.bulk.sd);
+    }
+  }
+}
+
+void load_solution_options(const Node& realm, RealmConfig& c) {
+  // CellularAutomataManager_3D.cpp:248-341 and, for remelting realms,
+  // CellularAutomataManager_3DRemelting.cpp:56-75
+  const Node* so = realm.FindValue("solution_options");
+  if (!so) return;
+  std::string name;
+  get_required(*so, "name", name);
+  const Node* opts = expect_sequence(*so, "options", true);
+  if (!opts) return;
+  for (size_t i = 0; i < opts->size(); i++) {
+    const Node& o = (*opts)[i];
+    if (expect_map(o, "output_microstructure_information", true)) {
+      const Node& f = *o.FindValue("output_microstructure_information");
+      get_required(f, "file_name", c.micro_file);
+      c.output_micro = true;
+    } else if (expect_map(o, "random_type", true)) {
+      // shuffled VTR ids only change output labels; not used on this path
+    } else if (expect_map(o, "phase_state", true)) {
+      const Node& p = *o.FindValue("phase_state");
+      get_if_present(p, "has_been_melted", c.initial_melted, c.initial_melted);
+    } else if (expect_map(o, "cell_grain_length_cutoff_factor", true)) {
+      const Node& p = *o.FindValue("cell_grain_length_cutoff_factor");
+      get_if_present(p, "factor_value", c.cutoff_factor, c.cutoff_factor);
+    } else if (expect_map(o, "parallel_specification", true)) {
+      // one device-resident grid: the decomposition axis has no effect
+    } else if (expect_map(o, "envelope", true)) {
+      const Node& p = *o.FindValue("envelope");
+      double f = -1;
+      get_if_present(p, "max_half_diagonal_length", f, f);
+      if (f > 0) c.trunc = (double)sqrtf(2) * f * 0.5;  // Octahedron.cpp:316-320
+    } else if (expect_map(o, "criterion_to_activate_grain", true)) {
+      const Node& p = *o.FindValue("criterion_to_activate_grain");
+      get_required(p, "using_nucleation_seed", c.using_nucleation_seed);
+    }
+  }
+}
+
+void load_physics(const Node& realm, RealmConfig& c) {
+  // ProblemPhysics_RappazGandin1993::load (ProblemPhysics.cpp:57-102)
+  const Node* p = realm.FindValue("problem_physics");
+  if (!p) throw std::runtime_error("parsing error: realm-problem_physics-RappazGandin");
+  get_required(*p, "type", c.physics);
+  if (c.physics != "RappazGandin")
+    throw std::runtime_error("parsing error: realm-problem_physics-RappazGandin");
+  get_if_present(*p, "initial_temperature", c.T0, c.T0);
+  get_if_present(*p, "liquidus_temperature", c.Tm, c.Tm);
+  get_if_present(*p, "melting_temperature", c.Tm, c.Tm);
+  get_if_present(*p, "solidus_temperature", c.Ts, c.Ts);
+  c.Tmin_nuc = c.Ts;
+  get_if_present(*p, "minimum_temperature_for_nucleation", c.Tmin_nuc, c.Tmin_nuc);
+  get_if_present(*p, "t_dot", c.Tdot, c.Tdot);
+  get_if_present(*p, "a1", c.a1, c.a1);
+  get_if_present(*p, "a2", c.a2, c.a2);
+  get_if_present(*p, "a3", c.a3, c.a3);
+  get_if_present(*p, "b1", c.b1, c.b1);
+  get_if_present(*p, "b2", c.b2, c.b2);
+  get_if_present(*p, "b3", c.b3, c.b3);
+}
+
+void load_output(const Node& realm, RealmConfig& c) {
+  // VtrManager::load (VtrManager.cpp:60-103): the node is required
+  const Node* out = realm.FindValue("output");
+  if (!out) throw std::runtime_error("parser error: realm-output");
+  get_if_present(*out, "output_frequency", c.output_frequency, c.output_frequency);
+  get_if_present(*out, "output_data_base_name", c.output_base, c.output_base);
+  get_if_present(*out, "output_time_interval", c.output_time_interval, c.output_time_interval);
+  if (const Node* vars = out->FindValue("output_variables")) {
+    for (size_t i = 0; i < vars->size(); i++) {
+      std::string v;
+      convert((*vars)[i], v);
+      c.output_variables.push_back(v);
+    }
+  }
+  if (c.output_variables.empty()) c.output_variables.push_back("grain_ID");
+}
+
+}  // namespace
+
+RealmConfig load_ca_realm(const Node& realm) {
+  RealmConfig c;
+  c.trunc = (double)sqrtf(3.0);  // Octahedron.cpp:25
+  get_if_present(realm, "name", c.name, c.name);
+  get_required(realm, "type", c.type);
+  get_if_present(realm, "dimension", c.dimension, c.dimension);
+  if (c.type != "cellular_automata" && c.type != "cellular_automata_remelting")
+    throw std::runtime_error("parser error: realm-type");
+  if (c.dimension != 3) throw std::runtime_error("only 3-D realms are supported");
+  get_if_present(realm, "random_seed", c.random_seed, c.random_seed);
+
+  // CellularAutomataManager_3D.cpp:79-190
+  const Node* domain = realm.FindValue("domain");
+  if (!domain) throw std::runtime_error("parser error: realm-domain");
+  get_required3(*domain, "original_point", c.origin);
+  get_if_present_no_default3(*domain, "lateral_sizes", c.lateral);
+  const Node* disc = realm.FindValue("discretization");
+  if (!disc) throw std::runtime_error("parser error: realm-discretization");
+  get_required(*disc, "cell_size", c.h);
+  if (c.lateral[0] < 0) {
+    get_required3(*disc, "number_cells", c.ncell);
+  } else {
+    for (int i = 0; i < 3; i++) {
+      c.ncell[i] = c.lateral[i] / c.h;  // truncation, as the reference
+      c.lateral[i] = c.ncell[i] * c.h;
+    }
+  }
+  if (c.ncell[0] < 1 || c.ncell[1] < 1 || c.ncell[2] < 1)
+    throw std::runtime_error("parser error: realm-discretization gives an empty grid");
+  load_nucleation(realm, c);
+  load_solution_options(realm, c);
+  load_output(realm, c);
+  load_physics(realm, c);
+  return c;
+}
+
+SimulationConfig load_simulation(const Node& doc) {
+  SimulationConfig s;
+  // Simulation::load (Simulation.cpp:36-74)
+  const Node* solvers = doc.FindValue("solvers");
+  if (!solvers) throw std::runtime_error("parser error: at least one solver should be specified!");
+  for (size_t i = 0; i < solvers->size(); i++) {
+    std::string name;
+    convert((*solvers)[i], name);
+    if (name == "cellular_automata") s.cellular_automata = true;
+    if (name == "finite_element_method") s.finite_element_method = true;
+  }
+  // CellularAutomataSolver::load (CellularAutomataSolver.cpp:40-130)
+  const Node* realms = doc.FindValue("realms");
+  if (!realms) throw std::runtime_error("parser error: realms");
+  for (size_t i = 0; i < realms->size(); i++) {
+    const Node& r = (*realms)[i];
+    std::string type;
+    get_required(r, "type", type);
+    if (type == "finite_element") continue;  // FE realms belong to the FE provider
+    s.realms.push_back(load_ca_realm(r));
+  }
+  const Node* tis = doc.FindValue("time_integrators");
+  if (!tis) throw std::runtime_error("parser error: tiem_integrations");
+  for (size_t i = 0; i < tis->size(); i++) {
+    const Node* st = (*tis)[i].FindValue("standard_time_integrator");
+    if (!st) continue;
+    get_required(*st, "termination_time", s.ti.termination_time);
+    get_required(*st, "time_step", s.ti.time_step);
+    get_if_present(*st, "termination_step_count", s.ti.termination_step_count,
+                   s.ti.termination_step_count);
+    get_if_present(*st, "time_step_factor_ca", s.ti.time_step_factor_ca, s.ti.time_step_factor_ca);
+    get_if_present(*st, "time_step_factor", s.ti.time_step_factor, s.ti.time_step_factor);
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+
+PlainRealm::PlainRealm(const RealmConfig& cfg, const TimeIntegratorConfig& ti)
+    : cfg_(cfg), ti_(ti) {
+  grid_ = Grid{cfg.ncell[0], cfg.ncell[1], cfg.ncell[2]};
+}
+
+PlainRealm::~PlainRealm() {
+  if (dom_) amca_domain_destroy(dom_);
+}
+
+void PlainRealm::prepare() {
+  if (prepared_) return;
+  // srand(rank*10000) in the constructor (CellularAutomataManager_3D.cpp:40)
+  GlibcRand rng(cfg_.random_seed);
+  std::unordered_map<int, double> cell_dt;
+  std::vector<int> sites;
+  // surface first, then bulk (:371-373)
+  std::vector<int> surf;
+  surface_cells(grid_, cfg_.surface_faces, surf);
+  const double h = cfg_.h;
+  SiteDraw ds = create_nucleation_sites(
+      (int)surf.size(), [&](int i) { return surf[i]; }, h * h, cfg_.surface.density,
+      cfg_.surface.mean, cfg_.surface.sd, rng, cell_dt, sites);
+  std::vector<int>().swap(surf);
+  const long long nb = (long long)grid_.nx * grid_.ny * grid_.nz;
+  SiteDraw db = create_nucleation_sites((int)nb, BulkCellAt{grid_}, h * h * h, cfg_.bulk.density,
+                                        cfg_.bulk.mean, cfg_.bulk.sd, rng, cell_dt, sites);
+  site_draws_ = (long long)ds.n_draws + db.n_draws;
+  n_surface_ = ds.n_new;
+  n_bulk_ = db.n_new;
+  site_cells_ = sites;
+  site_dt_.resize(sites.size());
+  for (size_t i = 0; i < sites.size(); i++) site_dt_[i] = cell_dt[sites[i]];
+  // One orientation row per possible nucleation event: in the plain path every
+  // site nucleates at most once, and rows are consumed in event order from the
+  // same rand() stream that drew the sites.
+  const int nrows = (int)sites.size();
+  euler_.resize(3 * (size_t)nrows);
+  l2g_.resize(9 * (size_t)nrows);
+  for (int o = 0; o < nrows; o++) {
+    random_orientation(rng, &euler_[3 * (size_t)o]);
+    rotation_from_euler(&euler_[3 * (size_t)o], &l2g_[9 * (size_t)o]);
+  }
+  prepared_ = true;
+}
+
+void PlainRealm::initialize() {
+  prepare();
+  const int nrows = (int)site_cells_.size();
+  const double h = cfg_.h;
+  amca_domain_desc d{};
+  d.nx = grid_.nx;
+  d.ny = grid_.ny;
+  d.nz = grid_.nz;
+  d.h = h;
+  d.x0 = cfg_.origin[0];
+  d.y0 = cfg_.origin[1];
+  d.z0 = cfg_.origin[2];
+  d.T0 = cfg_.T0;
+  d.Tm = cfg_.Tm;
+  d.Tdot = cfg_.Tdot;
+  d.a1 = cfg_.a1;
+  d.a2 = cfg_.a2;
+  d.a3 = cfg_.a3;
+  d.trunc = cfg_.trunc;
+  d.time_step_factor = ti_.time_step_factor_ca;
+  d.max_dt = ti_.time_step;
+  d.final_time = ti_.termination_time;
+  if (amca_domain_create(&d, nrows, &dom_) != AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_create: ") + amca_last_error());
+  if (amca_domain_set_sites(dom_, nrows, site_cells_.data(), site_dt_.data()) != AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_set_sites: ") + amca_last_error());
+  if (amca_domain_push_orientations(dom_, nrows, euler_.data(), l2g_.data()) != AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_push_orientations: ") + amca_last_error());
+}
+
+amca_counters PlainRealm::run(long long max_steps) {
+  amca_counters c{};
+  if (amca_domain_run(dom_, max_steps, &c) != AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_run: ") + amca_last_error());
+  return c;
+}
+
+}  // namespace amca_host

@@ -1,0 +1,88 @@
+// Device-side state of one CA domain and the kernels of the plain step.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace amca {
+
+// Constant description of a domain, passed by value to every kernel.
+struct DevParams {
+  int nx, ny, nz;
+  int nxG, nyG, nzG;
+  int nxyG;
+  int NG;                 // ghosted cell count
+  long long N;            // interior cell count
+  double h, x0, y0, z0;
+  double T0, Tm, Tdot, a1, a2, a3;
+  double trunc, tsf, max_dt, final_time;
+  double sqrt2f;          // (double)sqrtf(2), Octahedron.cpp:260
+  int off[26];            // 26-neighbour offsets in the reference's k,j,i order
+  int orient_cap;
+};
+
+// Mutable scalars that live on the device, so that a whole step (or many
+// steps) runs without a host round trip.
+struct DevCounters {
+  double t;               // time_
+  double dt;              // dT_ of the current step
+  double v;               // growth velocity of the current step
+  long long step;         // iStep_
+  long long n_solid;      // solid interior cells
+  int skip;               // 1: this step is past termination, every kernel returns
+  int done;               // no liquid cell left
+  int parity;             // which active-list buffer is the input of this step
+  int site_sel;           // which site buffer holds the live sites
+  int n_grains, n_orient;
+  int nA_in, nA_out;      // active list lengths (input / survivors)
+  int nW[2];              // capture-wave list lengths
+  int n_sites;
+  int step_nuc, step_cap, step_nA0;
+  int max_wave;
+  int err;                // 1: orientation table exhausted
+  int log_cap;
+  unsigned tickets[8];    // block tickets of the look-back kernels
+};
+
+struct StepLogRow {
+  double time_before, dt;
+  int n_nucleated, n_active_before, n_captured, n_active_after;
+  long long n_liquid;
+};
+
+// Device buffers of one domain (all owned by the handle).
+struct DevBuffers {
+  int* gid;               // cellGrainID_ [NG]
+  int* oid;               // cellOrientationID_ [NG]
+  int* claim;             // min claiming grain id per cell, INT_MAX when free [NG]
+  double4* env;           // grain envelope per cell: xc, yc, zc, L [NG]
+  int* listA[2];          // active lists (cells), ping-pong by step parity [N]
+  int* listW[2];          // capture-wave lists (cells) [N]
+  unsigned long long* masks;  // per wave entry: liquid bits | inside bits << 32 [N]
+  unsigned* res;          // per wave entry: captured bits | keep << 31 [N]
+  int* site_cell[2];      // live nucleation sites, ascending cell, ping-pong
+  double* site_dt[2];
+  double* l2g;            // orientation table, 9 doubles per row
+  unsigned* lb_flag;      // look-back state: epoch<<2 | state   [kLbSites * kMaxBlocks]
+  unsigned long long* lb_agg;
+  unsigned long long* lb_pref;
+  StepLogRow* log;
+  DevCounters* ctr;
+};
+
+constexpr int kBlock = 256;       // threads per block of the list kernels
+constexpr int kMaxBlocks = 1024;  // blocks of a look-back kernel
+constexpr int kLbSites = 4;       // look-back launch sites per step
+
+// Kernel launchers (plain_step.hip).  Each returns hipGetLastError().
+hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t s);
+hipError_t launch_nucleate(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
+hipError_t launch_claim(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                        hipStream_t s);
+hipError_t launch_resolve_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                               hipStream_t s);
+hipError_t launch_tail(const DevParams& P, const DevBuffers& B, hipStream_t s);
+hipError_t launch_envelope_selftest(int n, const double* env4, const double* l2g9,
+                                    const double* cell3, double h, double trunc, double sqrt2f,
+                                    int* inside, double* child4, hipStream_t s);
+
+}  // namespace amca

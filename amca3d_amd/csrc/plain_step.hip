@@ -1,0 +1,618 @@
+// Kernels of the plain CA step (realm type `cellular_automata`) for gfx950.
+//
+// One step is the reference's calculate_time_step + single_integration_step
+// (CellularAutomataManager_3D.cpp:440-669).  The reference sweeps a std::list of
+// active grains in creation order and appends every captured cell's new grain
+// to the same list, so new grains are swept later in the same step.  Grains are
+// always appended, so list order is grain-id order, and the sweep is a
+// breadth-first cascade of "waves":
+//   wave 0      = the active list after nucleation (ids ascending),
+//   wave w + 1  = the grains created while sweeping wave w (ids ascending).
+// Within a wave the first grain (smallest id) whose envelope holds a liquid
+// neighbour takes it; new ids follow (parent id, neighbour slot) order; a grain
+// stays active iff one of its liquid neighbours is outside its envelope and not
+// taken by a smaller id of the same wave.  The kernels compute exactly that:
+//   k_claim         : per wave entry, envelope tests of the 26 neighbours and an
+//                     atomicMin of the grain id into the neighbour's claim word;
+//   k_resolve_emit  : per entry, captured / keep decisions from the claims, a
+//                     single-pass decoupled look-back scan for output positions,
+//                     then the child envelopes and the survivor list;
+//   k_tail          : waves >= 2 (rare) in one work-group, then the time advance.
+// Claims need no reset: every claimed cell is captured in the same wave and is
+// never liquid again.
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#include "envelope.h"
+#include "plain_step.h"
+
+namespace amca {
+
+// ---------------------------------------------------------------------------
+// small device helpers
+
+__device__ __forceinline__ unsigned ld_acquire(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int ld_claim(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Growth velocity of the plain path: uniform undercooling at time t
+// (ProblemPhysics.cpp:117-135 with compute_cell_temperature :163-169).
+__device__ __forceinline__ double plain_velocity(const DevParams& P, double t) {
+  double T = P.T0 + P.Tdot * t;
+  double dT = P.Tm - T;
+  double v = 0.0;
+  if (dT > 0) {
+    v = P.a1 * dT + P.a2 * dT * dT + P.a3 * dT * dT * dT;
+    if (v < 0.0) v = 0.0;
+  }
+  return v;
+}
+
+__device__ __forceinline__ void slot_delta(int s, int& di, int& dj, int& dk) {
+  int idx = s < 13 ? s : s + 1;  // skip the centre of the 3x3x3 block
+  dk = idx / 9 - 1;
+  dj = (idx / 3) % 3 - 1;
+  di = idx % 3 - 1;
+}
+
+__device__ __forceinline__ void cell_ijk(const DevParams& P, int c, int& i, int& j, int& k) {
+  k = c / P.nxyG;
+  int rem = c - k * P.nxyG;
+  j = rem / P.nxG;
+  i = rem - j * P.nxG;
+}
+
+// Exclusive block scan of packed counter pairs (hi 32 | lo 32).  Both halves
+// stay below 2^32, so packed addition never carries between them.
+__device__ unsigned long long block_scan_excl(unsigned long long x, unsigned long long* total,
+                                              unsigned long long* sm) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned long long v = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned long long y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) sm[w] = v;
+  __syncthreads();
+  unsigned long long base = 0, tot = 0;
+  for (int q = 0; q < nw; q++) {
+    unsigned long long s = sm[q];
+    if (q < w) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + v - x;
+}
+
+// Single-pass decoupled look-back (called by thread 0): publishes this
+// block's aggregate, accumulates the predecessors' and returns the exclusive
+// prefix.  `epoch` makes entries of earlier launches invisible.
+__device__ unsigned long long lookback(unsigned* flag, unsigned long long* agg,
+                                       unsigned long long* pref, int bid, unsigned epoch,
+                                       unsigned long long aggregate) {
+  const unsigned tag = (epoch & 0x3fffffffu) << 2;
+  if (bid == 0) {
+    st_relaxed(&pref[0], aggregate);
+    st_release(&flag[0], tag | 2u);
+    return 0ull;
+  }
+  st_relaxed(&agg[bid], aggregate);
+  st_release(&flag[bid], tag | 1u);
+  unsigned long long excl = 0;
+  int p = bid - 1;
+  while (true) {
+    unsigned f = ld_acquire(&flag[p]);
+    if ((f & ~3u) != tag || (f & 3u) == 0) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    if ((f & 3u) == 2u) {
+      excl += ld_relaxed(&pref[p]);
+      break;
+    }
+    excl += ld_relaxed(&agg[p]);
+    p--;
+  }
+  st_relaxed(&pref[bid], excl + aggregate);
+  st_release(&flag[bid], tag | 2u);
+  return excl;
+}
+
+// Contiguous chunk of a list of n entries owned by block `bid` of `nb`.
+__device__ __forceinline__ void chunk_of(int n, int bid, int nb, int& lo, int& hi) {
+  int per = (n + nb - 1) / nb;
+  per = (per + kBlock - 1) / kBlock * kBlock;
+  long long l = (long long)bid * per;
+  lo = (int)(l < n ? l : n);
+  long long h = l + per;
+  hi = (int)(h < n ? h : n);
+}
+
+// ---------------------------------------------------------------------------
+// grid initialisation: CellularAutomataManager::initialize_cells and
+// CellularAutomataManager_3D::initialize_boundaries (ghost faces carry 0).
+
+__global__ void k_init_grid(DevParams P, DevBuffers B) {
+  long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.NG; c += stride) {
+    int k = (int)(c / P.nxyG);
+    int rem = (int)(c - (long long)k * P.nxyG);
+    int j = rem / P.nxG;
+    int i = rem - j * P.nxG;
+    bool ghost = i == 0 || j == 0 || k == 0 || i == P.nxG - 1 || j == P.nyG - 1 || k == P.nzG - 1;
+    int v = ghost ? 0 : -1;
+    B.gid[c] = v;
+    B.oid[c] = v;
+    B.claim[c] = INT_MAX;
+    B.env[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// nucleation (CellularAutomataManager::nucleate_at_nucleation_sites,
+// CellularAutomataManager.cpp:219-297) and the time step
+// (CellularAutomataManager_3D::calculate_time_step, :440-511).
+//
+// Sites are visited in ascending cell order; a liquid site whose undercooling
+// exceeds its critical value nucleates with the next orientation row and the
+// next grain id; a non-liquid site is dropped.  Ranks come from the look-back
+// scan, so ids follow cell order exactly as in the reference loop.
+
+__global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) {
+  DevCounters* ctr = B.ctr;
+  const double t = ctr->t;
+  const int skip = (t < P.final_time && !ctr->done) ? 0 : 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->skip = skip;
+  if (skip) return;
+
+  __shared__ int s_bid;
+  __shared__ unsigned long long s_sm[kBlock / 64];
+  __shared__ unsigned long long s_excl;
+  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[0], 1u);
+  __syncthreads();
+  const int bid = s_bid;
+
+  const int sel = ctr->site_sel;
+  const int n = ctr->n_sites;
+  const int ng0 = ctr->n_grains, no0 = ctr->n_orient, nA0 = ctr->nA_in;
+  const int par = ctr->parity;
+  const int* __restrict__ sc = B.site_cell[sel];
+  const double* __restrict__ sd = B.site_dt[sel];
+  int* __restrict__ nc = B.site_cell[sel ^ 1];
+  double* __restrict__ nd = B.site_dt[sel ^ 1];
+  int* __restrict__ Ain = B.listA[par];
+  const double T = P.T0 + P.Tdot * t;
+  const double under = P.Tm - T;
+
+  int lo, hi;
+  chunk_of(n, bid, gridDim.x, lo, hi);
+  unsigned long long mine = 0;
+  for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
+    int c = sc[i];
+    bool liq = B.gid[c] == -1;
+    bool nuc = liq && (under > sd[i]);
+    mine += ((unsigned long long)nuc << 32) | (unsigned long long)(liq && !nuc);
+  }
+  unsigned long long agg;
+  block_scan_excl(mine, &agg, s_sm);
+  if (threadIdx.x == 0)
+    s_excl = lookback(B.lb_flag, B.lb_agg, B.lb_pref, bid,
+                      (unsigned)(ctr->step * kLbSites + 0), agg);
+  __syncthreads();
+  unsigned long long run = s_excl;
+  for (int base = lo; base < hi; base += kBlock) {
+    int i = base + threadIdx.x;
+    int c = 0;
+    bool nuc = false, keep = false;
+    if (i < hi) {
+      c = sc[i];
+      bool liq = B.gid[c] == -1;
+      nuc = liq && (under > sd[i]);
+      keep = liq && !nuc;
+    }
+    unsigned long long tot;
+    unsigned long long off =
+        run + block_scan_excl(((unsigned long long)nuc << 32) | (unsigned long long)keep, &tot, s_sm);
+    if (nuc) {
+      int r = (int)(off >> 32);
+      int o = no0 + r;
+      if (o < P.orient_cap) {
+        int ci, cj, ck;
+        cell_ijk(P, c, ci, cj, ck);
+        B.env[c] = make_double4(centre_coord(P.x0, P.h, ci), centre_coord(P.y0, P.h, cj),
+                                centre_coord(P.z0, P.h, ck), 0.0);
+        B.gid[c] = ng0 + r;
+        B.oid[c] = o;
+        Ain[nA0 + r] = c;
+      }
+    }
+    if (keep) {
+      int k = (int)(off & 0xffffffffull);
+      nc[k] = c;
+      nd[k] = sd[i];
+    }
+    run += tot;
+  }
+  if (threadIdx.x == 0 && bid == (int)gridDim.x - 1) {
+    const int nn = (int)(run >> 32);
+    const int nk = (int)(run & 0xffffffffull);
+    if (no0 + nn > P.orient_cap) ctr->err = 1;
+    const int nA = nA0 + nn;
+    ctr->n_grains = ng0 + nn;
+    ctr->n_orient = no0 + nn;
+    ctr->nA_in = nA;
+    ctr->n_sites = nk;
+    ctr->site_sel = sel ^ 1;
+    ctr->step_nuc = nn;
+    ctr->step_nA0 = nA;
+    ctr->step_cap = 0;
+    ctr->nA_out = 0;
+    ctr->nW[0] = 0;
+    ctr->nW[1] = 0;
+    // maximum velocity over active grains, floor 1e-10 (:449-506), then
+    // dt = min(factor*h/maxV, maxDt) (:507)
+    const double v = plain_velocity(P, t);
+    double maxV = 1e-10;
+    if (nA > 0 && v > maxV) maxV = v;
+    const double a = P.tsf * P.h / maxV;
+    ctr->dt = (P.max_dt < a) ? P.max_dt : a;
+    ctr->v = v;
+    ctr->tickets[0] = 0u;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// capture waves
+
+__device__ __forceinline__ void wave_input(const DevParams& P, const DevBuffers& B, int wave,
+                                           const int*& list, int& n) {
+  const DevCounters* ctr = B.ctr;
+  if (wave == 0) {
+    list = B.listA[ctr->parity];
+    n = ctr->nA_in;
+  } else {
+    list = B.listW[(wave - 1) & 1];
+    n = ctr->nW[(wave - 1) & 1];
+  }
+}
+
+// Envelope tests of one wave entry.  Wave 0 first grows the grain
+// (Grain::grow, Grain.h:27, over the whole list as in :521-538).
+__device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
+                                                          int c, bool grow, double inc) {
+  const int key = B.gid[c];
+  const int o = B.oid[c];
+  double4 e = B.env[c];
+  if (grow) {
+    e.w = e.w + inc;
+    B.env[c].w = e.w;
+  }
+  int ci, cj, ck;
+  cell_ijk(P, c, ci, cj, ck);
+  double r[9];
+  const double* rp = B.l2g + 9 * (size_t)o;
+#pragma unroll
+  for (int q = 0; q < 9; q++) r[q] = rp[q];
+  unsigned lm = 0, im = 0;
+  int s = 0;
+#pragma unroll
+  for (int dk = -1; dk <= 1; dk++) {
+    const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++) {
+      const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
+#pragma unroll
+      for (int di = -1; di <= 1; di++) {
+        if (dk == 0 && dj == 0 && di == 0) continue;
+        const int nb = c + dk * P.nxyG + dj * P.nxG + di;
+        if (B.gid[nb] == -1) {
+          lm |= 1u << s;
+          const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
+          if (inside_envelope(r, e.w, gx, gy, gz)) {
+            im |= 1u << s;
+            atomicMin(&B.claim[nb], key);
+          }
+        }
+        s++;
+      }
+    }
+  }
+  return ((unsigned long long)im << 32) | lm;
+}
+
+__global__ void __launch_bounds__(kBlock) k_claim(DevParams P, DevBuffers B, int wave) {
+  const DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  const int* list;
+  int n;
+  wave_input(P, B, wave, list, n);
+  const double inc = ctr->dt * ctr->v;
+  const int stride = gridDim.x * blockDim.x;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
+    B.masks[i] = claim_entry(P, B, list[i], wave == 0, inc);
+}
+
+// captured slots and the keep bit of one entry
+__device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevBuffers& B, int c,
+                                                  unsigned long long m) {
+  const int key = B.gid[c];
+  unsigned lm = (unsigned)m, im = (unsigned)(m >> 32);
+  unsigned cap = 0, keep = 0;
+  while (lm) {
+    int s = __builtin_ctz(lm);
+    lm &= lm - 1;
+    const int cl = ld_claim(&B.claim[c + P.off[s]]);
+    if ((im >> s) & 1u) {
+      if (cl == key) cap |= 1u << s;
+    } else if (cl > key) {
+      keep = 1u;
+    }
+  }
+  return cap | (keep << 31);
+}
+
+// Octahedron::capture_cell via CellularAutomataManager_3D::capture_cell
+// (:676-707): the captured cell gets a new grain with the parent orientation.
+__device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffers& B, int c,
+                                              unsigned cap, int id0, int* __restrict__ kids,
+                                              int pos0) {
+  const int o = B.oid[c];
+  const double4 e = B.env[c];
+  int ci, cj, ck;
+  cell_ijk(P, c, ci, cj, ck);
+  double r[9];
+  const double* rp = B.l2g + 9 * (size_t)o;
+#pragma unroll
+  for (int q = 0; q < 9; q++) r[q] = rp[q];
+  int j = 0;
+  while (cap) {
+    int s = __builtin_ctz(cap);
+    cap &= cap - 1;
+    int di, dj, dk;
+    slot_delta(s, di, dj, dk);
+    const int nb = c + P.off[s];
+    const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
+    const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
+    const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
+    double out[4];
+    child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
+    B.env[nb] = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
+    B.gid[nb] = id0 + j;
+    B.oid[nb] = o;
+    kids[pos0 + j] = nb;
+    j++;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_resolve_emit(DevParams P, DevBuffers B, int wave) {
+  DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  const int site = wave == 0 ? 1 : 2;
+  __shared__ int s_bid;
+  __shared__ unsigned long long s_sm[kBlock / 64];
+  __shared__ unsigned long long s_excl;
+  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[site], 1u);
+  __syncthreads();
+  const int bid = s_bid;
+
+  const int* list;
+  int n;
+  wave_input(P, B, wave, list, n);
+  const int par = ctr->parity;
+  int* __restrict__ kids = B.listW[wave & 1];
+  int* __restrict__ surv = B.listA[par ^ 1];
+  const int gbase = ctr->n_grains;
+  const int sbase = ctr->nA_out;
+
+  int lo, hi;
+  chunk_of(n, bid, gridDim.x, lo, hi);
+  unsigned long long mine = 0;
+  for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
+    unsigned r = resolve_entry(P, B, list[i], B.masks[i]);
+    B.res[i] = r;
+    mine += ((unsigned long long)__builtin_popcount(r & 0x3ffffffu) << 32) | (r >> 31);
+  }
+  unsigned long long agg;
+  block_scan_excl(mine, &agg, s_sm);
+  if (threadIdx.x == 0)
+    s_excl = lookback(B.lb_flag + site * kMaxBlocks, B.lb_agg + site * kMaxBlocks,
+                      B.lb_pref + site * kMaxBlocks, bid,
+                      (unsigned)(ctr->step * kLbSites + site), agg);
+  __syncthreads();
+  unsigned long long run = s_excl;
+  for (int base = lo; base < hi; base += kBlock) {
+    int i = base + threadIdx.x;
+    unsigned r = 0;
+    int c = 0;
+    if (i < hi) {
+      r = B.res[i];
+      c = list[i];
+    }
+    const unsigned cap = r & 0x3ffffffu;
+    unsigned long long tot;
+    unsigned long long off =
+        run + block_scan_excl(((unsigned long long)__builtin_popcount(cap) << 32) | (r >> 31),
+                              &tot, s_sm);
+    if (r >> 31) surv[sbase + (int)(off & 0xffffffffull)] = c;
+    if (cap) {
+      const int pos = (int)(off >> 32);
+      emit_children(P, B, c, cap, gbase + pos, kids, pos);
+    }
+    run += tot;
+  }
+  if (threadIdx.x == 0 && bid == (int)gridDim.x - 1) {
+    const int nc = (int)(run >> 32);
+    const int nk = (int)(run & 0xffffffffull);
+    ctr->n_grains = gbase + nc;
+    ctr->nA_out = sbase + nk;
+    ctr->nW[wave & 1] = nc;
+    ctr->step_cap += nc;
+    if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
+    ctr->tickets[site] = 0u;
+  }
+}
+
+// Waves from 2 on, in one work-group (these are rare and short), then the end
+// of the step: time_ += dT_, iStep_++, liquid count (:650-659).
+constexpr int kTailBlock = 1024;
+
+__global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) {
+  DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ unsigned long long s_sm[kTailBlock / 64];
+  __shared__ int s_n, s_gbase, s_sbase;
+  int wave = 2;
+  while (true) {
+    const int in = (wave - 1) & 1;
+    if (threadIdx.x == 0) {
+      s_n = ctr->nW[in];
+      s_gbase = ctr->n_grains;
+      s_sbase = ctr->nA_out;
+    }
+    __syncthreads();
+    const int n = s_n;
+    if (n == 0) break;
+    const int* list = B.listW[in];
+    int* kids = B.listW[wave & 1];
+    int* surv = B.listA[ctr->parity ^ 1];
+    for (int i = threadIdx.x; i < n; i += kTailBlock) B.masks[i] = claim_entry(P, B, list[i], false, 0.0);
+    __threadfence();
+    __syncthreads();
+    unsigned long long run = 0;
+    for (int base = 0; base < n; base += kTailBlock) {
+      int i = base + threadIdx.x;
+      unsigned r = 0;
+      int c = 0;
+      if (i < n) {
+        c = list[i];
+        r = resolve_entry(P, B, c, B.masks[i]);
+      }
+      const unsigned cap = r & 0x3ffffffu;
+      unsigned long long tot;
+      unsigned long long off =
+          run + block_scan_excl(((unsigned long long)__builtin_popcount(cap) << 32) | (r >> 31),
+                                &tot, s_sm);
+      if (r >> 31) surv[s_sbase + (int)(off & 0xffffffffull)] = c;
+      if (cap) {
+        const int pos = (int)(off >> 32);
+        emit_children(P, B, c, cap, s_gbase + pos, kids, pos);
+      }
+      run += tot;
+    }
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      const int nc = (int)(run >> 32);
+      ctr->n_grains = s_gbase + nc;
+      ctr->nA_out = s_sbase + (int)(run & 0xffffffffull);
+      ctr->nW[wave & 1] = nc;
+      ctr->step_cap += nc;
+      if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
+    }
+    __syncthreads();
+    wave++;
+  }
+  if (threadIdx.x == 0) {
+    const double t0 = ctr->t;
+    const double dt = ctr->dt;
+    ctr->t = t0 + dt;
+    const long long st = ctr->step;
+    ctr->step = st + 1;
+    const long long solid = ctr->n_solid + ctr->step_nuc + ctr->step_cap;
+    ctr->n_solid = solid;
+    const long long liquid = P.N - solid;
+    if (liquid == 0) ctr->done = 1;
+    if (B.log && st < ctr->log_cap) {
+      StepLogRow row;
+      row.time_before = t0;
+      row.dt = dt;
+      row.n_nucleated = ctr->step_nuc;
+      row.n_active_before = ctr->step_nA0;
+      row.n_captured = ctr->step_cap;
+      row.n_active_after = ctr->nA_out;
+      row.n_liquid = liquid;
+      B.log[st] = row;
+    }
+    ctr->parity ^= 1;
+    ctr->nA_in = ctr->nA_out;
+    ctr->nA_out = 0;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+__global__ void k_envelope_selftest(int n, const double* env4, const double* l2g9,
+                                    const double* cell3, double h, double trunc, double sqrt2f,
+                                    int* inside, double* child4) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double* e = env4 + 4 * (size_t)i;
+  const double* r = l2g9 + 9 * (size_t)i;
+  const double* x = cell3 + 3 * (size_t)i;
+  double gx = x[0] - e[0], gy = x[1] - e[1], gz = x[2] - e[2];
+  bool in = inside_envelope(r, e[3], gx, gy, gz);
+  inside[i] = in ? 1 : 0;
+  double out[4] = {0.0, 0.0, 0.0, 0.0};
+  if (in) child_envelope(r, e[3], h, trunc, sqrt2f, gx, gy, gz, out);
+  child4[4 * (size_t)i + 0] = in ? out[0] + e[0] : 0.0;
+  child4[4 * (size_t)i + 1] = in ? out[1] + e[1] : 0.0;
+  child4[4 * (size_t)i + 2] = in ? out[2] + e[2] : 0.0;
+  child4[4 * (size_t)i + 3] = out[3];
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+
+hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t s) {
+  long long blocks = (P.NG + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_init_grid, dim3((unsigned)blocks), dim3(256), 0, s, P, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_nucleate(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_nucleate, dim3(blocks), dim3(kBlock), 0, s, P, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_claim(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                        hipStream_t s) {
+  hipLaunchKernelGGL(k_claim, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
+  return hipGetLastError();
+}
+
+hipError_t launch_resolve_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                               hipStream_t s) {
+  hipLaunchKernelGGL(k_resolve_emit, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
+  return hipGetLastError();
+}
+
+hipError_t launch_tail(const DevParams& P, const DevBuffers& B, hipStream_t s) {
+  hipLaunchKernelGGL(k_tail, dim3(1), dim3(kTailBlock), 0, s, P, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_envelope_selftest(int n, const double* env4, const double* l2g9,
+                                    const double* cell3, double h, double trunc, double sqrt2f,
+                                    int* inside, double* child4, hipStream_t s) {
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_envelope_selftest, dim3((n + 255) / 256), dim3(256), 0, s, n, env4, l2g9,
+                     cell3, h, trunc, sqrt2f, inside, child4);
+  return hipGetLastError();
+}
+
+}  // namespace amca

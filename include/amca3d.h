@@ -1,0 +1,204 @@
+/*
+ * amca3d.h -- C ABI of the MI355X (gfx950) cellular-automaton library.
+ *
+ * The reference (AMCA3D) has no FFI: its hot path sits behind the C++ virtual
+ * interface `CellularAutomataManager` (CA/include/CellularAutomataManager.h:20-246)
+ * and is driven by `CellularAutomataSolver` / `CafeSolver`.  The entry points
+ * below are what a C++ subclass of that interface (or any other host) binds to
+ * replace the per-cell loops; each cites the reference member it replaces.
+ *
+ * Conventions
+ *   - Every call returns an amca_status; no C++ exception crosses the ABI.
+ *     amca_last_error() gives a text for the last failure on this thread.
+ *   - Device memory is owned by the handle.  Host pointers passed in are only
+ *     borrowed for the duration of the call.
+ *   - One host thread per handle; work is ordered on the handle's own HIP
+ *     stream, handles are independent.
+ *   - Cell indices are "local-ghost" indices of the reference:
+ *     c = k*(nx+2)*(ny+2) + j*(nx+2) + i with i,j,k in [0, n+1], ghosts on the
+ *     faces (CellularAutomataManager_3D.cpp:876-888 at one rank).
+ */
+#ifndef AMCA3D_H
+#define AMCA3D_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  AMCA_OK = 0,
+  AMCA_ERR_INVALID = 1,     /* bad argument or state */
+  AMCA_ERR_HIP = 2,         /* a HIP runtime call failed */
+  AMCA_ERR_CAPACITY = 3,    /* a fixed-capacity table ran out */
+  AMCA_ERR_NO_DEVICE = 4,   /* no GPU visible */
+  AMCA_ERR_IO = 5,          /* file or parse error (host layer) */
+} amca_status;
+
+typedef struct amca_domain amca_domain;
+
+/* Domain description: geometry, kinetics and stepping constants of one realm.
+ * Fields mirror the reference members they come from. */
+typedef struct {
+  int32_t nx, ny, nz;         /* interior cells (nx_, ny_, nz_) */
+  double h;                   /* cell size (h_) */
+  double x0, y0, z0;          /* original point (x0_, y0_, z0_) */
+  double T0, Tm, Tdot;        /* initial, melting temperature, cooling rate */
+  double a1, a2, a3;          /* RappazGandin1993 kinetics (ProblemPhysics.cpp:117-135) */
+  double trunc;               /* envelope truncation coefficient (Octahedron.cpp:25,319) */
+  double time_step_factor;    /* timeStepFactor_ (CellularAutomataManager.cpp:44) */
+  double max_dt;              /* maxTimestep_ (time_step key) */
+  double final_time;          /* termination_time */
+} amca_domain_desc;
+
+/* Counters after a step or a run (replace the per-step log lines of
+ * CellularAutomataManager_3D.cpp:658-667). */
+typedef struct {
+  int64_t step;               /* iStep_ */
+  double time;                /* time_ */
+  double dt;                  /* dT_ of the last step taken */
+  int64_t n_liquid;           /* get_amount_of_liquid_cells (:2039-2061) */
+  int32_t n_grains;           /* grainVector_.size() */
+  int32_t n_orient;           /* orientationVector_.size() */
+  int32_t n_active;           /* activeGrainList_.size() */
+  int32_t n_sites;            /* live nucleation sites */
+  int32_t done;               /* solidificationDone_ or time >= termination_time */
+  int32_t max_wave;           /* deepest same-step capture cascade seen */
+} amca_counters;
+
+/* One row of the optional per-step log. */
+typedef struct {
+  double time_before, dt;
+  int32_t n_nucleated, n_active_before, n_captured, n_active_after;
+  int64_t n_liquid;
+} amca_step_record;
+
+/* ---- domain lifetime ---------------------------------------------------- */
+
+/* Allocates the ghosted grid on the current device and fills it like
+ * CellularAutomataManager_3D::setup_cells + initialize_boundaries
+ * (CellularAutomataManager_3D.cpp:404-435, 735-833; CellularAutomataManager.cpp:303-326):
+ * every cell liquid (-1), ghost faces grain/orientation 0.
+ * `orient_capacity` bounds the orientation table (nucleation events). */
+amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capacity,
+                               amca_domain** out);
+
+/* Frees the device memory (CellularAutomataManager.cpp:73-102). */
+amca_status amca_domain_destroy(amca_domain* d);
+
+/* ---- inputs drawn on the host -------------------------------------------- */
+
+/* Nucleation sites, sorted ascending by cell, with their critical
+ * undercooling: the output of create_nucleation_sites
+ * (CellularAutomataManager.cpp:153-213). */
+amca_status amca_domain_set_sites(amca_domain* d, int32_t n, const int32_t* cell,
+                                  const double* dT_crit);
+
+/* Orientation rows in nucleation-event order: Euler angles and the row-major
+ * local-to-global rotation of Octahedron::set_coordinate_transformation_matrix
+ * (Octahedron.cpp:281-310), drawn as in get_random_orientation_angle
+ * (CellularAutomataManager_3D.cpp:838-850).  Appends after rows already set. */
+amca_status amca_domain_push_orientations(amca_domain* d, int32_t n, const double* euler3,
+                                          const double* l2g9);
+
+/* ---- stepping ------------------------------------------------------------ */
+
+/* One calculate_time_step + single_integration_step
+ * (CellularAutomataManager_3D.cpp:440-669): nucleation, dt, growth, capture
+ * cascade, time advance, liquid count.  A step after termination is a no-op. */
+amca_status amca_domain_step(amca_domain* d, amca_counters* out);
+
+/* CellularAutomataManager::time_integration (CellularAutomataManager.cpp:134-147):
+ * steps until time >= termination_time or no liquid cell remains, or until
+ * `max_steps` steps were taken in this call (max_steps < 0: no limit). */
+amca_status amca_domain_run(amca_domain* d, int64_t max_steps, amca_counters* out);
+
+/* Same as amca_domain_run but takes exactly `n_steps` step launches without
+ * reading anything back in between (steps after termination are no-ops).
+ * Used for timing. */
+amca_status amca_domain_run_fixed(amca_domain* d, int64_t n_steps);
+
+amca_status amca_domain_counters(amca_domain* d, amca_counters* out);
+
+/* Per-step log: enable before stepping (capacity rows, 0 disables), then read. */
+amca_status amca_domain_enable_log(amca_domain* d, int32_t capacity);
+amca_status amca_domain_read_log(amca_domain* d, int32_t cap, amca_step_record* rows,
+                                 int32_t* n_rows);
+
+/* ---- outputs ------------------------------------------------------------- */
+
+/* Copies the state to host buffers (any pointer may be NULL), the data of
+ * output_microstructure_information (CellularAutomataManager_3D.cpp:1835-1913):
+ *   orient[ng], grain[ng]   cellOrientationID_, cellGrainID_  (ng = (nx+2)(ny+2)(nz+2))
+ *   env[4*ng]               per-cell grain envelope xc, yc, zc, L (FP64)
+ *   site_cell[n], site_dt[n] live nucleation sites (n <= site_cap) */
+amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain, double* env,
+                                 int32_t site_cap, int32_t* site_cell, double* site_dt,
+                                 int32_t* n_sites);
+
+/* Device pointers of the grain-ID fields and the stream (for callers that keep
+ * the data on the GPU).  Valid until amca_domain_destroy. */
+amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
+                                      void** stream);
+
+/* ---- kernel timing (HIP events on the handle's stream) -------------------- */
+
+/* Enables accumulation of per-kernel durations (milliseconds) for the named
+ * kernels of the step; amca_domain_kernel_times returns totals and launch
+ * counts in a fixed order given by amca_kernel_name(i). */
+amca_status amca_domain_enable_timing(amca_domain* d, int32_t on);
+int32_t amca_kernel_count(void);
+const char* amca_kernel_name(int32_t i);
+amca_status amca_domain_kernel_times(amca_domain* d, double* ms, int64_t* launches);
+
+/* ---- device arithmetic self-check ---------------------------------------- */
+
+/* Runs the device capture test (Octahedron::cell_is_captured, Octahedron.cpp:98-128)
+ * and child-envelope construction (Octahedron::capture_cell, Octahedron.cpp:133-275)
+ * on n inputs: env4[4n] parent centre+L, l2g9[9n], cell3[3n]; writes
+ * inside[n] and child4[4n] (child centre and L, only where inside). */
+amca_status amca_envelope_selftest(int32_t n, const double* env4, const double* l2g9,
+                                   const double* cell3, double h, double trunc,
+                                   int32_t* inside, double* child4);
+
+const char* amca_last_error(void);
+const char* amca_version(void);
+
+/* ---- realm layer: YAML input deck -> device domain ------------------------
+ * The reference's Simulation / CellularAutomataSolver layer
+ * (Simulation.cpp:36-93, CellularAutomataSolver.cpp:40-202) with the same
+ * realm keys; host RNG draws (sites, orientations) happen here. */
+typedef struct amca_sim amca_sim;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  double h;
+  double termination_time, time_step;
+  int32_t n_sites, n_surface_sites, n_bulk_sites;
+  int64_t n_site_draws;
+} amca_sim_info;
+
+/* Simulation::load on a YAML document held in memory / in a file. */
+amca_status amca_sim_create(const char* yaml_text, amca_sim** out);
+amca_status amca_sim_create_from_file(const char* path, amca_sim** out);
+/* The host half of CellularAutomataManager_3D::initialize (:366-379): the
+ * rand()-stream draws of nucleation sites and orientation rows.  Needs no GPU. */
+amca_status amca_sim_prepare(amca_sim* s);
+/* Simulation::initialize: prepare (if not done) and upload to a new domain. */
+amca_status amca_sim_initialize(amca_sim* s);
+/* Simulation::run -> time_integration; max_steps < 0 runs to termination. */
+amca_status amca_sim_run(amca_sim* s, int64_t max_steps, amca_counters* out);
+amca_domain* amca_sim_domain(amca_sim* s);
+amca_status amca_sim_get_info(amca_sim* s, amca_sim_info* info);
+/* The sites and orientation rows drawn at initialisation (for checking). */
+amca_status amca_sim_initial_sites(amca_sim* s, int32_t cap, int32_t* cell, double* dT_crit,
+                                   double* euler3, int32_t* n);
+amca_status amca_sim_destroy(amca_sim* s);
+const char* amca_sim_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* AMCA3D_H */

@@ -1,0 +1,68 @@
+/* ORACLE -- test infrastructure only (CPU restatement of the reference path).
+ * Only tests/, the smoke check and the benchmark's CPU-baseline leg load this. */
+#ifndef AMCA_ORACLE_H
+#define AMCA_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int32_t nx, ny, nz;
+  double h, x0, y0, z0;
+  double ns, dTs_mean, dTs_sd;   /* surface site density / Gaussian undercooling */
+  double nv, dTv_mean, dTv_sd;   /* bulk */
+  int32_t surface_mode;          /* 0: the reference's six-face list, 1: z-min face only */
+  uint32_t seed;                 /* srand() seed; 0 is the np=1 reference value */
+  double T0, Tm, Tdot, a1, a2, a3;
+  double trunc;                  /* envelope truncation coefficient (float-rounded) */
+  double tsf, max_dt, final_time;
+  int32_t max_steps;             /* < 0: run until final_time or full solidification */
+} oracle_plain_params;
+
+typedef struct {
+  double time_before, dt;
+  int32_t n_nucleated, n_active_before, n_captured, n_active_after, n_liquid, max_wave;
+} oracle_step_record;
+
+typedef struct {
+  /* caller-provided buffers (any may be NULL) */
+  int32_t *ori, *gid;            /* [ng] */
+  double *nuc_dt;                /* [ng] */
+  double *grain_env;             /* [4*grain_cap] xc,yc,zc,L */
+  int32_t *grain_cell, *grain_orient;
+  uint8_t *grain_active;
+  int32_t grain_cap;
+  double *euler;                 /* [3*orient_cap] */
+  int32_t orient_cap;
+  int32_t *site_cell; double *site_dt; int32_t site_cap;
+  oracle_step_record *steps; int32_t steps_cap;
+  /* results */
+  int32_t ng, n_steps, n_records, done, n_grains, n_orient, n_active, n_sites;
+  int64_t n_site_draws;
+  double time;
+} oracle_plain_out;
+
+int oracle_plain_run(const oracle_plain_params* prm, oracle_plain_out* out);
+/* same loop with explicit sites and orientation angles (consumed in
+ * nucleation order) instead of the rand() draws */
+int oracle_plain_run_with(const oracle_plain_params* prm, int n_sites, const int32_t* site_cell,
+                          const double* site_dt, int n_orient, const double* euler3,
+                          oracle_plain_out* out);
+void oracle_rotation(const double* euler3, double* l2g9);
+int oracle_capture_test(const double* env4, const double* l2g9, const double* cell3);
+void oracle_child_envelope(const double* gc3, double L, double h, double trunc,
+                           const double* l2g9, double* out4);
+void oracle_glibc_rand(uint32_t seed, int n, int32_t* out);
+void oracle_std_normal(double mean, double sd, int n, double* out);
+void oracle_minstd(int n, int64_t* out);
+float oracle_sqrtf(float x);
+/* capture test + child envelope over n independent inputs */
+void oracle_envelope_batch(int n, const double* env4, const double* l2g9, const double* cell3,
+                           double h, double trunc, int32_t* inside, double* child4);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,219 @@
+"""GPU parity of the plain CA step (realm type `cellular_automata`).
+
+Every case runs the HIP path through the C ABI and the CPU oracle on the same
+input deck (same seed, same host-drawn sites and orientations) and requires
+bit-exact equality of the grain-ID fields (cellOrientationID_, cellGrainID_),
+of the FP64 envelope (centre and half-diagonal) of every grain, of the live
+nucleation sites and of the per-step log (time, dt, counts).  Full-size runs are
+checked through size-independent invariants.
+"""
+import numpy as np
+import pytest
+
+from amca3d_amd import decks
+
+pytestmark = pytest.mark.gpu
+
+TRUNC = float(np.sqrt(np.float32(3.0)))
+
+
+def case(nx, ny, nz, h, *, surface=(0.0, 2.0, 0.5), faces="all", bulk=(0.0, 3.0, 1.0),
+         T0=-0.2, Tdot=-20.0, time_step=1e-3, termination_time=17.81, seed=0, lateral=None):
+    text = decks.plain_deck(nx, ny, nz, h, surface=surface, surface_faces=faces, bulk=bulk,
+                            T0=T0, Tdot=Tdot, time_step=time_step,
+                            termination_time=termination_time, random_seed=seed or None,
+                            lateral=lateral)
+    prm = dict(nx=nx, ny=ny, nz=nz, h=h, ns=surface[0], dTs_mean=surface[1], dTs_sd=surface[2],
+               nv=bulk[0], dTv_mean=bulk[1], dTv_sd=bulk[2], surface_mode=1 if faces == "z_min" else 0,
+               seed=seed, T0=T0, Tdot=Tdot, max_dt=time_step, final_time=termination_time)
+    return text, prm
+
+
+def run_gpu(amca, text, log_cap=1 << 16):
+    s = amca.Simulation(text).initialize()
+    d = s.domain
+    d.enable_log(log_cap)
+    c = s.run(-1)
+    out = d.download()
+    out["log"] = d.read_log()
+    out["counters"] = c
+    out["initial_sites"] = s.initial_sites()
+    s.close()
+    return out
+
+
+def compare(oracle, prm_kw, g, r=None):
+    p = oracle.plain_params(**prm_kw)
+    if r is None:
+        r = oracle.run_plain(p)
+    assert g["counters"]["step"] == r["n_steps"]
+    assert g["counters"]["time"] == r["time"]
+    assert g["counters"]["n_grains"] == r["n_grains"]
+    assert g["counters"]["n_orient"] == r["n_orient"]
+    assert g["counters"]["n_active"] == r["n_active"]
+    assert np.array_equal(g["ori"], r["ori"]), "orientation (grain-ID) field differs"
+    assert np.array_equal(g["gid"], r["gid"]), "grain id field differs"
+    # envelopes, per cell
+    env_cell = np.zeros((len(r["ori"]), 4))
+    env_cell[r["grain_cell"]] = r["env"]
+    cells = r["grain_cell"]
+    assert np.array_equal(g["env"][cells], env_cell[cells]), "FP64 envelopes differ"
+    # live sites and the erased ones (dT set to 0)
+    sc, sd, _ = g["initial_sites"]
+    live = dict(zip(g["sites_cell"].tolist(), g["sites_dt"].tolist()))
+    nuc = np.full(len(r["ori"]), -1.0)
+    nuc[sc] = [live.get(c, 0.0) for c in sc.tolist()]
+    assert np.array_equal(nuc, r["nuc_dt"]), "nucleation-site table differs"
+    # per-step log
+    lg, st = g["log"], r["steps"]
+    n = min(len(lg), len(st))
+    assert n == r["n_steps"]
+    for f in ("time_before", "dt", "n_nucleated", "n_active_before", "n_captured",
+              "n_active_after", "n_liquid"):
+        assert np.array_equal(lg[f][:n], st[f][:n].astype(lg[f].dtype)), f
+    return r
+
+
+def test_device_envelope_matches_oracle(amca, oracle):
+    """Capture test and child envelope on 400k random inputs, many on or near
+    the envelope boundary: identical booleans and identical doubles."""
+    rng = np.random.default_rng(20261202)
+    n = 400_000
+    h = 2e-6
+    eul = rng.uniform([0, 0, 0], [2 * np.pi, np.pi, np.pi / 2], size=(512, 3))
+    R = oracle.rotations(eul)[rng.integers(0, 512, n)]
+    centre = rng.uniform(0, 40 * h, size=(n, 3))
+    L = rng.uniform(0.0, 4 * h, size=n)
+    L[:1000] = 0.0
+    cells = (np.floor(centre / h + rng.integers(-2, 3, size=(n, 3))) + 0.5) * h
+    env = np.column_stack([centre, L])
+    # a quarter exactly on the boundary: L = ||R^T (x - c)||_1 evaluated in order
+    k = n // 4
+    d = cells[:k] - centre[:k]
+    Rm = R[:k].reshape(-1, 3, 3)
+    loc = (d[:, 0:1] * Rm[:, 0, :] + d[:, 1:2] * Rm[:, 1, :]) + d[:, 2:3] * Rm[:, 2, :]
+    env[:k, 3] = np.abs(loc).sum(axis=1)
+    gi, gc = amca.envelope_selftest(env, R, cells, h, TRUNC)
+    oi, oc = oracle.envelope_batch(env, R, cells, h, TRUNC)
+    assert np.array_equal(gi, oi)
+    assert gi.sum() > n // 4
+    assert np.array_equal(gc, oc)
+
+
+def test_nucleation_example_parity(amca, oracle):
+    """The reference's nucleation example deck (60^3, 270 bulk draws) to full
+    solidification."""
+    g = run_gpu(amca, decks.nucleation_example_deck())
+    prm = dict(nx=60, ny=60, nz=60, h=0.0005, nv=1e7, dTv_mean=3.0, dTv_sd=1.0,
+               ns=0.0, dTs_mean=2.0, dTs_sd=0.5, T0=-0.2, Tdot=-20.0, max_dt=1e-3,
+               final_time=17.81)
+    r = compare(oracle, prm, g)
+    assert r["done"] == 1 and g["counters"]["n_liquid"] == 0
+
+
+SMALL = [
+    dict(nx=24, ny=24, nz=24, h=2e-6, surface=(1e9, 1.0, 0.5), faces="z_min",
+         bulk=(5e13, 3.0, 1.0), Tdot=-200.0, time_step=1e-4),
+    dict(nx=17, ny=23, nz=11, h=2e-6, surface=(2e9, 1.0, 0.5), faces="all",
+         bulk=(8e13, 3.0, 1.0), Tdot=-500.0, time_step=1e-4),
+    dict(nx=20, ny=9, nz=24, h=5e-6, surface=(0.0, 1.0, 0.5), faces="all",
+         bulk=(3e12, 2.0, 1.0), Tdot=-2000.0, time_step=1e-4, T0=-1.0),
+]
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2, 3, 11])
+@pytest.mark.parametrize("ci", range(len(SMALL)))
+def test_small_grids_all_seeds(amca, oracle, seed, ci):
+    text, prm = case(**SMALL[ci], seed=seed)
+    g = run_gpu(amca, text)
+    compare(oracle, prm, g)
+
+
+def test_uniform_cooling_96_parity(amca, oracle):
+    """The 512^3 uniform-cooling workload's physics (2e-6 m cells, -200 K/s,
+    5e13 m^-3 bulk, 1e9 m^-2 on z = 0) on a 96^3 grid, to full solidification."""
+    text, prm = case(96, 96, 96, 2e-6, surface=(1e9, 1.0, 0.5), faces="z_min",
+                     bulk=(5e13, 3.0, 1.0), Tdot=-200.0, time_step=1e-4, termination_time=1.0)
+    g = run_gpu(amca, text)
+    r = compare(oracle, prm, g)
+    assert r["done"] == 1
+
+
+def test_given_sites_identity_nucleus(amca, oracle):
+    """Low-level ABI with explicit sites and orientation rows; one identity
+    nucleus grows to full solidification of a 21^3 box."""
+    n = 21
+    p = oracle.plain_params(nx=n, ny=n, nz=n, h=1e-6, T0=-5.0, Tdot=-200.0, max_dt=1e-4,
+                            final_time=1.0)
+    c = oracle.cell_index(p, 11, 11, 11)
+    eul = np.zeros((1, 3))
+    r = oracle.run_plain(p, given=([c], [0.5], eul))
+    d = amca.OwnedDomain(nx=n, ny=n, nz=n, h=1e-6, orient_capacity=1, T0=-5.0, Tdot=-200.0,
+                         max_dt=1e-4, final_time=1.0)
+    d.set_sites([c], [0.5])
+    d.push_orientations(eul, oracle.rotations(eul))
+    cnt = d.run(-1)
+    out = d.download()
+    assert cnt["step"] == r["n_steps"] and cnt["done"] == 1
+    assert np.array_equal(out["ori"], r["ori"]) and np.array_equal(out["gid"], r["gid"])
+    env_cell = np.zeros((len(r["ori"]), 4))
+    env_cell[r["grain_cell"]] = r["env"]
+    assert np.array_equal(out["env"][r["grain_cell"]], env_cell[r["grain_cell"]])
+    d.close()
+
+
+def test_stepwise_equals_run_and_repeatable(amca, oracle):
+    text, prm = case(**SMALL[1], seed=5)
+    a = run_gpu(amca, text)
+    s = amca.Simulation(text).initialize()
+    d = s.domain
+    while True:
+        c = d.step()
+        if c["done"]:
+            break
+    b = d.download()
+    s.close()
+    assert np.array_equal(a["ori"], b["ori"]) and np.array_equal(a["gid"], b["gid"])
+    assert np.array_equal(a["env"], b["env"])
+    a2 = run_gpu(amca, text)
+    assert np.array_equal(a["env"], a2["env"]) and np.array_equal(a["gid"], a2["gid"])
+
+
+def test_edge_cases(amca, oracle):
+    # no sites: every step is empty until the termination time
+    text, prm = case(6, 5, 4, 1e-3, termination_time=0.0105, time_step=1e-3)
+    g = run_gpu(amca, text)
+    r = compare(oracle, prm, g)
+    assert g["counters"]["step"] == 11 and g["counters"]["n_liquid"] == 6 * 5 * 4
+    # a single cell
+    text, prm = case(1, 1, 1, 1e-3, bulk=(1e12, 1.0, 0.1))
+    g = run_gpu(amca, text)
+    compare(oracle, prm, g)
+    assert g["counters"]["n_grains"] == 1 and g["counters"]["n_active"] == 0
+    # a line of cells with a saturated site density (every cell a site)
+    text, prm = case(40, 1, 1, 1e-4, bulk=(1e13, 3.0, 1.0))
+    g = run_gpu(amca, text)
+    compare(oracle, prm, g)
+
+
+def test_full_size_512_invariants(amca):
+    """512^3 cells (the benchmark workload) for 120 steps: counters agree with
+    the downloaded fields, ids are a permutation, orientations are in range."""
+    s = amca.Simulation(decks.uniform_cooling_512_deck()).initialize()
+    d = s.domain
+    c = s.run(120)
+    assert c["step"] == 120
+    out = d.download(env=False)
+    n = 512
+    ori = out["ori"].reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
+    gid = out["gid"].reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
+    solid = ori >= 0
+    nsol = int(solid.sum())
+    assert nsol == n ** 3 - c["n_liquid"] == c["n_grains"]
+    assert c["n_grains"] > 100_000
+    g = gid[solid]
+    assert g.min() == 0 and g.max() == c["n_grains"] - 1
+    assert np.bincount(g, minlength=c["n_grains"]).max() == 1
+    assert ori.max() < c["n_orient"]
+    assert ((gid < 0) == (ori < 0)).all()
+    s.close()

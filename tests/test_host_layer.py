@@ -1,0 +1,119 @@
+"""Host layer of libamca3d (no GPU): input-deck parsing with the reference's
+key semantics and the host RNG draws (sites, orientations), checked against
+the oracle's draws from the same deck."""
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+from amca3d_amd import decks
+from amca3d_amd._lib import CLI_PATH, LIB_PATH, SIGNATURES, AmcaError
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def test_library_exports_every_declared_symbol(amca):
+    """Every function declared in include/amca3d.h is exported and bound."""
+    hdr = open(os.path.join(ROOT, "include", "amca3d.h")).read()
+    hdr = re.sub(r"/\*.*?\*/", "", hdr, flags=re.S)
+    names = set(re.findall(r"\b(amca_[a-z0-9_]+)\s*\(", hdr))
+    assert len(names) >= 25
+    lib = amca.load()
+    for n in names:
+        assert hasattr(lib, n), n
+        assert n in SIGNATURES, n
+    out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
+    exported = set(re.findall(r" T (amca_\w+)", out))
+    assert names <= exported
+
+
+def test_version(amca):
+    assert b"gfx950" in amca.load().amca_version()
+
+
+def test_nucleation_example_deck_semantics(amca):
+    """lateral_sizes wins over number_cells by truncation (0.03/5e-4 -> 60) and
+    `10e6` reads as 1e7: the example draws 270 bulk sites."""
+    s = amca.Simulation(decks.nucleation_example_deck()).prepare()
+    inf = s.info()
+    assert (inf["nx"], inf["ny"], inf["nz"]) == (60, 60, 60)
+    assert inf["h"] == 0.0005
+    assert inf["n_site_draws"] == 270
+    assert inf["termination_time"] == 17.81 and inf["time_step"] == 0.001
+
+
+@pytest.mark.parametrize("faces,seed", [("all", 0), ("all", 3), ("z_min", 0)])
+def test_host_draws_match_oracle(amca, oracle, faces, seed):
+    """Sites (cells, critical undercooling) and orientation angles drawn by the
+    host layer equal the oracle's draws bit for bit."""
+    n = 24
+    text = decks.plain_deck(n, n + 4, n - 2, 2e-6, surface=(1e9, 1.0, 0.5), surface_faces=faces,
+                            bulk=(5e13, 3.0, 1.0), Tdot=-200.0, time_step=1e-4,
+                            random_seed=seed if seed else None)
+    s = amca.Simulation(text).prepare()
+    cell, dt, eu = s.initial_sites()
+    p = oracle.plain_params(nx=n, ny=n + 4, nz=n - 2, h=2e-6, ns=1e9, dTs_mean=1.0, dTs_sd=0.5,
+                            nv=5e13, dTv_mean=3.0, dTv_sd=1.0, Tdot=-200.0, max_dt=1e-4,
+                            surface_mode=1 if faces == "z_min" else 0, seed=seed, max_steps=0)
+    r = oracle.run_plain(p, want_grains=False)
+    assert np.array_equal(cell, r["sites_cell"])
+    assert np.array_equal(dt, r["sites_dt"])
+    assert s.info()["n_site_draws"] == r["n_site_draws"]
+    # orientation rows: the oracle draws them only at nucleation, so compare with
+    # a run long enough to nucleate everything
+    p.max_steps = -1
+    p.final_time = 0.05
+    r2 = oracle.run_plain(p, want_grains=False)
+    k = r2["n_orient"]
+    assert k > 0
+    assert np.array_equal(eu[:k], r2["euler"][:k])
+
+
+def test_surface_rows_skip_the_reference_way(amca, oracle):
+    """With nx < ny the six-face list skips interior row j = nx+1 (the row test
+    of get_surface_cells compares with the x extent)."""
+    text = decks.plain_deck(6, 12, 5, 1e-4, surface=(4e7, 1.0, 0.5), bulk=(0.0, 3.0, 1.0))
+    s = amca.Simulation(text).prepare()
+    cell, dt, _ = s.initial_sites()
+    p = oracle.plain_params(nx=6, ny=12, nz=5, h=1e-4, ns=4e7, dTs_mean=1.0, dTs_sd=0.5,
+                            max_steps=0)
+    r = oracle.run_plain(p, want_grains=False)
+    assert np.array_equal(cell, r["sites_cell"]) and np.array_equal(dt, r["sites_dt"])
+    j = (cell // 8) % 14
+    assert not np.any(j == 7)
+
+
+@pytest.mark.parametrize("mutate,msg", [
+    (lambda t: t.replace("    problem_physics:", "    problem_physicsX:"), "problem_physics"),
+    (lambda t: t.replace("    output:", "    outputX:"), "realm-output"),
+    (lambda t: t.replace("      cell_size:", "      cell_sizeX:"), "cell_size"),
+    (lambda t: t.replace("type: RappazGandin", "type: Other"), "RappazGandin"),
+    (lambda t: t.replace("type: cellular_automata\n", "type: lattice_boltzmann\n"), "realm-type"),
+    (lambda t: t.replace("      termination_time:", "      termination_timeX:"), "termination_time"),
+    (lambda t: t.replace("          mean: 3.0", "          mean: three"), "invalid scalar"),
+])
+def test_deck_errors_are_reported(amca, mutate, msg):
+    text = mutate(decks.plain_deck(4, 4, 4, 1e-3, bulk=(1e9, 3.0, 1.0)))
+    with pytest.raises(AmcaError, match=msg):
+        amca.Simulation(text)
+
+
+def test_yaml_scalar_conversions(amca):
+    """yaml-cpp 0.3 conversions: integers with base detection, flexible-case
+    booleans, flow lists with or without spaces, comments."""
+    text = decks.plain_deck(4, 4, 4, 1e-3, bulk=(1e9, 3.0, 1.0))
+    text = text.replace("number_cells: [4, 4, 4]", "number_cells: [0x4,04,4]  # hex and octal")
+    s = amca.Simulation(text).prepare()
+    inf = s.info()
+    assert (inf["nx"], inf["ny"], inf["nz"]) == (4, 4, 4)
+    text2 = text.replace("number_cells: [0x4,04,4]", "number_cells: [010, 4, 4]")
+    assert amca.Simulation(text2).prepare().info()["nx"] == 8
+
+
+def test_cli_help_and_missing_input():
+    r = subprocess.run([CLI_PATH, "-h"], capture_output=True, text=True)
+    assert r.returncode == 0 and "input-deck" in r.stderr
+    r = subprocess.run([CLI_PATH, "-i", "/nonexistent/deck.i"], capture_output=True, text=True)
+    assert r.returncode == 0 and "does not exist" in r.stderr
