@@ -403,9 +403,9 @@ amca_status amca_domain_read_log(amca_domain* d, int32_t cap, amca_step_record* 
   if (!d || !n_rows) return fail(AMCA_ERR_INVALID, "null argument");
   amca::DevCounters c;
   AMCA_TRY(read_counters(d, &c));
-  long long n = std::min<long long>(c.step, c.log_cap);
-  n = std::min<long long>(n, cap);
-  *n_rows = (int32_t)n;
+  const long long avail = std::min<long long>(c.step, c.log_cap);
+  *n_rows = (int32_t)avail;  // rows available; at most `cap` are copied
+  const long long n = std::min<long long>(avail, std::max(cap, 0));
   if (n > 0 && rows && d->B.log) {
     std::vector<amca::StepLogRow> tmp((size_t)n);
     HIP_TRY(hipMemcpy(tmp.data(), d->B.log, sizeof(amca::StepLogRow) * n, hipMemcpyDeviceToHost));
