@@ -71,6 +71,7 @@ SIGNATURES = {
     "amca_domain_enable_log": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_domain_read_log": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(StepRecord), I32P]),
     "amca_domain_download": (C.c_int, [C.c_void_p, I32P, I32P, F64P, C.c_int32, I32P, F64P, I32P]),
+    "amca_domain_download_grains": (C.c_int, [C.c_void_p, C.c_int32, F64P, I32P, I32P, I32P]),
     "amca_domain_device_fields": (C.c_int, [C.c_void_p, VPP, VPP, VPP, VPP]),
     "amca_domain_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_kernel_count": (C.c_int32, []),

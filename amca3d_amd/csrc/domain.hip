@@ -231,7 +231,10 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   ALLOC(B.gid, (size_t)NG);
   ALLOC(B.oid, (size_t)NG);
   ALLOC(B.claim, (size_t)NG);
-  ALLOC(B.env, (size_t)NG);
+  ALLOC(B.liq, (size_t)(NG >> 5) + 2);
+  ALLOC(B.clm, (size_t)(NG >> 5) + 2);
+  ALLOC(B.genv, N);
+  ALLOC(B.gco, N);
   ALLOC(B.listA[0], N);
   ALLOC(B.listA[1], N);
   ALLOC(B.listW[0], N);
@@ -261,8 +264,8 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   if (hipGetDeviceProperties(&prop, d->device) == hipSuccess && prop.multiProcessorCount > 0)
     cus = prop.multiProcessorCount;
   long long want = (P.N + amca::kBlock - 1) / amca::kBlock;
-  d->list_blocks = (int)std::max(1LL, std::min<long long>(want, 8LL * cus));
-  d->emit_blocks = (int)std::max(1LL, std::min<long long>(want, amca::kMaxBlocks));
+  d->list_blocks = (int)std::max(8LL, std::min<long long>((want + 7) / 8 * 8, 16LL * cus));
+  d->emit_blocks = (int)std::max(8LL, std::min<long long>((want + 7) / 8 * 8, std::min(16LL * cus, (long long)amca::kMaxBlocks)));
   d->nuc_blocks = 1;
   HIP_TRY(hipStreamSynchronize(d->stream));
   *out = d;
@@ -427,12 +430,23 @@ amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain
                                  int32_t* n_sites) {
   if (!d) return fail(AMCA_ERR_INVALID, "null domain");
   const size_t NG = (size_t)d->P.NG;
-  HIP_TRY(hipStreamSynchronize(d->stream));
-  if (orient) HIP_TRY(hipMemcpy(orient, d->B.oid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
-  if (grain) HIP_TRY(hipMemcpy(grain, d->B.gid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
-  if (env) HIP_TRY(hipMemcpy(env, d->B.env, sizeof(double) * 4 * NG, hipMemcpyDeviceToHost));
   amca::DevCounters c;
   AMCA_TRY(read_counters(d, &c));
+  if (orient) HIP_TRY(hipMemcpy(orient, d->B.oid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
+  if (grain) HIP_TRY(hipMemcpy(grain, d->B.gid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
+  if (env) {
+    // grain records are stored by grain id; scatter them to their cells
+    std::memset(env, 0, sizeof(double) * 4 * NG);
+    const size_t ng = (size_t)std::max(c.n_grains, 0);
+    if (ng > 0) {
+      std::vector<double> ge(4 * ng);
+      std::vector<int32_t> gc(2 * ng);
+      HIP_TRY(hipMemcpy(ge.data(), d->B.genv, sizeof(double) * 4 * ng, hipMemcpyDeviceToHost));
+      HIP_TRY(hipMemcpy(gc.data(), d->B.gco, sizeof(int32_t) * 2 * ng, hipMemcpyDeviceToHost));
+      for (size_t g = 0; g < ng; g++)
+        std::memcpy(env + 4 * (size_t)gc[2 * g], &ge[4 * g], sizeof(double) * 4);
+    }
+  }
   if (n_sites) *n_sites = c.n_sites;
   int m = std::min(c.n_sites, site_cap);
   if (m > 0 && site_cell)
@@ -442,12 +456,32 @@ amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain
   return AMCA_OK;
 }
 
+amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env4, int32_t* cell,
+                                        int32_t* orient, int32_t* n_grains) {
+  if (!d || !n_grains) return fail(AMCA_ERR_INVALID, "null argument");
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  *n_grains = c.n_grains;
+  const size_t m = (size_t)std::max(0, std::min(cap, c.n_grains));
+  if (m == 0) return AMCA_OK;
+  if (env4) HIP_TRY(hipMemcpy(env4, d->B.genv, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
+  if (cell || orient) {
+    std::vector<int32_t> co(2 * m);
+    HIP_TRY(hipMemcpy(co.data(), d->B.gco, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost));
+    for (size_t g = 0; g < m; g++) {
+      if (cell) cell[g] = co[2 * g];
+      if (orient) orient[g] = co[2 * g + 1];
+    }
+  }
+  return AMCA_OK;
+}
+
 amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
                                       void** stream) {
   if (!d) return fail(AMCA_ERR_INVALID, "null domain");
   if (orient) *orient = d->B.oid;
   if (grain) *grain = d->B.gid;
-  if (env) *env = d->B.env;
+  if (env) *env = d->B.genv;
   if (stream) *stream = d->stream;
   return AMCA_OK;
 }

@@ -38,6 +38,7 @@ struct DevCounters {
   int n_sites;
   int step_nuc, step_cap, step_nA0;
   int max_wave;
+  int wave_gbase;         // first new grain id of the running wave
   int err;                // 1: orientation table exhausted
   int log_cap;
   unsigned tickets[8];    // block tickets of the look-back kernels
@@ -50,13 +51,22 @@ struct StepLogRow {
 };
 
 // Device buffers of one domain (all owned by the handle).
+//
+// Cell-indexed (ghosted grid, NG cells): the two grain-ID fields, the claim
+// word of the capture waves and a 1-bit liquid map.  Grain-indexed (grain id =
+// creation order, capacity N): envelope, cell and orientation.  Lists hold
+// grain ids, so list order is id order and grain records are read in
+// ascending address order.
 struct DevBuffers {
   int* gid;               // cellGrainID_ [NG]
   int* oid;               // cellOrientationID_ [NG]
-  int* claim;             // min claiming grain id per cell, INT_MAX when free [NG]
-  double4* env;           // grain envelope per cell: xc, yc, zc, L [NG]
-  int* listA[2];          // active lists (cells), ping-pong by step parity [N]
-  int* listW[2];          // capture-wave lists (cells) [N]
+  int* claim;             // smallest claiming grain id per cell, INT_MAX when free [NG]
+  unsigned* liq;          // bit c set <=> cell c is liquid [NG/32 + 2]
+  unsigned* clm;          // bit c set <=> cell c was claimed in the running wave [NG/32 + 2]
+  double4* genv;          // grain envelope: xc, yc, zc, L [N]
+  int2* gco;              // grain -> (cell, orientation) [N]
+  int* listA[2];          // active lists (grain ids), ping-pong by step parity [N]
+  int* listW[2];          // capture-wave lists (grain ids) [N]
   unsigned long long* masks;  // per wave entry: liquid bits | inside bits << 32 [N]
   unsigned* res;          // per wave entry: captured bits | keep << 31 [N]
   int* site_cell[2];      // live nucleation sites, ascending cell, ping-pong
@@ -70,7 +80,8 @@ struct DevBuffers {
 };
 
 constexpr int kBlock = 256;       // threads per block of the list kernels
-constexpr int kMaxBlocks = 1024;  // blocks of a look-back kernel
+constexpr int kItems = 4;         // consecutive list entries per thread in scan passes
+constexpr int kMaxBlocks = 4096;  // blocks of a look-back kernel
 constexpr int kLbSites = 4;       // look-back launch sites per step
 
 // Kernel launchers (plain_step.hip).  Each returns hipGetLastError().
@@ -78,6 +89,7 @@ hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t
 hipError_t launch_nucleate(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
 hipError_t launch_claim(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                         hipStream_t s);
+// k_resolve, k_scan and k_emit of one wave (`blocks` <= kMaxBlocks chunks)
 hipError_t launch_resolve_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                                hipStream_t s);
 hipError_t launch_tail(const DevParams& P, const DevBuffers& B, hipStream_t s);

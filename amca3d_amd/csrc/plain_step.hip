@@ -12,11 +12,13 @@
 // neighbour takes it; new ids follow (parent id, neighbour slot) order; a grain
 // stays active iff one of its liquid neighbours is outside its envelope and not
 // taken by a smaller id of the same wave.  The kernels compute exactly that:
-//   k_claim         : per wave entry, envelope tests of the 26 neighbours and an
-//                     atomicMin of the grain id into the neighbour's claim word;
-//   k_resolve_emit  : per entry, captured / keep decisions from the claims, a
-//                     single-pass decoupled look-back scan for output positions,
-//                     then the child envelopes and the survivor list;
+//   k_claim         : per wave entry, liquid test of the 26 neighbours on the
+//                     1-bit liquid map, envelope tests, and an atomicMin of the
+//                     grain id into each enclosed neighbour's claim word;
+//   k_resolve       : per entry, captured / keep decisions from the claims and
+//                     per-chunk totals; k_scan turns the totals into output
+//                     positions; k_emit writes the child envelopes and the
+//                     survivor list;
 //   k_tail          : waves >= 2 (rare) in one work-group, then the time advance.
 // Claims need no reset: every claimed cell is captured in the same wave and is
 // never liquid again.
@@ -43,7 +45,12 @@ __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long lon
 __device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long long v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ int ld_claim(const int* p) {
+// loads that bypass the compute unit's L1: words that atomics update during
+// the same kernel (claims, liquid map) are read at the L2
+__device__ __forceinline__ int ld_coherent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_coherent(const unsigned* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
@@ -74,6 +81,34 @@ __device__ __forceinline__ void cell_ijk(const DevParams& P, int c, int& i, int&
   i = rem - j * P.nxG;
 }
 
+__device__ __forceinline__ void clear_bit(unsigned* map, int c) {
+  atomicAnd(&map[c >> 5], ~(1u << (c & 31)));
+}
+
+// Three bits of cells p-1, p, p+1 from a 1-bit-per-cell map.
+template <bool kCoherent>
+__device__ __forceinline__ unsigned bits3(const unsigned* __restrict__ liq, int p) {
+  const int q = p - 1;
+  const int w = q >> 5;
+  unsigned lo, hi;
+  if (kCoherent) {
+    lo = ld_coherent(&liq[w]);
+    hi = ld_coherent(&liq[w + 1]);
+  } else {
+    lo = liq[w];
+    hi = liq[w + 1];
+  }
+  const unsigned long long both = ((unsigned long long)hi << 32) | lo;
+  return (unsigned)(both >> (q & 31)) & 7u;
+}
+
+// wave-wide sum
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
 // Exclusive block scan of packed counter pairs (hi 32 | lo 32).  Both halves
 // stay below 2^32, so packed addition never carries between them.
 __device__ unsigned long long block_scan_excl(unsigned long long x, unsigned long long* total,
@@ -98,67 +133,113 @@ __device__ unsigned long long block_scan_excl(unsigned long long x, unsigned lon
   return base + v - x;
 }
 
-// Single-pass decoupled look-back (called by thread 0): publishes this
-// block's aggregate, accumulates the predecessors' and returns the exclusive
-// prefix.  `epoch` makes entries of earlier launches invisible.
+// Single-pass decoupled look-back, run by the 64 lanes of the block's first
+// wave: publishes this block's aggregate, then probes 64 predecessors at a
+// time, summing aggregates back to the nearest published inclusive prefix.
+// Returns the exclusive prefix (to every lane).  `epoch` makes entries of
+// earlier launches invisible.
 __device__ unsigned long long lookback(unsigned* flag, unsigned long long* agg,
                                        unsigned long long* pref, int bid, unsigned epoch,
                                        unsigned long long aggregate) {
+  const int lane = threadIdx.x;
   const unsigned tag = (epoch & 0x3fffffffu) << 2;
   if (bid == 0) {
-    st_relaxed(&pref[0], aggregate);
-    st_release(&flag[0], tag | 2u);
+    if (lane == 0) {
+      st_relaxed(&pref[0], aggregate);
+      st_release(&flag[0], tag | 2u);
+    }
     return 0ull;
   }
-  st_relaxed(&agg[bid], aggregate);
-  st_release(&flag[bid], tag | 1u);
+  if (lane == 0) {
+    st_relaxed(&agg[bid], aggregate);
+    st_release(&flag[bid], tag | 1u);
+  }
   unsigned long long excl = 0;
-  int p = bid - 1;
+  int base = bid - 1;
   while (true) {
-    unsigned f = ld_acquire(&flag[p]);
-    if ((f & ~3u) != tag || (f & 3u) == 0) {
+    const int p = base - lane;
+    bool ready, is_pref;
+    if (p >= 0) {
+      const unsigned f = ld_acquire(&flag[p]);
+      ready = (f & ~3u) == tag && (f & 3u) != 0u;
+      is_pref = ready && (f & 3u) == 2u;
+    } else {
+      ready = true;  // before block 0: an empty prefix
+      is_pref = true;
+    }
+    const unsigned long long rm = __ballot(ready);
+    const unsigned long long pm = __ballot(is_pref);
+    unsigned long long need = ~0ull;
+    int first = 64;
+    if (pm) {
+      first = __builtin_ctzll(pm);
+      need = (first == 63) ? ~0ull : ((1ull << (first + 1)) - 1ull);
+    }
+    if ((rm & need) != need) {
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    if ((f & 3u) == 2u) {
-      excl += ld_relaxed(&pref[p]);
-      break;
-    }
-    excl += ld_relaxed(&agg[p]);
-    p--;
+    unsigned long long v = 0;
+    if (p >= 0 && lane <= first) v = (lane == first) ? ld_relaxed(&pref[p]) : ld_relaxed(&agg[p]);
+    excl += wave_sum(v);
+    if (pm) break;
+    base -= 64;
   }
-  st_relaxed(&pref[bid], excl + aggregate);
-  st_release(&flag[bid], tag | 2u);
+  if (lane == 0) {
+    st_relaxed(&pref[bid], excl + aggregate);
+    st_release(&flag[bid], tag | 2u);
+  }
   return excl;
 }
 
-// Contiguous chunk of a list of n entries owned by block `bid` of `nb`.
-__device__ __forceinline__ void chunk_of(int n, int bid, int nb, int& lo, int& hi) {
+// Contiguous chunk of a list of n entries owned by block `bid` of `nb`,
+// rounded to whole tiles of `tile` entries.
+__device__ __forceinline__ void chunk_of(int n, int bid, int nb, int tile, int& lo, int& hi) {
   int per = (n + nb - 1) / nb;
-  per = (per + kBlock - 1) / kBlock * kBlock;
+  per = (per + tile - 1) / tile * tile;
   long long l = (long long)bid * per;
   lo = (int)(l < n ? l : n);
   long long h = l + per;
   hi = (int)(h < n ? h : n);
 }
 
+// Block index with the eight XCDs each owning a contiguous range: work-groups
+// are dispatched to XCDs round-robin, so block b runs on XCD b % 8.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x;
+  if (nb & 7) return blockIdx.x;
+  return (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+}
+
 // ---------------------------------------------------------------------------
 // grid initialisation: CellularAutomataManager::initialize_cells and
 // CellularAutomataManager_3D::initialize_boundaries (ghost faces carry 0).
 
+__device__ __forceinline__ bool is_ghost(const DevParams& P, long long c) {
+  int k = (int)(c / P.nxyG);
+  int rem = (int)(c - (long long)k * P.nxyG);
+  int j = rem / P.nxG;
+  int i = rem - j * P.nxG;
+  return i == 0 || j == 0 || k == 0 || i == P.nxG - 1 || j == P.nyG - 1 || k == P.nzG - 1;
+}
+
 __global__ void k_init_grid(DevParams P, DevBuffers B) {
-  long long stride = (long long)gridDim.x * blockDim.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.NG; c += stride) {
-    int k = (int)(c / P.nxyG);
-    int rem = (int)(c - (long long)k * P.nxyG);
-    int j = rem / P.nxG;
-    int i = rem - j * P.nxG;
-    bool ghost = i == 0 || j == 0 || k == 0 || i == P.nxG - 1 || j == P.nyG - 1 || k == P.nzG - 1;
-    int v = ghost ? 0 : -1;
+    int v = is_ghost(P, c) ? 0 : -1;
     B.gid[c] = v;
     B.oid[c] = v;
     B.claim[c] = INT_MAX;
-    B.env[c] = make_double4(0.0, 0.0, 0.0, 0.0);
+  }
+  const long long nw = (long long)(P.NG >> 5) + 2;
+  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += stride) {
+    unsigned bits = 0;
+    for (int b = 0; b < 32; b++) {
+      long long c = w * 32 + b;
+      if (c < P.NG && !is_ghost(P, c)) bits |= 1u << b;
+    }
+    B.liq[w] = bits;
+    B.clm[w] = 0u;
   }
 }
 
@@ -199,7 +280,7 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
   const double under = P.Tm - T;
 
   int lo, hi;
-  chunk_of(n, bid, gridDim.x, lo, hi);
+  chunk_of(n, bid, gridDim.x, kBlock, lo, hi);
   unsigned long long mine = 0;
   for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
     int c = sc[i];
@@ -209,9 +290,11 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
   }
   unsigned long long agg;
   block_scan_excl(mine, &agg, s_sm);
-  if (threadIdx.x == 0)
-    s_excl = lookback(B.lb_flag, B.lb_agg, B.lb_pref, bid,
-                      (unsigned)(ctr->step * kLbSites + 0), agg);
+  if (threadIdx.x < 64) {
+    unsigned long long e = lookback(B.lb_flag, B.lb_agg, B.lb_pref, bid,
+                                    (unsigned)(ctr->step * kLbSites + 0), agg);
+    if (threadIdx.x == 0) s_excl = e;
+  }
   __syncthreads();
   unsigned long long run = s_excl;
   for (int base = lo; base < hi; base += kBlock) {
@@ -233,11 +316,14 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
       if (o < P.orient_cap) {
         int ci, cj, ck;
         cell_ijk(P, c, ci, cj, ck);
-        B.env[c] = make_double4(centre_coord(P.x0, P.h, ci), centre_coord(P.y0, P.h, cj),
-                                centre_coord(P.z0, P.h, ck), 0.0);
-        B.gid[c] = ng0 + r;
+        const int id = ng0 + r;
+        B.genv[id] = make_double4(centre_coord(P.x0, P.h, ci), centre_coord(P.y0, P.h, cj),
+                                  centre_coord(P.z0, P.h, ck), 0.0);
+        B.gco[id] = make_int2(c, o);
+        B.gid[c] = id;
         B.oid[c] = o;
-        Ain[nA0 + r] = c;
+        clear_bit(B.liq, c);
+        Ain[nA0 + r] = id;
       }
     }
     if (keep) {
@@ -278,8 +364,8 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
 // ---------------------------------------------------------------------------
 // capture waves
 
-__device__ __forceinline__ void wave_input(const DevParams& P, const DevBuffers& B, int wave,
-                                           const int*& list, int& n) {
+__device__ __forceinline__ void wave_input(const DevBuffers& B, int wave, const int*& list,
+                                           int& n) {
   const DevCounters* ctr = B.ctr;
   if (wave == 0) {
     list = B.listA[ctr->parity];
@@ -290,45 +376,68 @@ __device__ __forceinline__ void wave_input(const DevParams& P, const DevBuffers&
   }
 }
 
-// Envelope tests of one wave entry.  Wave 0 first grows the grain
+// Envelope tests of one wave entry (grain id g).  Wave 0 first grows the grain
 // (Grain::grow, Grain.h:27, over the whole list as in :521-538).
+// the 26 neighbour bits of cell c from a 1-bit map, in slot order (k, j, i)
+template <bool kCoherent>
+__device__ __forceinline__ unsigned neighbour_bits(const DevParams& P, const unsigned* map, int c) {
+  unsigned m = 0;
+  int s = 0;
+#pragma unroll
+  for (int dk = -1; dk <= 1; dk++) {
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++) {
+      const unsigned b = bits3<kCoherent>(map, c + dk * P.nxyG + dj * P.nxG);
+      if (dk == 0 && dj == 0) {
+        m |= (b & 1u) << s;
+        m |= ((b >> 2) & 1u) << (s + 1);
+        s += 2;
+      } else {
+        m |= b << s;
+        s += 3;
+      }
+    }
+  }
+  return m;
+}
+
+// Entries of wave w >= 1 are exactly the cells captured in wave w-1, whose
+// claimed bits are cleared here (not while wave w-1 still reads the map).
+template <bool kCoherent>
 __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
-                                                          int c, bool grow, double inc) {
-  const int key = B.gid[c];
-  const int o = B.oid[c];
-  double4 e = B.env[c];
+                                                          int g, bool grow, double inc) {
+  const int2 co = B.gco[g];
+  const int c = co.x;
+  const int o = co.y;
+  if (!grow) clear_bit(B.clm, c);
+  double4 e = B.genv[g];
   if (grow) {
     e.w = e.w + inc;
-    B.env[c].w = e.w;
+    B.genv[g].w = e.w;
   }
+  const unsigned lm = neighbour_bits<kCoherent>(P, B.liq, c);
+  if (lm == 0 || e.w == 0) return lm;  // no liquid neighbour, or an envelope that captures nothing
   int ci, cj, ck;
   cell_ijk(P, c, ci, cj, ck);
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
 #pragma unroll
   for (int q = 0; q < 9; q++) r[q] = rp[q];
-  unsigned lm = 0, im = 0;
-  int s = 0;
-#pragma unroll
-  for (int dk = -1; dk <= 1; dk++) {
+  unsigned im = 0;
+  unsigned rest = lm;
+  while (rest) {
+    const int s = __builtin_ctz(rest);
+    rest &= rest - 1;
+    int di, dj, dk;
+    slot_delta(s, di, dj, dk);
+    const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
+    const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
     const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
-#pragma unroll
-    for (int dj = -1; dj <= 1; dj++) {
-      const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
-#pragma unroll
-      for (int di = -1; di <= 1; di++) {
-        if (dk == 0 && dj == 0 && di == 0) continue;
-        const int nb = c + dk * P.nxyG + dj * P.nxG + di;
-        if (B.gid[nb] == -1) {
-          lm |= 1u << s;
-          const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
-          if (inside_envelope(r, e.w, gx, gy, gz)) {
-            im |= 1u << s;
-            atomicMin(&B.claim[nb], key);
-          }
-        }
-        s++;
-      }
+    if (inside_envelope(r, e.w, gx, gy, gz)) {
+      im |= 1u << s;
+      const int nb = c + P.off[s];
+      atomicMin(&B.claim[nb], g);
+      atomicOr(&B.clm[nb >> 5], 1u << (nb & 31));
     }
   }
   return ((unsigned long long)im << 32) | lm;
@@ -339,39 +448,61 @@ __global__ void __launch_bounds__(kBlock) k_claim(DevParams P, DevBuffers B, int
   if (ctr->skip) return;
   const int* list;
   int n;
-  wave_input(P, B, wave, list, n);
+  wave_input(B, wave, list, n);
   const double inc = ctr->dt * ctr->v;
-  const int stride = gridDim.x * blockDim.x;
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride)
-    B.masks[i] = claim_entry(P, B, list[i], wave == 0, inc);
+  int lo, hi;
+  chunk_of(n, xcd_block(), gridDim.x, kBlock, lo, hi);
+  for (int i = lo + threadIdx.x; i < hi; i += kBlock)
+    B.masks[i] = claim_entry<false>(P, B, list[i], wave == 0, inc);
 }
 
-// captured slots and the keep bit of one entry
-__device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevBuffers& B, int c,
+// Captured slots and the keep bit of one entry.  A liquid slot nobody claimed
+// in this wave needs no claim word (its claim is "none", larger than any id);
+// only claimed slots read theirs, all loads issued before any is used.
+template <bool kCoherent>
+__device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevBuffers& B, int g,
                                                   unsigned long long m) {
-  const int key = B.gid[c];
-  unsigned lm = (unsigned)m, im = (unsigned)(m >> 32);
-  unsigned cap = 0, keep = 0;
-  while (lm) {
-    int s = __builtin_ctz(lm);
-    lm &= lm - 1;
-    const int cl = ld_claim(&B.claim[c + P.off[s]]);
-    if ((im >> s) & 1u) {
-      if (cl == key) cap |= 1u << s;
-    } else if (cl > key) {
-      keep = 1u;
+  const unsigned lm = (unsigned)m, im = (unsigned)(m >> 32);
+  if (lm == 0) return 0u;
+  const int c = B.gco[g].x;
+  const unsigned cm = neighbour_bits<kCoherent>(P, B.clm, c) & lm;
+  // an unclaimed liquid neighbour outside the envelope keeps the grain active
+  unsigned keep = (lm & ~im & ~cm) ? 1u : 0u;
+  const unsigned need = keep ? (cm & im) : cm;
+  if (need == 0) return keep << 31;
+  int cl[26];
+#pragma unroll
+  for (int s = 0; s < 26; s++) {
+    cl[s] = INT_MAX;
+    if ((need >> s) & 1u) {
+      const int* w = &B.claim[c + P.off[s]];
+      cl[s] = kCoherent ? ld_coherent(w) : *w;
+    }
+  }
+  unsigned cap = 0;
+#pragma unroll
+  for (int s = 0; s < 26; s++) {
+    if ((need >> s) & 1u) {
+      if ((im >> s) & 1u) {
+        if (cl[s] == g) cap |= 1u << s;
+      } else if (cl[s] > g) {
+        keep = 1u;
+      }
     }
   }
   return cap | (keep << 31);
 }
 
 // Octahedron::capture_cell via CellularAutomataManager_3D::capture_cell
-// (:676-707): the captured cell gets a new grain with the parent orientation.
-__device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffers& B, int c,
+// (:676-707): each captured cell gets a new grain (ids id0, id0+1, ... in slot
+// order) with the parent orientation.
+__device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffers& B, int g,
                                               unsigned cap, int id0, int* __restrict__ kids,
                                               int pos0) {
-  const int o = B.oid[c];
-  const double4 e = B.env[c];
+  const int2 co = B.gco[g];
+  const int c = co.x;
+  const int o = co.y;
+  const double4 e = B.genv[g];
   int ci, cj, ck;
   cell_ijk(P, c, ci, cj, ck);
   double r[9];
@@ -380,7 +511,7 @@ __device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffe
   for (int q = 0; q < 9; q++) r[q] = rp[q];
   int j = 0;
   while (cap) {
-    int s = __builtin_ctz(cap);
+    const int s = __builtin_ctz(cap);
     cap &= cap - 1;
     int di, dj, dk;
     slot_delta(s, di, dj, dk);
@@ -390,79 +521,136 @@ __device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffe
     const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
     double out[4];
     child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
-    B.env[nb] = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
-    B.gid[nb] = id0 + j;
+    const int id = id0 + j;
+    B.genv[id] = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
+    B.gco[id] = make_int2(nb, o);
+    B.gid[nb] = id;
     B.oid[nb] = o;
-    kids[pos0 + j] = nb;
+    clear_bit(B.liq, nb);
+    kids[pos0 + j] = id;
     j++;
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_resolve_emit(DevParams P, DevBuffers B, int wave) {
-  DevCounters* ctr = B.ctr;
-  if (ctr->skip) return;
-  const int site = wave == 0 ? 1 : 2;
-  __shared__ int s_bid;
-  __shared__ unsigned long long s_sm[kBlock / 64];
-  __shared__ unsigned long long s_excl;
-  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[site], 1u);
-  __syncthreads();
-  const int bid = s_bid;
+// The list of wave entries is cut into `nb` contiguous chunks (nb <= grid):
+// k_resolve decides every entry and writes per-chunk totals, k_scan turns the
+// totals into absolute output positions, k_emit writes survivors and children.
+__device__ __forceinline__ int wave_chunks(int n, int grid) {
+  constexpr int kTile = kBlock * kItems;
+  return max(1, min(grid, (n + kTile - 1) / kTile));
+}
 
+__global__ void __launch_bounds__(kBlock) k_resolve(DevParams P, DevBuffers B, int wave) {
+  const DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ unsigned long long s_sm[kBlock / 64];
   const int* list;
   int n;
-  wave_input(P, B, wave, list, n);
-  const int par = ctr->parity;
-  int* __restrict__ kids = B.listW[wave & 1];
-  int* __restrict__ surv = B.listA[par ^ 1];
-  const int gbase = ctr->n_grains;
-  const int sbase = ctr->nA_out;
-
+  wave_input(B, wave, list, n);
+  const int nb = wave_chunks(n, gridDim.x);
+  const int bid = xcd_block();
+  if (bid >= nb) return;
   int lo, hi;
-  chunk_of(n, bid, gridDim.x, lo, hi);
+  chunk_of(n, bid, nb, kBlock * kItems, lo, hi);
   unsigned long long mine = 0;
   for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
-    unsigned r = resolve_entry(P, B, list[i], B.masks[i]);
+    const unsigned r = resolve_entry<false>(P, B, list[i], B.masks[i]);
     B.res[i] = r;
     mine += ((unsigned long long)__builtin_popcount(r & 0x3ffffffu) << 32) | (r >> 31);
   }
-  unsigned long long agg;
-  block_scan_excl(mine, &agg, s_sm);
-  if (threadIdx.x == 0)
-    s_excl = lookback(B.lb_flag + site * kMaxBlocks, B.lb_agg + site * kMaxBlocks,
-                      B.lb_pref + site * kMaxBlocks, bid,
-                      (unsigned)(ctr->step * kLbSites + site), agg);
-  __syncthreads();
-  unsigned long long run = s_excl;
-  for (int base = lo; base < hi; base += kBlock) {
-    int i = base + threadIdx.x;
-    unsigned r = 0;
-    int c = 0;
-    if (i < hi) {
-      r = B.res[i];
-      c = list[i];
-    }
-    const unsigned cap = r & 0x3ffffffu;
-    unsigned long long tot;
-    unsigned long long off =
-        run + block_scan_excl(((unsigned long long)__builtin_popcount(cap) << 32) | (r >> 31),
-                              &tot, s_sm);
-    if (r >> 31) surv[sbase + (int)(off & 0xffffffffull)] = c;
-    if (cap) {
-      const int pos = (int)(off >> 32);
-      emit_children(P, B, c, cap, gbase + pos, kids, pos);
-    }
-    run += tot;
+  unsigned long long tot;
+  block_scan_excl(mine, &tot, s_sm);
+  if (threadIdx.x == 0) B.lb_agg[kMaxBlocks + bid] = tot;  // chunk totals
+}
+
+constexpr int kScanBlock = 1024;
+
+__global__ void __launch_bounds__(kScanBlock) k_scan(DevParams P, DevBuffers B, int wave, int grid) {
+  DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ unsigned long long s_sm[kScanBlock / 64];
+  const int* list;
+  int n;
+  wave_input(B, wave, list, n);
+  const int nb = wave_chunks(n, grid);
+  // absolute bases: children ids and wave-list positions start at n_grains / 0,
+  // survivors at nA_out; stored as (child offset << 32 | survivor position)
+  const int gbase = ctr->n_grains;
+  const int sbase = ctr->nA_out;
+  constexpr int kPer = kMaxBlocks / kScanBlock;
+  unsigned long long v[kPer];
+  unsigned long long mine = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int b = threadIdx.x * kPer + q;
+    v[q] = b < nb ? B.lb_agg[kMaxBlocks + b] : 0ull;
+    mine += v[q];
   }
-  if (threadIdx.x == 0 && bid == (int)gridDim.x - 1) {
-    const int nc = (int)(run >> 32);
-    const int nk = (int)(run & 0xffffffffull);
+  unsigned long long tot;
+  unsigned long long off = block_scan_excl(mine, &tot, s_sm) + (unsigned long long)sbase;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const int b = threadIdx.x * kPer + q;
+    if (b < nb) B.lb_pref[kMaxBlocks + b] = off;
+    off += v[q];
+  }
+  if (threadIdx.x == 0) {
+    const int nc = (int)(tot >> 32);
+    const int nk = (int)(tot & 0xffffffffull);
+    ctr->wave_gbase = gbase;
     ctr->n_grains = gbase + nc;
     ctr->nA_out = sbase + nk;
     ctr->nW[wave & 1] = nc;
     ctr->step_cap += nc;
     if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
-    ctr->tickets[site] = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_emit(DevParams P, DevBuffers B, int wave) {
+  const DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ unsigned long long s_sm[kBlock / 64];
+  const int* list;
+  int n;
+  wave_input(B, wave, list, n);
+  const int nb = wave_chunks(n, gridDim.x);
+  const int bid = xcd_block();
+  if (bid >= nb) return;
+  int* __restrict__ kids = B.listW[wave & 1];
+  int* __restrict__ surv = B.listA[ctr->parity ^ 1];
+  const int gbase = ctr->wave_gbase;
+  constexpr int kTile = kBlock * kItems;
+  int lo, hi;
+  chunk_of(n, bid, nb, kTile, lo, hi);
+  unsigned long long run = B.lb_pref[kMaxBlocks + bid];
+  for (int base = lo; base < hi; base += kTile) {
+    unsigned r[kItems];
+    int g[kItems];
+    unsigned long long cnt = 0;
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      const int i = base + threadIdx.x * kItems + q;
+      r[q] = 0;
+      g[q] = 0;
+      if (i < hi) {
+        r[q] = B.res[i];
+        g[q] = list[i];
+      }
+      cnt += ((unsigned long long)__builtin_popcount(r[q] & 0x3ffffffu) << 32) | (r[q] >> 31);
+    }
+    unsigned long long tot;
+    unsigned long long off = run + block_scan_excl(cnt, &tot, s_sm);
+#pragma unroll
+    for (int q = 0; q < kItems; q++) {
+      const unsigned cap = r[q] & 0x3ffffffu;
+      if (r[q] >> 31) surv[(int)(off & 0xffffffffull)] = g[q];
+      if (cap) {
+        const int pos = (int)(off >> 32);
+        emit_children(P, B, g[q], cap, gbase + pos, kids, pos);
+      }
+      off += ((unsigned long long)__builtin_popcount(cap) << 32) | (r[q] >> 31);
+    }
+    run += tot;
   }
 }
 
@@ -489,27 +677,28 @@ __global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) 
     const int* list = B.listW[in];
     int* kids = B.listW[wave & 1];
     int* surv = B.listA[ctr->parity ^ 1];
-    for (int i = threadIdx.x; i < n; i += kTailBlock) B.masks[i] = claim_entry(P, B, list[i], false, 0.0);
+    for (int i = threadIdx.x; i < n; i += kTailBlock)
+      B.masks[i] = claim_entry<true>(P, B, list[i], false, 0.0);
     __threadfence();
     __syncthreads();
     unsigned long long run = 0;
     for (int base = 0; base < n; base += kTailBlock) {
-      int i = base + threadIdx.x;
+      const int i = base + threadIdx.x;
       unsigned r = 0;
-      int c = 0;
+      int g = 0;
       if (i < n) {
-        c = list[i];
-        r = resolve_entry(P, B, c, B.masks[i]);
+        g = list[i];
+        r = resolve_entry<true>(P, B, g, B.masks[i]);
       }
       const unsigned cap = r & 0x3ffffffu;
       unsigned long long tot;
       unsigned long long off =
           run + block_scan_excl(((unsigned long long)__builtin_popcount(cap) << 32) | (r >> 31),
                                 &tot, s_sm);
-      if (r >> 31) surv[s_sbase + (int)(off & 0xffffffffull)] = c;
+      if (r >> 31) surv[s_sbase + (int)(off & 0xffffffffull)] = g;
       if (cap) {
         const int pos = (int)(off >> 32);
-        emit_children(P, B, c, cap, s_gbase + pos, kids, pos);
+        emit_children(P, B, g, cap, s_gbase + pos, kids, pos);
       }
       run += tot;
     }
@@ -597,7 +786,9 @@ hipError_t launch_claim(const DevParams& P, const DevBuffers& B, int wave, int b
 
 hipError_t launch_resolve_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                                hipStream_t s) {
-  hipLaunchKernelGGL(k_resolve_emit, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
+  hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
+  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, s, P, B, wave, blocks);
+  hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
   return hipGetLastError();
 }
 

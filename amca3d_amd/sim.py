@@ -69,6 +69,18 @@ class Domain:
             out["env"] = e.reshape(-1, 4)
         return out
 
+    def download_grains(self):
+        n = C.c_int32(0)
+        check(load().amca_domain_download_grains(self.h, 0, None, None, None, C.byref(n)), "grains")
+        m = int(n.value)
+        env = np.empty((max(m, 1), 4), np.float64)
+        cell = np.empty(max(m, 1), np.int32)
+        ori = np.empty(max(m, 1), np.int32)
+        check(load().amca_domain_download_grains(self.h, m, _ptr(env, C.c_double),
+                                                 _ptr(cell, C.c_int32), _ptr(ori, C.c_int32),
+                                                 C.byref(n)), "grains")
+        return dict(env=env[:m], cell=cell[:m], orient=ori[:m])
+
     def enable_timing(self, on=True):
         check(load().amca_domain_enable_timing(self.h, int(bool(on))), "enable_timing")
 

@@ -28,8 +28,9 @@ def algorithmic_bytes(log):
 
     k_claim_wave0        48 B per active grain: centre+L read (32), orientation
                          id (4), grain id (4), grown L written (8)
-    k_resolve_emit_wave0 44 B per capture: child envelope (32), orientation (4),
-                         grain id (4), wave-list entry (4); 4 B per survivor
+    k_resolve_emit_wave0 (k_resolve + k_scan + k_emit) 44 B per capture: child
+                         envelope (32), orientation (4), grain id (4), wave-list
+                         entry (4); 4 B per survivor-list entry
     k_claim_wave1        40 B per wave-1 grain (no growth write)
     k_resolve_emit_wave1 4 B per survivor of wave 1
     k_nucleate           12 B per live site (cell index + critical undercooling)
@@ -64,8 +65,8 @@ def cpu_baseline(seconds_target=20.0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=400)
-    ap.add_argument("--warmup", type=int, default=30)
+    ap.add_argument("--steps", type=int, default=450)
+    ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")

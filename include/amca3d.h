@@ -138,8 +138,14 @@ amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain
                                  int32_t site_cap, int32_t* site_cell, double* site_dt,
                                  int32_t* n_sites);
 
-/* Device pointers of the grain-ID fields and the stream (for callers that keep
- * the data on the GPU).  Valid until amca_domain_destroy. */
+/* Grain records in grain-id order (grainVector_ of the reference): envelope
+ * xc, yc, zc, L, owning cell and orientation; copies min(cap, n) rows. */
+amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env4, int32_t* cell,
+                                        int32_t* orient, int32_t* n_grains);
+
+/* Device pointers of the grain-ID fields, of the grain envelopes (double4 per
+ * grain id) and the stream (for callers that keep the data on the GPU).
+ * Valid until amca_domain_destroy. */
 amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
                                       void** stream);
 

@@ -35,6 +35,7 @@ def run_gpu(amca, text, log_cap=1 << 16):
     d.enable_log(log_cap)
     c = s.run(-1)
     out = d.download()
+    out["grains"] = d.download_grains()
     out["log"] = d.read_log()
     out["counters"] = c
     out["initial_sites"] = s.initial_sites()
@@ -58,6 +59,10 @@ def compare(oracle, prm_kw, g, r=None):
     env_cell[r["grain_cell"]] = r["env"]
     cells = r["grain_cell"]
     assert np.array_equal(g["env"][cells], env_cell[cells]), "FP64 envelopes differ"
+    # grain records in grain-id order (grainVector_)
+    assert np.array_equal(g["grains"]["env"], r["env"])
+    assert np.array_equal(g["grains"]["cell"], r["grain_cell"])
+    assert np.array_equal(g["grains"]["orient"], r["grain_orient"])
     # live sites and the erased ones (dT set to 0)
     sc, sd, _ = g["initial_sites"]
     live = dict(zip(g["sites_cell"].tolist(), g["sites_dt"].tolist()))
