@@ -1,0 +1,178 @@
+// Device helpers shared by the CA step kernels: coherent loads, 1-bit cell
+// maps, slot geometry, block scans, the decoupled look-back and XCD-aware
+// block numbering.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#include "plain_step.h"
+
+namespace amca {
+
+__device__ __forceinline__ unsigned ld_acquire(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_release(unsigned* p, unsigned v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void st_relaxed(unsigned long long* p, unsigned long long v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// loads that bypass the compute unit's L1: words that atomics update during
+// the same kernel (claims, liquid map) are read at the L2
+__device__ __forceinline__ int ld_coherent(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ unsigned ld_coherent(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ void slot_delta(int s, int& di, int& dj, int& dk) {
+  int idx = s < 13 ? s : s + 1;  // skip the centre of the 3x3x3 block
+  dk = idx / 9 - 1;
+  dj = (idx / 3) % 3 - 1;
+  di = idx % 3 - 1;
+}
+
+__device__ __forceinline__ void cell_ijk(const DevParams& P, int c, int& i, int& j, int& k) {
+  k = c / P.nxyG;
+  int rem = c - k * P.nxyG;
+  j = rem / P.nxG;
+  i = rem - j * P.nxG;
+}
+
+__device__ __forceinline__ void clear_bit(unsigned* map, int c) {
+  atomicAnd(&map[c >> 5], ~(1u << (c & 31)));
+}
+
+// Three bits of cells p-1, p, p+1 from a 1-bit-per-cell map.
+template <bool kCoherent>
+__device__ __forceinline__ unsigned bits3(const unsigned* __restrict__ liq, int p) {
+  const int q = p - 1;
+  const int w = q >> 5;
+  unsigned lo, hi;
+  if (kCoherent) {
+    lo = ld_coherent(&liq[w]);
+    hi = ld_coherent(&liq[w + 1]);
+  } else {
+    lo = liq[w];
+    hi = liq[w + 1];
+  }
+  const unsigned long long both = ((unsigned long long)hi << 32) | lo;
+  return (unsigned)(both >> (q & 31)) & 7u;
+}
+
+// wave-wide sum
+__device__ __forceinline__ unsigned long long wave_sum(unsigned long long v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  return v;
+}
+
+// Exclusive block scan of packed counter pairs (hi 32 | lo 32).  Both halves
+// stay below 2^32, so packed addition never carries between them.
+__device__ inline unsigned long long block_scan_excl(unsigned long long x, unsigned long long* total,
+                                              unsigned long long* sm) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, nw = blockDim.x >> 6;
+  unsigned long long v = x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    unsigned long long y = __shfl_up(v, o, 64);
+    if (lane >= o) v += y;
+  }
+  if (lane == 63) sm[w] = v;
+  __syncthreads();
+  unsigned long long base = 0, tot = 0;
+  for (int q = 0; q < nw; q++) {
+    unsigned long long s = sm[q];
+    if (q < w) base += s;
+    tot += s;
+  }
+  __syncthreads();
+  *total = tot;
+  return base + v - x;
+}
+
+// Single-pass decoupled look-back, run by the 64 lanes of the block's first
+// wave: publishes this block's aggregate, then probes 64 predecessors at a
+// time, summing aggregates back to the nearest published inclusive prefix.
+// Returns the exclusive prefix (to every lane).  `epoch` makes entries of
+// earlier launches invisible.
+__device__ inline unsigned long long lookback(unsigned* flag, unsigned long long* agg,
+                                       unsigned long long* pref, int bid, unsigned epoch,
+                                       unsigned long long aggregate) {
+  const int lane = threadIdx.x;
+  const unsigned tag = (epoch & 0x3fffffffu) << 2;
+  if (bid == 0) {
+    if (lane == 0) {
+      st_relaxed(&pref[0], aggregate);
+      st_release(&flag[0], tag | 2u);
+    }
+    return 0ull;
+  }
+  if (lane == 0) {
+    st_relaxed(&agg[bid], aggregate);
+    st_release(&flag[bid], tag | 1u);
+  }
+  unsigned long long excl = 0;
+  int base = bid - 1;
+  while (true) {
+    const int p = base - lane;
+    bool ready, is_pref;
+    if (p >= 0) {
+      const unsigned f = ld_acquire(&flag[p]);
+      ready = (f & ~3u) == tag && (f & 3u) != 0u;
+      is_pref = ready && (f & 3u) == 2u;
+    } else {
+      ready = true;  // before block 0: an empty prefix
+      is_pref = true;
+    }
+    const unsigned long long rm = __ballot(ready);
+    const unsigned long long pm = __ballot(is_pref);
+    unsigned long long need = ~0ull;
+    int first = 64;
+    if (pm) {
+      first = __builtin_ctzll(pm);
+      need = (first == 63) ? ~0ull : ((1ull << (first + 1)) - 1ull);
+    }
+    if ((rm & need) != need) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    unsigned long long v = 0;
+    if (p >= 0 && lane <= first) v = (lane == first) ? ld_relaxed(&pref[p]) : ld_relaxed(&agg[p]);
+    excl += wave_sum(v);
+    if (pm) break;
+    base -= 64;
+  }
+  if (lane == 0) {
+    st_relaxed(&pref[bid], excl + aggregate);
+    st_release(&flag[bid], tag | 2u);
+  }
+  return excl;
+}
+
+// Contiguous chunk of a list of n entries owned by block `bid` of `nb`,
+// rounded to whole tiles of `tile` entries.
+__device__ __forceinline__ void chunk_of(int n, int bid, int nb, int tile, int& lo, int& hi) {
+  int per = (n + nb - 1) / nb;
+  per = (per + tile - 1) / tile * tile;
+  long long l = (long long)bid * per;
+  lo = (int)(l < n ? l : n);
+  long long h = l + per;
+  hi = (int)(h < n ? h : n);
+}
+
+// Block index with the eight XCDs each owning a contiguous range: work-groups
+// are dispatched to XCDs round-robin, so block b runs on XCD b % 8.
+__device__ __forceinline__ int xcd_block() {
+  const int nb = gridDim.x;
+  if (nb & 7) return blockIdx.x;
+  return (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
+}
+
+
+}  // namespace amca

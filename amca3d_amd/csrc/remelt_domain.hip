@@ -1,0 +1,573 @@
+// C ABI of a coupled CA-FE (remelting) domain (include/amca3d.h, amca_remelt_*).
+//
+// The reference's CafeSolver owns a FiniteElementManager that streams text
+// frames and a MapVoxelManager that interpolates them per cell every step
+// (CafeSolver.cpp:159-219).  Here the FE frames, the voxel map and the whole CA
+// state are resident in HBM; the host only maps voxels once, refills the
+// orientation rows it draws from the rand() stream, and reads counters back
+// between batches of steps.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/amca3d.h"
+#include "host/voxel_map.h"
+#include "plain_step.h"
+#include "remelt_step.h"
+
+namespace {
+
+thread_local std::string g_rm_error;
+
+amca_status rm_fail(amca_status st, const std::string& msg) {
+  g_rm_error = msg;
+  return st;
+}
+
+#define RM_HIP(expr)                                                                      \
+  do {                                                                                    \
+    hipError_t e_ = (expr);                                                               \
+    if (e_ != hipSuccess)                                                                 \
+      return rm_fail(AMCA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));    \
+  } while (0)
+
+#define RM_TRY(expr)              \
+  do {                            \
+    amca_status s_ = (expr);      \
+    if (s_ != AMCA_OK) return s_; \
+  } while (0)
+
+}  // namespace
+
+struct amca_remelt {
+  hipStream_t stream = nullptr;
+  amca_remelt_desc desc{};
+  amca::RmParams P{};
+  amca::RmBuffers B{};
+  std::vector<void*> allocs;
+  bool fe_ready = false, state_ready = false;
+  int site_blocks = 1;
+  int rows_pushed = 0;
+  amca_orientation_source source = nullptr;
+  void* source_user = nullptr;
+  size_t grain_cap = 0;  // records genv/gco can hold
+  std::vector<int32_t> host_tag;
+  std::vector<double> host_par;
+
+  template <class T>
+  amca_status alloc(T** p, size_t count) {
+    void* q = nullptr;
+    const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+    hipError_t e = hipMalloc(&q, bytes);
+    if (e != hipSuccess)
+      return rm_fail(AMCA_ERR_HIP, "hipMalloc(" + std::to_string(bytes) + "): " + hipGetErrorString(e));
+    allocs.push_back(q);
+    *p = static_cast<T*>(q);
+    return AMCA_OK;
+  }
+  void release(void* p) {
+    if (!p) return;
+    for (auto& a : allocs)
+      if (a == p) {
+        (void)hipFree(a);
+        a = nullptr;
+      }
+    allocs.erase(std::remove(allocs.begin(), allocs.end(), nullptr), allocs.end());
+  }
+  ~amca_remelt() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    for (void* p : allocs) (void)hipFree(p);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+amca_status rm_counters(amca_remelt* d, amca::RmCounters* c) {
+  RM_HIP(hipMemcpyAsync(c, d->B.ctr, sizeof(*c), hipMemcpyDeviceToHost, d->stream));
+  RM_HIP(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+amca_status rm_put_counters(amca_remelt* d, const amca::RmCounters& c) {
+  RM_HIP(hipMemcpyAsync(d->B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream));
+  RM_HIP(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+void rm_fill(const amca_remelt* d, const amca::RmCounters& c, amca_remelt_counters* o) {
+  if (!o) return;
+  o->step = c.step;
+  o->time = c.t;
+  o->time_fe = c.time_fe;
+  o->dt = c.dt;
+  o->max_dt = c.max_dt;
+  o->n_grains = c.n_grains;
+  o->n_orient = c.n_orient;
+  o->frames_exhausted = c.exhausted;
+  o->done = (c.exhausted || !(c.tcafe < d->desc.final_time)) ? 1 : 0;
+}
+
+amca_status push_rows(amca_remelt* d, int n, const double* l2g9) {
+  if (d->rows_pushed + n > d->P.orient_cap)
+    return rm_fail(AMCA_ERR_CAPACITY, "orientation table capacity exceeded");
+  if (n > 0)
+    RM_HIP(hipMemcpyAsync(d->B.l2g + 9 * (size_t)d->rows_pushed, l2g9, sizeof(double) * 9 * n,
+                          hipMemcpyHostToDevice, d->stream));
+  d->rows_pushed += n;
+  RM_HIP(hipStreamSynchronize(d->stream));
+  return AMCA_OK;
+}
+
+// make sure `need` rows exist, pulling from the orientation source
+amca_status ensure_rows(amca_remelt* d, long long need) {
+  if (need <= d->rows_pushed) return AMCA_OK;
+  if (!d->source)
+    return rm_fail(AMCA_ERR_CAPACITY, "orientation rows exhausted and no orientation source set");
+  need = std::min<long long>(need, d->P.orient_cap);
+  const int n = (int)(need - d->rows_pushed);
+  if (n <= 0) return AMCA_OK;
+  std::vector<double> eu(3 * (size_t)n), r(9 * (size_t)n);
+  if (d->source(d->source_user, n, eu.data(), r.data()) != 0)
+    return rm_fail(AMCA_ERR_INVALID, "orientation source failed");
+  return push_rows(d, n, r.data());
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* amca_remelt_last_error(void) { return g_rm_error.c_str(); }
+
+amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capacity,
+                               amca_remelt** out) {
+  if (!desc || !out) return rm_fail(AMCA_ERR_INVALID, "null argument");
+  *out = nullptr;
+  if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1 || !(desc->h > 0))
+    return rm_fail(AMCA_ERR_INVALID, "bad grid");
+  if (desc->b1 != 1.0 || desc->b2 != 2.0 || desc->b3 != 3.0)
+    return rm_fail(AMCA_ERR_INVALID,
+                   "kinetics exponents b1, b2, b3 must be 1, 2, 3 on the device path");
+  const long long nxG = desc->nx + 2LL, nyG = desc->ny + 2LL, nzG = desc->nz + 2LL;
+  const long long NG = nxG * nyG * nzG;
+  if (NG > (long long)INT_MAX) return rm_fail(AMCA_ERR_INVALID, "grid exceeds int32 cell indices");
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return rm_fail(AMCA_ERR_NO_DEVICE, "no HIP device visible");
+  auto* d = new amca_remelt();
+  d->desc = *desc;
+  if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+    delete d;
+    return rm_fail(AMCA_ERR_HIP, "stream creation failed");
+  }
+  amca::RmParams& P = d->P;
+  P.nx = desc->nx; P.ny = desc->ny; P.nz = desc->nz;
+  P.nxG = (int)nxG; P.nyG = (int)nyG; P.nzG = (int)nzG;
+  P.nxyG = (int)(nxG * nyG);
+  P.NG = (int)NG;
+  P.N = (long long)desc->nx * desc->ny * desc->nz;
+  P.h = desc->h; P.x0 = desc->x0; P.y0 = desc->y0; P.z0 = desc->z0;
+  P.Tm = desc->Tm; P.Ts = desc->Ts; P.Tmin_nuc = desc->Tmin_nuc;
+  P.a1 = desc->a1; P.a2 = desc->a2; P.a3 = desc->a3;
+  P.trunc = desc->trunc;
+  P.tsf = desc->time_step_factor;
+  P.sqrt2f = (double)sqrtf(2.0f);
+  P.cutoff = desc->cutoff_factor * desc->h;  // cellGrainLegnthCutoff_ *= h_
+  P.final_time = desc->final_time;
+  P.void_temp = desc->void_temperature;
+  P.orient_cap = orient_capacity;
+  {
+    int s = 0;
+    for (int dk = -1; dk <= 1; dk++)
+      for (int dj = -1; dj <= 1; dj++)
+        for (int di = -1; di <= 1; di++) {
+          if (!dk && !dj && !di) continue;
+          P.off[s++] = dk * P.nxyG + dj * P.nxG + di;
+        }
+  }
+  amca::RmBuffers& B = d->B;
+  const size_t ng = (size_t)NG, nc = (size_t)P.N + 1;
+  amca_status st = AMCA_OK;
+  auto A = [&](auto** p, size_t n) {
+    if (st == AMCA_OK) st = d->alloc(p, n);
+  };
+  A(&B.gid, ng);
+  A(&B.oid, ng);
+  A(&B.T, ng);
+  A(&B.cooling, ng);
+  A(&B.melted, ng);
+  A(&B.flag, ng);
+  A(&B.tag, ng);
+  A(&B.par, 3 * ng);
+  A(&B.claim, ng);
+  A(&B.msk, ng);
+  A(&B.genv, nc);
+  A(&B.gco, nc);
+  d->grain_cap = nc;
+  A(&B.l2g, 9 * (size_t)std::max(orient_capacity, 1));
+  A(&B.rowcnt, (size_t)P.nx * P.ny);
+  A(&B.rowoff, (size_t)P.nx * P.ny);
+  A(&B.lb_flag, (size_t)amca::kMaxBlocks);
+  A(&B.lb_agg, (size_t)amca::kMaxBlocks);
+  A(&B.lb_pref, (size_t)amca::kMaxBlocks);
+  A(&B.ctr, 1);
+  A(&B.site_cell, 1);
+  A(&B.site_dt, 1);
+  if (st != AMCA_OK) {
+    delete d;
+    return st;
+  }
+  B.log = nullptr;
+  B.frames = nullptr;
+  B.frame_time = nullptr;
+  B.theta = nullptr;
+  // every cell liquid without a grain until a state is loaded
+  (void)hipMemsetAsync(B.gid, 0xff, sizeof(int) * ng, d->stream);
+  (void)hipMemsetAsync(B.oid, 0xff, sizeof(int) * ng, d->stream);
+  (void)hipMemsetAsync(B.melted, desc->initial_melted ? 1 : 0, ng, d->stream);
+  (void)hipMemsetAsync(B.lb_flag, 0, sizeof(unsigned) * amca::kMaxBlocks, d->stream);
+  amca::RmCounters c{};
+  c.min_t_key = ~0ull;
+  if (hipMemcpyAsync(B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream) != hipSuccess ||
+      amca::rm_launch_init(P, B, d->stream) != hipSuccess ||
+      hipStreamSynchronize(d->stream) != hipSuccess) {
+    delete d;
+    return rm_fail(AMCA_ERR_HIP, "initialisation failed");
+  }
+  *out = d;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_destroy(amca_remelt* d) {
+  delete d;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t fnz,
+                               const double* node_xyz, int32_t n_frames, const double* frame_time,
+                               const double* frame_theta) {
+  if (!d || fnx < 2 || fny < 2 || fnz < 2 || !node_xyz || n_frames < 2 || !frame_time ||
+      !frame_theta)
+    return rm_fail(AMCA_ERR_INVALID, "FE mesh needs >= 2 nodes per axis and >= 2 frames");
+  amca::RmParams& P = d->P;
+  amca_host::FeMesh mesh;
+  mesh.fnx = fnx;
+  mesh.fny = fny;
+  mesh.fnz = fnz;
+  const size_t nodes = (size_t)fnx * fny * fnz;
+  mesh.xyz.assign(node_xyz, node_xyz + 3 * nodes);
+  amca_host::VoxelMapParams vp;
+  vp.nx = P.nx; vp.ny = P.ny; vp.nz = P.nz;
+  vp.h = P.h; vp.x0 = P.x0; vp.y0 = P.y0; vp.z0 = P.z0;
+  for (int q = 0; q < 3; q++) vp.bin[q] = d->desc.bin[q] > 0 ? d->desc.bin[q] : 10;
+  vp.tol = d->desc.tolerance;
+  vp.max_iter = d->desc.max_iterations;
+  if (!amca_host::map_voxels(mesh, vp, d->host_tag, d->host_par))
+    return rm_fail(AMCA_ERR_INVALID, "no CA cell lies in the FE mesh");
+  P.fnx = fnx;
+  P.fny = fny;
+  P.fnz = fnz;
+  P.nodes = (int)nodes;
+  P.ncx = fnx - 1;
+  P.ncy = fny - 1;
+  P.n_frames = n_frames;
+  d->release((void*)d->B.frames);
+  d->release((void*)d->B.frame_time);
+  d->release(d->B.theta);
+  double *fr = nullptr, *ft = nullptr, *th = nullptr;
+  RM_TRY(d->alloc(&fr, nodes * (size_t)n_frames));
+  RM_TRY(d->alloc(&ft, (size_t)n_frames));
+  RM_TRY(d->alloc(&th, nodes));
+  std::vector<double> rel(n_frames);
+  for (int f = 0; f < n_frames; f++) rel[f] = frame_time[f] - frame_time[0];  // inTimeOffset_
+  RM_HIP(hipMemcpyAsync(fr, frame_theta, sizeof(double) * nodes * n_frames, hipMemcpyHostToDevice, d->stream));
+  RM_HIP(hipMemcpyAsync(ft, rel.data(), sizeof(double) * n_frames, hipMemcpyHostToDevice, d->stream));
+  // thetaArray_ starts as theta_[1], the first frame
+  RM_HIP(hipMemcpyAsync(th, frame_theta, sizeof(double) * nodes, hipMemcpyHostToDevice, d->stream));
+  RM_HIP(hipMemcpyAsync(d->B.tag, d->host_tag.data(), sizeof(int32_t) * P.NG, hipMemcpyHostToDevice, d->stream));
+  RM_HIP(hipMemcpyAsync(d->B.par, d->host_par.data(), sizeof(double) * 3 * (size_t)P.NG,
+                        hipMemcpyHostToDevice, d->stream));
+  d->B.frames = fr;
+  d->B.frame_time = ft;
+  d->B.theta = th;
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  c.frame = 1;
+  c.in_t1 = rel[0];
+  c.in_t0 = rel[1];
+  c.time_fe = 0.0;
+  c.max_dt = c.in_t0 - c.in_t1;  // get_output_time_step_from_txt_file
+  RM_TRY(rm_put_counters(d, c));
+  d->fe_ready = true;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_voxel_map(amca_remelt* d, int32_t* tag, double* par) {
+  if (!d || !d->fe_ready) return rm_fail(AMCA_ERR_INVALID, "FE mesh not set");
+  if (tag) std::memcpy(tag, d->host_tag.data(), sizeof(int32_t) * d->P.NG);
+  if (par) std::memcpy(par, d->host_par.data(), sizeof(double) * 3 * (size_t)d->P.NG);
+  return AMCA_OK;
+}
+
+amca_status amca_fe_map_voxels(int32_t nx, int32_t ny, int32_t nz, double h, double x0, double y0,
+                               double z0, const int32_t* bin, double tolerance,
+                               int32_t max_iterations, int32_t fnx, int32_t fny, int32_t fnz,
+                               const double* node_xyz, int32_t* tag, double* par) {
+  if (nx < 1 || ny < 1 || nz < 1 || fnx < 2 || fny < 2 || fnz < 2 || !node_xyz || !tag || !par)
+    return rm_fail(AMCA_ERR_INVALID, "bad arguments");
+  amca_host::FeMesh mesh;
+  mesh.fnx = fnx;
+  mesh.fny = fny;
+  mesh.fnz = fnz;
+  mesh.xyz.assign(node_xyz, node_xyz + 3 * (size_t)fnx * fny * fnz);
+  amca_host::VoxelMapParams vp;
+  vp.nx = nx; vp.ny = ny; vp.nz = nz;
+  vp.h = h; vp.x0 = x0; vp.y0 = y0; vp.z0 = z0;
+  for (int q = 0; q < 3; q++) vp.bin[q] = (bin && bin[q] > 0) ? bin[q] : 10;
+  vp.tol = tolerance;
+  vp.max_iter = max_iterations;
+  std::vector<int32_t> t;
+  std::vector<double> p;
+  const bool any = amca_host::map_voxels(mesh, vp, t, p);
+  std::memcpy(tag, t.data(), sizeof(int32_t) * t.size());
+  std::memcpy(par, p.data(), sizeof(double) * p.size());
+  return any ? AMCA_OK : rm_fail(AMCA_ERR_INVALID, "no CA cell lies in the FE mesh");
+}
+
+amca_status amca_remelt_set_state(amca_remelt* d, const int32_t* orient, const int32_t* grain,
+                                  const uint8_t* melted, int32_t n_orient, int32_t n_grains,
+                                  const int32_t* grain_cell, const int32_t* grain_orient) {
+  if (!d || !orient || !grain || !melted || n_grains < 0 || (n_grains > 0 && (!grain_cell || !grain_orient)))
+    return rm_fail(AMCA_ERR_INVALID, "bad state arrays");
+  amca::RmParams& P = d->P;
+  if (n_orient < 0 || n_orient > d->rows_pushed)
+    return rm_fail(AMCA_ERR_INVALID, "push the loaded orientation rows before the state");
+  for (int g = 0; g < n_grains; g++)
+    if (grain_orient[g] < 0 || grain_orient[g] >= n_orient)
+      return rm_fail(AMCA_ERR_INVALID, "grain orientation out of range");
+  // Every interior cell owns at most one grain record and no record is shared:
+  // the device grows a record once per flagged cell, as the reference does when
+  // each cell carries its own grain.  Records for grainless cells are added by
+  // captures and nucleation, so the table needs n_grains + (grainless cells).
+  std::vector<uint8_t> owned((size_t)std::max(n_grains, 1), 0);
+  long long grainless = 0;
+  for (int c = 0; c < P.NG; c++) {
+    const int k = c / P.nxyG, j = (c % P.nxyG) / P.nxG, i = (c % P.nxyG) % P.nxG;
+    if (i == 0 || j == 0 || k == 0 || i == P.nxG - 1 || j == P.nyG - 1 || k == P.nzG - 1) continue;
+    const int g = grain[c];
+    if (orient[c] < -1 || orient[c] >= n_orient)
+      return rm_fail(AMCA_ERR_INVALID, "cell orientation out of range");
+    if (g < 0) {
+      grainless++;
+      continue;
+    }
+    if (g >= n_grains) return rm_fail(AMCA_ERR_INVALID, "cell grain id out of range");
+    if (owned[g]) return rm_fail(AMCA_ERR_INVALID, "a grain record is shared by two cells");
+    owned[g] = 1;
+  }
+  const size_t need = (size_t)n_grains + (size_t)grainless;
+  if (need > d->grain_cap) {
+    RM_HIP(hipStreamSynchronize(d->stream));
+    d->release(d->B.genv);
+    d->release(d->B.gco);
+    d->B.genv = nullptr;
+    d->B.gco = nullptr;
+    RM_TRY(d->alloc(&d->B.genv, need));
+    RM_TRY(d->alloc(&d->B.gco, need));
+    d->grain_cap = need;
+  }
+  // retrieve_grain_information_before_simulation (:312-332): ghosts liquid,
+  // other cells without an orientation take their grain's
+  std::vector<int32_t> o(orient, orient + P.NG);
+  const int nxG = P.nxG, nyG = P.nyG, nzG = P.nzG;
+  for (int c = 0; c < P.NG; c++) {
+    const int k = c / P.nxyG, j = (c % P.nxyG) / nxG, i = (c % P.nxyG) % nxG;
+    if (i == 0 || j == 0 || k == 0 || i == nxG - 1 || j == nyG - 1 || k == nzG - 1) {
+      o[c] = -1;
+      continue;
+    }
+    if (o[c] > -1) continue;
+    const int g = grain[c];
+    if (g < 0 || g >= n_grains)
+      return rm_fail(AMCA_ERR_INVALID, "a cell without orientation has no grain record");
+    o[c] = grain_orient[g];
+  }
+  // grain records: centre reset to the cell centre, zero length (:282-300)
+  std::vector<double> env(4 * (size_t)std::max(n_grains, 1));
+  std::vector<int32_t> co(2 * (size_t)std::max(n_grains, 1));
+  for (int g = 0; g < n_grains; g++) {
+    const int c = grain_cell[g];
+    if (c < 0 || c >= P.NG) return rm_fail(AMCA_ERR_INVALID, "grain cell out of range");
+    const int k = c / P.nxyG, j = (c % P.nxyG) / nxG, i = (c % P.nxyG) % nxG;
+    env[4 * (size_t)g + 0] = P.x0 + P.h * (i - 0.5);
+    env[4 * (size_t)g + 1] = P.y0 + P.h * (j - 0.5);
+    env[4 * (size_t)g + 2] = P.z0 + P.h * (k - 0.5);
+    env[4 * (size_t)g + 3] = 0.0;
+    co[2 * (size_t)g] = c;
+    co[2 * (size_t)g + 1] = grain_orient[g];
+  }
+  RM_HIP(hipMemcpyAsync(d->B.oid, o.data(), sizeof(int32_t) * P.NG, hipMemcpyHostToDevice, d->stream));
+  RM_HIP(hipMemcpyAsync(d->B.gid, grain, sizeof(int32_t) * P.NG, hipMemcpyHostToDevice, d->stream));
+  RM_HIP(hipMemcpyAsync(d->B.melted, melted, P.NG, hipMemcpyHostToDevice, d->stream));
+  if (n_grains > 0) {
+    RM_HIP(hipMemcpyAsync(d->B.genv, env.data(), sizeof(double) * 4 * n_grains, hipMemcpyHostToDevice, d->stream));
+    RM_HIP(hipMemcpyAsync(d->B.gco, co.data(), sizeof(int32_t) * 2 * n_grains, hipMemcpyHostToDevice, d->stream));
+  }
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  c.n_grains = n_grains;
+  c.n_orient = n_orient;
+  RM_TRY(rm_put_counters(d, c));
+  d->state_ready = true;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_push_orientations(amca_remelt* d, int32_t n, const double* euler3,
+                                          const double* l2g9) {
+  (void)euler3;
+  if (!d || n < 0 || (n > 0 && !l2g9)) return rm_fail(AMCA_ERR_INVALID, "bad rows");
+  return push_rows(d, n, l2g9);
+}
+
+amca_status amca_remelt_set_orientation_source(amca_remelt* d, amca_orientation_source src,
+                                               void* user) {
+  if (!d) return rm_fail(AMCA_ERR_INVALID, "null domain");
+  d->source = src;
+  d->source_user = user;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_set_sites(amca_remelt* d, int32_t n, const int32_t* cell,
+                                  const double* dT_crit) {
+  if (!d || n < 0 || (n > 0 && (!cell || !dT_crit))) return rm_fail(AMCA_ERR_INVALID, "bad sites");
+  for (int i = 0; i < n; i++) {
+    if (cell[i] < 0 || cell[i] >= d->P.NG) return rm_fail(AMCA_ERR_INVALID, "site out of range");
+    if (i > 0 && cell[i] <= cell[i - 1]) return rm_fail(AMCA_ERR_INVALID, "sites must ascend");
+  }
+  d->release(d->B.site_cell);
+  d->release(d->B.site_dt);
+  RM_TRY(d->alloc(&d->B.site_cell, (size_t)std::max(n, 1)));
+  RM_TRY(d->alloc(&d->B.site_dt, (size_t)std::max(n, 1)));
+  if (n > 0) {
+    RM_HIP(hipMemcpyAsync(d->B.site_cell, cell, sizeof(int32_t) * n, hipMemcpyHostToDevice, d->stream));
+    RM_HIP(hipMemcpyAsync(d->B.site_dt, dT_crit, sizeof(double) * n, hipMemcpyHostToDevice, d->stream));
+  }
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  c.n_sites = n;
+  RM_TRY(rm_put_counters(d, c));
+  d->site_blocks = std::max(1, std::min((n + 255) / 256, (int)amca::kMaxBlocks));
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_enable_log(amca_remelt* d, int32_t capacity) {
+  if (!d || capacity < 0) return rm_fail(AMCA_ERR_INVALID, "bad arguments");
+  RM_HIP(hipStreamSynchronize(d->stream));
+  d->release(d->B.log);
+  d->B.log = nullptr;
+  if (capacity > 0) RM_TRY(d->alloc(&d->B.log, (size_t)capacity));
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  c.log_cap = capacity;
+  return rm_put_counters(d, c);
+}
+
+amca_status amca_remelt_read_log(amca_remelt* d, int32_t cap, amca_remelt_step_record* rows,
+                                 int32_t* n_rows) {
+  if (!d || !n_rows) return rm_fail(AMCA_ERR_INVALID, "null argument");
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  const long long avail = std::min<long long>(c.step, c.log_cap);
+  *n_rows = (int32_t)avail;
+  const long long n = std::min<long long>(avail, std::max(cap, 0));
+  if (n > 0 && rows && d->B.log) {
+    std::vector<amca::RmLogRow> tmp((size_t)n);
+    RM_HIP(hipMemcpy(tmp.data(), d->B.log, sizeof(amca::RmLogRow) * n, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < n; i++) {
+      rows[i].time_before = tmp[i].time_before;
+      rows[i].dt = tmp[i].dt;
+      rows[i].n_nucleated = tmp[i].n_nucleated;
+      rows[i].n_captured = tmp[i].n_captured;
+      rows[i].n_fresh = tmp[i].n_fresh;
+      rows[i].n_flagged = tmp[i].n_flagged;
+    }
+  }
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_counters* out) {
+  if (!d || !d->fe_ready) return rm_fail(AMCA_ERR_INVALID, "FE mesh not set");
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  const long long start = c.step;
+  long long batch = 4;
+  while (true) {
+    if (c.err == 1) return rm_fail(AMCA_ERR_CAPACITY, "orientation table exhausted");
+    if (c.err == 2) return rm_fail(AMCA_ERR_INVALID, "FE time ratio below zero");
+    if (c.exhausted || !(c.tcafe < d->desc.final_time)) break;
+    const long long taken = c.step - start;
+    if (max_steps >= 0 && taken >= max_steps) break;
+    long long k = batch;
+    if (max_steps >= 0) k = std::min<long long>(k, max_steps - taken);
+    // each live site nucleates at most once per step
+    const long long room = d->P.orient_cap - (long long)c.n_orient;
+    if (c.n_sites > 0) k = std::max(1LL, std::min<long long>(k, room / c.n_sites));
+    RM_TRY(ensure_rows(d, (long long)c.n_orient + k * c.n_sites));
+    for (long long q = 0; q < k; q++)
+      RM_HIP(amca::rm_launch_step(d->P, d->B, d->site_blocks, d->stream));
+    RM_TRY(rm_counters(d, &c));
+    batch = std::min<long long>(batch * 2, 64);
+  }
+  rm_fill(d, c, out);
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_get_counters(amca_remelt* d, amca_remelt_counters* out) {
+  if (!d || !out) return rm_fail(AMCA_ERR_INVALID, "null argument");
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  rm_fill(d, c, out);
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_download(amca_remelt* d, int32_t* orient, int32_t* grain, double* T,
+                                 uint8_t* melted, uint8_t* cooling, int32_t grain_cap,
+                                 double* genv, int32_t* gcell, int32_t* gorient, int32_t* n_grains) {
+  if (!d) return rm_fail(AMCA_ERR_INVALID, "null domain");
+  const size_t ng = (size_t)d->P.NG;
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  if (orient) RM_HIP(hipMemcpy(orient, d->B.oid, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
+  if (grain) RM_HIP(hipMemcpy(grain, d->B.gid, sizeof(int32_t) * ng, hipMemcpyDeviceToHost));
+  if (T) RM_HIP(hipMemcpy(T, d->B.T, sizeof(double) * ng, hipMemcpyDeviceToHost));
+  if (melted) RM_HIP(hipMemcpy(melted, d->B.melted, ng, hipMemcpyDeviceToHost));
+  if (cooling) RM_HIP(hipMemcpy(cooling, d->B.cooling, ng, hipMemcpyDeviceToHost));
+  if (n_grains) *n_grains = c.n_grains;
+  const size_t m = (size_t)std::max(0, std::min(grain_cap, c.n_grains));
+  if (m > 0) {
+    if (genv) RM_HIP(hipMemcpy(genv, d->B.genv, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
+    if (gcell || gorient) {
+      std::vector<int32_t> co(2 * m);
+      RM_HIP(hipMemcpy(co.data(), d->B.gco, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost));
+      for (size_t g = 0; g < m; g++) {
+        if (gcell) gcell[g] = co[2 * g];
+        if (gorient) gorient[g] = co[2 * g + 1];
+      }
+    }
+  }
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_theta(amca_remelt* d, double* theta) {
+  if (!d || !d->fe_ready || !theta) return rm_fail(AMCA_ERR_INVALID, "FE mesh not set");
+  RM_HIP(hipStreamSynchronize(d->stream));
+  RM_HIP(hipMemcpy(theta, d->B.theta, sizeof(double) * d->P.nodes, hipMemcpyDeviceToHost));
+  return AMCA_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,78 @@
+// Device-side state of a coupled CA-FE (remelting) domain and its kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace amca {
+
+struct RmParams {
+  int nx, ny, nz;
+  int nxG, nyG, nzG;
+  int nxyG;
+  int NG;
+  long long N;
+  double h, x0, y0, z0;
+  double Tm, Ts, Tmin_nuc, a1, a2, a3;   // kinetics with exponents 1, 2, 3
+  double trunc, tsf, sqrt2f, cutoff, final_time, void_temp;
+  int off[26];
+  int fnx, fny, fnz, nodes, ncx, ncy;    // structured FE mesh
+  int n_frames;
+  int orient_cap;
+};
+
+struct RmCounters {
+  double t;          // CA time_
+  double tcafe;      // CafeSolver time
+  double dt;         // dT_ of the running step
+  double max_dt;     // maxTimestep_ = dT_FE
+  double time_fe;    // FiniteElementManager time_
+  double in_t0, in_t1;  // inTimeSeries_[0], [1]
+  double ratio;      // nodal time-interpolation ratio
+  long long step;
+  unsigned long long min_t_key;  // ordered key of the minimum flagged temperature
+  int frame;         // frame held as theta_[0]; frame-1 is theta_[1]
+  int skip, exhausted;
+  int n_grains, n_orient, n_sites;
+  int step_nuc, step_cap, step_fresh, n_flagged;
+  int err;
+  int log_cap;
+  unsigned tickets[4];
+};
+
+struct RmLogRow {
+  double time_before, dt;
+  int n_nucleated, n_captured, n_fresh, n_flagged;
+};
+
+struct RmBuffers {
+  int* gid;               // cellGrainID_ [NG]
+  int* oid;               // cellOrientationID_ [NG]
+  double* T;              // cellTemperature_ [NG]
+  unsigned char* cooling; // cooling_ [NG]
+  unsigned char* melted;  // hasBeenMelted_ [NG]
+  unsigned char* flag;    // cellLiquidNeigh [NG]
+  int* tag;               // element tag [NG]
+  double* par;            // parametric coordinates, 3 per cell [3*NG]
+  int* claim;             // smallest claiming sweep key, INT_MAX when free [NG]
+  unsigned* msk;          // per flagged cell: enclosed / captured neighbour slots [NG]
+  double4* genv;          // grain envelope by grain id [N]
+  int2* gco;              // grain -> (cell, orientation) [N]
+  const double* frames;   // nodal frames [n_frames * nodes]
+  const double* frame_time;  // [n_frames], relative to the first
+  double* theta;          // thetaArray_ [nodes]
+  int* site_cell;         // nucleation sites, ascending cell (never erased here)
+  double* site_dt;
+  double* l2g;            // orientation rows, 9 doubles each
+  int* rowcnt;            // fresh captures per (i, j) row, x-major row order [nx*ny]
+  int* rowoff;
+  unsigned* lb_flag;
+  unsigned long long* lb_agg;
+  unsigned long long* lb_pref;
+  RmLogRow* log;
+  RmCounters* ctr;
+};
+
+hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s);
+hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s);
+
+}  // namespace amca

@@ -1,0 +1,576 @@
+// Kernels of the coupled CA-FE step (realm type `cellular_automata_remelting`
+// driven by CafeSolver::time_integration_weak_coupling, CafeSolver.cpp:178-219).
+//
+// One step, in the reference's order:
+//   k_rm_phase    temperature of every cell from the FE nodal field (hex8
+//                 interpolation, MapVoxelManager::evaluate_temperature), cooling
+//                 flag, melting (checkup_current_temperature_to_update_phase_state,
+//                 CellularAutomataManager_3DRemelting.cpp:338-404)
+//   k_rm_liquid   flags the solid neighbours of undercooled liquid cells
+//                 (checkup_liquid_cell_neigh_to_growth, :485-542)
+//   k_rm_nucleate nucleation sites, remelting variant (:936-999)
+//   k_rm_mint     length cutoff and the minimum flagged temperature (:656-727)
+//   k_rm_dt       dt = min(factor*h/v(minT), dT_FE)
+//   k_rm_claim    growth of every flagged grain and claims on enclosed liquid
+//                 neighbours (single_integration_step, :544-652)
+//   k_rm_resolve  captures: the claim with the smallest x-major sweep key wins
+//   k_rm_rowscan  grain ids for cells that never had a grain, in sweep order
+//   k_rm_emit     child envelopes (capture_cell, :807-847)
+//   k_rm_finish   time advance and FE frame selection
+//                 (FiniteElementManager::pseudo_single_integration_step,
+//                 LoadDataFromFile.cpp:207-313)
+//   k_rm_nodal    nodal time interpolation of the FE field
+// The reference sweeps cells with i outermost and k innermost, and a capture in
+// the sweep is final, so a liquid cell goes to the first flagged cell in that
+// order whose envelope holds it: key (i*nyG + j)*nzG + k.  The three sweep
+// kernels walk one thread per (i, j) row over k, which keeps loads coalesced
+// and makes per-row counts come out in sweep order.
+#include <hip/hip_runtime.h>
+#include <climits>
+
+#include "device_util.h"
+#include "envelope.h"
+#include "remelt_step.h"
+
+namespace amca {
+
+namespace {
+
+constexpr int kRmBlock = 256;
+
+__device__ __forceinline__ void rm_ijk(const RmParams& P, int c, int& i, int& j, int& k) {
+  k = c / P.nxyG;
+  int rem = c - k * P.nxyG;
+  j = rem / P.nxG;
+  i = rem - j * P.nxG;
+}
+
+__device__ __forceinline__ bool rm_ghost_ijk(const RmParams& P, int i, int j, int k) {
+  return i == 0 || j == 0 || k == 0 || i == P.nxG - 1 || j == P.nyG - 1 || k == P.nzG - 1;
+}
+
+// exponents are 1, 2 and 3: pow(x, 1) = x, pow(x, 2) = x*x (glibc's pow is
+// exact for both); the cube term only matters when a3 != 0
+__device__ __forceinline__ double rm_velocity(const RmParams& P, double Tc) {
+  const double dTu = P.Tm - Tc;
+  double v = 0.0;
+  if (dTu > 0) {
+    v = P.a1 * dTu + P.a2 * (dTu * dTu) + P.a3 * (dTu * dTu * dTu);
+    if (v < 0.0) v = 0.0;
+  }
+  return v;
+}
+
+// order-preserving map of a double to an unsigned key (for atomicMin)
+__device__ __forceinline__ unsigned long long dkey(double x) {
+  unsigned long long b = (unsigned long long)__double_as_longlong(x);
+  return (b >> 63) ? ~b : (b | 0x8000000000000000ull);
+}
+__device__ __forceinline__ double dkey_inv(unsigned long long k) {
+  unsigned long long b = (k >> 63) ? (k & 0x7fffffffffffffffull) : ~k;
+  return __longlong_as_double((long long)b);
+}
+
+// MapVoxelManager::evaluate_temperature (MapVoxelManager.cpp:746-799) on a
+// structured mesh: element e has corner node ((ek*fny + ej)*fnx + ei).
+__device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B, int c, double& out) {
+  const double nm = P.void_temp + 0.00001;
+  const int e = B.tag[c];
+  if (e < 0) return false;
+  const int ei = e % P.ncx;
+  const int ej = (e / P.ncx) % P.ncy;
+  const int ek = e / (P.ncx * P.ncy);
+  const int nxy = P.fnx * P.fny;
+  int nid[8];
+  nid[0] = nxy * ek + P.fnx * ej + ei;
+  nid[1] = nid[0] + 1;
+  nid[2] = nid[1] + P.fnx;
+  nid[3] = nid[2] - 1;
+  nid[4] = nid[0] + nxy;
+  nid[5] = nid[1] + nxy;
+  nid[6] = nid[2] + nxy;
+  nid[7] = nid[3] + nxy;
+  double th[8];
+#pragma unroll
+  for (int m = 0; m < 8; m++) th[m] = B.theta[nid[m]];
+#pragma unroll
+  for (int m = 0; m < 8; m++)
+    if (th[m] < nm) return false;
+  const double xi = B.par[3 * (size_t)c], eta = B.par[3 * (size_t)c + 1],
+               zeta = B.par[3 * (size_t)c + 2];
+  double sf[8];
+  sf[0] = 0.125 * (1.0 - eta) * (1.0 - xi) * (1.0 - zeta);
+  sf[1] = 0.125 * (1.0 - eta) * (1.0 + xi) * (1.0 - zeta);
+  sf[2] = 0.125 * (1.0 + eta) * (1.0 + xi) * (1.0 - zeta);
+  sf[3] = 0.125 * (1.0 + eta) * (1.0 - xi) * (1.0 - zeta);
+  sf[4] = 0.125 * (1.0 - eta) * (1.0 - xi) * (1.0 + zeta);
+  sf[5] = 0.125 * (1.0 - eta) * (1.0 + xi) * (1.0 + zeta);
+  sf[6] = 0.125 * (1.0 + eta) * (1.0 + xi) * (1.0 + zeta);
+  sf[7] = 0.125 * (1.0 + eta) * (1.0 - xi) * (1.0 + zeta);
+  double s = 0.0;
+#pragma unroll
+  for (int m = 0; m < 8; m++) s += sf[m] * th[m];
+  out = s;
+  return true;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+
+__global__ void k_rm_init(RmParams P, RmBuffers B) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.NG; c += stride) {
+    B.T[c] = -1.0;  // nullMatTemp_ (CellularAutomataManager_3DRemelting.cpp:29)
+    B.cooling[c] = 0;
+    B.flag[c] = 0;
+    B.claim[c] = INT_MAX;
+    B.msk[c] = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  const int skip = (ctr->tcafe < P.final_time && !ctr->exhausted) ? 0 : 1;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->skip = skip;
+  if (skip) return;
+  const bool later = ctr->step > 0;
+  const double dt_prev = ctr->dt;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
+    const int c = (int)cc;
+    double Tn = -1.0;
+    const bool mat = rm_eval_T(P, B, c, Tn);
+    if (later) {
+      double rate = 410;
+      const double Tp = B.T[c];
+      if (Tp > 0) rate = (Tn - Tp) / dt_prev;
+      if (Tn <= 0) rate = 0;
+      B.cooling[c] = mat ? (rate < 0) : 0;
+    }
+    B.T[c] = Tn;
+    B.flag[c] = 0;
+    int i, j, k;
+    rm_ijk(P, c, i, j, k);
+    if (rm_ghost_ijk(P, i, j, k)) continue;
+    if (mat && Tn < P.Tm) continue;
+    B.oid[c] = -1;
+    B.melted[c] = mat ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B) {
+  const RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  const double tc = ctr->tcafe, maxdt = ctr->max_dt;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
+    const int c = (int)cc;
+    int i, j, k;
+    rm_ijk(P, c, i, j, k);
+    if (rm_ghost_ijk(P, i, j, k)) continue;
+    const double Tc = B.T[c];
+    const bool mat = !(Tc < -1.0 + 0.01);
+    const int o = B.oid[c];
+    if (o == -1 && mat && tc >= maxdt) B.melted[c] = 1;
+    if (!mat) {
+      B.melted[c] = 0;
+      continue;
+    }
+    if (o != -1) continue;
+    if (Tc >= P.Tm) continue;
+#pragma unroll
+    for (int s = 0; s < 26; s++) {
+      const int nb = c + P.off[s];
+      if (B.oid[nb] > -1) B.flag[nb] = 1;
+    }
+  }
+}
+
+// nucleate_at_nucleation_sites, remelting variant: sites stay in the list;
+// a nucleus reuses the cell's grain record when it has one
+__global__ void __launch_bounds__(kRmBlock) k_rm_nucleate(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ int s_bid;
+  __shared__ unsigned long long s_sm[kRmBlock / 64];
+  __shared__ unsigned long long s_excl;
+  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[0], 1u);
+  __syncthreads();
+  const int bid = s_bid;
+  const int n = ctr->n_sites;
+  const int ng0 = ctr->n_grains, no0 = ctr->n_orient;
+  auto decide = [&](int i, int& c, bool& nuc, bool& fresh) {
+    c = B.site_cell[i];
+    const double Tc = B.T[c];
+    nuc = false;
+    fresh = false;
+    if (B.cooling[c] && B.oid[c] == -1 && B.melted[c] && (Tc > P.Tmin_nuc)) {
+      const double dTu = P.Tm - Tc;
+      if (dTu > B.site_dt[i]) {
+        nuc = true;
+        fresh = B.gid[c] == -1;
+      }
+    }
+  };
+  int lo, hi;
+  chunk_of(n, bid, gridDim.x, kRmBlock, lo, hi);
+  unsigned long long mine = 0;
+  for (int i = lo + threadIdx.x; i < hi; i += kRmBlock) {
+    int c;
+    bool nuc, fresh;
+    decide(i, c, nuc, fresh);
+    mine += ((unsigned long long)nuc << 32) | (unsigned long long)fresh;
+  }
+  unsigned long long agg;
+  block_scan_excl(mine, &agg, s_sm);
+  if (threadIdx.x < 64) {
+    unsigned long long e =
+        lookback(B.lb_flag, B.lb_agg, B.lb_pref, bid, (unsigned)(ctr->step * 4 + 0), agg);
+    if (threadIdx.x == 0) s_excl = e;
+  }
+  __syncthreads();
+  unsigned long long run = s_excl;
+  for (int base = lo; base < hi; base += kRmBlock) {
+    const int i = base + threadIdx.x;
+    int c = 0;
+    bool nuc = false, fresh = false;
+    if (i < hi) decide(i, c, nuc, fresh);
+    unsigned long long tot;
+    const unsigned long long off =
+        run + block_scan_excl(((unsigned long long)nuc << 32) | (unsigned long long)fresh, &tot, s_sm);
+    if (nuc) {
+      const int o = no0 + (int)(off >> 32);
+      if (o < P.orient_cap) {
+        int ci, cj, ck;
+        rm_ijk(P, c, ci, cj, ck);
+        const double4 e = make_double4(centre_coord(P.x0, P.h, ci), centre_coord(P.y0, P.h, cj),
+                                       centre_coord(P.z0, P.h, ck), 0.0);
+        int g = B.gid[c];
+        if (fresh) {
+          g = ng0 + (int)(off & 0xffffffffull);
+          B.gid[c] = g;
+        }
+        B.genv[g] = e;
+        if (fresh)
+          B.gco[g] = make_int2(c, o);
+        else
+          B.gco[g].y = o;  // a reused record keeps its cell
+        B.oid[c] = o;
+      }
+    }
+    run += tot;
+  }
+  if (threadIdx.x == 0 && bid == (int)gridDim.x - 1) {
+    const int nn = (int)(run >> 32);
+    const int nf = (int)(run & 0xffffffffull);
+    if (no0 + nn > P.orient_cap) ctr->err = 1;
+    ctr->n_orient = no0 + nn;
+    ctr->n_grains = ng0 + nf;
+    ctr->step_nuc = nn;
+    ctr->tickets[0] = 0u;
+  }
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ unsigned long long s_min;
+  __shared__ int s_cnt;
+  if (threadIdx.x == 0) {
+    s_min = ~0ull;
+    s_cnt = 0;
+  }
+  __syncthreads();
+  unsigned long long mk = ~0ull;
+  int cnt = 0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
+    const int c = (int)cc;
+    if (!B.flag[c]) continue;
+    cnt++;
+    int i, j, k;
+    rm_ijk(P, c, i, j, k);
+    if (rm_ghost_ijk(P, i, j, k)) continue;
+    const int g = B.gid[c];
+    if (g < 0) continue;
+    const double L = B.genv[g].w;
+    if (!(L < P.cutoff)) B.genv[g].w = P.h;  // length reset (:688-694)
+    const double Tc = B.T[c];
+    if (Tc <= P.Ts) continue;
+    const unsigned long long k2 = dkey(Tc);
+    if (k2 < mk) mk = k2;
+  }
+  if (mk != ~0ull) atomicMin(&s_min, mk);
+  if (cnt) atomicAdd(&s_cnt, cnt);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (s_min != ~0ull) atomicMin(&ctr->min_t_key, s_min);
+    if (s_cnt) atomicAdd(&ctr->n_flagged, s_cnt);
+  }
+}
+
+__global__ void k_rm_dt(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  double minT = P.Tm;
+  if (ctr->min_t_key != ~0ull) {
+    const double m = dkey_inv(ctr->min_t_key);
+    if (minT > m) minT = m;
+  }
+  const double v = rm_velocity(P, minT);
+  const double maxV = (v > 0 ? v : 0);
+  const double a = P.tsf * P.h / maxV;
+  const double dtca = (ctr->max_dt < a) ? ctr->max_dt : a;  // std::min(a, maxDt)
+  const double dtfe = ctr->max_dt;
+  ctr->dt = ((dtfe) < (dtca) ? (dtfe) : (dtca));  // __min(dT_FE, dT_CA)
+  ctr->step_cap = 0;
+  ctr->step_fresh = 0;
+}
+
+// one thread per (i, j) row of interior cells, k innermost
+__device__ __forceinline__ bool rm_row(const RmParams& P, int& i, int& j) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= P.nx * P.ny) return false;
+  i = 1 + t % P.nx;
+  j = 1 + t / P.nx;
+  return true;
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) {
+  const RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  int i, j;
+  if (!rm_row(P, i, j)) return;
+  const double dt = ctr->dt;
+  const double vts = rm_velocity(P, P.Ts + 0.01);
+  for (int k = 1; k <= P.nz; k++) {
+    const int c = k * P.nxyG + j * P.nxG + i;
+    if (!B.flag[c]) continue;
+    const int g = B.gid[c];
+    if (g < 0) continue;
+    double4 e = B.genv[g];
+    const double Tc = B.T[c];
+    const double vel = (Tc < P.Ts) ? vts : rm_velocity(P, Tc);
+    e.w = e.w + dt * vel;  // Grain::grow
+    B.genv[g].w = e.w;
+    const int o = B.gco[g].y;
+    double r[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) r[q] = B.l2g[9 * (size_t)o + q];
+    const int key = (i * P.nyG + j) * P.nzG + k;
+    unsigned m = 0;
+#pragma unroll
+    for (int s = 0; s < 26; s++) {
+      int di, dj, dk;
+      slot_delta(s, di, dj, dk);
+      const int nb = c + P.off[s];
+      if (B.oid[nb] > -1) continue;
+      const double Tn = B.T[nb];
+      if (Tn > P.Tm) continue;
+      if (!(Tn > -1.0)) continue;
+      if (rm_ghost_ijk(P, i + di, j + dj, k + dk)) continue;
+      const double gx = centre_coord(P.x0, P.h, i + di) - e.x;
+      const double gy = centre_coord(P.y0, P.h, j + dj) - e.y;
+      const double gz = centre_coord(P.z0, P.h, k + dk) - e.z;
+      if (inside_envelope(r, e.w, gx, gy, gz)) {
+        m |= 1u << s;
+        atomicMin(&B.claim[nb], key);
+      }
+    }
+    B.msk[c] = m;
+  }
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_resolve(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  int i, j;
+  if (!rm_row(P, i, j)) return;
+  int fresh = 0, caps = 0;
+  for (int k = 1; k <= P.nz; k++) {
+    const int c = k * P.nxyG + j * P.nxG + i;
+    if (!B.flag[c]) continue;
+    if (B.gid[c] < 0) continue;
+    unsigned m = B.msk[c];
+    if (!m) continue;
+    const int key = (i * P.nyG + j) * P.nzG + k;
+    unsigned cap = 0;
+    while (m) {
+      const int s = __builtin_ctz(m);
+      m &= m - 1;
+      const int nb = c + P.off[s];
+      if (B.claim[nb] == key) {
+        cap |= 1u << s;
+        if (B.gid[nb] < 0) fresh++;
+      }
+    }
+    B.msk[c] = cap;
+    caps += __builtin_popcount(cap);
+  }
+  B.rowcnt[(i - 1) * P.ny + (j - 1)] = fresh;
+  if (fresh) atomicAdd(&ctr->step_fresh, fresh);
+  if (caps) atomicAdd(&ctr->step_cap, caps);
+}
+
+constexpr int kRmScan = 1024;
+
+__global__ void __launch_bounds__(kRmScan) k_rm_rowscan(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip || ctr->step_fresh == 0) return;
+  __shared__ unsigned long long s_sm[kRmScan / 64];
+  const int nrow = P.nx * P.ny;
+  const int per = (nrow + kRmScan - 1) / kRmScan;
+  const int lo = min(nrow, threadIdx.x * per), hi = min(nrow, lo + per);
+  unsigned long long mine = 0;
+  for (int r = lo; r < hi; r++) mine += (unsigned long long)B.rowcnt[r];
+  unsigned long long tot;
+  unsigned long long off = block_scan_excl(mine, &tot, s_sm) + (unsigned long long)ctr->n_grains;
+  for (int r = lo; r < hi; r++) {
+    B.rowoff[r] = (int)off;
+    off += (unsigned long long)B.rowcnt[r];
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) ctr->n_grains += (int)tot;
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
+  const RmCounters* ctr = B.ctr;
+  if (ctr->skip || ctr->step_cap == 0) return;
+  int i, j;
+  if (!rm_row(P, i, j)) return;
+  int next = ctr->step_fresh ? B.rowoff[(i - 1) * P.ny + (j - 1)] : 0;
+  for (int k = 1; k <= P.nz; k++) {
+    const int c = k * P.nxyG + j * P.nxG + i;
+    if (!B.flag[c]) continue;
+    const int g = B.gid[c];
+    if (g < 0) continue;
+    unsigned cap = B.msk[c];
+    if (!cap) continue;
+    const double4 e = B.genv[g];
+    const int o = B.gco[g].y;
+    double r[9];
+#pragma unroll
+    for (int q = 0; q < 9; q++) r[q] = B.l2g[9 * (size_t)o + q];
+    while (cap) {
+      const int s = __builtin_ctz(cap);
+      cap &= cap - 1;
+      int di, dj, dk;
+      slot_delta(s, di, dj, dk);
+      const int nb = c + P.off[s];
+      const double gx = centre_coord(P.x0, P.h, i + di) - e.x;
+      const double gy = centre_coord(P.y0, P.h, j + dj) - e.y;
+      const double gz = centre_coord(P.z0, P.h, k + dk) - e.z;
+      double out[4];
+      child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
+      const double4 ce = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
+      int id = B.gid[nb];
+      if (id < 0) {
+        id = next++;
+        B.gid[nb] = id;
+        B.gco[id] = make_int2(nb, o);
+      } else {
+        B.gco[id].y = o;
+      }
+      B.genv[id] = ce;
+      B.oid[nb] = o;
+      B.claim[nb] = INT_MAX;
+    }
+  }
+}
+
+__global__ void k_rm_finish(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  const double t0 = ctr->t;
+  const double dt = ctr->dt;
+  ctr->t = t0 + dt;
+  const long long st = ctr->step;
+  ctr->step = st + 1;
+  ctr->tcafe = ctr->tcafe + dt;
+  if (B.log && st < ctr->log_cap) {
+    RmLogRow row;
+    row.time_before = t0;
+    row.dt = dt;
+    row.n_nucleated = ctr->step_nuc;
+    row.n_captured = ctr->step_cap;
+    row.n_fresh = ctr->step_fresh;
+    row.n_flagged = ctr->n_flagged;
+    B.log[st] = row;
+  }
+  ctr->n_flagged = 0;
+  ctr->min_t_key = ~0ull;
+  // FiniteElementManager::pseudo_single_integration_step
+  double tf = ctr->time_fe;
+  if (tf == 0) tf = ctr->in_t1;
+  tf += dt;
+  ctr->time_fe = tf;
+  double in0 = ctr->in_t0, in1 = ctr->in_t1;
+  int f = ctr->frame;
+  while (tf >= in0) {
+    if (f + 1 >= P.n_frames) {
+      ctr->exhausted = 1;
+      break;
+    }
+    f++;
+    in1 = in0;
+    in0 = B.frame_time[f];
+  }
+  ctr->frame = f;
+  ctr->in_t0 = in0;
+  ctr->in_t1 = in1;
+  if (!ctr->exhausted) {
+    ctr->ratio = (tf - in1) / (in0 - in1);
+    if (ctr->ratio < 0.0) ctr->err = 2;
+    ctr->max_dt = in0 - tf;  // get_max_time_step
+  }
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_nodal(RmParams P, RmBuffers B) {
+  const RmCounters* ctr = B.ctr;
+  if (ctr->skip || ctr->exhausted) return;
+  const double r = ctr->ratio;
+  const double* th0 = B.frames + (size_t)ctr->frame * P.nodes;
+  const double* th1 = th0 - P.nodes;
+  const int stride = gridDim.x * blockDim.x;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.nodes; n += stride) {
+    const double a = th0[n], b = th1[n];
+    double v;
+    if (a <= P.void_temp) v = P.void_temp;
+    else if (b <= P.void_temp) v = a;
+    else v = r * (a - b) + b;
+    B.theta[n] = v;
+  }
+}
+
+// ---------------------------------------------------------------------------
+
+hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s) {
+  long long blocks = (P.NG + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(k_rm_init, dim3((unsigned)blocks), dim3(256), 0, s, P, B);
+  return hipGetLastError();
+}
+
+hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s) {
+  long long cb = (P.NG + kRmBlock - 1) / kRmBlock;
+  if (cb > 16384) cb = 16384;
+  const unsigned cells = (unsigned)cb;
+  const unsigned rows = (unsigned)((P.nx * P.ny + kRmBlock - 1) / kRmBlock);
+  long long nbk = (P.nodes + kRmBlock - 1) / kRmBlock;
+  if (nbk > 8192) nbk = 8192;
+  hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B);
+  hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+  return hipGetLastError();
+}
+
+}  // namespace amca

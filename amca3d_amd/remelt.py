@@ -1,0 +1,240 @@
+"""Python handle over the coupled CA-FE (remelting) C ABI (amca_remelt_*)."""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import AmcaError, load
+
+I32P = C.POINTER(C.c_int32)
+F64P = C.POINTER(C.c_double)
+U8P = C.POINTER(C.c_uint8)
+
+
+class RemeltDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("h", C.c_double), ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+        ("Tm", C.c_double), ("Ts", C.c_double), ("Tmin_nuc", C.c_double),
+        ("a1", C.c_double), ("a2", C.c_double), ("a3", C.c_double),
+        ("b1", C.c_double), ("b2", C.c_double), ("b3", C.c_double),
+        ("trunc", C.c_double), ("time_step_factor", C.c_double), ("cutoff_factor", C.c_double),
+        ("final_time", C.c_double), ("void_temperature", C.c_double),
+        ("bin", C.c_int32 * 3), ("tolerance", C.c_double), ("max_iterations", C.c_int32),
+        ("initial_melted", C.c_int32),
+    ]
+
+
+class RemeltCounters(C.Structure):
+    _fields_ = [("step", C.c_int64), ("time", C.c_double), ("time_fe", C.c_double),
+                ("dt", C.c_double), ("max_dt", C.c_double), ("n_grains", C.c_int32),
+                ("n_orient", C.c_int32), ("done", C.c_int32), ("frames_exhausted", C.c_int32)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class RemeltStepRecord(C.Structure):
+    _fields_ = [("time_before", C.c_double), ("dt", C.c_double), ("n_nucleated", C.c_int32),
+                ("n_captured", C.c_int32), ("n_fresh", C.c_int32), ("n_flagged", C.c_int32)]
+
+
+ORIENT_SOURCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, F64P, F64P)
+
+SIGNATURES = {
+    "amca_remelt_create": (C.c_int, [C.POINTER(RemeltDesc), C.c_int32, C.POINTER(C.c_void_p)]),
+    "amca_remelt_destroy": (C.c_int, [C.c_void_p]),
+    "amca_remelt_set_fe": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_int32, F64P, C.c_int32,
+                                     F64P, F64P]),
+    "amca_remelt_voxel_map": (C.c_int, [C.c_void_p, I32P, F64P]),
+    "amca_fe_map_voxels": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                     C.c_double, C.c_double, I32P, C.c_double, C.c_int32,
+                                     C.c_int32, C.c_int32, C.c_int32, F64P, I32P, F64P]),
+    "amca_remelt_set_state": (C.c_int, [C.c_void_p, I32P, I32P, U8P, C.c_int32, C.c_int32, I32P,
+                                        I32P]),
+    "amca_remelt_push_orientations": (C.c_int, [C.c_void_p, C.c_int32, F64P, F64P]),
+    "amca_remelt_set_orientation_source": (C.c_int, [C.c_void_p, ORIENT_SOURCE, C.c_void_p]),
+    "amca_remelt_set_sites": (C.c_int, [C.c_void_p, C.c_int32, I32P, F64P]),
+    "amca_remelt_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(RemeltCounters)]),
+    "amca_remelt_get_counters": (C.c_int, [C.c_void_p, C.POINTER(RemeltCounters)]),
+    "amca_remelt_enable_log": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_remelt_read_log": (C.c_int, [C.c_void_p, C.c_int32, C.POINTER(RemeltStepRecord), I32P]),
+    "amca_remelt_download": (C.c_int, [C.c_void_p, I32P, I32P, F64P, U8P, U8P, C.c_int32, F64P,
+                                       I32P, I32P, I32P]),
+    "amca_remelt_theta": (C.c_int, [C.c_void_p, F64P]),
+    "amca_remelt_last_error": (C.c_char_p, []),
+}
+
+_bound = False
+
+
+def lib():
+    global _bound
+    L = load()
+    if not _bound:
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _bound = True
+    return L
+
+
+def _check(status, where):
+    if status != 0:
+        raise AmcaError(f"{where}: status {status}: {lib().amca_remelt_last_error().decode()}")
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+def map_voxels(grid, fe, bin=(5, 5, 5), tol=1e-16, max_iter=20):
+    """Host-only voxel mapping (element tag and parametric coordinates per cell)."""
+    nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
+    ng = (nx + 2) * (ny + 2) * (nz + 2)
+    tag = np.empty(ng, np.int32)
+    par = np.empty(3 * ng, np.float64)
+    b = (C.c_int32 * 3)(*bin)
+    xyz = np.ascontiguousarray(fe["xyz"], np.float64)
+    _check(lib().amca_fe_map_voxels(nx, ny, nz, grid["h"], grid["x0"], grid["y0"], grid["z0"], b,
+                                    tol, max_iter, *fe["shape"], _p(xyz, C.c_double),
+                                    _p(tag, C.c_int32), _p(par, C.c_double)), "map_voxels")
+    return tag, par.reshape(-1, 3)
+
+
+def rotations(euler):
+    """Row-major local-to-global rotations (Z-X-Z Euler), numpy (host plumbing)."""
+    e = np.asarray(euler, np.float64).reshape(-1, 3)
+    ca, sa = np.cos(e[:, 0]), np.sin(e[:, 0])
+    cb, sb = np.cos(e[:, 1]), np.sin(e[:, 1])
+    cg, sg = np.cos(e[:, 2]), np.sin(e[:, 2])
+    return np.stack([ca * cg - sa * cb * sg, -ca * sg - sa * cb * cg, sa * sb,
+                     sa * cg + ca * cb * sg, -sa * sg + ca * cb * cg, -ca * sb,
+                     sb * sg, sb * cg, cb], axis=1)
+
+
+class RemeltDomain:
+    def __init__(self, desc_kw, orient_capacity):
+        d = RemeltDesc()
+        for k, v in desc_kw.items():
+            if k == "bin":
+                for q in range(3):
+                    d.bin[q] = v[q]
+            else:
+                setattr(d, k, v)
+        self.grid = (d.nx, d.ny, d.nz)
+        self.ng = (d.nx + 2) * (d.ny + 2) * (d.nz + 2)
+        self.h = C.c_void_p()
+        _check(lib().amca_remelt_create(C.byref(d), int(orient_capacity), C.byref(self.h)), "create")
+        self._keep = []
+
+    def set_fe(self, fe):
+        xyz = np.ascontiguousarray(fe["xyz"], np.float64)
+        tt = np.ascontiguousarray(fe["times"], np.float64)
+        th = np.ascontiguousarray(fe["theta"], np.float64)
+        self.nodes = int(np.prod(fe["shape"]))
+        _check(lib().amca_remelt_set_fe(self.h, *fe["shape"], _p(xyz, C.c_double), len(tt),
+                                        _p(tt, C.c_double), _p(th, C.c_double)), "set_fe")
+
+    def voxel_map(self):
+        tag = np.empty(self.ng, np.int32)
+        par = np.empty(3 * self.ng, np.float64)
+        _check(lib().amca_remelt_voxel_map(self.h, _p(tag, C.c_int32), _p(par, C.c_double)), "voxel_map")
+        return tag, par.reshape(-1, 3)
+
+    def push_orientations(self, euler, l2g):
+        e = np.ascontiguousarray(euler, np.float64).reshape(-1)
+        r = np.ascontiguousarray(l2g, np.float64).reshape(-1)
+        _check(lib().amca_remelt_push_orientations(self.h, len(r) // 9, _p(e, C.c_double),
+                                                   _p(r, C.c_double)), "push_orientations")
+
+    def set_orientation_source(self, fn):
+        """fn(n) -> (euler[n,3], l2g[n,9]) continuing the rand() stream."""
+        def cb(user, n, eu, r):
+            try:
+                a, b = fn(int(n))
+                C.memmove(eu, np.ascontiguousarray(a, np.float64).ctypes.data, 8 * 3 * n)
+                C.memmove(r, np.ascontiguousarray(b, np.float64).ctypes.data, 8 * 9 * n)
+                return 0
+            except Exception:
+                return 1
+        f = ORIENT_SOURCE(cb)
+        self._keep.append(f)
+        _check(lib().amca_remelt_set_orientation_source(self.h, f, None), "set_source")
+
+    def set_state(self, st):
+        o = np.ascontiguousarray(st["ori"], np.int32)
+        g = np.ascontiguousarray(st["gid"], np.int32)
+        m = np.ascontiguousarray(st["melted"], np.uint8)
+        gc = np.ascontiguousarray(st["grain_cell"], np.int32)
+        go = np.ascontiguousarray(st["grain_orient"], np.int32)
+        _check(lib().amca_remelt_set_state(self.h, _p(o, C.c_int32), _p(g, C.c_int32),
+                                           _p(m, C.c_uint8), len(st["euler"]), len(gc),
+                                           _p(gc, C.c_int32), _p(go, C.c_int32)), "set_state")
+
+    def set_sites(self, cells, dts):
+        c = np.ascontiguousarray(cells, np.int32)
+        d = np.ascontiguousarray(dts, np.float64)
+        _check(lib().amca_remelt_set_sites(self.h, len(c), _p(c, C.c_int32), _p(d, C.c_double)),
+               "set_sites")
+
+    def enable_log(self, cap):
+        _check(lib().amca_remelt_enable_log(self.h, int(cap)), "enable_log")
+
+    def read_log(self):
+        n = C.c_int32(0)
+        _check(lib().amca_remelt_read_log(self.h, 0, None, C.byref(n)), "read_log")
+        m = int(n.value)
+        rows = (RemeltStepRecord * max(m, 1))()
+        _check(lib().amca_remelt_read_log(self.h, m, rows, C.byref(n)), "read_log")
+        return np.array([(r.time_before, r.dt, r.n_nucleated, r.n_captured, r.n_fresh, r.n_flagged)
+                         for r in rows[:m]],
+                        dtype=[("time_before", "f8"), ("dt", "f8"), ("n_nucleated", "i4"),
+                               ("n_captured", "i4"), ("n_fresh", "i4"), ("n_flagged", "i4")])
+
+    def run(self, max_steps=-1):
+        c = RemeltCounters()
+        _check(lib().amca_remelt_run(self.h, int(max_steps), C.byref(c)), "run")
+        return c.as_dict()
+
+    def counters(self):
+        c = RemeltCounters()
+        _check(lib().amca_remelt_get_counters(self.h, C.byref(c)), "counters")
+        return c.as_dict()
+
+    def download(self, grain_cap=None):
+        ng = self.ng
+        out = dict(ori=np.empty(ng, np.int32), gid=np.empty(ng, np.int32),
+                   T=np.empty(ng, np.float64), melted=np.empty(ng, np.uint8),
+                   cooling=np.empty(ng, np.uint8))
+        n = C.c_int32(0)
+        _check(lib().amca_remelt_download(self.h, None, None, None, None, None, 0, None, None, None,
+                                          C.byref(n)), "download")
+        m = int(n.value) if grain_cap is None else min(grain_cap, int(n.value))
+        env = np.empty(4 * max(m, 1), np.float64)
+        gc = np.empty(max(m, 1), np.int32)
+        go = np.empty(max(m, 1), np.int32)
+        _check(lib().amca_remelt_download(self.h, _p(out["ori"], C.c_int32), _p(out["gid"], C.c_int32),
+                                          _p(out["T"], C.c_double), _p(out["melted"], C.c_uint8),
+                                          _p(out["cooling"], C.c_uint8), m, _p(env, C.c_double),
+                                          _p(gc, C.c_int32), _p(go, C.c_int32), C.byref(n)),
+               "download")
+        out.update(grain_env=env[:4 * m].reshape(-1, 4), grain_cell=gc[:m], grain_orient=go[:m],
+                   n_grains=int(n.value))
+        return out
+
+    def theta(self):
+        t = np.empty(self.nodes, np.float64)
+        _check(lib().amca_remelt_theta(self.h, _p(t, C.c_double)), "theta")
+        return t
+
+    def close(self):
+        if self.h:
+            lib().amca_remelt_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

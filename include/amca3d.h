@@ -172,6 +172,93 @@ amca_status amca_envelope_selftest(int32_t n, const double* env4, const double* 
 const char* amca_last_error(void);
 const char* amca_version(void);
 
+/* ---- coupled CA-FE (remelting) domain --------------------------------------
+ * The `cellular_automata_remelting` realm driven by CafeSolver
+ * (CafeSolver.cpp:159-219, CellularAutomataManager_3DRemelting.cpp) with the
+ * FE temperature provider (LoadDataFromFile.cpp, MapVoxelManager.cpp) on the
+ * device. */
+typedef struct amca_remelt amca_remelt;
+
+typedef struct {
+  int32_t nx, ny, nz;
+  double h, x0, y0, z0;
+  double Tm, Ts, Tmin_nuc;        /* melting, solidus, minimum_temperature_for_nucleation */
+  double a1, a2, a3, b1, b2, b3;  /* ProblemPhysics.cpp:142-158; b must be 1, 2, 3 */
+  double trunc;                   /* envelope truncation coefficient */
+  double time_step_factor;        /* time_step_factor_ca */
+  double cutoff_factor;           /* cell_grain_length_cutoff_factor */
+  double final_time;              /* termination_time */
+  double void_temperature;        /* transfers.void_temperature */
+  int32_t bin[3];                 /* transfers.bin_size */
+  double tolerance;               /* transfers.tolerance */
+  int32_t max_iterations;         /* transfers.max_iterations */
+  int32_t initial_melted;         /* phase_state.has_been_melted */
+} amca_remelt_desc;
+
+typedef struct {
+  int64_t step;
+  double time, time_fe, dt, max_dt;
+  int32_t n_grains, n_orient, done, frames_exhausted;
+} amca_remelt_counters;
+
+typedef struct {
+  double time_before, dt;
+  int32_t n_nucleated, n_captured, n_fresh, n_flagged;
+} amca_remelt_step_record;
+
+/* Supplies n more orientation rows (Euler angles and row-major local-to-global
+ * rotation) continuing the host's rand() stream; returns 0 on success. */
+typedef int (*amca_orientation_source)(void* user, int32_t n, double* euler3, double* l2g9);
+
+/* CellularAutomataManager_3DRemelting::initialize (:81-135) storage. */
+amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capacity,
+                               amca_remelt** out);
+amca_status amca_remelt_destroy(amca_remelt* d);
+/* A structured hex FE mesh and its nodal frames (the data of
+ * load_mesh_and_first_two_steps_results_from_txt_file / load_next_step_nodal_data,
+ * LoadDataFromFile.cpp:21-191, after offsets and scales); maps every cell to its
+ * element and parametric coordinates (MapVoxelManager.cpp:140-699) on the host. */
+amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t fnz,
+                               const double* node_xyz, int32_t n_frames, const double* frame_time,
+                               const double* frame_theta);
+amca_status amca_remelt_voxel_map(amca_remelt* d, int32_t* tag, double* par);
+/* The same voxel mapping without a domain or a GPU: tag[NG], par[3*NG]
+ * (MapVoxelManager::execute_with_focus_on_CA_region, MapVoxelManager.cpp:140-482). */
+amca_status amca_fe_map_voxels(int32_t nx, int32_t ny, int32_t nz, double h, double x0, double y0,
+                               double z0, const int32_t* bin, double tolerance,
+                               int32_t max_iterations, int32_t fnx, int32_t fny, int32_t fnz,
+                               const double* node_xyz, int32_t* tag, double* par);
+/* load_microstructure_information + retrieve_grain_information_before_simulation
+ * (CellularAutomataManager_3DRemelting.cpp:165-332): per-cell orientation, grain
+ * id and melted flag; grain records (cell, orientation), centres reset to the
+ * cell centre with zero length.  Orientation rows 0..n_orient-1 must be pushed first. */
+amca_status amca_remelt_set_state(amca_remelt* d, const int32_t* orient, const int32_t* grain,
+                                  const uint8_t* melted, int32_t n_orient, int32_t n_grains,
+                                  const int32_t* grain_cell, const int32_t* grain_orient);
+amca_status amca_remelt_push_orientations(amca_remelt* d, int32_t n, const double* euler3,
+                                          const double* l2g9);
+amca_status amca_remelt_set_orientation_source(amca_remelt* d, amca_orientation_source src,
+                                               void* user);
+/* Nucleation sites (criterion_to_activate_grain.using_nucleation_seed). */
+amca_status amca_remelt_set_sites(amca_remelt* d, int32_t n, const int32_t* cell,
+                                  const double* dT_crit);
+/* CafeSolver::time_integration_weak_coupling (CafeSolver.cpp:178-219) until
+ * termination_time, the end of the frames, or max_steps steps (< 0: no limit). */
+amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_counters* out);
+amca_status amca_remelt_get_counters(amca_remelt* d, amca_remelt_counters* out);
+amca_status amca_remelt_enable_log(amca_remelt* d, int32_t capacity);
+amca_status amca_remelt_read_log(amca_remelt* d, int32_t cap, amca_remelt_step_record* rows,
+                                 int32_t* n_rows);
+/* cellOrientationID_, cellGrainID_, cellTemperature_, hasBeenMelted_, cooling_
+ * per cell; grain records (envelope, cell, orientation) by grain id. */
+amca_status amca_remelt_download(amca_remelt* d, int32_t* orient, int32_t* grain, double* T,
+                                 uint8_t* melted, uint8_t* cooling, int32_t grain_cap,
+                                 double* genv, int32_t* gcell, int32_t* gorient,
+                                 int32_t* n_grains);
+/* thetaArray_: the nodal field the next step interpolates from. */
+amca_status amca_remelt_theta(amca_remelt* d, double* theta);
+const char* amca_remelt_last_error(void);
+
 /* ---- realm layer: YAML input deck -> device domain ------------------------
  * The reference's Simulation / CellularAutomataSolver layer
  * (Simulation.cpp:36-93, CellularAutomataSolver.cpp:40-202) with the same

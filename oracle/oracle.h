@@ -62,6 +62,69 @@ float oracle_sqrtf(float x);
 void oracle_envelope_batch(int n, const double* env4, const double* l2g9, const double* cell3,
                            double h, double trunc, int32_t* inside, double* child4);
 
+/* ---- coupled CA-FE (remelting) path ---------------------------------- */
+typedef struct {
+  int32_t nx, ny, nz;
+  double h, x0, y0, z0;
+  /* structured FE mesh: node (i,j,k) has id (k*fny + j)*fnx + i */
+  int32_t fnx, fny, fnz;
+  const double* node_xyz;        /* [3*nodes], after offset and length scale */
+  int32_t n_frames;
+  const double* frame_time;      /* [n_frames], after time scale */
+  const double* frame_theta;     /* [n_frames*nodes] */
+  int32_t bin[3];
+  double tol;
+  int32_t max_iter;
+  double void_temp;
+  /* physics */
+  double Tm, Ts, Tmin_nuc, a1, a2, a3, b1, b2, b3;
+  double trunc, tsf, cutoff_factor, final_time;
+  int32_t max_steps;
+  int32_t initial_melted;        /* phase_state.has_been_melted */
+  /* nucleation sites (criterion_to_activate_grain.using_nucleation_seed) */
+  int32_t use_sites;
+  double ns, dTs_mean, dTs_sd, nv, dTv_mean, dTv_sd;
+  uint32_t seed;
+  /* loaded microstructure (NULL init_ori: none) */
+  const int32_t* init_ori;       /* [ng] */
+  const int32_t* init_gid;       /* [ng] */
+  const uint8_t* init_melted;    /* [ng] */
+  int32_t n_init_orient;
+  const double* init_euler;      /* [3*n_init_orient] */
+  int32_t n_init_grains;
+  const int32_t* init_grain_cell;
+  const int32_t* init_grain_orient;
+} oracle_remelt_params;
+
+typedef struct {
+  double time_before, dt;
+  int32_t n_nucleated, n_captured, n_fresh, n_flagged, n_liquid;
+} oracle_remelt_step;
+
+typedef struct {
+  int32_t *ori, *gid;            /* [ng] */
+  double* T;                     /* [ng] */
+  uint8_t *melted, *cooling;     /* [ng] */
+  int32_t* tag;                  /* [ng] element tags */
+  double* par;                   /* [3*ng] parametric coordinates */
+  double* theta;                 /* [nodes] interpolated nodal field of the last step */
+  double* grain_env;             /* [4*grain_cap] */
+  int32_t *grain_cell, *grain_orient;
+  int32_t grain_cap;
+  double* euler;
+  int32_t orient_cap;
+  oracle_remelt_step* steps;
+  int32_t steps_cap;
+  int32_t* site_cell;            /* [site_cap] nucleation sites in list order */
+  double* site_dt;               /* their critical undercoolings */
+  int32_t site_cap;
+  /* results */
+  int32_t n_steps, n_records, n_grains, n_orient, frames_exhausted, n_sites;
+  double time, time_fe;
+} oracle_remelt_out;
+
+int oracle_remelt_run(const oracle_remelt_params* prm, oracle_remelt_out* out);
+
 #ifdef __cplusplus
 }
 #endif

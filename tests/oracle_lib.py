@@ -192,3 +192,147 @@ def rotations(euler):
     for i in range(len(euler)):
         lib().oracle_rotation(_p(euler[i], C.c_double), _p(out[i], C.c_double))
     return out
+
+
+# ---- coupled CA-FE (remelting) path ---------------------------------------
+
+U8P = C.POINTER(C.c_uint8)
+
+
+class RemeltParams(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32),
+        ("h", C.c_double), ("x0", C.c_double), ("y0", C.c_double), ("z0", C.c_double),
+        ("fnx", C.c_int32), ("fny", C.c_int32), ("fnz", C.c_int32),
+        ("node_xyz", F64P), ("n_frames", C.c_int32), ("frame_time", F64P), ("frame_theta", F64P),
+        ("bin", C.c_int32 * 3), ("tol", C.c_double), ("max_iter", C.c_int32),
+        ("void_temp", C.c_double),
+        ("Tm", C.c_double), ("Ts", C.c_double), ("Tmin_nuc", C.c_double),
+        ("a1", C.c_double), ("a2", C.c_double), ("a3", C.c_double),
+        ("b1", C.c_double), ("b2", C.c_double), ("b3", C.c_double),
+        ("trunc", C.c_double), ("tsf", C.c_double), ("cutoff_factor", C.c_double),
+        ("final_time", C.c_double), ("max_steps", C.c_int32), ("initial_melted", C.c_int32),
+        ("use_sites", C.c_int32), ("ns", C.c_double), ("dTs_mean", C.c_double),
+        ("dTs_sd", C.c_double), ("nv", C.c_double), ("dTv_mean", C.c_double),
+        ("dTv_sd", C.c_double), ("seed", C.c_uint32),
+        ("init_ori", I32P), ("init_gid", I32P), ("init_melted", U8P),
+        ("n_init_orient", C.c_int32), ("init_euler", F64P),
+        ("n_init_grains", C.c_int32), ("init_grain_cell", I32P), ("init_grain_orient", I32P),
+    ]
+
+
+class RemeltStep(C.Structure):
+    _fields_ = [("time_before", C.c_double), ("dt", C.c_double), ("n_nucleated", C.c_int32),
+                ("n_captured", C.c_int32), ("n_fresh", C.c_int32), ("n_flagged", C.c_int32),
+                ("n_liquid", C.c_int32)]
+
+
+class RemeltOut(C.Structure):
+    _fields_ = [
+        ("ori", I32P), ("gid", I32P), ("T", F64P), ("melted", U8P), ("cooling", U8P),
+        ("tag", I32P), ("par", F64P), ("theta", F64P), ("grain_env", F64P),
+        ("grain_cell", I32P), ("grain_orient", I32P), ("grain_cap", C.c_int32),
+        ("euler", F64P), ("orient_cap", C.c_int32),
+        ("steps", C.POINTER(RemeltStep)), ("steps_cap", C.c_int32),
+        ("site_cell", I32P), ("site_dt", F64P), ("site_cap", C.c_int32),
+        ("n_steps", C.c_int32), ("n_records", C.c_int32), ("n_grains", C.c_int32),
+        ("n_orient", C.c_int32), ("frames_exhausted", C.c_int32), ("n_sites", C.c_int32),
+        ("time", C.c_double), ("time_fe", C.c_double),
+    ]
+
+
+def run_remelt(case, steps_cap=100000, grain_cap=None):
+    """Run the remelting oracle on a case dict (see remelt_case)."""
+    L = lib()
+    if not hasattr(L, "_remelt_bound"):
+        L.oracle_remelt_run.argtypes = [C.POINTER(RemeltParams), C.POINTER(RemeltOut)]
+        L.oracle_remelt_run.restype = C.c_int
+        L._remelt_bound = True
+    g = case["grid"]
+    nx, ny, nz = g["nx"], g["ny"], g["nz"]
+    ng = (nx + 2) * (ny + 2) * (nz + 2)
+    keep = []
+
+    def arr(a, t, dt):
+        a = np.ascontiguousarray(a, dt)
+        keep.append(a)
+        return a.ctypes.data_as(C.POINTER(t))
+
+    fe = case["fe"]
+    p = RemeltParams()
+    for k in ("nx", "ny", "nz", "h", "x0", "y0", "z0"):
+        setattr(p, k, g[k])
+    p.fnx, p.fny, p.fnz = fe["shape"]
+    p.node_xyz = arr(fe["xyz"], C.c_double, np.float64)
+    p.n_frames = len(fe["times"])
+    p.frame_time = arr(fe["times"], C.c_double, np.float64)
+    p.frame_theta = arr(fe["theta"], C.c_double, np.float64)
+    for i in range(3):
+        p.bin[i] = case["bin"][i]
+    p.tol, p.max_iter, p.void_temp = case["tol"], case["max_iter"], case["void_temp"]
+    ph = case["physics"]
+    for k in ("Tm", "Ts", "Tmin_nuc", "a1", "a2", "a3", "b1", "b2", "b3"):
+        setattr(p, k, ph[k])
+    for k in ("trunc", "tsf", "cutoff_factor", "final_time", "max_steps", "initial_melted"):
+        setattr(p, k, case[k])
+    nuc = case.get("nucleation")
+    p.use_sites = 1 if nuc else 0
+    if nuc:
+        p.ns, p.dTs_mean, p.dTs_sd = nuc["surface"]
+        p.nv, p.dTv_mean, p.dTv_sd = nuc["bulk"]
+    p.seed = case.get("seed", 0)
+    st = case.get("state")
+    if st is not None:
+        p.init_ori = arr(st["ori"], C.c_int32, np.int32)
+        p.init_gid = arr(st["gid"], C.c_int32, np.int32)
+        p.init_melted = arr(st["melted"], C.c_uint8, np.uint8)
+        p.n_init_orient = len(st["euler"])
+        p.init_euler = arr(st["euler"], C.c_double, np.float64)
+        p.n_init_grains = len(st["grain_cell"])
+        p.init_grain_cell = arr(st["grain_cell"], C.c_int32, np.int32)
+        p.init_grain_orient = arr(st["grain_orient"], C.c_int32, np.int32)
+    if grain_cap is None:
+        grain_cap = nx * ny * nz + 16
+    nnodes = int(np.prod(fe["shape"]))
+    out_arrays = dict(ori=np.empty(ng, np.int32), gid=np.empty(ng, np.int32),
+                      T=np.empty(ng, np.float64), melted=np.empty(ng, np.uint8),
+                      cooling=np.empty(ng, np.uint8), tag=np.empty(ng, np.int32),
+                      par=np.empty(3 * ng, np.float64), theta=np.empty(nnodes, np.float64),
+                      grain_env=np.empty(4 * grain_cap, np.float64),
+                      grain_cell=np.empty(grain_cap, np.int32),
+                      grain_orient=np.empty(grain_cap, np.int32),
+                      euler=np.empty(3 * grain_cap, np.float64),
+                      site_cell=np.empty(nx * ny * nz, np.int32),
+                      site_dt=np.empty(nx * ny * nz, np.float64))
+    recs = (RemeltStep * steps_cap)()
+    o = RemeltOut()
+    tmap = dict(ori=C.c_int32, gid=C.c_int32, T=C.c_double, melted=C.c_uint8, cooling=C.c_uint8,
+                tag=C.c_int32, par=C.c_double, theta=C.c_double, grain_env=C.c_double,
+                grain_cell=C.c_int32, grain_orient=C.c_int32, euler=C.c_double,
+                site_cell=C.c_int32, site_dt=C.c_double)
+    for k, a in out_arrays.items():
+        setattr(o, k, a.ctypes.data_as(C.POINTER(tmap[k])))
+    o.grain_cap = grain_cap
+    o.orient_cap = grain_cap
+    o.steps = recs
+    o.steps_cap = steps_cap
+    o.site_cap = nx * ny * nz
+    rc = L.oracle_remelt_run(C.byref(p), C.byref(o))
+    assert rc == 0, f"remelt oracle failed ({rc})"
+    n = o.n_grains
+    res = {k: v for k, v in out_arrays.items()}
+    res["par"] = res["par"].reshape(-1, 3)
+    res["grain_env"] = res["grain_env"][:4 * n].reshape(-1, 4)
+    res["grain_cell"] = res["grain_cell"][:n]
+    res["grain_orient"] = res["grain_orient"][:n]
+    res["euler"] = res["euler"][:3 * o.n_orient].reshape(-1, 3)
+    res["site_cell"] = res["site_cell"][:o.n_sites]
+    res["site_dt"] = res["site_dt"][:o.n_sites]
+    res["steps"] = np.array([(r.time_before, r.dt, r.n_nucleated, r.n_captured, r.n_fresh,
+                              r.n_flagged, r.n_liquid) for r in recs[:o.n_records]],
+                            dtype=[("time_before", "f8"), ("dt", "f8"), ("n_nucleated", "i4"),
+                                   ("n_captured", "i4"), ("n_fresh", "i4"), ("n_flagged", "i4"),
+                                   ("n_liquid", "i4")])
+    res.update(n_steps=o.n_steps, n_grains=n, n_orient=o.n_orient, time=o.time, time_fe=o.time_fe,
+               frames_exhausted=o.frames_exhausted)
+    return res

@@ -1,0 +1,207 @@
+"""Coupled CA-FE (remelting) path on the GPU, bit-exact against the oracle.
+
+Every case runs the same inputs through the device domain (amca_remelt_* C
+ABI) and the oracle, then compares the per-cell fields (orientation, grain id,
+temperature, melted, cooling), the grain records, the nodal field, the step log
+and the counters word for word."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import remelt_cases as rc
+
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("oracle")]
+
+IDENT = np.array([1.0, 0, 0, 0, 1.0, 0, 0, 0, 1.0])
+
+
+def _desc(case):
+    g, ph = case["grid"], case["physics"]
+    d = {k: g[k] for k in ("nx", "ny", "nz", "h", "x0", "y0", "z0")}
+    d.update({k: ph[k] for k in ("Tm", "Ts", "Tmin_nuc", "a1", "a2", "a3", "b1", "b2", "b3")})
+    d.update(trunc=case["trunc"], time_step_factor=case["tsf"], cutoff_factor=case["cutoff_factor"],
+             final_time=case["final_time"], void_temperature=case["void_temp"], bin=case["bin"],
+             tolerance=case["tol"], max_iterations=case["max_iter"],
+             initial_melted=case["initial_melted"])
+    return d
+
+
+def run_device(case, orc, chunks=None):
+    """Run `case` on the device.  Nucleation sites and the orientation rows drawn
+    after the loaded ones are the host-side inputs of the path; they come from
+    the oracle run so both sides consume the same rand() stream."""
+    from amca3d_amd.remelt import RemeltDomain
+    st = case["state"]
+    n0 = len(st["euler"])
+    extra = orc["euler"][n0:]
+    n_sites = len(orc["site_cell"]) if case.get("nucleation") else 0
+    cap = n0 + len(extra) + 64 * max(n_sites, 1) + 16
+    d = RemeltDomain(_desc(case), cap)
+    d.push_orientations(st["euler"], ol.rotations(st["euler"]))
+    rows = ol.rotations(extra) if len(extra) else np.zeros((0, 9))
+    pos = [0]
+
+    def source(n):
+        take = rows[pos[0]:pos[0] + n]
+        pos[0] += len(take)
+        pad = n - len(take)
+        eu = np.zeros((n, 3))
+        r = np.concatenate([take, np.tile(IDENT, (pad, 1))]) if pad else take
+        return eu, r
+
+    d.set_orientation_source(source)
+    d.set_fe(case["fe"])
+    d.set_state(st)
+    if n_sites:
+        d.set_sites(orc["site_cell"], orc["site_dt"])
+    d.enable_log(100000)
+    if chunks:
+        for k in chunks:
+            d.run(k)
+        c = d.counters()
+    else:
+        c = d.run(case["max_steps"])
+    out = d.download()
+    out.update(theta=d.theta(), log=d.read_log(), counters=c, tag_par=d.voxel_map())
+    d.close()
+    return out
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64) if a.dtype == np.float64 else a
+
+
+def compare(case, orc, dev):
+    c = dev["counters"]
+    assert c["step"] == orc["n_steps"]
+    assert c["n_grains"] == orc["n_grains"]
+    assert c["n_orient"] == orc["n_orient"]
+    assert c["frames_exhausted"] == orc["frames_exhausted"]
+    assert bits(np.float64(c["time"])) == bits(np.float64(orc["time"]))
+    assert bits(np.float64(c["time_fe"])) == bits(np.float64(orc["time_fe"]))
+    tag, par = dev["tag_par"]
+    np.testing.assert_array_equal(tag, orc["tag"])
+    np.testing.assert_array_equal(bits(par), bits(orc["par"]))
+    np.testing.assert_array_equal(dev["ori"], orc["ori"])
+    np.testing.assert_array_equal(dev["gid"], orc["gid"])
+    np.testing.assert_array_equal(bits(dev["T"]), bits(orc["T"]))
+    np.testing.assert_array_equal(dev["melted"], orc["melted"])
+    np.testing.assert_array_equal(dev["cooling"], orc["cooling"])
+    np.testing.assert_array_equal(bits(dev["theta"]), bits(orc["theta"]))
+    n = orc["n_grains"]
+    np.testing.assert_array_equal(dev["grain_cell"], orc["grain_cell"][:n])
+    np.testing.assert_array_equal(dev["grain_orient"], orc["grain_orient"][:n])
+    np.testing.assert_array_equal(bits(dev["grain_env"]), bits(orc["grain_env"][:n]))
+    lg, so = dev["log"], orc["steps"]
+    assert len(lg) == len(so)
+    for k in ("n_nucleated", "n_captured", "n_fresh", "n_flagged"):
+        np.testing.assert_array_equal(lg[k], so[k], err_msg=k)
+    for k in ("time_before", "dt"):
+        np.testing.assert_array_equal(bits(lg[k]), bits(so[k]), err_msg=k)
+
+
+def _holes(case, frac, seed):
+    """Cells that stay solid but lose their grain record id (gid = -1): when
+    they melt and are recaptured, the capture appends a fresh record."""
+    st = case["state"]
+    rng = np.random.default_rng(seed)
+    cells = st["grain_cell"]
+    pick = cells[rng.uniform(size=len(cells)) < frac]
+    st["gid"] = st["gid"].copy()
+    st["gid"][pick] = -1
+    return case
+
+
+@pytest.mark.parametrize("fresh", [False, True])
+def test_single_molten_cell(amca, fresh):
+    case = rc.single_cell_case(fresh=fresh)
+    orc = ol.run_remelt(case)
+    dev = run_device(case, orc)
+    compare(case, orc, dev)
+    assert dev["ori"][case["hot_cell"]] == case["state"]["ori"][case["hot_cell"] - 1]
+
+
+@pytest.fixture(scope="module")
+def track_nuc():
+    nuc = dict(surface=(0.0, 1.0, 0.5), bulk=(2e15, 5.0, 1.0))
+    case = rc.track_case(nx=80, ny=24, nz=20, n_frames=26, P=30.0, nucleation=nuc)
+    return case, ol.run_remelt(case)
+
+
+def test_track_with_nucleation(amca, track_nuc):
+    case, orc = track_nuc
+    assert orc["steps"]["n_captured"].sum() > 1000
+    assert orc["steps"]["n_nucleated"].sum() > 50
+    compare(case, orc, run_device(case, orc))
+
+
+def test_track_in_chunks_equals_one_run(amca, track_nuc):
+    case, orc = track_nuc
+    dev = run_device(case, orc, chunks=[1, 2, 5, 13, 40, 1000])
+    compare(case, orc, dev)
+
+
+def test_track_with_fresh_captures(amca):
+    nuc = dict(surface=(2e10, 2.0, 0.5), bulk=(2e15, 5.0, 1.0))
+    case = _holes(rc.track_case(nx=64, ny=20, nz=16, n_frames=22, P=30.0, nucleation=nuc,
+                                seed=11), 0.3, 4)
+    orc = ol.run_remelt(case)
+    assert orc["steps"]["n_fresh"].sum() > 100
+    compare(case, orc, run_device(case, orc))
+
+
+def test_track_without_nucleation_stops_at_max_steps(amca):
+    case = rc.track_case(nx=48, ny=16, nz=14, n_frames=16, P=30.0, max_steps=37)
+    orc = ol.run_remelt(case)
+    assert orc["n_steps"] == 37
+    compare(case, orc, run_device(case, orc))
+
+
+def test_track_runs_off_the_end_of_the_frames(amca):
+    case = rc.track_case(nx=40, ny=14, nz=12, n_frames=6, P=30.0)
+    orc = ol.run_remelt(case)
+    assert orc["frames_exhausted"] == 1
+    compare(case, orc, run_device(case, orc))
+
+
+def test_termination_time(amca):
+    case = rc.track_case(nx=40, ny=14, nz=12, n_frames=20, P=30.0, final_time=7.3e-5)
+    orc = ol.run_remelt(case)
+    assert orc["frames_exhausted"] == 0 and orc["n_steps"] > 5
+    compare(case, orc, run_device(case, orc))
+
+
+def test_distorted_mesh_and_hot_track(amca):
+    """A non-affine FE mesh (Newton mapping) and a pool that melts through the
+    top surface, with melted-from-the-start cells."""
+    case = rc.track_case(nx=56, ny=20, nz=16, n_frames=14, P=80.0, seed=5)
+    rng = np.random.default_rng(1)
+    case["fe"]["xyz"] = case["fe"]["xyz"] + rng.uniform(-0.2, 0.2, case["fe"]["xyz"].shape) * 2.5e-6
+    case["initial_melted"] = 1
+    case["state"]["melted"] = np.ones_like(case["state"]["melted"])
+    orc = ol.run_remelt(case)
+    assert orc["steps"]["n_captured"].sum() > 500
+    compare(case, orc, run_device(case, orc))
+
+
+def test_state_validation(amca):
+    from amca3d_amd._lib import AmcaError
+    from amca3d_amd.remelt import RemeltDomain
+    case = rc.single_cell_case()
+    st = dict(case["state"])
+    d = RemeltDomain(_desc(case), len(st["euler"]) + 8)
+    d.push_orientations(st["euler"], ol.rotations(st["euler"]))
+    d.set_fe(case["fe"])
+    shared = st["gid"].copy()
+    shared[st["grain_cell"][1]] = shared[st["grain_cell"][0]]
+    with pytest.raises(AmcaError, match="shared"):
+        d.set_state(dict(st, gid=shared))
+    bad = st["grain_orient"].copy()
+    bad[0] = len(st["euler"])
+    with pytest.raises(AmcaError, match="orientation"):
+        d.set_state(dict(st, grain_orient=bad))
+    d.set_state(st)
+    with pytest.raises(AmcaError, match="ascend"):
+        d.set_sites([30, 20], [1.0, 1.0])
+    d.close()
