@@ -292,7 +292,8 @@ def run_remelt(case, steps_cap=100000, grain_cap=None):
         p.init_grain_cell = arr(st["grain_cell"], C.c_int32, np.int32)
         p.init_grain_orient = arr(st["grain_orient"], C.c_int32, np.int32)
     if grain_cap is None:
-        grain_cap = nx * ny * nz + 16
+        n_loaded = len(st["grain_cell"]) if st is not None else 0
+        grain_cap = n_loaded + nx * ny * nz + 16
     nnodes = int(np.prod(fe["shape"]))
     out_arrays = dict(ori=np.empty(ng, np.int32), gid=np.empty(ng, np.int32),
                       T=np.empty(ng, np.float64), melted=np.empty(ng, np.uint8),
@@ -320,6 +321,7 @@ def run_remelt(case, steps_cap=100000, grain_cap=None):
     rc = L.oracle_remelt_run(C.byref(p), C.byref(o))
     assert rc == 0, f"remelt oracle failed ({rc})"
     n = o.n_grains
+    assert n <= grain_cap and o.n_orient <= grain_cap, "oracle output buffers too small"
     res = {k: v for k, v in out_arrays.items()}
     res["par"] = res["par"].reshape(-1, 3)
     res["grain_env"] = res["grain_env"][:4 * n].reshape(-1, 4)
