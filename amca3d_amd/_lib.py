@@ -40,7 +40,7 @@ class StepRecord(C.Structure):
         ("time_before", C.c_double), ("dt", C.c_double),
         ("n_nucleated", C.c_int32), ("n_active_before", C.c_int32),
         ("n_captured", C.c_int32), ("n_active_after", C.c_int32),
-        ("n_liquid", C.c_int64),
+        ("n_liquid", C.c_int64), ("n_wave1", C.c_int32), ("max_wave", C.c_int32),
     ]
 
 
@@ -81,6 +81,7 @@ SIGNATURES = {
                                          I32P, F64P]),
     "amca_last_error": (C.c_char_p, []),
     "amca_version": (C.c_char_p, []),
+    "amca_copy_probe": (C.c_int, [C.c_int64, C.c_int32, F64P, F64P]),
     "amca_sim_create": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "amca_sim_create_from_file": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "amca_sim_prepare": (C.c_int, [C.c_void_p]),

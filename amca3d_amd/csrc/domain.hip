@@ -35,9 +35,10 @@ amca_status fail(amca_status st, const std::string& msg) {
       return fail(AMCA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
-enum KernelId { K_NUCLEATE = 0, K_CLAIM0, K_EMIT0, K_CLAIM1, K_EMIT1, K_TAIL, K_COUNT };
-const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim_wave0", "k_resolve_emit_wave0",
-                                     "k_claim_wave1", "k_resolve_emit_wave1", "k_tail"};
+enum KernelId { K_NUCLEATE = 0, K_CLAIM, K_RESOLVE, K_SCAN, K_EMIT, K_TAIL, K_COUNT };
+// named as the kernels are, so event times line up with a kernel trace
+const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim", "k_resolve",
+                                     "k_scan",     "k_emit",  "k_tail"};
 
 }  // namespace
 
@@ -153,10 +154,13 @@ amca_status enqueue_step(amca_domain* d) {
   const amca::DevBuffers& B = d->B;
   hipStream_t s = d->stream;
   AMCA_TRY(timed(d, K_NUCLEATE, [&] { return amca::launch_nucleate(P, B, d->nuc_blocks, s); }));
-  AMCA_TRY(timed(d, K_CLAIM0, [&] { return amca::launch_claim(P, B, 0, d->list_blocks, s); }));
-  AMCA_TRY(timed(d, K_EMIT0, [&] { return amca::launch_resolve_emit(P, B, 0, d->emit_blocks, s); }));
-  AMCA_TRY(timed(d, K_CLAIM1, [&] { return amca::launch_claim(P, B, 1, d->list_blocks, s); }));
-  AMCA_TRY(timed(d, K_EMIT1, [&] { return amca::launch_resolve_emit(P, B, 1, d->emit_blocks, s); }));
+  for (int wave = 0; wave < 2; wave++) {
+    AMCA_TRY(timed(d, K_CLAIM, [&] { return amca::launch_claim(P, B, wave, d->list_blocks, s); }));
+    AMCA_TRY(timed(d, K_RESOLVE,
+                   [&] { return amca::launch_resolve(P, B, wave, d->emit_blocks, s); }));
+    AMCA_TRY(timed(d, K_SCAN, [&] { return amca::launch_scan(P, B, wave, d->emit_blocks, s); }));
+    AMCA_TRY(timed(d, K_EMIT, [&] { return amca::launch_emit(P, B, wave, d->emit_blocks, s); }));
+  }
   AMCA_TRY(timed(d, K_TAIL, [&] { return amca::launch_tail(P, B, s); }));
   return AMCA_OK;
 }
@@ -420,6 +424,8 @@ amca_status amca_domain_read_log(amca_domain* d, int32_t cap, amca_step_record* 
       rows[i].n_captured = tmp[i].n_captured;
       rows[i].n_active_after = tmp[i].n_active_after;
       rows[i].n_liquid = tmp[i].n_liquid;
+      rows[i].n_wave1 = tmp[i].n_wave1;
+      rows[i].max_wave = tmp[i].max_wave;
     }
   }
   return AMCA_OK;

@@ -38,6 +38,7 @@ struct DevCounters {
   int n_sites;
   int step_nuc, step_cap, step_nA0;
   int max_wave;
+  int step_wave;          // deepest capturing wave of the running step
   int wave_gbase;         // first new grain id of the running wave
   int err;                // 1: orientation table exhausted
   int log_cap;
@@ -48,6 +49,7 @@ struct StepLogRow {
   double time_before, dt;
   int n_nucleated, n_active_before, n_captured, n_active_after;
   long long n_liquid;
+  int n_wave1, max_wave;
 };
 
 // Device buffers of one domain (all owned by the handle).
@@ -89,9 +91,14 @@ hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t
 hipError_t launch_nucleate(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
 hipError_t launch_claim(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                         hipStream_t s);
-// k_resolve, k_scan and k_emit of one wave (`blocks` <= kMaxBlocks chunks)
-hipError_t launch_resolve_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
-                               hipStream_t s);
+// k_resolve, k_scan and k_emit of one wave, launched in that order with the
+// same `blocks` (<= kMaxBlocks chunks)
+hipError_t launch_resolve(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                          hipStream_t s);
+hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                       hipStream_t s);
+hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                       hipStream_t s);
 hipError_t launch_tail(const DevParams& P, const DevBuffers& B, hipStream_t s);
 hipError_t launch_envelope_selftest(int n, const double* env4, const double* l2g9,
                                     const double* cell3, double h, double trunc, double sqrt2f,

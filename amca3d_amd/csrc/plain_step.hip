@@ -179,6 +179,7 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
     ctr->step_nuc = nn;
     ctr->step_nA0 = nA;
     ctr->step_cap = 0;
+    ctr->step_wave = 0;
     ctr->nA_out = 0;
     ctr->nW[0] = 0;
     ctr->nW[1] = 0;
@@ -436,6 +437,7 @@ __global__ void __launch_bounds__(kScanBlock) k_scan(DevParams P, DevBuffers B, 
     ctr->nW[wave & 1] = nc;
     ctr->step_cap += nc;
     if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
+    if (nc > 0 && ctr->step_wave < wave + 1) ctr->step_wave = wave + 1;
   }
 }
 
@@ -496,6 +498,7 @@ __global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) 
   if (ctr->skip) return;
   __shared__ unsigned long long s_sm[kTailBlock / 64];
   __shared__ int s_n, s_gbase, s_sbase;
+  const int n_wave1 = ctr->nW[0];  // wave-0 captures, before wave 2 reuses the slot
   int wave = 2;
   while (true) {
     const int in = (wave - 1) & 1;
@@ -544,6 +547,7 @@ __global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) 
       ctr->nW[wave & 1] = nc;
       ctr->step_cap += nc;
       if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
+      if (nc > 0 && ctr->step_wave < wave + 1) ctr->step_wave = wave + 1;
     }
     __syncthreads();
     wave++;
@@ -567,6 +571,8 @@ __global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) 
       row.n_captured = ctr->step_cap;
       row.n_active_after = ctr->nA_out;
       row.n_liquid = liquid;
+      row.n_wave1 = n_wave1;
+      row.max_wave = ctr->step_wave;
       B.log[st] = row;
     }
     ctr->parity ^= 1;
@@ -617,10 +623,20 @@ hipError_t launch_claim(const DevParams& P, const DevBuffers& B, int wave, int b
   return hipGetLastError();
 }
 
-hipError_t launch_resolve_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
-                               hipStream_t s) {
+hipError_t launch_resolve(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                          hipStream_t s) {
   hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
+  return hipGetLastError();
+}
+
+hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                       hipStream_t s) {
   hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, s, P, B, wave, blocks);
+  return hipGetLastError();
+}
+
+hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
+                       hipStream_t s) {
   hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
   return hipGetLastError();
 }

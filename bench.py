@@ -24,32 +24,35 @@ METRIC = "CA cell-updates/sec + % HBM peak; batched+surrogate vs plain GPU at fi
 
 
 def algorithmic_bytes(log):
-    """Bytes per kernel over a step log ('Algorithmic bytes').
+    """Algorithmic HBM bytes per kernel over a step log (unit figures of the
+    plain step: 48 B per active grain, 44 B per capture, 12 B per live site).
 
-    k_claim_wave0        48 B per active grain: centre+L read (32), orientation
-                         id (4), grain id (4), grown L written (8)
-    k_resolve_emit_wave0 (k_resolve + k_scan + k_emit) 44 B per capture: child
-                         envelope (32), orientation (4), grain id (4), wave-list
-                         entry (4); 4 B per survivor-list entry
-    k_claim_wave1        40 B per wave-1 grain (no growth write)
-    k_resolve_emit_wave1 4 B per survivor of wave 1
-    k_nucleate           12 B per live site (cell index + critical undercooling)
+    k_claim    wave 0: 48 B per active grain (centre + L read 32, orientation id
+               4, grain id 4, grown L written 8); wave 1: 40 B per wave-1 grain
+               (no growth write)
+    k_resolve  4 B per survivor-list entry written
+    k_emit     44 B per capture: child envelope 32, orientation 4, grain id 4,
+               wave-list entry 4 (captures of waves >= 2, made in k_tail, are a
+               handful per step and are counted here too)
+    k_nucleate 12 B per live site (cell index 4 + critical undercooling 8)
     """
     A = log["n_active_before"].astype(np.float64)
+    W1 = log["n_wave1"].astype(np.float64)
     C = log["n_captured"].astype(np.float64)
     Aout = log["n_active_after"].astype(np.float64)
     return {
-        "k_claim_wave0": 48.0 * A,
-        "k_resolve_emit_wave0": 44.0 * C + 4.0 * Aout,
-        "k_claim_wave1": 40.0 * C,
+        "k_claim": 48.0 * A + 40.0 * W1,
+        "k_resolve": 4.0 * Aout,
+        "k_emit": 44.0 * C,
     }
 
 
 def cpu_baseline(seconds_target=20.0):
-    """The oracle (one core) on the same physics at 128^3, to full solidification."""
+    """The oracle (one core) on the same physics at 160^3, to full solidification
+    (about 10 s of CPU work)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    n = 128
+    n = 160
     p = oracle_lib.plain_params(nx=n, ny=n, nz=n, h=2e-6, ns=1e9, dTs_mean=1.0, dTs_sd=0.5,
                                 surface_mode=1, nv=5e13, dTv_mean=3.0, dTv_sd=1.0, Tdot=-200.0,
                                 max_dt=1e-4, final_time=1.0)
@@ -62,6 +65,30 @@ def cpu_baseline(seconds_target=20.0):
                       f"{n}^3, {r['n_steps']} steps to full solidification, {wall:.1f} s"}
 
 
+def copy_probe(mib=2048, reps=20):
+    """Measured stream-copy rate of the device (k_copy_probe, read + write)."""
+    import ctypes as C
+    from amca3d_amd import load
+    g, ms = C.c_double(0.0), C.c_double(0.0)
+    st = load().amca_copy_probe(mib << 20, reps, C.byref(g), C.byref(ms))
+    if st != 0:
+        return None
+    return {"gbps": g.value, "ms_per_copy": ms.value, "bytes": mib << 20}
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed PMC summary
+    (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel factor)."""
+    path = os.path.join(ROOT, "profiles", "pmc_traffic_plain_uniform512.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    k = t.get("kernels", {}).get(kernel)
+    return None if k is None else k.get("bytes_per_launch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,6 +97,8 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run the stream-copy probe (for counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs the CA step on "
@@ -78,6 +107,10 @@ def main():
 
     import amca3d_amd
     from amca3d_amd import decks
+
+    if args.copy_probe:
+        print(json.dumps({"copy_probe": copy_probe()}))
+        return
 
     n = args.size
     deck = decks.uniform_cooling_512_deck(n=n)
@@ -125,6 +158,7 @@ def main():
         ab = algorithmic_bytes(log)
         roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": dom_k,
                 "kernel_ms_total": total[dom_k], "launches": kt[dom_k][1],
+                "avg_launch_us": 1e3 * total[dom_k] / max(kt[dom_k][1], 1),
                 "kernel_share_of_wall": total[dom_k] / (1e3 * wall)}
         if dom_k in ab:
             bytes_total = float(ab[dom_k].sum())
@@ -133,7 +167,10 @@ def main():
                         bytes_per_launch=bytes_total / max(kt[dom_k][1], 1))
         else:
             roof.update(achieved=None, frac=None)
-        roof["traffic"] = None
+        roof["traffic"] = measured_traffic(dom_k)
+        cp = copy_probe()
+        if cp:
+            roof["measured_copy_gbs"] = cp["gbps"]
         result["roofline"] = roof
         result["kernel_ms"] = {k: round(v, 3) for k, v in total.items()}
     if not args.no_cpu_baseline:

@@ -73,6 +73,8 @@ typedef struct {
   double time_before, dt;
   int32_t n_nucleated, n_active_before, n_captured, n_active_after;
   int64_t n_liquid;
+  int32_t n_wave1;    /* grains captured by the grains active at the step's start */
+  int32_t max_wave;   /* deepest capture wave that captured a cell (0: none) */
 } amca_step_record;
 
 /* ---- domain lifetime ---------------------------------------------------- */
@@ -168,6 +170,12 @@ amca_status amca_domain_kernel_times(amca_domain* d, double* ms, int64_t* launch
 amca_status amca_envelope_selftest(int32_t n, const double* env4, const double* l2g9,
                                    const double* cell3, double h, double trunc,
                                    int32_t* inside, double* child4);
+
+/* Device stream-copy rate: `reps` copies of a `bytes` buffer by the kernel
+ * k_copy_probe (read + write counted), average GB/s and ms per copy.  No
+ * counterpart in the reference: it is the measured bandwidth roof and the
+ * known-traffic kernel that hardware byte counters are scaled against. */
+amca_status amca_copy_probe(int64_t bytes, int32_t reps, double* gbps, double* ms);
 
 const char* amca_last_error(void);
 const char* amca_version(void);

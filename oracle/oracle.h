@@ -24,6 +24,7 @@ typedef struct {
 typedef struct {
   double time_before, dt;
   int32_t n_nucleated, n_active_before, n_captured, n_active_after, n_liquid, max_wave;
+  int32_t n_wave1; /* captures made by grains active at the step's start */
 } oracle_step_record;
 
 typedef struct {

@@ -273,7 +273,7 @@ struct PlainCA {
     }
     int nActive0 = (int)active.size();
     for (int g : active) grains[g].wave = 0;
-    int captures = 0, maxWave = 0;
+    int captures = 0, maxWave = 0, wave1 = 0;
     int nb[26];
     auto it = active.begin();
     while (it != active.end()) {
@@ -291,6 +291,7 @@ struct PlainCA {
             Grain par = grains[g];  // grains may reallocate on push_back
             capture(par, n);
             captures++;
+            if (par.wave == 0) wave1++;
             maxWave = std::max(maxWave, par.wave + 1);
           } else {
             open = true;
@@ -321,6 +322,7 @@ struct PlainCA {
       rec->n_active_after = (int)active.size();
       rec->n_liquid = liquid;
       rec->max_wave = maxWave;
+      rec->n_wave1 = wave1;
     }
     return 0;
   }

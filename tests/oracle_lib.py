@@ -35,7 +35,7 @@ class StepRecord(C.Structure):
         ("time_before", C.c_double), ("dt", C.c_double),
         ("n_nucleated", C.c_int32), ("n_active_before", C.c_int32),
         ("n_captured", C.c_int32), ("n_active_after", C.c_int32),
-        ("n_liquid", C.c_int32), ("max_wave", C.c_int32),
+        ("n_liquid", C.c_int32), ("max_wave", C.c_int32), ("n_wave1", C.c_int32),
     ]
 
 
@@ -147,10 +147,11 @@ def run_plain(prm, steps_cap=200000, want_grains=True, given=None):
     ng_ = out.n_grains
     n_rec = out.n_records
     steps = np.array([(r.time_before, r.dt, r.n_nucleated, r.n_active_before, r.n_captured,
-                       r.n_active_after, r.n_liquid, r.max_wave) for r in recs[:n_rec]],
+                       r.n_active_after, r.n_liquid, r.max_wave, r.n_wave1) for r in recs[:n_rec]],
                      dtype=[("time_before", "f8"), ("dt", "f8"), ("n_nucleated", "i4"),
                             ("n_active_before", "i4"), ("n_captured", "i4"),
-                            ("n_active_after", "i4"), ("n_liquid", "i4"), ("max_wave", "i4")])
+                            ("n_active_after", "i4"), ("n_liquid", "i4"), ("max_wave", "i4"),
+                            ("n_wave1", "i4")])
     res = dict(ori=ori, gid=gid, nuc_dt=nuc, steps=steps, n_steps=out.n_steps, time=out.time,
                done=out.done, n_grains=ng_, n_orient=out.n_orient, n_active=out.n_active,
                sites_cell=scell[:out.n_sites].copy(), sites_dt=sdt[:out.n_sites].copy(),

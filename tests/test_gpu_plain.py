@@ -74,7 +74,7 @@ def compare(oracle, prm_kw, g, r=None):
     n = min(len(lg), len(st))
     assert n == r["n_steps"]
     for f in ("time_before", "dt", "n_nucleated", "n_active_before", "n_captured",
-              "n_active_after", "n_liquid"):
+              "n_active_after", "n_liquid", "n_wave1", "max_wave"):
         assert np.array_equal(lg[f][:n], st[f][:n].astype(lg[f].dtype)), f
     return r
 
