@@ -44,25 +44,73 @@ __device__ __forceinline__ void cell_ijk(const DevParams& P, int c, int& i, int&
   i = rem - j * P.nxG;
 }
 
-__device__ __forceinline__ void clear_bit(unsigned* map, int c) {
-  atomicAnd(&map[c >> 5], ~(1u << (c & 31)));
+__device__ __forceinline__ unsigned long long ld_coherent(const unsigned long long* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Three bits of cells p-1, p, p+1 from a 1-bit-per-cell map.
-template <bool kCoherent>
-__device__ __forceinline__ unsigned bits3(const unsigned* __restrict__ liq, int p) {
-  const int q = p - 1;
-  const int w = q >> 5;
-  unsigned lo, hi;
-  if (kCoherent) {
-    lo = ld_coherent(&liq[w]);
-    hi = ld_coherent(&liq[w + 1]);
-  } else {
-    lo = liq[w];
-    hi = liq[w + 1];
+// ---- 4x4x4 brick map: word 0 liquid, word 1 claimed ----------------------
+__device__ __forceinline__ int brick_word(const DevParams& P, int i, int j, int k) {
+  return 2 * (((k >> 2) * P.nbY + (j >> 2)) * P.nbX + (i >> 2));
+}
+__device__ __forceinline__ unsigned long long brick_bit(int i, int j, int k) {
+  return 1ull << (((k & 3) << 4) | ((j & 3) << 2) | (i & 3));
+}
+__device__ __forceinline__ void brick_clear(const DevParams& P, unsigned long long* bm, int word,
+                                            int i, int j, int k) {
+  atomicAnd(&bm[brick_word(P, i, j, k) + word], ~brick_bit(i, j, k));
+}
+__device__ __forceinline__ void brick_set(const DevParams& P, unsigned long long* bm, int word,
+                                          int i, int j, int k) {
+  atomicOr(&bm[brick_word(P, i, j, k) + word], brick_bit(i, j, k));
+}
+
+// The 26 neighbour bits of interior cell (i, j, k) from word `kWord` of the
+// brick map, in slot order (k, j, i, centre skipped).  The 3x3x3 block spans
+// bricks bx0..bx1 (bx1 = bx0 or bx0 + 1) per axis: the 8 corner words are
+// loaded once and each of the 9 rows takes its 3 bits from the two x-adjacent
+// nibbles.
+template <int kWord, bool kCoherent>
+__device__ __forceinline__ unsigned brick_bits26(const DevParams& P, const unsigned long long* bm,
+                                                 int i, int j, int k) {
+  const int bx0 = (i - 1) >> 2, bx1 = (i + 1) >> 2;
+  const int by0 = (j - 1) >> 2, by1 = (j + 1) >> 2;
+  const int bz0 = (k - 1) >> 2, bz1 = (k + 1) >> 2;
+  unsigned long long w[8];
+#pragma unroll
+  for (int q = 0; q < 8; q++) {
+    const int bx = (q & 1) ? bx1 : bx0;
+    const int by = (q & 2) ? by1 : by0;
+    const int bz = (q & 4) ? bz1 : bz0;
+    const unsigned long long* p = bm + 2 * ((bz * P.nbY + by) * P.nbX + bx) + kWord;
+    w[q] = kCoherent ? ld_coherent(p) : *p;
   }
-  const unsigned long long both = ((unsigned long long)hi << 32) | lo;
-  return (unsigned)(both >> (q & 31)) & 7u;
+  const int xs = (i - 1) & 3;  // first x of the block inside brick bx0
+  unsigned m = 0;
+  int s = 0;
+#pragma unroll
+  for (int dk = -1; dk <= 1; dk++) {
+    const int z = k + dk;
+    const bool hz = (z >> 2) != bz0;
+#pragma unroll
+    for (int dj = -1; dj <= 1; dj++) {
+      const int y = j + dj;
+      const bool hy = (y >> 2) != by0;
+      const unsigned long long lo = hz ? (hy ? w[6] : w[4]) : (hy ? w[2] : w[0]);
+      const unsigned long long hi = hz ? (hy ? w[7] : w[5]) : (hy ? w[3] : w[1]);
+      const int sh = ((z & 3) << 4) | ((y & 3) << 2);
+      const unsigned row8 = (unsigned)((lo >> sh) & 0xfull) | ((unsigned)((hi >> sh) & 0xfull) << 4);
+      const unsigned b = (row8 >> xs) & 7u;
+      if (dk == 0 && dj == 0) {
+        m |= (b & 1u) << s;
+        m |= ((b >> 2) & 1u) << (s + 1);
+        s += 2;
+      } else {
+        m |= b << s;
+        s += 3;
+      }
+    }
+  }
+  return m;
 }
 
 // wave-wide sum

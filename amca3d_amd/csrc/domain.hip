@@ -204,6 +204,9 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   P.nxG = (int)nxG; P.nyG = (int)nyG; P.nzG = (int)nzG;
   P.nxyG = (int)(nxG * nyG);
   P.NG = (int)NG;
+  P.nbX = (int)((nxG + 3) / 4);
+  P.nbY = (int)((nyG + 3) / 4);
+  P.nbZ = (int)((nzG + 3) / 4);
   P.N = (long long)desc->nx * desc->ny * desc->nz;
   P.h = desc->h; P.x0 = desc->x0; P.y0 = desc->y0; P.z0 = desc->z0;
   P.T0 = desc->T0; P.Tm = desc->Tm; P.Tdot = desc->Tdot;
@@ -235,8 +238,7 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   ALLOC(B.gid, (size_t)NG);
   ALLOC(B.oid, (size_t)NG);
   ALLOC(B.claim, (size_t)NG);
-  ALLOC(B.liq, (size_t)(NG >> 5) + 2);
-  ALLOC(B.clm, (size_t)(NG >> 5) + 2);
+  ALLOC(B.bmap, 2 * (size_t)P.nbX * P.nbY * P.nbZ);
   ALLOC(B.genv, N);
   ALLOC(B.gco, N);
   ALLOC(B.listA[0], N);

@@ -18,6 +18,7 @@ struct DevParams {
   double sqrt2f;          // (double)sqrtf(2), Octahedron.cpp:260
   int off[26];            // 26-neighbour offsets in the reference's k,j,i order
   int orient_cap;
+  int nbX, nbY, nbZ;      // 4x4x4-cell bricks per axis of the liquid / claimed map
 };
 
 // Mutable scalars that live on the device, so that a whole step (or many
@@ -54,8 +55,12 @@ struct StepLogRow {
 
 // Device buffers of one domain (all owned by the handle).
 //
-// Cell-indexed (ghosted grid, NG cells): the two grain-ID fields, the claim
-// word of the capture waves and a 1-bit liquid map.  Grain-indexed (grain id =
+// Cell-indexed (ghosted grid, NG cells): the two grain-ID fields and the claim
+// word of the capture waves.  The liquid and claimed states are 1 bit per cell
+// in 4x4x4-cell bricks: brick b holds two 64-bit words, [2b] liquid and [2b+1]
+// claimed, bit ((k&3)*16 + (j&3)*4 + (i&3)); bricks run x fastest.  A cell's 26
+// neighbours then lie in at most 8 bricks and usually in two or three cache
+// lines, against nine rows of a cell-linear map.  Grain-indexed (grain id =
 // creation order, capacity N): envelope, cell and orientation.  Lists hold
 // grain ids, so list order is id order and grain records are read in
 // ascending address order.
@@ -63,8 +68,7 @@ struct DevBuffers {
   int* gid;               // cellGrainID_ [NG]
   int* oid;               // cellOrientationID_ [NG]
   int* claim;             // smallest claiming grain id per cell, INT_MAX when free [NG]
-  unsigned* liq;          // bit c set <=> cell c is liquid [NG/32 + 2]
-  unsigned* clm;          // bit c set <=> cell c was claimed in the running wave [NG/32 + 2]
+  unsigned long long* bmap;  // brick words: liquid, claimed [2 * nbX*nbY*nbZ]
   double4* genv;          // grain envelope: xc, yc, zc, L [N]
   int2* gco;              // grain -> (cell, orientation) [N]
   int* listA[2];          // active lists (grain ids), ping-pong by step parity [N]

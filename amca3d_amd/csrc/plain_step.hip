@@ -64,15 +64,19 @@ __global__ void k_init_grid(DevParams P, DevBuffers B) {
     B.oid[c] = v;
     B.claim[c] = INT_MAX;
   }
-  const long long nw = (long long)(P.NG >> 5) + 2;
-  for (long long w = (long long)blockIdx.x * blockDim.x + threadIdx.x; w < nw; w += stride) {
-    unsigned bits = 0;
-    for (int b = 0; b < 32; b++) {
-      long long c = w * 32 + b;
-      if (c < P.NG && !is_ghost(P, c)) bits |= 1u << b;
+  // brick map: every interior cell liquid, nothing claimed
+  const long long nb = (long long)P.nbX * P.nbY * P.nbZ;
+  for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += stride) {
+    const int bx = (int)(b % P.nbX);
+    const int by = (int)((b / P.nbX) % P.nbY);
+    const int bz = (int)(b / ((long long)P.nbX * P.nbY));
+    unsigned long long bits = 0;
+    for (int q = 0; q < 64; q++) {
+      const int i = 4 * bx + (q & 3), j = 4 * by + ((q >> 2) & 3), k = 4 * bz + (q >> 4);
+      if (i >= 1 && i <= P.nx && j >= 1 && j <= P.ny && k >= 1 && k <= P.nz) bits |= 1ull << q;
     }
-    B.liq[w] = bits;
-    B.clm[w] = 0u;
+    B.bmap[2 * b] = bits;
+    B.bmap[2 * b + 1] = 0ull;
   }
 }
 
@@ -155,7 +159,7 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
         B.gco[id] = make_int2(c, o);
         B.gid[c] = id;
         B.oid[c] = o;
-        clear_bit(B.liq, c);
+        brick_clear(P, B.bmap, 0, ci, cj, ck);
         Ain[nA0 + r] = id;
       }
     }
@@ -212,29 +216,6 @@ __device__ __forceinline__ void wave_input(const DevBuffers& B, int wave, const 
 
 // Envelope tests of one wave entry (grain id g).  Wave 0 first grows the grain
 // (Grain::grow, Grain.h:27, over the whole list as in :521-538).
-// the 26 neighbour bits of cell c from a 1-bit map, in slot order (k, j, i)
-template <bool kCoherent>
-__device__ __forceinline__ unsigned neighbour_bits(const DevParams& P, const unsigned* map, int c) {
-  unsigned m = 0;
-  int s = 0;
-#pragma unroll
-  for (int dk = -1; dk <= 1; dk++) {
-#pragma unroll
-    for (int dj = -1; dj <= 1; dj++) {
-      const unsigned b = bits3<kCoherent>(map, c + dk * P.nxyG + dj * P.nxG);
-      if (dk == 0 && dj == 0) {
-        m |= (b & 1u) << s;
-        m |= ((b >> 2) & 1u) << (s + 1);
-        s += 2;
-      } else {
-        m |= b << s;
-        s += 3;
-      }
-    }
-  }
-  return m;
-}
-
 // Entries of wave w >= 1 are exactly the cells captured in wave w-1, whose
 // claimed bits are cleared here (not while wave w-1 still reads the map).
 template <bool kCoherent>
@@ -243,16 +224,16 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
   const int2 co = B.gco[g];
   const int c = co.x;
   const int o = co.y;
-  if (!grow) clear_bit(B.clm, c);
+  int ci, cj, ck;
+  cell_ijk(P, c, ci, cj, ck);
+  if (!grow) brick_clear(P, B.bmap, 1, ci, cj, ck);
   double4 e = B.genv[g];
   if (grow) {
     e.w = e.w + inc;
     B.genv[g].w = e.w;
   }
-  const unsigned lm = neighbour_bits<kCoherent>(P, B.liq, c);
+  const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
   if (lm == 0 || e.w == 0) return lm;  // no liquid neighbour, or an envelope that captures nothing
-  int ci, cj, ck;
-  cell_ijk(P, c, ci, cj, ck);
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
 #pragma unroll
@@ -271,7 +252,7 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
       im |= 1u << s;
       const int nb = c + P.off[s];
       atomicMin(&B.claim[nb], g);
-      atomicOr(&B.clm[nb >> 5], 1u << (nb & 31));
+      brick_set(P, B.bmap, 1, ci + di, cj + dj, ck + dk);
     }
   }
   return ((unsigned long long)im << 32) | lm;
@@ -299,7 +280,9 @@ __device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevB
   const unsigned lm = (unsigned)m, im = (unsigned)(m >> 32);
   if (lm == 0) return 0u;
   const int c = B.gco[g].x;
-  const unsigned cm = neighbour_bits<kCoherent>(P, B.clm, c) & lm;
+  int ci, cj, ck;
+  cell_ijk(P, c, ci, cj, ck);
+  const unsigned cm = brick_bits26<1, kCoherent>(P, B.bmap, ci, cj, ck) & lm;
   // an unclaimed liquid neighbour outside the envelope keeps the grain active
   unsigned keep = (lm & ~im & ~cm) ? 1u : 0u;
   const unsigned need = keep ? (cm & im) : cm;
@@ -360,7 +343,7 @@ __device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffe
     B.gco[id] = make_int2(nb, o);
     B.gid[nb] = id;
     B.oid[nb] = o;
-    clear_bit(B.liq, nb);
+    brick_clear(P, B.bmap, 0, ci + di, cj + dj, ck + dk);
     kids[pos0 + j] = id;
     j++;
   }
