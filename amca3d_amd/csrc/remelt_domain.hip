@@ -12,10 +12,14 @@
 #include <climits>
 #include <cmath>
 #include <cstring>
+#include <memory>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/amca3d.h"
+#include "host/generators.h"
+#include "host/sites.h"
 #include "host/voxel_map.h"
 #include "plain_step.h"
 #include "remelt_step.h"
@@ -55,9 +59,18 @@ struct amca_remelt {
   int rows_pushed = 0;
   amca_orientation_source source = nullptr;
   void* source_user = nullptr;
+  std::unique_ptr<amca_host::GlibcRand> rng;  // the rand() stream after amca_remelt_draw_sites
+  std::vector<double> euler;                  // Euler angles of every row, 3 per row
   size_t grain_cap = 0;  // records genv/gco can hold
   std::vector<int32_t> host_tag;
   std::vector<double> host_par;
+  // per-kernel event timing (off by default)
+  bool timing = false;
+  std::vector<hipEvent_t> ev_pool;
+  std::vector<std::pair<int, size_t>> ev_pending;
+  size_t ev_used = 0;
+  double kernel_ms[amca::kRmKernels] = {};
+  int64_t kernel_launches[amca::kRmKernels] = {};
 
   template <class T>
   amca_status alloc(T** p, size_t count) {
@@ -81,6 +94,7 @@ struct amca_remelt {
   }
   ~amca_remelt() {
     if (stream) (void)hipStreamSynchronize(stream);
+    for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
   }
@@ -127,15 +141,60 @@ amca_status push_rows(amca_remelt* d, int n, const double* l2g9) {
 // make sure `need` rows exist, pulling from the orientation source
 amca_status ensure_rows(amca_remelt* d, long long need) {
   if (need <= d->rows_pushed) return AMCA_OK;
-  if (!d->source)
+  if (!d->source && !d->rng)
     return rm_fail(AMCA_ERR_CAPACITY, "orientation rows exhausted and no orientation source set");
   need = std::min<long long>(need, d->P.orient_cap);
   const int n = (int)(need - d->rows_pushed);
   if (n <= 0) return AMCA_OK;
   std::vector<double> eu(3 * (size_t)n), r(9 * (size_t)n);
-  if (d->source(d->source_user, n, eu.data(), r.data()) != 0)
+  if (d->rng) {
+    for (int q = 0; q < n; q++) {
+      amca_host::random_orientation(*d->rng, &eu[3 * (size_t)q]);
+      amca_host::rotation_from_euler(&eu[3 * (size_t)q], &r[9 * (size_t)q]);
+    }
+  } else if (d->source(d->source_user, n, eu.data(), r.data()) != 0) {
     return rm_fail(AMCA_ERR_INVALID, "orientation source failed");
+  }
+  d->euler.insert(d->euler.end(), eu.begin(), eu.end());
   return push_rows(d, n, r.data());
+}
+
+amca_status rm_resolve_events(amca_remelt* d) {
+  RM_HIP(hipStreamSynchronize(d->stream));
+  for (auto& p : d->ev_pending) {
+    float ms = 0.f;
+    RM_HIP(hipEventElapsedTime(&ms, d->ev_pool[p.second], d->ev_pool[p.second + 1]));
+    d->kernel_ms[p.first] += ms;
+    d->kernel_launches[p.first] += 1;
+  }
+  d->ev_pending.clear();
+  d->ev_used = 0;
+  return AMCA_OK;
+}
+
+// one step; with timing on, each kernel sits between two events on the stream
+amca_status rm_enqueue_step(amca_remelt* d) {
+  if (!d->timing) {
+    RM_HIP(amca::rm_launch_step(d->P, d->B, d->site_blocks, d->stream));
+    return AMCA_OK;
+  }
+  for (int q = 0; q < amca::kRmKernels; q++) {
+    if (d->ev_used + 2 > d->ev_pool.size()) {
+      if (d->ev_pool.size() >= (1u << 16)) RM_TRY(rm_resolve_events(d));
+      while (d->ev_used + 2 > d->ev_pool.size()) {
+        hipEvent_t e;
+        RM_HIP(hipEventCreate(&e));
+        d->ev_pool.push_back(e);
+      }
+    }
+    const size_t i = d->ev_used;
+    d->ev_used += 2;
+    RM_HIP(hipEventRecord(d->ev_pool[i], d->stream));
+    RM_HIP(amca::rm_launch_kernel(q, d->P, d->B, d->site_blocks, d->stream));
+    RM_HIP(hipEventRecord(d->ev_pool[i + 1], d->stream));
+    d->ev_pending.emplace_back(q, i);
+  }
+  return AMCA_OK;
 }
 
 }  // namespace
@@ -143,6 +202,33 @@ amca_status ensure_rows(amca_remelt* d, long long need) {
 extern "C" {
 
 const char* amca_remelt_last_error(void) { return g_rm_error.c_str(); }
+
+amca_status amca_remelt_enable_timing(amca_remelt* d, int32_t on) {
+  if (!d) return rm_fail(AMCA_ERR_INVALID, "null domain");
+  RM_TRY(rm_resolve_events(d));
+  d->timing = on != 0;
+  for (int q = 0; q < amca::kRmKernels; q++) {
+    d->kernel_ms[q] = 0.0;
+    d->kernel_launches[q] = 0;
+  }
+  return AMCA_OK;
+}
+
+int32_t amca_remelt_kernel_count(void) { return amca::kRmKernels; }
+
+const char* amca_remelt_kernel_name(int32_t i) {
+  return (i >= 0 && i < amca::kRmKernels) ? amca::kRmKernelNames[i] : "";
+}
+
+amca_status amca_remelt_kernel_times(amca_remelt* d, double* ms, int64_t* launches) {
+  if (!d || !ms || !launches) return rm_fail(AMCA_ERR_INVALID, "null argument");
+  RM_TRY(rm_resolve_events(d));
+  for (int q = 0; q < amca::kRmKernels; q++) {
+    ms[q] = d->kernel_ms[q];
+    launches[q] = d->kernel_launches[q];
+  }
+  return AMCA_OK;
+}
 
 amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capacity,
                                amca_remelt** out) {
@@ -429,9 +515,17 @@ amca_status amca_remelt_set_state(amca_remelt* d, const int32_t* orient, const i
 
 amca_status amca_remelt_push_orientations(amca_remelt* d, int32_t n, const double* euler3,
                                           const double* l2g9) {
-  (void)euler3;
-  if (!d || n < 0 || (n > 0 && !l2g9)) return rm_fail(AMCA_ERR_INVALID, "bad rows");
-  return push_rows(d, n, l2g9);
+  if (!d || n < 0 || (n > 0 && !l2g9 && !euler3)) return rm_fail(AMCA_ERR_INVALID, "bad rows");
+  if (!l2g9 && n > 0) {
+    // Octahedron::set_coordinate_transformation_matrix on the host (glibc cos/sin)
+    std::vector<double> r(9 * (size_t)n);
+    for (int q = 0; q < n; q++) amca_host::rotation_from_euler(euler3 + 3 * q, &r[9 * (size_t)q]);
+    RM_TRY(push_rows(d, n, r.data()));
+  } else {
+    RM_TRY(push_rows(d, n, l2g9));
+  }
+  for (int q = 0; q < 3 * n; q++) d->euler.push_back(euler3 ? euler3[q] : 0.0);
+  return AMCA_OK;
 }
 
 amca_status amca_remelt_set_orientation_source(amca_remelt* d, amca_orientation_source src,
@@ -439,6 +533,65 @@ amca_status amca_remelt_set_orientation_source(amca_remelt* d, amca_orientation_
   if (!d) return rm_fail(AMCA_ERR_INVALID, "null domain");
   d->source = src;
   d->source_user = user;
+  d->rng.reset();
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_draw_sites(amca_remelt* d, uint32_t seed, double ns, double dTs_mean,
+                                   double dTs_sd, double nv, double dTv_mean, double dTv_sd,
+                                   int32_t* n_sites) {
+  if (!d) return rm_fail(AMCA_ERR_INVALID, "null domain");
+  amca_host::Grid g{d->P.nx, d->P.ny, d->P.nz};
+  auto rng = std::make_unique<amca_host::GlibcRand>(seed);
+  std::unordered_map<int, double> cell_dt;
+  std::vector<int> sites;
+  const double h = d->P.h;
+  {
+    std::vector<int> ids;
+    amca_host::surface_cells(g, amca_host::SurfaceFaces::kReferenceSix, ids);
+    amca_host::create_nucleation_sites((int)ids.size(), [&](int i) { return ids[i]; }, h * h, ns,
+                                       dTs_mean, dTs_sd, *rng, cell_dt, sites);
+  }
+  amca_host::create_nucleation_sites((int)(g.nx * (long long)g.ny * g.nz),
+                                     amca_host::BulkCellAt{g}, h * h * h, nv, dTv_mean, dTv_sd,
+                                     *rng, cell_dt, sites);
+  std::vector<double> dts(sites.size());
+  for (size_t q = 0; q < sites.size(); q++) dts[q] = cell_dt[sites[q]];
+  RM_TRY(amca_remelt_set_sites(d, (int32_t)sites.size(), sites.data(), dts.data()));
+  d->source = nullptr;
+  d->source_user = nullptr;
+  d->rng = std::move(rng);
+  if (n_sites) *n_sites = (int32_t)sites.size();
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_euler(amca_remelt* d, int32_t cap, double* euler3, int32_t* n) {
+  if (!d || !n) return rm_fail(AMCA_ERR_INVALID, "null argument");
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  *n = c.n_orient;
+  const int m = std::max(0, std::min(std::min(cap, c.n_orient), (int)(d->euler.size() / 3)));
+  if (euler3 && m > 0) std::memcpy(euler3, d->euler.data(), sizeof(double) * 3 * m);
+  return AMCA_OK;
+}
+
+amca_status amca_rosenthal_frames(int64_t n_nodes, const double* node_xyz, int32_t n_frames,
+                                  double t0, double dt, double P, double eta, double v, double k,
+                                  double alpha, double T0, double x_start, double y_c,
+                                  double z_top, double r_min, double* out) {
+  if (n_nodes < 1 || n_frames < 1 || !node_xyz || !out || !(k > 0) || !(alpha > 0))
+    return rm_fail(AMCA_ERR_INVALID, "bad arguments");
+  amca_host::RosenthalParams p{P, eta, v, k, alpha, T0, x_start, y_c, z_top, r_min};
+  amca_host::rosenthal_frames(n_nodes, node_xyz, n_frames, t0, dt, p, out);
+  return AMCA_OK;
+}
+
+amca_status amca_voronoi_substrate(int32_t nx, int32_t ny, int32_t nz, double h, double x0,
+                                   double y0, double z0, int32_t n_seeds, uint64_t seed,
+                                   int32_t* label, double* euler3) {
+  if (nx < 1 || ny < 1 || nz < 1 || n_seeds < 1 || !label || !euler3 || !(h > 0))
+    return rm_fail(AMCA_ERR_INVALID, "bad arguments");
+  amca_host::voronoi_labels(nx, ny, nz, h, x0, y0, z0, n_seeds, seed, label, euler3);
   return AMCA_OK;
 }
 
@@ -518,8 +671,7 @@ amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_count
     const long long room = d->P.orient_cap - (long long)c.n_orient;
     if (c.n_sites > 0) k = std::max(1LL, std::min<long long>(k, room / c.n_sites));
     RM_TRY(ensure_rows(d, (long long)c.n_orient + k * c.n_sites));
-    for (long long q = 0; q < k; q++)
-      RM_HIP(amca::rm_launch_step(d->P, d->B, d->site_blocks, d->stream));
+    for (long long q = 0; q < k; q++) RM_TRY(rm_enqueue_step(d));
     RM_TRY(rm_counters(d, &c));
     batch = std::min<long long>(batch * 2, 64);
   }

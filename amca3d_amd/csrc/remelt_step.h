@@ -73,6 +73,11 @@ struct RmBuffers {
 };
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s);
+// One step: kernels 0..kRmKernels-1 in order on stream s.
+constexpr int kRmKernels = 11;
+extern const char* const kRmKernelNames[kRmKernels];
+hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
+                            hipStream_t s);
 hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s);
 
 }  // namespace amca

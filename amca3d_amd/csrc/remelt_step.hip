@@ -552,25 +552,42 @@ hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s) 
   return hipGetLastError();
 }
 
-hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s) {
+const char* const kRmKernelNames[kRmKernels] = {
+    "k_rm_phase", "k_rm_liquid", "k_rm_nucleate", "k_rm_mint",   "k_rm_dt",   "k_rm_claim",
+    "k_rm_resolve", "k_rm_rowscan", "k_rm_emit", "k_rm_finish", "k_rm_nodal"};
+
+hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
+                            hipStream_t s) {
   long long cb = (P.NG + kRmBlock - 1) / kRmBlock;
   if (cb > 16384) cb = 16384;
   const unsigned cells = (unsigned)cb;
   const unsigned rows = (unsigned)((P.nx * P.ny + kRmBlock - 1) / kRmBlock);
   long long nbk = (P.nodes + kRmBlock - 1) / kRmBlock;
   if (nbk > 8192) nbk = 8192;
-  hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B);
-  hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+  if (nbk < 1) nbk = 1;
+  switch (q) {
+    case 0: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 1: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 2: hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B); break;
+    case 3: hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 4: hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B); break;
+    case 5: hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 6: hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 7: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
+    case 8: hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 9: hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B); break;
+    case 10: hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B); break;
+    default: return hipErrorInvalidValue;
+  }
   return hipGetLastError();
+}
+
+hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s) {
+  for (int q = 0; q < kRmKernels; q++) {
+    const hipError_t e = rm_launch_kernel(q, P, B, site_blocks, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace amca

@@ -61,6 +61,20 @@ SIGNATURES = {
     "amca_remelt_download": (C.c_int, [C.c_void_p, I32P, I32P, F64P, U8P, U8P, C.c_int32, F64P,
                                        I32P, I32P, I32P]),
     "amca_remelt_theta": (C.c_int, [C.c_void_p, F64P]),
+    "amca_remelt_draw_sites": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_double, C.c_double, I32P]),
+    "amca_remelt_euler": (C.c_int, [C.c_void_p, C.c_int32, F64P, I32P]),
+    "amca_remelt_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_remelt_kernel_count": (C.c_int32, []),
+    "amca_remelt_kernel_name": (C.c_char_p, [C.c_int32]),
+    "amca_remelt_kernel_times": (C.c_int, [C.c_void_p, F64P, C.POINTER(C.c_int64)]),
+    "amca_rosenthal_frames": (C.c_int, [C.c_int64, F64P, C.c_int32, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, C.c_double, C.c_double,
+                                        C.c_double, C.c_double, F64P]),
+    "amca_voronoi_substrate": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_int32, C.c_uint64, I32P,
+                                         F64P]),
     "amca_remelt_last_error": (C.c_char_p, []),
 }
 
@@ -102,15 +116,30 @@ def map_voxels(grid, fe, bin=(5, 5, 5), tol=1e-16, max_iter=20):
     return tag, par.reshape(-1, 3)
 
 
-def rotations(euler):
-    """Row-major local-to-global rotations (Z-X-Z Euler), numpy (host plumbing)."""
-    e = np.asarray(euler, np.float64).reshape(-1, 3)
-    ca, sa = np.cos(e[:, 0]), np.sin(e[:, 0])
-    cb, sb = np.cos(e[:, 1]), np.sin(e[:, 1])
-    cg, sg = np.cos(e[:, 2]), np.sin(e[:, 2])
-    return np.stack([ca * cg - sa * cb * sg, -ca * sg - sa * cb * cg, sa * sb,
-                     sa * cg + ca * cb * sg, -sa * sg + ca * cb * cg, -ca * sb,
-                     sb * sg, sb * cg, cb], axis=1)
+def rosenthal_frames(xyz, times, *, P, eta, v, k, alpha, T0, x_start, y_c, z_top, r_min):
+    """Nodal frames [len(times), n_nodes] of a moving point source (host, glibc exp).
+    `times` must be evenly spaced."""
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    times = np.asarray(times, np.float64)
+    n = len(xyz)
+    dt = float(times[1] - times[0]) if len(times) > 1 else 0.0
+    out = np.empty((len(times), n), np.float64)
+    _check(lib().amca_rosenthal_frames(n, _p(xyz, C.c_double), len(times), float(times[0]), dt,
+                                       P, eta, v, k, alpha, T0, x_start, y_c, z_top, r_min,
+                                       _p(out, C.c_double)), "rosenthal_frames")
+    return out
+
+
+def voronoi_substrate(grid, n_seeds, seed):
+    """(label[NG] with ghosts -1, euler[n_seeds, 3]) of a Voronoi substrate (host)."""
+    nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
+    ng = (nx + 2) * (ny + 2) * (nz + 2)
+    lab = np.empty(ng, np.int32)
+    eu = np.empty((n_seeds, 3), np.float64)
+    _check(lib().amca_voronoi_substrate(nx, ny, nz, grid["h"], grid["x0"], grid["y0"], grid["z0"],
+                                        n_seeds, seed, _p(lab, C.c_int32), _p(eu, C.c_double)),
+           "voronoi_substrate")
+    return lab, eu
 
 
 class RemeltDomain:
@@ -142,11 +171,14 @@ class RemeltDomain:
         _check(lib().amca_remelt_voxel_map(self.h, _p(tag, C.c_int32), _p(par, C.c_double)), "voxel_map")
         return tag, par.reshape(-1, 3)
 
-    def push_orientations(self, euler, l2g):
+    def push_orientations(self, euler, l2g=None):
+        """Append orientation rows; without `l2g` the library computes the
+        rotations on the host (glibc cos/sin, as the reference does)."""
         e = np.ascontiguousarray(euler, np.float64).reshape(-1)
-        r = np.ascontiguousarray(l2g, np.float64).reshape(-1)
-        _check(lib().amca_remelt_push_orientations(self.h, len(r) // 9, _p(e, C.c_double),
-                                                   _p(r, C.c_double)), "push_orientations")
+        r = None if l2g is None else np.ascontiguousarray(l2g, np.float64).reshape(-1)
+        _check(lib().amca_remelt_push_orientations(self.h, len(e) // 3, _p(e, C.c_double),
+                                                   None if r is None else _p(r, C.c_double)),
+               "push_orientations")
 
     def set_orientation_source(self, fn):
         """fn(n) -> (euler[n,3], l2g[n,9]) continuing the rand() stream."""
@@ -177,6 +209,22 @@ class RemeltDomain:
         d = np.ascontiguousarray(dts, np.float64)
         _check(lib().amca_remelt_set_sites(self.h, len(c), _p(c, C.c_int32), _p(d, C.c_double)),
                "set_sites")
+
+    def draw_sites(self, seed, surface, bulk):
+        """The reference sampler on the host rand() stream: surface = (density,
+        mean, sd) on the six faces, bulk = (density, mean, sd); later nuclei take
+        their orientations from the same stream."""
+        n = C.c_int32(0)
+        _check(lib().amca_remelt_draw_sites(self.h, seed, *surface, *bulk, C.byref(n)), "draw_sites")
+        return int(n.value)
+
+    def euler(self):
+        n = C.c_int32(0)
+        _check(lib().amca_remelt_euler(self.h, 0, None, C.byref(n)), "euler")
+        out = np.empty((max(int(n.value), 1), 3), np.float64)
+        _check(lib().amca_remelt_euler(self.h, int(n.value), _p(out, C.c_double), C.byref(n)),
+               "euler")
+        return out[:int(n.value)]
 
     def enable_log(self, cap):
         _check(lib().amca_remelt_enable_log(self.h, int(cap)), "enable_log")
@@ -222,6 +270,20 @@ class RemeltDomain:
         out.update(grain_env=env[:4 * m].reshape(-1, 4), grain_cell=gc[:m], grain_orient=go[:m],
                    n_grains=int(n.value))
         return out
+
+    def enable_timing(self, on=True):
+        _check(lib().amca_remelt_enable_timing(self.h, 1 if on else 0), "enable_timing")
+
+    def kernel_times(self):
+        """{kernel name: (total ms, launches)} since timing was enabled."""
+        L = lib()
+        n = L.amca_remelt_kernel_count()
+        ms = np.zeros(n, np.float64)
+        cnt = np.zeros(n, np.int64)
+        _check(L.amca_remelt_kernel_times(self.h, _p(ms, C.c_double), _p(cnt, C.c_int64)),
+               "kernel_times")
+        return {L.amca_remelt_kernel_name(i).decode(): (float(ms[i]), int(cnt[i]))
+                for i in range(n)}
 
     def theta(self):
         t = np.empty(self.nodes, np.float64)

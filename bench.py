@@ -76,10 +76,10 @@ def copy_probe(mib=2048, reps=20):
     return {"gbps": g.value, "ms_per_copy": ms.value, "bytes": mib << 20}
 
 
-def measured_traffic(kernel):
+def measured_traffic(kernel, summary="pmc_traffic_plain_uniform512.json"):
     """HBM bytes per launch of `kernel` from the committed PMC summary
     (FETCH_SIZE + WRITE_SIZE per dispatch, scaled by the copy-kernel factor)."""
-    path = os.path.join(ROOT, "profiles", "pmc_traffic_plain_uniform512.json")
+    path = os.path.join(ROOT, "profiles", summary)
     try:
         with open(path) as f:
             t = json.load(f)
@@ -87,6 +87,122 @@ def measured_traffic(kernel):
         return None
     k = t.get("kernels", {}).get(kernel)
     return None if k is None else k.get("bytes_per_launch")
+
+
+def remelt_algorithmic_bytes(P, log):
+    """Algorithmic bytes per kernel of the coupled step over a step log.
+
+    k_rm_phase   46 B per cell (element tag 4, parametric coordinates 24, T read
+                 8 and written 8, cooling and liquid-neighbour flags 1 + 1) and
+                 8 B per FE node (the nodal field)
+    k_rm_liquid  13 B per interior cell (T 8, orientation 4, melted flag 1)
+    k_rm_mint    1 B per cell (flag) + 20 B per flagged cell (grain id 4,
+                 length 8, T 8)
+    k_rm_claim   1 B per interior cell + 56 B per flagged cell (grain id 4,
+                 envelope read 32 and length written 8, T 8, orientation 4)
+    k_rm_emit    1 B per interior cell + 44 B per capture
+    k_rm_nodal   24 B per node (two frames read, nodal field written)
+    """
+    ng = float((P["nx"] + 2) * (P["ny"] + 2) * (P["nz"] + 2))
+    n = float(P["nx"] * P["ny"] * P["nz"])
+    nodes = float(P["nodes"])
+    F = log["n_flagged"].astype(np.float64)
+    C = log["n_captured"].astype(np.float64)
+    one = np.ones_like(F)
+    return {
+        "k_rm_phase": (46.0 * ng + 8.0 * nodes) * one,
+        "k_rm_liquid": 13.0 * n * one,
+        "k_rm_mint": 1.0 * ng + 20.0 * F,
+        "k_rm_claim": 1.0 * n + 56.0 * F,
+        "k_rm_emit": 1.0 * n + 44.0 * C,
+        "k_rm_nodal": 24.0 * nodes * one,
+    }
+
+
+def pbf_cpu_baseline(steps=300):
+    """The oracle (one core) on a 200 x 80 x 40 sample of the same track physics."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from amca3d_amd import pbf
+    w = pbf.track(nx=200, ny=80, nz=40, n_frames=40)
+    d = w["desc"]
+    case = dict(grid=w["grid"], fe=w["fe"], bin=d["bin"], tol=d["tolerance"],
+                max_iter=d["max_iterations"], void_temp=d["void_temperature"],
+                physics={k: d[k] for k in ("Tm", "Ts", "Tmin_nuc", "a1", "a2", "a3", "b1", "b2",
+                                           "b3")},
+                trunc=d["trunc"], tsf=d["time_step_factor"], cutoff_factor=d["cutoff_factor"],
+                final_time=d["final_time"], max_steps=steps, initial_melted=0,
+                nucleation=dict(surface=w["surface"], bulk=w["bulk"]), seed=w["nuc_seed"],
+                state=w["state"])
+    t0 = time.perf_counter()
+    r = oracle_lib.run_remelt(case)
+    wall = time.perf_counter() - t0
+    n = 200 * 80 * 40
+    return {"value": n * r["n_steps"] / wall, "unit": "cell-updates/s", "cores": 1, "kind": "port",
+            "sample": f"CPU oracle (sequential restatement, g++ -O2) on the same track physics "
+                      f"over 200x80x40 cells, {r['n_steps']} steps, {wall:.1f} s"}
+
+
+def pbf_main(args):
+    """Coupled CA-FE single track: 1000 x 400 x 200 cells of 2.5e-6 m."""
+    from amca3d_amd import pbf
+    t_setup = time.perf_counter()
+    w = pbf.track(n_frames=args.frames)
+    d = pbf.domain(w)
+    setup_s = time.perf_counter() - t_setup
+    d.enable_log(args.warmup + args.steps + 8)
+    if args.warmup > 0:
+        d.run(args.warmup)
+    if not args.no_timing:
+        d.enable_timing(True)
+    c0 = d.counters()
+    t0 = time.perf_counter()
+    c = d.run(args.steps)
+    wall = time.perf_counter() - t0
+    executed = int(c["step"] - c0["step"])
+    g = w["grid"]
+    cells = g["nx"] * g["ny"] * g["nz"]
+    value = cells * executed / wall
+    log = d.read_log()[int(c0["step"]):int(c["step"])]
+    result = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall / max(executed, 1), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (Rosenthal frames and Voronoi substrate from the library's host "
+                "generators, sites from its rand() stream, seed 0)",
+        "config": {"workload": f"coupled CA-FE single track, {g['nx']}x{g['ny']}x{g['nz']} cells "
+                               "of 2.5e-6 m, FE nodes every 1e-5 m, Rosenthal 200 W 1 m/s, "
+                               "2000 Voronoi grains, bulk 1e13 m^-3 N(5,1)",
+                   "cells": cells, "fe_nodes": int(np.prod(w["fe"]["shape"])),
+                   "frames": args.frames, "executed_steps": executed,
+                   "sim_time_s": c["time"], "captures": int(log["n_captured"].sum()),
+                   "nucleated": int(log["n_nucleated"].sum()), "setup_s": setup_s},
+    }
+    if not args.no_timing:
+        kt = d.kernel_times()
+        total = {k: v[0] for k, v in kt.items()}
+        dom_k = max(total, key=total.get)
+        Pn = dict(nx=g["nx"], ny=g["ny"], nz=g["nz"], nodes=int(np.prod(w["fe"]["shape"])))
+        ab = remelt_algorithmic_bytes(Pn, log)
+        roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": dom_k,
+                "kernel_ms_total": total[dom_k], "launches": kt[dom_k][1],
+                "avg_launch_us": 1e3 * total[dom_k] / max(kt[dom_k][1], 1),
+                "kernel_share_of_wall": total[dom_k] / (1e3 * wall)}
+        if dom_k in ab and kt[dom_k][1] == len(log):
+            bytes_total = float(ab[dom_k].sum())
+            achieved = bytes_total / (total[dom_k] * 1e-3) / 1e9
+            roof.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS,
+                        bytes_per_launch=bytes_total / max(kt[dom_k][1], 1))
+        else:
+            roof.update(achieved=None, frac=None)
+        roof["traffic"] = measured_traffic(dom_k, "pmc_traffic_pbf_track.json")
+        result["roofline"] = roof
+        result["kernel_ms"] = {k: round(v, 3) for k, v in total.items()}
+    d.close()
+    if not args.no_cpu_baseline:
+        result["cpu_baseline"] = pbf_cpu_baseline()
+    print(json.dumps(result))
 
 
 def main():
@@ -99,6 +215,10 @@ def main():
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the stream-copy probe (for counter calibration)")
+    ap.add_argument("--workload", choices=("plain512", "pbf"), default="plain512",
+                    help="plain512: uniform cooling 512^3 (default); pbf: coupled CA-FE track "
+                         "1000x400x200")
+    ap.add_argument("--frames", type=int, default=100, help="pbf: FE frames (10 us apart)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs the CA step on "
@@ -110,6 +230,9 @@ def main():
 
     if args.copy_probe:
         print(json.dumps({"copy_probe": copy_probe()}))
+        return
+    if args.workload == "pbf":
+        pbf_main(args)
         return
 
     n = args.size

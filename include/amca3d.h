@@ -243,6 +243,8 @@ amca_status amca_fe_map_voxels(int32_t nx, int32_t ny, int32_t nz, double h, dou
 amca_status amca_remelt_set_state(amca_remelt* d, const int32_t* orient, const int32_t* grain,
                                   const uint8_t* melted, int32_t n_orient, int32_t n_grains,
                                   const int32_t* grain_cell, const int32_t* grain_orient);
+/* Appends n orientation rows (Euler angles; l2g9 may be NULL, then the
+ * rotations are computed on the host as Octahedron.cpp:281-310 does). */
 amca_status amca_remelt_push_orientations(amca_remelt* d, int32_t n, const double* euler3,
                                           const double* l2g9);
 amca_status amca_remelt_set_orientation_source(amca_remelt* d, amca_orientation_source src,
@@ -250,6 +252,17 @@ amca_status amca_remelt_set_orientation_source(amca_remelt* d, amca_orientation_
 /* Nucleation sites (criterion_to_activate_grain.using_nucleation_seed). */
 amca_status amca_remelt_set_sites(amca_remelt* d, int32_t n, const int32_t* cell,
                                   const double* dT_crit);
+/* The host half of CellularAutomataManager_3DRemelting::initialize: srand(seed),
+ * then create_nucleation_sites for the surface (six faces, h^2 per cell) and the
+ * bulk (h^3 per cell) populations (CellularAutomataManager.cpp:153-213), and
+ * sets the orientation source to the continuing rand() stream
+ * (get_random_orientation_angle, CellularAutomataManager_3D.cpp:838-850).
+ * Replaces any sites and source set before. */
+amca_status amca_remelt_draw_sites(amca_remelt* d, uint32_t seed, double ns, double dTs_mean,
+                                   double dTs_sd, double nv, double dTv_mean, double dTv_sd,
+                                   int32_t* n_sites);
+/* Euler angles of every orientation row pushed or drawn so far. */
+amca_status amca_remelt_euler(amca_remelt* d, int32_t cap, double* euler3, int32_t* n);
 /* CafeSolver::time_integration_weak_coupling (CafeSolver.cpp:178-219) until
  * termination_time, the end of the frames, or max_steps steps (< 0: no limit). */
 amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_counters* out);
@@ -265,6 +278,29 @@ amca_status amca_remelt_download(amca_remelt* d, int32_t* orient, int32_t* grain
                                  int32_t* n_grains);
 /* thetaArray_: the nodal field the next step interpolates from. */
 amca_status amca_remelt_theta(amca_remelt* d, double* theta);
+/* Per-kernel device time from HIP events around every launch (off by default);
+ * enabling resets the totals.  Kernel i is named amca_remelt_kernel_name(i). */
+amca_status amca_remelt_enable_timing(amca_remelt* d, int32_t on);
+int32_t amca_remelt_kernel_count(void);
+const char* amca_remelt_kernel_name(int32_t i);
+amca_status amca_remelt_kernel_times(amca_remelt* d, double* ms, int64_t* launches);
+
+/* Input generators (host only, no GPU).  Nodal frames of a Rosenthal moving
+ * point source, T0 + eta*P/(2 pi k R) exp(-v (R + xi)/(2 alpha)) with
+ * xi = x - (x_start + v t), R floored at r_min, for t = t0 + f*dt; out is
+ * [n_frames][n_nodes].  No counterpart in the reference, which reads such
+ * frames from text files (LoadDataFromFile.cpp:21-191). */
+amca_status amca_rosenthal_frames(int64_t n_nodes, const double* node_xyz, int32_t n_frames,
+                                  double t0, double dt, double P, double eta, double v, double k,
+                                  double alpha, double T0, double x_start, double y_c,
+                                  double z_top, double r_min, double* out);
+/* Voronoi substrate: label[NG] = nearest of n_seeds seed points (interior
+ * cells; ghosts -1), euler3[3*n_seeds] uniform on SO(3), from a splitmix64
+ * stream of `seed`.  No counterpart in the reference, which loads such a
+ * microstructure from grainInfo text files (:165-307). */
+amca_status amca_voronoi_substrate(int32_t nx, int32_t ny, int32_t nz, double h, double x0,
+                                   double y0, double z0, int32_t n_seeds, uint64_t seed,
+                                   int32_t* label, double* euler3);
 const char* amca_remelt_last_error(void);
 
 /* ---- realm layer: YAML input deck -> device domain ------------------------

@@ -26,6 +26,27 @@ def _desc(case):
     return d
 
 
+def run_device_own_sampler(case, orc):
+    """Run `case` with the library's own host sampler and rand() stream
+    (amca_remelt_draw_sites) instead of inputs taken from the oracle run."""
+    from amca3d_amd.remelt import RemeltDomain
+    st = case["state"]
+    n0 = len(st["euler"])
+    nuc = case["nucleation"]
+    d = RemeltDomain(_desc(case), n0 + 200000)
+    d.push_orientations(st["euler"])  # rotations computed by the library
+    d.set_fe(case["fe"])
+    d.set_state(st)
+    n_sites = d.draw_sites(case["seed"], nuc["surface"], nuc["bulk"])
+    d.enable_log(100000)
+    c = d.run(case["max_steps"])
+    out = d.download()
+    out.update(theta=d.theta(), log=d.read_log(), counters=c, tag_par=d.voxel_map(),
+               euler=d.euler(), n_sites=n_sites)
+    d.close()
+    return out
+
+
 def run_device(case, orc, chunks=None):
     """Run `case` on the device.  Nucleation sites and the orientation rows drawn
     after the loaded ones are the host-side inputs of the path; they come from
@@ -134,6 +155,16 @@ def test_track_with_nucleation(amca, track_nuc):
     assert orc["steps"]["n_captured"].sum() > 1000
     assert orc["steps"]["n_nucleated"].sum() > 50
     compare(case, orc, run_device(case, orc))
+
+
+def test_track_with_the_library_sampler(amca, track_nuc):
+    """Sites and nucleus orientations drawn by the library's host rand() stream
+    give the oracle's run, including every Euler angle."""
+    case, orc = track_nuc
+    dev = run_device_own_sampler(case, orc)
+    assert dev["n_sites"] == len(orc["site_cell"])
+    compare(case, orc, dev)
+    np.testing.assert_array_equal(bits(dev["euler"]), bits(orc["euler"]))
 
 
 def test_track_in_chunks_equals_one_run(amca, track_nuc):
