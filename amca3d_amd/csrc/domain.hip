@@ -554,3 +554,339 @@ amca_status amca_envelope_selftest(int32_t n, const double* env4, const double* 
 }
 
 }  // extern "C"
+
+// ===========================================================================
+// Batches: many independent plain domains stepped together.  Every kernel of
+// a step is launched once for the whole batch; its blocks are split among the
+// domains by a per-kernel (domain, block, count) map, so small domains that
+// could not fill the device alone share it.  The steps are captured in a HIP
+// graph of `steps_per_graph` steps that is replayed until every domain has
+// finished; a finished domain's kernels return at once.  Results per domain
+// do not depend on the block split (every kernel's result is independent of
+// its block count), so each domain stays identical to its own run.
+
+struct amca_batch {
+  std::vector<amca_domain*> doms;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  amca::DevParams* dP = nullptr;
+  amca::DevBuffers* dB = nullptr;
+  int4* maps = nullptr;  // nuc | list | emit | scan, back to back
+  int4* status = nullptr;
+  amca::BatchLaunch L{};
+  amca_batch_schedule sched{};
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  int graph_steps = 0;
+  int cus = 256;
+  double last_ms_per_step = 0.0;
+
+  void drop_graph() {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+    graph_steps = 0;
+  }
+  ~amca_batch() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
+    if (dP) (void)hipFree(dP);
+    if (dB) (void)hipFree(dB);
+    if (maps) (void)hipFree(maps);
+    if (status) (void)hipFree(status);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+};
+
+namespace {
+
+// Per-domain block counts of the list kernels under a schedule: a budget of
+// `list_blocks` blocks (and emit_blocks for resolve/emit) shared in proportion
+// to the domains' cell counts, at least `min_blocks` each, capped by what a
+// domain can use (one block per 256 cells of its grid, kMaxBlocks for the
+// scanned kernels).
+void batch_geometry(const amca_batch* b, std::vector<int>& nl, std::vector<int>& ne) {
+  const size_t n = b->doms.size();
+  double total = 0.0;
+  for (auto* d : b->doms) total += (double)d->P.N;
+  nl.resize(n);
+  ne.resize(n);
+  for (size_t i = 0; i < n; i++) {
+    const double share = (double)b->doms[i]->P.N / total;
+    const long long cap = std::max(1LL, (b->doms[i]->P.N + amca::kBlock - 1) / amca::kBlock);
+    long long l = (long long)std::llround(share * b->sched.list_blocks);
+    long long e = (long long)std::llround(share * b->sched.emit_blocks);
+    l = std::max<long long>(b->sched.min_blocks, std::min(l, cap));
+    e = std::max<long long>(b->sched.min_blocks, std::min(std::min(e, cap), (long long)amca::kMaxBlocks));
+    nl[i] = (int)std::max(1LL, l);
+    ne[i] = (int)std::max(1LL, e);
+  }
+}
+
+// Block order within one launch: domain-major in the given domain order
+// (`interleave` = 0), or round-robin over domains so that every domain's
+// blocks are spread over the whole launch (`interleave` = 1).
+void build_map(const std::vector<int>& order, const std::vector<int>& cnt, int interleave,
+               std::vector<int4>& out) {
+  out.clear();
+  if (!interleave) {
+    for (int d : order)
+      for (int k = 0; k < cnt[d]; k++) out.push_back(make_int4(d, k, cnt[d], 0));
+    return;
+  }
+  int left = 1;
+  for (int k = 0; left; k++) {
+    left = 0;
+    for (int d : order)
+      if (k < cnt[d]) {
+        out.push_back(make_int4(d, k, cnt[d], 0));
+        left = 1;
+      }
+  }
+}
+
+amca_status batch_apply_schedule(amca_batch* b) {
+  const int n = (int)b->doms.size();
+  std::vector<int> nl, ne, nn(n), one(n, 1);
+  batch_geometry(b, nl, ne);
+  for (int i = 0; i < n; i++) nn[i] = b->doms[i]->nuc_blocks;
+  std::vector<int> order(n);
+  for (int i = 0; i < n; i++) order[i] = i;
+  // larger domains first: their blocks start earliest
+  std::stable_sort(order.begin(), order.end(),
+                   [&](int a, int c) { return b->doms[a]->P.N > b->doms[c]->P.N; });
+  std::vector<int4> mn, ml, me, ms;
+  build_map(order, nn, b->sched.interleave, mn);
+  build_map(order, nl, b->sched.interleave, ml);
+  build_map(order, ne, b->sched.interleave, me);
+  build_map(order, one, 0, ms);
+  for (auto& m : ms) m.z = ne[m.x];  // the scan of domain d spans its emit blocks
+  const size_t total = mn.size() + ml.size() + me.size() + ms.size();
+  if (b->maps) HIP_TRY(hipFree(b->maps));
+  b->maps = nullptr;
+  HIP_TRY(hipMalloc(&b->maps, sizeof(int4) * total));
+  std::vector<int4> all;
+  all.insert(all.end(), mn.begin(), mn.end());
+  all.insert(all.end(), ml.begin(), ml.end());
+  all.insert(all.end(), me.begin(), me.end());
+  all.insert(all.end(), ms.begin(), ms.end());
+  HIP_TRY(hipMemcpy(b->maps, all.data(), sizeof(int4) * total, hipMemcpyHostToDevice));
+  b->L.map_nuc = b->maps;
+  b->L.map_list = b->maps + mn.size();
+  b->L.map_emit = b->L.map_list + ml.size();
+  b->L.map_scan = b->L.map_emit + me.size();
+  b->L.n_nuc = (int)mn.size();
+  b->L.n_list = (int)ml.size();
+  b->L.n_emit = (int)me.size();
+  b->L.n_one = (int)ms.size();
+  b->drop_graph();
+  return AMCA_OK;
+}
+
+amca_status batch_capture(amca_batch* b, int steps) {
+  b->drop_graph();
+  HIP_TRY(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = hipSuccess;
+  for (int q = 0; q < steps && e == hipSuccess; q++) e = amca::launch_batch_step(b->L, b->stream);
+  if (e == hipSuccess) e = amca::launch_batch_status(b->L, (int)b->doms.size(), b->status, b->stream);
+  hipGraph_t g = nullptr;
+  const hipError_t e2 = hipStreamEndCapture(b->stream, &g);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return fail(AMCA_ERR_HIP, std::string("graph capture: ") +
+                                  hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  b->graph = g;
+  HIP_TRY(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+  b->graph_steps = steps;
+  return AMCA_OK;
+}
+
+// Replays the graph until every domain has finished or `max_graphs` replays
+// ran; returns the number of finished domains and the elapsed time.
+amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, double* ms,
+                         long long* replays) {
+  if (!b->exec || b->graph_steps != b->sched.steps_per_graph)
+    AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
+  const int n = (int)b->doms.size();
+  std::vector<int4> st(n);
+  hipEvent_t e0, e1;
+  HIP_TRY(hipEventCreate(&e0));
+  HIP_TRY(hipEventCreate(&e1));
+  HIP_TRY(hipEventRecord(e0, b->stream));
+  long long r = 0;
+  int fin = 0;
+  while (max_graphs < 0 || r < max_graphs) {
+    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+    r++;
+    HIP_TRY(hipMemcpyAsync(st.data(), b->status, sizeof(int4) * n, hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    fin = 0;
+    for (int i = 0; i < n; i++) {
+      if (st[i].y) {
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        return fail(AMCA_ERR_CAPACITY, "orientation table exhausted in domain " + std::to_string(i));
+      }
+      fin += st[i].x;
+    }
+    if (fin == n) break;
+  }
+  HIP_TRY(hipEventRecord(e1, b->stream));
+  HIP_TRY(hipEventSynchronize(e1));
+  float t = 0.f;
+  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (finished) *finished = fin;
+  if (ms) *ms = t;
+  if (replays) *replays = r;
+  return AMCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+amca_status amca_batch_create(amca_domain* const* doms, int32_t n, amca_batch** out) {
+  if (!doms || n < 1 || !out) return fail(AMCA_ERR_INVALID, "need at least one domain");
+  *out = nullptr;
+  auto* b = new amca_batch();
+  for (int i = 0; i < n; i++) {
+    if (!doms[i]) {
+      delete b;
+      return fail(AMCA_ERR_INVALID, "null domain in batch");
+    }
+    if (doms[i]->timing) {
+      delete b;
+      return fail(AMCA_ERR_INVALID, "disable per-kernel timing before batching a domain");
+    }
+    b->doms.push_back(doms[i]);
+  }
+  amca_status st = AMCA_OK;
+  do {
+    if (hipGetDevice(&b->device) != hipSuccess) { st = fail(AMCA_ERR_HIP, "hipGetDevice"); break; }
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, b->device) == hipSuccess && prop.multiProcessorCount > 0)
+      b->cus = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) {
+      st = fail(AMCA_ERR_HIP, "stream creation failed");
+      break;
+    }
+    std::vector<amca::DevParams> P(n);
+    std::vector<amca::DevBuffers> B(n);
+    for (int i = 0; i < n; i++) {
+      // every domain's own queued work must be done before the batch stream takes over
+      if (hipStreamSynchronize(b->doms[i]->stream) != hipSuccess) { st = fail(AMCA_ERR_HIP, "sync"); break; }
+      P[i] = b->doms[i]->P;
+      B[i] = b->doms[i]->B;
+    }
+    if (st != AMCA_OK) break;
+    if (hipMalloc(&b->dP, sizeof(amca::DevParams) * n) != hipSuccess ||
+        hipMalloc(&b->dB, sizeof(amca::DevBuffers) * n) != hipSuccess ||
+        hipMalloc(&b->status, sizeof(int4) * n) != hipSuccess) {
+      st = fail(AMCA_ERR_HIP, "batch allocation failed");
+      break;
+    }
+    if (hipMemcpy(b->dP, P.data(), sizeof(amca::DevParams) * n, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(b->dB, B.data(), sizeof(amca::DevBuffers) * n, hipMemcpyHostToDevice) != hipSuccess) {
+      st = fail(AMCA_ERR_HIP, "batch upload failed");
+      break;
+    }
+    b->L.P = b->dP;
+    b->L.B = b->dB;
+    b->sched.list_blocks = 16 * b->cus;
+    b->sched.emit_blocks = 16 * b->cus;
+    b->sched.min_blocks = 1;
+    b->sched.interleave = 0;
+    b->sched.steps_per_graph = 16;
+    st = batch_apply_schedule(b);
+  } while (0);
+  if (st != AMCA_OK) {
+    delete b;
+    return st;
+  }
+  *out = b;
+  return AMCA_OK;
+}
+
+amca_status amca_batch_destroy(amca_batch* b) {
+  delete b;
+  return AMCA_OK;
+}
+
+amca_status amca_batch_set_schedule(amca_batch* b, const amca_batch_schedule* s) {
+  if (!b || !s || s->list_blocks < 1 || s->emit_blocks < 1 || s->min_blocks < 1 ||
+      s->steps_per_graph < 1 || s->steps_per_graph > 4096)
+    return fail(AMCA_ERR_INVALID, "bad schedule");
+  HIP_TRY(hipStreamSynchronize(b->stream));
+  b->sched = *s;
+  return batch_apply_schedule(b);
+}
+
+amca_status amca_batch_get_schedule(amca_batch* b, amca_batch_schedule* s) {
+  if (!b || !s) return fail(AMCA_ERR_INVALID, "null argument");
+  *s = b->sched;
+  return AMCA_OK;
+}
+
+amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finished, double* ms,
+                           int64_t* replays) {
+  if (!b) return fail(AMCA_ERR_INVALID, "null batch");
+  int fin = 0;
+  double t = 0.0;
+  long long r = 0;
+  AMCA_TRY(batch_replay(b, max_graphs, &fin, &t, &r));
+  if (n_finished) *n_finished = fin;
+  if (ms) *ms = t;
+  if (replays) *replays = r;
+  return AMCA_OK;
+}
+
+// The autotuner runs the real batch: each candidate schedule takes
+// `trial_graphs` graph replays in turn, for `rounds` rounds, and the schedule
+// with the least total time wins.  Schedules do not change results, so the
+// trial steps are ordinary steps of the run.
+amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rounds,
+                                amca_batch_schedule* chosen) {
+  if (!b || trial_graphs < 1 || rounds < 1) return fail(AMCA_ERR_INVALID, "bad arguments");
+  std::vector<amca_batch_schedule> cand;
+  for (int budget : {4, 16, 64})
+    for (int inter : {0, 1}) {
+      amca_batch_schedule s = b->sched;
+      s.list_blocks = budget * b->cus;
+      s.emit_blocks = std::min(budget * b->cus, (int)amca::kMaxBlocks * 8);
+      s.min_blocks = 1;
+      s.interleave = inter;
+      cand.push_back(s);
+    }
+  std::vector<double> tot(cand.size(), 0.0);
+  bool all_done = false;
+  for (int r = 0; r < rounds && !all_done; r++)
+    for (size_t c = 0; c < cand.size() && !all_done; c++) {
+      b->sched.list_blocks = cand[c].list_blocks;
+      b->sched.emit_blocks = cand[c].emit_blocks;
+      b->sched.min_blocks = cand[c].min_blocks;
+      b->sched.interleave = cand[c].interleave;
+      AMCA_TRY(batch_apply_schedule(b));
+      int fin = 0;
+      double ms = 0.0;
+      long long rep = 0;
+      AMCA_TRY(batch_replay(b, trial_graphs, &fin, &ms, &rep));
+      tot[c] += ms / std::max(1LL, rep);
+      if (fin == (int)b->doms.size()) all_done = true;
+    }
+  size_t best = 0;
+  for (size_t c = 1; c < cand.size(); c++)
+    if (tot[c] < tot[best]) best = c;
+  b->sched.list_blocks = cand[best].list_blocks;
+  b->sched.emit_blocks = cand[best].emit_blocks;
+  b->sched.min_blocks = cand[best].min_blocks;
+  b->sched.interleave = cand[best].interleave;
+  AMCA_TRY(batch_apply_schedule(b));
+  if (chosen) *chosen = b->sched;
+  return AMCA_OK;
+}
+
+}  // extern "C"

@@ -104,6 +104,21 @@ hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int bl
 hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                        hipStream_t s);
 hipError_t launch_tail(const DevParams& P, const DevBuffers& B, hipStream_t s);
+
+// One step of many domains in one launch per kernel (device arrays of the
+// domains' parameters and buffers; per-block (domain, block, count) maps).
+struct BatchLaunch {
+  const DevParams* P;
+  const DevBuffers* B;
+  const int4* map_nuc;   // nucleation geometry per domain
+  const int4* map_list;  // claim geometry
+  const int4* map_emit;  // resolve / emit geometry
+  const int4* map_scan;  // one block per domain (scan and tail), z = emit block count
+  int n_nuc, n_list, n_emit, n_one;
+};
+hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s);
+// out[d] = (finished, error, step & 0x7fffffff, step >> 31) for each of n domains
+hipError_t launch_batch_status(const BatchLaunch& L, int n, int4* out, hipStream_t s);
 hipError_t launch_envelope_selftest(int n, const double* env4, const double* l2g9,
                                     const double* cell3, double h, double trunc, double sqrt2f,
                                     int* inside, double* child4, hipStream_t s);

@@ -90,11 +90,15 @@ __global__ void k_init_grid(DevParams P, DevBuffers B) {
 // next grain id; a non-liquid site is dropped.  Ranks come from the look-back
 // scan, so ids follow cell order exactly as in the reference loop.
 
-__global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) {
+// Kernel bodies take the domain by reference and their block number / block
+// count explicitly, so that one launch can also serve many domains (the
+// batched kernels `kb_*` map each block to a (domain, block, count) entry).
+__device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffers& B, int blk,
+                                              int nblk) {
   DevCounters* ctr = B.ctr;
   const double t = ctr->t;
   const int skip = (t < P.final_time && !ctr->done) ? 0 : 1;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->skip = skip;
+  if (blk == 0 && threadIdx.x == 0) ctr->skip = skip;
   if (skip) return;
 
   __shared__ int s_bid;
@@ -117,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
   const double under = P.Tm - T;
 
   int lo, hi;
-  chunk_of(n, bid, gridDim.x, kBlock, lo, hi);
+  chunk_of(n, bid, nblk, kBlock, lo, hi);
   unsigned long long mine = 0;
   for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
     int c = sc[i];
@@ -170,7 +174,7 @@ __global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) 
     }
     run += tot;
   }
-  if (threadIdx.x == 0 && bid == (int)gridDim.x - 1) {
+  if (threadIdx.x == 0 && bid == nblk - 1) {
     const int nn = (int)(run >> 32);
     const int nk = (int)(run & 0xffffffffull);
     if (no0 + nn > P.orient_cap) ctr->err = 1;
@@ -258,7 +262,8 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
   return ((unsigned long long)im << 32) | lm;
 }
 
-__global__ void __launch_bounds__(kBlock) k_claim(DevParams P, DevBuffers B, int wave) {
+__device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers& B, int wave,
+                                           int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   const int* list;
@@ -266,7 +271,7 @@ __global__ void __launch_bounds__(kBlock) k_claim(DevParams P, DevBuffers B, int
   wave_input(B, wave, list, n);
   const double inc = ctr->dt * ctr->v;
   int lo, hi;
-  chunk_of(n, xcd_block(), gridDim.x, kBlock, lo, hi);
+  chunk_of(n, blk, nblk, kBlock, lo, hi);
   for (int i = lo + threadIdx.x; i < hi; i += kBlock)
     B.masks[i] = claim_entry<false>(P, B, list[i], wave == 0, inc);
 }
@@ -357,15 +362,16 @@ __device__ __forceinline__ int wave_chunks(int n, int grid) {
   return max(1, min(grid, (n + kTile - 1) / kTile));
 }
 
-__global__ void __launch_bounds__(kBlock) k_resolve(DevParams P, DevBuffers B, int wave) {
+__device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffers& B, int wave,
+                                             int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   __shared__ unsigned long long s_sm[kBlock / 64];
   const int* list;
   int n;
   wave_input(B, wave, list, n);
-  const int nb = wave_chunks(n, gridDim.x);
-  const int bid = xcd_block();
+  const int nb = wave_chunks(n, nblk);
+  const int bid = blk;
   if (bid >= nb) return;
   int lo, hi;
   chunk_of(n, bid, nb, kBlock * kItems, lo, hi);
@@ -382,7 +388,8 @@ __global__ void __launch_bounds__(kBlock) k_resolve(DevParams P, DevBuffers B, i
 
 constexpr int kScanBlock = 1024;
 
-__global__ void __launch_bounds__(kScanBlock) k_scan(DevParams P, DevBuffers B, int wave, int grid) {
+__device__ __forceinline__ void scan_body(const DevParams& P, const DevBuffers& B, int wave,
+                                          int grid) {
   DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   __shared__ unsigned long long s_sm[kScanBlock / 64];
@@ -424,15 +431,16 @@ __global__ void __launch_bounds__(kScanBlock) k_scan(DevParams P, DevBuffers B, 
   }
 }
 
-__global__ void __launch_bounds__(kBlock) k_emit(DevParams P, DevBuffers B, int wave) {
+__device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& B, int wave,
+                                          int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   __shared__ unsigned long long s_sm[kBlock / 64];
   const int* list;
   int n;
   wave_input(B, wave, list, n);
-  const int nb = wave_chunks(n, gridDim.x);
-  const int bid = xcd_block();
+  const int nb = wave_chunks(n, nblk);
+  const int bid = blk;
   if (bid >= nb) return;
   int* __restrict__ kids = B.listW[wave & 1];
   int* __restrict__ surv = B.listA[ctr->parity ^ 1];
@@ -476,7 +484,7 @@ __global__ void __launch_bounds__(kBlock) k_emit(DevParams P, DevBuffers B, int 
 // of the step: time_ += dT_, iStep_++, liquid count (:650-659).
 constexpr int kTailBlock = 1024;
 
-__global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) {
+__device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& B) {
   DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   __shared__ unsigned long long s_sm[kTailBlock / 64];
@@ -587,6 +595,89 @@ __global__ void k_envelope_selftest(int n, const double* env4, const double* l2g
 
 // ---------------------------------------------------------------------------
 // launchers
+
+// ---------------------------------------------------------------------------
+// single-domain kernels: the whole grid serves one domain; list kernels give
+// each XCD a contiguous eighth of the list
+
+__global__ void __launch_bounds__(kBlock) k_nucleate(DevParams P, DevBuffers B) {
+  nucleate_body(P, B, blockIdx.x, gridDim.x);
+}
+__global__ void __launch_bounds__(kBlock) k_claim(DevParams P, DevBuffers B, int wave) {
+  claim_body(P, B, wave, xcd_block(), gridDim.x);
+}
+__global__ void __launch_bounds__(kBlock) k_resolve(DevParams P, DevBuffers B, int wave) {
+  resolve_body(P, B, wave, xcd_block(), gridDim.x);
+}
+__global__ void __launch_bounds__(kScanBlock) k_scan(DevParams P, DevBuffers B, int wave, int grid) {
+  scan_body(P, B, wave, grid);
+}
+__global__ void __launch_bounds__(kBlock) k_emit(DevParams P, DevBuffers B, int wave) {
+  emit_body(P, B, wave, xcd_block(), gridDim.x);
+}
+__global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) { tail_body(P, B); }
+
+// ---------------------------------------------------------------------------
+// batched kernels: block b serves domain map[b].x as its block map[b].y of
+// map[b].z (the per-domain launch geometry of the single-domain kernels)
+
+__global__ void __launch_bounds__(kBlock) kb_nucleate(const DevParams* Ps, const DevBuffers* Bs,
+                                                      const int4* map) {
+  const int4 m = map[blockIdx.x];
+  nucleate_body(Ps[m.x], Bs[m.x], m.y, m.z);
+}
+__global__ void __launch_bounds__(kBlock) kb_claim(const DevParams* Ps, const DevBuffers* Bs,
+                                                   const int4* map, int wave) {
+  const int4 m = map[blockIdx.x];
+  claim_body(Ps[m.x], Bs[m.x], wave, m.y, m.z);
+}
+__global__ void __launch_bounds__(kBlock) kb_resolve(const DevParams* Ps, const DevBuffers* Bs,
+                                                     const int4* map, int wave) {
+  const int4 m = map[blockIdx.x];
+  resolve_body(Ps[m.x], Bs[m.x], wave, m.y, m.z);
+}
+__global__ void __launch_bounds__(kScanBlock) kb_scan(const DevParams* Ps, const DevBuffers* Bs,
+                                                      const int4* map, int wave) {
+  const int4 m = map[blockIdx.x];
+  scan_body(Ps[m.x], Bs[m.x], wave, m.z);
+}
+__global__ void __launch_bounds__(kBlock) kb_emit(const DevParams* Ps, const DevBuffers* Bs,
+                                                  const int4* map, int wave) {
+  const int4 m = map[blockIdx.x];
+  emit_body(Ps[m.x], Bs[m.x], wave, m.y, m.z);
+}
+__global__ void __launch_bounds__(kTailBlock) kb_tail(const DevParams* Ps, const DevBuffers* Bs,
+                                                      const int4* map) {
+  const int4 m = map[blockIdx.x];
+  tail_body(Ps[m.x], Bs[m.x]);
+}
+
+// per-domain status after a batch of steps: (finished, error, steps low, steps high)
+__global__ void kb_status(const DevParams* Ps, const DevBuffers* Bs, int n, int4* out) {
+  const int d = blockIdx.x * blockDim.x + threadIdx.x;
+  if (d >= n) return;
+  const DevCounters* c = Bs[d].ctr;
+  const int fin = (c->done || !(c->t < Ps[d].final_time)) ? 1 : 0;
+  const unsigned long long st = (unsigned long long)c->step;
+  out[d] = make_int4(fin, c->err, (int)(st & 0x7fffffffull), (int)(st >> 31));
+}
+
+hipError_t launch_batch_status(const BatchLaunch& L, int n, int4* out, hipStream_t s) {
+  hipLaunchKernelGGL(kb_status, dim3((n + 63) / 64), dim3(64), 0, s, L.P, L.B, n, out);
+  return hipGetLastError();
+}
+
+hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
+  hipLaunchKernelGGL(kb_nucleate, dim3(L.n_nuc), dim3(kBlock), 0, s, L.P, L.B, L.map_nuc);
+  for (int wave = 0; wave < 2; wave++) {
+    hipLaunchKernelGGL(kb_claim, dim3(L.n_list), dim3(kBlock), 0, s, L.P, L.B, L.map_list, wave);
+    hipLaunchKernelGGL(kb_resolve, dim3(L.n_emit), dim3(kBlock), 0, s, L.P, L.B, L.map_emit, wave);
+    hipLaunchKernelGGL(kb_scan, dim3(L.n_one), dim3(kScanBlock), 0, s, L.P, L.B, L.map_scan, wave);
+    hipLaunchKernelGGL(kb_emit, dim3(L.n_emit), dim3(kBlock), 0, s, L.P, L.B, L.map_emit, wave);
+  }
+  hipLaunchKernelGGL(kb_tail, dim3(L.n_one), dim3(kTailBlock), 0, s, L.P, L.B, L.map_scan);
+  return hipGetLastError();
+}
 
 hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t s) {
   long long blocks = (P.NG + 255) / 256;

@@ -161,6 +161,38 @@ int32_t amca_kernel_count(void);
 const char* amca_kernel_name(int32_t i);
 amca_status amca_domain_kernel_times(amca_domain* d, double* ms, int64_t* launches);
 
+/* ---- batches of plain domains ------------------------------------------------
+ * Many independent domains (own sizes, seeds, cooling rates, site densities)
+ * stepped together: one launch per kernel for the whole batch, captured in a
+ * HIP graph of `steps_per_graph` steps and replayed until every domain has
+ * finished.  No counterpart in the reference, which runs one realm per
+ * process; each domain's results equal its own amca_domain_run.  Domains stay
+ * owned by the caller (create, sites and orientation rows as usual, then
+ * batch them; read them back with amca_domain_download afterwards). */
+typedef struct amca_batch amca_batch;
+
+typedef struct {
+  int32_t list_blocks;      /* claim blocks, shared in proportion to domain cell counts */
+  int32_t emit_blocks;      /* resolve / emit blocks, same sharing */
+  int32_t min_blocks;       /* at least this many per domain */
+  int32_t interleave;       /* 0: blocks domain by domain (largest first), 1: round robin */
+  int32_t steps_per_graph;  /* steps captured in one graph replay */
+} amca_batch_schedule;
+
+amca_status amca_batch_create(amca_domain* const* domains, int32_t n, amca_batch** out);
+amca_status amca_batch_destroy(amca_batch* b);
+amca_status amca_batch_set_schedule(amca_batch* b, const amca_batch_schedule* s);
+amca_status amca_batch_get_schedule(amca_batch* b, amca_batch_schedule* s);
+/* Replays until all domains finished (max_graphs < 0) or max_graphs replays;
+ * returns the finished-domain count, device milliseconds and replays. */
+amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finished, double* ms,
+                           int64_t* replays);
+/* Tries the candidate schedules on the running batch (trial_graphs replays
+ * each, `rounds` rounds) and keeps the fastest; the trial steps are ordinary
+ * steps of the run. */
+amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rounds,
+                                amca_batch_schedule* chosen);
+
 /* ---- device arithmetic self-check ---------------------------------------- */
 
 /* Runs the device capture test (Octahedron::cell_is_captured, Octahedron.cpp:98-128)
