@@ -193,6 +193,38 @@ amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finishe
 amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rounds,
                                 amca_batch_schedule* chosen);
 
+/* ---- temperature surrogate (MLP) ----------------------------------------------
+ * A 5-128-128-128-128-1 network with SiLU on the hidden layers: inputs
+ * (x - v t, y, z, P, v) normalised by the caller, output the normalised
+ * temperature.  BF16 operands on the matrix cores, FP32 accumulation, FP32
+ * master weights trained with Adam on the mean squared error.  No counterpart
+ * in the reference, whose temperatures come only from FE frames.  The
+ * parameter vector is W1[5][128] b1[128] W2..W4[128][128] b2..b4[128]
+ * W5[128] b5, weights stored [fan_in][fan_out]. */
+typedef struct amca_mlp amca_mlp;
+int32_t amca_mlp_param_count(void);
+amca_status amca_mlp_create(uint64_t seed, amca_mlp** out);
+amca_status amca_mlp_destroy(amca_mlp* m);
+amca_status amca_mlp_set_params(amca_mlp* m, const float* params);
+amca_status amca_mlp_get_params(amca_mlp* m, float* params);
+/* Host rows x5[n][5] -> t[n]. */
+amca_status amca_mlp_forward(amca_mlp* m, int32_t n, const float* x5, float* t);
+/* Gradient of mean((mlp(x) - y)^2) over n host rows; loss = that mean. */
+amca_status amca_mlp_gradients(amca_mlp* m, int32_t n, const float* x5, const float* y,
+                               float* grads, float* loss);
+/* `steps` Adam steps over random contiguous batches of device-resident samples
+ * (dX[n][5], dY[n], pre-shuffled); optional per-step loss trace (host). */
+amca_status amca_mlp_train_device(amca_mlp* m, int64_t n, const float* dX, const float* dY,
+                                  int32_t batch, int32_t steps, float lr, uint64_t seed,
+                                  float* loss_trace);
+/* Device rows dX[n][5] -> dT[n]. */
+amca_status amca_mlp_forward_device(amca_mlp* m, int64_t n, const float* dX, float* dT);
+/* The tiled BF16 GEMM alone: C[M][N] = opA(A) opB(B) with FP32 host matrices
+ * (row-major; trans_a: A is K x M, trans_b: B is N x K). */
+amca_status amca_mlp_gemm_selftest(int32_t M, int32_t N, int32_t K, int32_t trans_a,
+                                   int32_t trans_b, const float* A, const float* B, float* C);
+const char* amca_mlp_last_error(void);
+
 /* ---- device arithmetic self-check ---------------------------------------- */
 
 /* Runs the device capture test (Octahedron::cell_is_captured, Octahedron.cpp:98-128)

@@ -1,0 +1,76 @@
+"""Surrogate network kernels on the GPU against the CPU oracle: the BF16 MFMA
+GEMM in every transpose form, the fused forward, and the backward pass."""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as mo  # noqa: E402
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("ta,tb", [(False, False), (True, False), (False, True), (True, True)])
+@pytest.mark.parametrize("M,N,K", [(64, 32, 16), (200, 130, 70), (5, 128, 333), (257, 1, 128)])
+def test_gemm_matches_bf16_reference(amca, ta, tb, M, N, K):
+    from amca3d_amd.surrogate import gemm_selftest
+    rng = np.random.default_rng(M * 7 + N * 3 + K)
+    A = rng.uniform(-1, 1, (K, M) if ta else (M, K))
+    B = rng.uniform(-1, 1, (N, K) if tb else (K, N))
+    got = gemm_selftest(A, B, ta, tb)
+    a = mo.bf16(A)
+    b = mo.bf16(B)
+    ref = (a.T if ta else a) @ (b.T if tb else b)
+    # FP32 accumulation of exact BF16 products
+    np.testing.assert_allclose(got, ref, rtol=0, atol=1e-5 * K + 1e-6)
+
+
+def _params(seed):
+    rng = np.random.default_rng(seed)
+    off, n = mo.layout()
+    p = np.empty(n)
+    for (w, b), fi, fo in zip(off, mo.FAN_IN, mo.FAN_OUT):
+        a = np.sqrt(6.0 / fi)
+        p[w:b] = rng.uniform(-a, a, fi * fo)
+        p[b:b + fo] = rng.uniform(-0.1, 0.1, fo)
+    return p.astype(np.float32)
+
+
+def test_forward_matches_oracle(amca):
+    from amca3d_amd.surrogate import MLP
+    m = MLP(1)
+    p = _params(5)
+    m.set_params(p)
+    rng = np.random.default_rng(2)
+    X = rng.uniform(-1, 1, (1000, 5)).astype(np.float32)
+    t = m.forward(X)
+    ref, _ = mo.forward(p.astype(np.float64), X)
+    # the oracle rounds at the same points; FP32 accumulation order and the
+    # occasional BF16 tie broken differently remain
+    np.testing.assert_allclose(t, ref, rtol=0, atol=2e-2 * (np.abs(ref).max() + 1))
+    assert np.corrcoef(t, ref)[0, 1] > 0.9999
+    m.close()
+
+
+def test_gradients_match_oracle(amca):
+    from amca3d_amd.surrogate import MLP
+    m = MLP(1)
+    p = _params(7)
+    m.set_params(p)
+    rng = np.random.default_rng(4)
+    X = rng.uniform(-1, 1, (777, 5)).astype(np.float32)
+    Y = rng.uniform(-1, 1, 777).astype(np.float32)
+    g, loss = m.gradients(X, Y)
+    ref, rloss = mo.gradients(p.astype(np.float64), X, Y)
+    assert loss == pytest.approx(rloss, rel=1e-3)
+    off, _ = mo.layout()
+    for l, ((w, b), fo) in enumerate(zip(off, mo.FAN_OUT)):
+        for lo, hi, what in ((w, b, "W"), (b, b + fo, "b")):
+            err = np.linalg.norm(g[lo:hi] - ref[lo:hi])
+            assert err <= 2e-2 * np.linalg.norm(ref[lo:hi]) + 1e-7, (l, what, err)
+    # and against the exact float64 gradient, to BF16 accuracy
+    g64, _ = mo.gradients64(p.astype(np.float64), X, Y)
+    assert np.linalg.norm(g - g64) < 5e-2 * np.linalg.norm(g64)
+    m.close()

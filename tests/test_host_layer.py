@@ -12,6 +12,7 @@ from amca3d_amd import decks
 from amca3d_amd._lib import CLI_PATH, LIB_PATH, SIGNATURES, AmcaError
 from amca3d_amd.batch import SIGNATURES as BATCH_SIGNATURES
 from amca3d_amd.remelt import SIGNATURES as RM_SIGNATURES
+from amca3d_amd.surrogate import SIGNATURES as MLP_SIGNATURES
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
@@ -25,7 +26,7 @@ def test_library_exports_every_declared_symbol(amca):
     lib = amca.load()
     for n in names:
         assert hasattr(lib, n), n
-        assert n in SIGNATURES or n in RM_SIGNATURES or n in BATCH_SIGNATURES, n
+        assert any(n in t for t in (SIGNATURES, RM_SIGNATURES, BATCH_SIGNATURES, MLP_SIGNATURES)), n
     out = subprocess.run(["nm", "-D", "--defined-only", LIB_PATH], capture_output=True, text=True).stdout
     exported = set(re.findall(r" T (amca_\w+)", out))
     assert names <= exported
