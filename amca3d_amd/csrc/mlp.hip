@@ -235,6 +235,10 @@ struct amca_mlp {
   float* dT = nullptr;           // [cap]
   float* dZ[2] = {};             // ping-pong [cap][128]
   float* loss = nullptr;
+  // device-resident training set (amca_surrogate_sample)
+  float* setX = nullptr;
+  float* setY = nullptr;
+  long long set_n = 0;
   std::vector<void*> allocs;
   ~amca_mlp() {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -320,12 +324,12 @@ hipError_t gemm(hipStream_t s, const TA* A, int lda, const TB* B, int ldb, int M
   return hipGetLastError();
 }
 
-// forward of n rows already in h->X; keeps Z and activations for a backward pass
-amca_status forward(amca_mlp* h, int n) {
+// forward of n rows already in h->X; keeps Z (for a backward pass) when keep_z
+amca_status forward(amca_mlp* h, int n, bool keep_z = true) {
   const Layout& L = h->lay;
   hipStream_t s = h->stream;
   for (int l = 0; l < 4; l++) {
-    EpiArgs e{nullptr, kHid, h->params + L.b[l], h->Z[l], h->Aact[l]};
+    EpiArgs e{nullptr, kHid, h->params + L.b[l], keep_z ? h->Z[l] : nullptr, h->Aact[l]};
     if (l == 0)
       MLP_HIP((gemm<float, float, false, false, kEpiBiasSilu>(s, h->X, kIn, h->params + L.w[0], kHid,
                                                                n, kHid, kIn, e)));
@@ -531,7 +535,7 @@ amca_status amca_mlp_forward_device(amca_mlp* h, int64_t n, const float* dX, flo
     const int m = (int)std::min<int64_t>(chunk, n - r);
     MLP_HIP(hipMemcpyAsync(h->X, dX + r * kIn, sizeof(float) * kIn * m, hipMemcpyDeviceToDevice,
                            h->stream));
-    MLP_TRY(forward(h, m));
+    MLP_TRY(forward(h, m, false));
     MLP_HIP(hipMemcpyAsync(dT + r, h->T, sizeof(float) * m, hipMemcpyDeviceToDevice, h->stream));
   }
   MLP_HIP(hipStreamSynchronize(h->stream));
@@ -575,3 +579,234 @@ extern "C" amca_status amca_mlp_gemm_selftest(int32_t M, int32_t N, int32_t K, i
   if (s) (void)hipStreamDestroy(s);
   return st;
 }
+
+// ===========================================================================
+// Training data and lattice evaluation of the temperature surrogate.
+
+#include "surrogate.h"
+
+namespace mlpk {
+
+struct Sampler {
+  amca_surrogate_desc d;
+  amca_scan s[16];
+  int n_scans;
+};
+
+__device__ __forceinline__ unsigned long long splitmix(unsigned long long& st) {
+  unsigned long long z = (st += 0x9e3779b97f4a7c15ull);
+  z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+  z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double unit(unsigned long long& st) {
+  return (double)(splitmix(st) >> 11) * (1.0 / 9007199254740992.0);
+}
+
+__device__ __forceinline__ double rosenthal(const amca_surrogate_desc& d, const amca_scan& s,
+                                            double x, double y, double z, double t) {
+  const double xi = x - (s.x_start + s.v * t);
+  const double dy = y - s.y_c, dz = z - d.z_top;
+  double R = sqrt(xi * xi + dy * dy + dz * dz);
+  if (R < d.r_min) R = d.r_min;
+  return d.T0 + d.eta * s.P / (2.0 * 3.14159265358979323846 * d.k) / R *
+                    exp(-s.v * (R + xi) / (2.0 * d.alpha));
+}
+
+// the coupled domain's FE path at (x, y, z, t): nodal frames of the source at
+// t_f and t_f+1, linear in time (void rule), hex8 in space
+__device__ double fe_temperature(const amca_surrogate_desc& d, const amca_scan& s, double x,
+                                 double y, double z, double t) {
+  int e[3];
+  double p[3];
+  const double o[3] = {d.fx0, d.fy0, d.fz0};
+  const int n[3] = {d.fnx, d.fny, d.fnz};
+  const double q[3] = {x, y, z};
+  for (int a = 0; a < 3; a++) {
+    int i = (int)floor((q[a] - o[a]) / d.fdx);
+    i = i < 0 ? 0 : (i > n[a] - 2 ? n[a] - 2 : i);
+    e[a] = i;
+    p[a] = 2.0 * ((q[a] - (o[a] + i * d.fdx)) / d.fdx) - 1.0;
+  }
+  int f = (int)floor(t / d.frame_dt);
+  f = f < 0 ? 0 : (f > d.n_frames - 2 ? d.n_frames - 2 : f);
+  const double r = (t - f * d.frame_dt) / d.frame_dt;
+  const double xi = p[0], eta = p[1], zeta = p[2];
+  const double sf[8] = {
+      0.125 * (1.0 - eta) * (1.0 - xi) * (1.0 - zeta), 0.125 * (1.0 - eta) * (1.0 + xi) * (1.0 - zeta),
+      0.125 * (1.0 + eta) * (1.0 + xi) * (1.0 - zeta), 0.125 * (1.0 + eta) * (1.0 - xi) * (1.0 - zeta),
+      0.125 * (1.0 - eta) * (1.0 - xi) * (1.0 + zeta), 0.125 * (1.0 - eta) * (1.0 + xi) * (1.0 + zeta),
+      0.125 * (1.0 + eta) * (1.0 + xi) * (1.0 + zeta), 0.125 * (1.0 + eta) * (1.0 - xi) * (1.0 + zeta)};
+  const int di[8] = {0, 1, 1, 0, 0, 1, 1, 0}, dj[8] = {0, 0, 1, 1, 0, 0, 1, 1},
+            dk[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+  double T = 0.0;
+  for (int m = 0; m < 8; m++) {
+    const double nx = d.fx0 + (e[0] + di[m]) * d.fdx, ny = d.fy0 + (e[1] + dj[m]) * d.fdx,
+                 nz = d.fz0 + (e[2] + dk[m]) * d.fdx;
+    const double th1 = rosenthal(d, s, nx, ny, nz, f * d.frame_dt);
+    const double th0 = rosenthal(d, s, nx, ny, nz, (f + 1) * d.frame_dt);
+    const double v = d.void_temperature;
+    const double th = (th0 <= v) ? v : ((th1 <= v) ? th0 : r * (th0 - th1) + th1);
+    T += sf[m] * th;
+  }
+  return T;
+}
+
+__device__ __forceinline__ void inputs(const amca_surrogate_desc& d, const amca_scan& s, double x,
+                                       double y, double z, double t, float* o) {
+  o[0] = (float)((x - s.x_start - s.v * t) / d.xi_scale);
+  o[1] = (float)((y - s.y_c) / d.y_scale);
+  o[2] = (float)((z - d.z_top) / d.z_scale);
+  o[3] = (float)(s.P / d.P_scale);
+  o[4] = (float)(s.v / d.v_scale);
+}
+
+__device__ __host__ inline double to_target(const amca_surrogate_desc& d, double T) {
+  const double u = (fmin(T, d.T_clip) - d.T_ref) / d.T_scale;
+  return d.log_target ? log1p(fmax(u, 0.0)) : u;
+}
+__device__ __host__ inline double from_target(const amca_surrogate_desc& d, double y) {
+  return d.T_ref + d.T_scale * (d.log_target ? expm1(y) : y);
+}
+
+__global__ void k_sample(Sampler S, long long n, double near_fraction, unsigned long long seed,
+                         float* __restrict__ X, float* __restrict__ Y) {
+  const amca_surrogate_desc& d = S.d;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    unsigned long long st = seed ^ ((unsigned long long)i * 0xd1b54a32d192ed03ull);
+    const amca_scan& s = S.s[i % S.n_scans];
+    const double t = unit(st) * (d.n_frames - 1) * d.frame_dt;
+    const double lx = (d.fnx - 1) * d.fdx, ly = (d.fny - 1) * d.fdx, lz = (d.fnz - 1) * d.fdx;
+    double x, y, z;
+    if (unit(st) < near_fraction) {
+      // close behind the source, where the field varies: 40 xi_scale behind,
+      // 4 ahead, 8 y_scale across, 8 z_scale deep
+      x = s.x_start + s.v * t + (unit(st) * 44.0 - 40.0) * d.xi_scale;
+      y = s.y_c + (unit(st) * 2.0 - 1.0) * 8.0 * d.y_scale;
+      z = d.z_top - unit(st) * 8.0 * d.z_scale;
+    } else {
+      x = d.fx0 + unit(st) * lx;
+      y = d.fy0 + unit(st) * ly;
+      z = d.fz0 + unit(st) * lz;
+    }
+    x = fmin(fmax(x, d.fx0), d.fx0 + lx);
+    y = fmin(fmax(y, d.fy0), d.fy0 + ly);
+    z = fmin(fmax(z, d.fz0), d.fz0 + lz);
+    const double T = fe_temperature(d, s, x, y, z, t);
+    inputs(d, s, x, y, z, t, X + 5 * i);
+    Y[i] = (float)to_target(d, T);
+  }
+}
+
+__global__ void k_lattice_inputs(amca_surrogate_desc d, amca_scan s, SurLattice g, long long r0,
+                                 int m, float* __restrict__ X) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  const long long q = r0 + r;
+  const int i = (int)(q % g.nxi);
+  const int j = (int)((q / g.nxi) % g.ny);
+  const int k = (int)(q / ((long long)g.nxi * g.ny));
+  // a lattice point is (xi, y, z) in the moving frame: feed x = xi + x_start at t = 0
+  inputs(d, s, g.xi0 + i * g.dxi + s.x_start, g.y0 + j * g.dy, g.z0 + k * g.dz, 0.0, X + 5 * r);
+}
+
+__global__ void k_denorm(const float* __restrict__ t, int m, amca_surrogate_desc d,
+                         float* __restrict__ out) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= m) return;
+  out[r] = (float)from_target(d, (double)t[r]);
+}
+
+__global__ void k_eval_points(amca_surrogate_desc d, amca_scan s, int n, const double* __restrict__ p,
+                              float* __restrict__ X, double* __restrict__ Tfe) {
+  const int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= n) return;
+  const double x = p[4 * r], y = p[4 * r + 1], z = p[4 * r + 2], t = p[4 * r + 3];
+  inputs(d, s, x, y, z, t, X + 5 * r);
+  Tfe[r] = fe_temperature(d, s, x, y, z, t);
+}
+
+}  // namespace mlpk
+
+const char* surrogate_error() { return g_mlp_error.c_str(); }
+
+amca_status surrogate_eval_lattice(amca_mlp* h, const amca_surrogate_desc& d, const amca_scan& s,
+                                   const SurLattice& g, float* out) {
+  const long long total = (long long)g.nxi * g.ny * g.nz;
+  const int chunk = 1 << 20;
+  MLP_TRY(ensure_cap(h, (int)std::min<long long>(total, chunk)));
+  for (long long r0 = 0; r0 < total; r0 += chunk) {
+    const int m = (int)std::min<long long>(chunk, total - r0);
+    hipLaunchKernelGGL(k_lattice_inputs, dim3((m + 255) / 256), dim3(256), 0, h->stream, d, s, g, r0,
+                       m, h->X);
+    MLP_HIP(hipGetLastError());
+    MLP_TRY(forward(h, m, false));
+    hipLaunchKernelGGL(k_denorm, dim3((m + 255) / 256), dim3(256), 0, h->stream, h->T, m, d,
+                       out + r0);
+    MLP_HIP(hipGetLastError());
+  }
+  MLP_HIP(hipStreamSynchronize(h->stream));
+  return AMCA_OK;
+}
+
+extern "C" {
+
+amca_status amca_surrogate_sample(amca_mlp* h, const amca_surrogate_desc* d, const amca_scan* scans,
+                                  int32_t n_scans, int64_t n, double near_fraction, uint64_t seed) {
+  if (!h || !d || !scans || n_scans < 1 || n_scans > 16 || n < 1)
+    return mlp_fail(AMCA_ERR_INVALID, "bad arguments (1..16 scans)");
+  if (d->fnx < 2 || d->fny < 2 || d->fnz < 2 || d->n_frames < 2 || !(d->fdx > 0) ||
+      !(d->frame_dt > 0) || !(d->xi_scale > 0) || !(d->T_scale > 0))
+    return mlp_fail(AMCA_ERR_INVALID, "bad surrogate description");
+  MLP_HIP(hipStreamSynchronize(h->stream));
+  mlp_release(h, h->setX);
+  mlp_release(h, h->setY);
+  h->setX = h->setY = nullptr;
+  h->set_n = 0;
+  MLP_TRY(mlp_alloc(h, &h->setX, (size_t)n * kIn));
+  MLP_TRY(mlp_alloc(h, &h->setY, (size_t)n));
+  Sampler S;
+  S.d = *d;
+  S.n_scans = n_scans;
+  for (int i = 0; i < n_scans; i++) S.s[i] = scans[i];
+  const long long blocks = std::min<long long>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_sample, dim3((unsigned)blocks), dim3(256), 0, h->stream, S, (long long)n,
+                     near_fraction, (unsigned long long)seed, h->setX, h->setY);
+  MLP_HIP(hipGetLastError());
+  MLP_HIP(hipStreamSynchronize(h->stream));
+  h->set_n = n;
+  return AMCA_OK;
+}
+
+amca_status amca_surrogate_train(amca_mlp* h, int32_t batch, int32_t steps, float lr, uint64_t seed,
+                                 float* loss_trace) {
+  if (!h || !h->set_n) return mlp_fail(AMCA_ERR_INVALID, "no training set: call amca_surrogate_sample");
+  return amca_mlp_train_device(h, h->set_n, h->setX, h->setY, batch, steps, lr, seed, loss_trace);
+}
+
+amca_status amca_surrogate_eval(amca_mlp* h, const amca_surrogate_desc* d, const amca_scan* s,
+                                int32_t n, const double* xyzt, double* Ts, double* Tfe) {
+  if (!h || !d || !s || n < 1 || !xyzt || !Ts || !Tfe) return mlp_fail(AMCA_ERR_INVALID, "bad arguments");
+  MLP_TRY(ensure_cap(h, n));
+  double *dp = nullptr, *dfe = nullptr;
+  MLP_HIP(hipMalloc(&dp, sizeof(double) * 4 * n));
+  MLP_HIP(hipMalloc(&dfe, sizeof(double) * n));
+  amca_status st = AMCA_OK;
+  std::vector<float> t(n);
+  do {
+    if (hipMemcpy(dp, xyzt, sizeof(double) * 4 * n, hipMemcpyHostToDevice) != hipSuccess) { st = mlp_fail(AMCA_ERR_HIP, "copy"); break; }
+    hipLaunchKernelGGL(k_eval_points, dim3((n + 255) / 256), dim3(256), 0, h->stream, *d, *s, n, dp,
+                       h->X, dfe);
+    if ((st = forward(h, n, false)) != AMCA_OK) break;
+    if (hipMemcpyAsync(t.data(), h->T, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipMemcpyAsync(Tfe, dfe, sizeof(double) * n, hipMemcpyDeviceToHost, h->stream) != hipSuccess ||
+        hipStreamSynchronize(h->stream) != hipSuccess) { st = mlp_fail(AMCA_ERR_HIP, "readback"); break; }
+    for (int i = 0; i < n; i++) Ts[i] = from_target(*d, (double)t[i]);
+  } while (0);
+  (void)hipFree(dp);
+  (void)hipFree(dfe);
+  return st;
+}
+
+}  // extern "C"

@@ -23,6 +23,7 @@
 #include "host/voxel_map.h"
 #include "plain_step.h"
 #include "remelt_step.h"
+#include "surrogate.h"
 
 namespace {
 
@@ -64,6 +65,7 @@ struct amca_remelt {
   size_t grain_cap = 0;  // records genv/gco can hold
   std::vector<int32_t> host_tag;
   std::vector<double> host_par;
+  std::vector<double> node_xyz;
   // per-kernel event timing (off by default)
   bool timing = false;
   std::vector<hipEvent_t> ev_pool;
@@ -347,6 +349,7 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   mesh.fnz = fnz;
   const size_t nodes = (size_t)fnx * fny * fnz;
   mesh.xyz.assign(node_xyz, node_xyz + 3 * nodes);
+  d->node_xyz = mesh.xyz;
   amca_host::VoxelMapParams vp;
   vp.nx = P.nx; vp.ny = P.ny; vp.nz = P.nz;
   vp.h = P.h; vp.x0 = P.x0; vp.y0 = P.y0; vp.z0 = P.z0;
@@ -648,6 +651,7 @@ amca_status amca_remelt_read_log(amca_remelt* d, int32_t cap, amca_remelt_step_r
       rows[i].n_captured = tmp[i].n_captured;
       rows[i].n_fresh = tmp[i].n_fresh;
       rows[i].n_flagged = tmp[i].n_flagged;
+      rows[i].n_fallback = tmp[i].n_fallback;
     }
   }
   return AMCA_OK;
@@ -712,6 +716,96 @@ amca_status amca_remelt_download(amca_remelt* d, int32_t* orient, int32_t* grain
       }
     }
   }
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_use_surrogate(amca_remelt* d, amca_mlp* m, const amca_surrogate_desc* sd,
+                                      const amca_scan* scan, double band, double resid) {
+  if (!d || !d->fe_ready) return rm_fail(AMCA_ERR_INVALID, "FE mesh not set");
+  if (!m || !sd || !scan || !(band >= 0) || !(resid >= 0))
+    return rm_fail(AMCA_ERR_INVALID, "bad surrogate arguments");
+  amca::RmParams& P = d->P;
+  if (sd->fnx != P.fnx || sd->fny != P.fny || sd->fnz != P.fnz)
+    return rm_fail(AMCA_ERR_INVALID, "surrogate mesh differs from the domain's FE mesh");
+  // the FE mesh must be the uniform lattice the description names
+  for (int n = 0; n < P.nodes; n++) {
+    const int i = n % P.fnx, j = (n / P.fnx) % P.fny, k = n / (P.fnx * P.fny);
+    const double e[3] = {sd->fx0 + i * sd->fdx, sd->fy0 + j * sd->fdx, sd->fz0 + k * sd->fdx};
+    for (int q = 0; q < 3; q++)
+      if (std::fabs(d->node_xyz[3 * (size_t)n + q] - e[q]) > 1e-9 * sd->fdx)
+        return rm_fail(AMCA_ERR_INVALID, "the FE mesh is not the uniform lattice of the description");
+  }
+  RM_HIP(hipStreamSynchronize(d->stream));
+  // time span of the frames
+  std::vector<double> ft(P.n_frames);
+  RM_HIP(hipMemcpy(ft.data(), d->B.frame_time, sizeof(double) * P.n_frames, hipMemcpyDeviceToHost));
+  const double tmax = ft.back();
+  const double h = P.h;
+  const double xlo = std::min(P.x0 - 1.0 * h, sd->fx0), xhi = std::max(P.x0 + (P.nx + 1) * h, sd->fx0 + (P.fnx - 1) * sd->fdx);
+  const double a = xlo - scan->x_start - std::max(0.0, scan->v) * tmax - 2 * h;
+  const double b = xhi - scan->x_start - std::min(0.0, scan->v) * tmax + 2 * h;
+  const int nxi = (int)std::ceil((b - a) / h) + 2;
+  d->release(const_cast<float*>(d->B.sur_cell));
+  d->release(const_cast<float*>(d->B.sur_node));
+  d->release(d->B.node_bad);
+  d->release(d->B.elem_fb);
+  d->release(d->B.sur_stats);
+  d->release(d->B.fb_slots);
+  float *tc = nullptr, *tn = nullptr;
+  unsigned char *nb = nullptr, *eb = nullptr;
+  double* stats = nullptr;
+  RM_TRY(d->alloc(&tc, (size_t)nxi * P.nyG * P.nzG));
+  RM_TRY(d->alloc(&tn, (size_t)nxi * P.fny * P.fnz));
+  RM_TRY(d->alloc(&nb, (size_t)P.nodes));
+  RM_TRY(d->alloc(&eb, (size_t)(P.fnx - 1) * (P.fny - 1) * (P.fnz - 1)));
+  RM_TRY(d->alloc(&stats, 8));
+  int* slots = nullptr;
+  RM_TRY(d->alloc(&slots, 256));
+  RM_HIP(hipMemset(slots, 0, sizeof(int) * 256));
+  // cell lattice: y, z at the cell centres (ghosts included); node lattice at the nodes
+  SurLattice gc{a, h, nxi, P.y0 - 0.5 * h, h, P.nyG, P.z0 - 0.5 * h, h, P.nzG};
+  SurLattice gn{a, h, nxi, sd->fy0, sd->fdx, P.fny, sd->fz0, sd->fdx, P.fnz};
+  amca_status st = surrogate_eval_lattice(m, *sd, *scan, gc, tc);
+  if (st == AMCA_OK) st = surrogate_eval_lattice(m, *sd, *scan, gn, tn);
+  if (st != AMCA_OK) return rm_fail(st, std::string("surrogate evaluation: ") + surrogate_error());
+  d->B.sur_cell = tc;
+  d->B.sur_node = tn;
+  d->B.node_bad = nb;
+  d->B.elem_fb = eb;
+  d->B.sur_stats = stats;
+  d->B.fb_slots = slots;
+  P.sur = 1;
+  P.sur_xi0 = a;
+  P.sur_dxi = h;
+  P.sur_nxi = nxi;
+  P.sur_nxi_nodes = nxi;
+  P.sur_x_start = scan->x_start;
+  P.sur_v = scan->v;
+  P.sur_band = band;
+  P.sur_resid = resid;
+  P.fx0 = sd->fx0;
+  P.fy0 = sd->fy0;
+  P.fz0 = sd->fz0;
+  P.fdx = sd->fdx;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6) {
+  if (!d || !out6 || !d->P.sur) return rm_fail(AMCA_ERR_INVALID, "no surrogate in use");
+  RM_HIP(hipMemsetAsync(d->B.sur_stats, 0, sizeof(double) * 8, d->stream));
+  // fallback flags for the current time, as the next step's phase would see them
+  RM_HIP(amca::rm_launch_kernel(0, d->P, d->B, d->site_blocks, d->stream));
+  RM_HIP(amca::rm_launch_kernel(1, d->P, d->B, d->site_blocks, d->stream));
+  RM_HIP(amca::rm_launch_sur_error(d->P, d->B, d->stream));
+  double s[8];
+  RM_HIP(hipMemcpyAsync(s, d->B.sur_stats, sizeof(s), hipMemcpyDeviceToHost, d->stream));
+  RM_HIP(hipStreamSynchronize(d->stream));
+  out6[0] = s[1] > 0 ? std::sqrt(s[0] / s[1]) : 0.0;  // RMS over all cells with material
+  out6[1] = s[1];
+  out6[2] = s[3] > 0 ? std::sqrt(s[2] / s[3]) : 0.0;  // RMS over the cells the surrogate serves
+  out6[3] = s[3];
+  out6[4] = s[4];                                     // max |error| over those cells
+  out6[5] = s[5];                                     // cells on the FE path
   return AMCA_OK;
 }
 

@@ -18,6 +18,13 @@ struct RmParams {
   int fnx, fny, fnz, nodes, ncx, ncy;    // structured FE mesh
   int n_frames;
   int orient_cap;
+  // temperature surrogate (sur = 1): moving-frame tables on a xi lattice of
+  // spacing sur_dxi from sur_xi0; FE fallback within sur_band of Tm and in
+  // elements whose nodes disagree with the surrogate by more than sur_resid
+  int sur;
+  double sur_xi0, sur_dxi, sur_x_start, sur_v, sur_band, sur_resid;
+  int sur_nxi, sur_nxi_nodes;
+  double fx0, fy0, fz0, fdx;             // uniform node lattice of the structured mesh
 };
 
 struct RmCounters {
@@ -36,12 +43,14 @@ struct RmCounters {
   int step_nuc, step_cap, step_fresh, n_flagged;
   int err;
   int log_cap;
+  int n_fallback;      // cells of the running step that took the FE path
   unsigned tickets[4];
 };
 
 struct RmLogRow {
   double time_before, dt;
   int n_nucleated, n_captured, n_fresh, n_flagged;
+  int n_fallback, pad;
 };
 
 struct RmBuffers {
@@ -70,14 +79,22 @@ struct RmBuffers {
   unsigned long long* lb_pref;
   RmLogRow* log;
   RmCounters* ctr;
+  const float* sur_cell;       // [nzG][nyG][sur_nxi] surrogate temperature at cell (y, z)
+  const float* sur_node;       // [fnz][fny][sur_nxi_nodes] at node (y, z)
+  unsigned char* node_bad;     // [nodes]
+  unsigned char* elem_fb;      // [elements]
+  double* sur_stats;           // [8] error statistics
+  int* fb_slots;               // [256] per-step fallback cell counts by block slot
 };
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s);
 // One step: kernels 0..kRmKernels-1 in order on stream s.
-constexpr int kRmKernels = 11;
+constexpr int kRmKernels = 13;
 extern const char* const kRmKernelNames[kRmKernels];
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s);
 hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s);
+// surrogate error statistics into B.sur_stats (zeroed by the caller)
+hipError_t rm_launch_sur_error(const RmParams& P, const RmBuffers& B, hipStream_t s);
 
 }  // namespace amca

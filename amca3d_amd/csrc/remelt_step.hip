@@ -116,6 +116,20 @@ __device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B,
 
 }  // namespace
 
+// surrogate temperature of cell (i, j, k) at FE time t from the moving-frame
+// table (linear in xi)
+__device__ __forceinline__ double rm_sur_T(const RmParams& P, const RmBuffers& B, int i, int j,
+                                           int k, double t) {
+  const double xi = P.x0 + P.h * (i - 0.5) - P.sur_x_start - P.sur_v * t;
+  double u = (xi - P.sur_xi0) / P.sur_dxi;
+  int i0 = (int)floor(u);
+  i0 = i0 < 0 ? 0 : (i0 > P.sur_nxi - 2 ? P.sur_nxi - 2 : i0);
+  double f = u - i0;
+  f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+  const float* row = B.sur_cell + ((size_t)k * P.nyG + j) * P.sur_nxi;
+  return (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
+}
+
 // ---------------------------------------------------------------------------
 
 __global__ void k_rm_init(RmParams P, RmBuffers B) {
@@ -134,13 +148,33 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
   const int skip = (ctr->tcafe < P.final_time && !ctr->exhausted) ? 0 : 1;
   if (blockIdx.x == 0 && threadIdx.x == 0) ctr->skip = skip;
   if (skip) return;
+  int n_fb = 0;
   const bool later = ctr->step > 0;
   const double dt_prev = ctr->dt;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
     const int c = (int)cc;
     double Tn = -1.0;
-    const bool mat = rm_eval_T(P, B, c, Tn);
+    bool mat;
+    if (!P.sur) {
+      mat = rm_eval_T(P, B, c, Tn);
+    } else {
+      const int e = B.tag[c];
+      bool fe = e < 0 || B.elem_fb[e];
+      if (!fe) {
+        int si, sj, sk;
+        rm_ijk(P, c, si, sj, sk);
+        Tn = rm_sur_T(P, B, si, sj, sk, ctr->time_fe);
+        fe = fabs(Tn - P.Tm) < P.sur_band;
+      }
+      if (fe) {
+        mat = rm_eval_T(P, B, c, Tn);
+        if (!mat) Tn = -1.0;
+        if (e >= 0) n_fb++;
+      } else {
+        mat = true;
+      }
+    }
     if (later) {
       double rate = 410;
       const double Tp = B.T[c];
@@ -157,6 +191,95 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
     B.oid[c] = -1;
     B.melted[c] = mat ? 1 : 0;
   }
+  if (P.sur) {
+    // block sum, one atomic per block into one of 256 slots (summed by k_rm_finish)
+    __shared__ int s_fb[kRmBlock / 64];
+    int v = n_fb;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s_fb[threadIdx.x >> 6] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int t = 0;
+#pragma unroll
+      for (int q = 0; q < kRmBlock / 64; q++) t += s_fb[q];
+      if (t) atomicAdd(&B.fb_slots[blockIdx.x & 255], t);
+    }
+  }
+}
+
+// Surrogate fallback, per step: a node is bad when the surrogate at the node
+// differs from the FE nodal field by more than sur_resid (or the node is
+// void); an element falls back to the FE path when any of its nodes is bad.
+__global__ void __launch_bounds__(kRmBlock) k_rm_sur_nodes(RmParams P, RmBuffers B) {
+  const RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  const double t = ctr->time_fe;
+  const int stride = gridDim.x * blockDim.x;
+  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.nodes; n += stride) {
+    const int ni = n % P.fnx, nj = (n / P.fnx) % P.fny, nk = n / (P.fnx * P.fny);
+    const double xi = P.fx0 + P.fdx * ni - P.sur_x_start - P.sur_v * t;
+    double u = (xi - P.sur_xi0) / P.sur_dxi;
+    int i0 = (int)floor(u);
+    i0 = i0 < 0 ? 0 : (i0 > P.sur_nxi_nodes - 2 ? P.sur_nxi_nodes - 2 : i0);
+    double f = u - i0;
+    f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+    const float* row = B.sur_node + ((size_t)nk * P.fny + nj) * P.sur_nxi_nodes;
+    const double Ts = (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
+    const double th = B.theta[n];
+    B.node_bad[n] = (fabs(Ts - th) > P.sur_resid || th < P.void_temp + 0.00001) ? 1 : 0;
+  }
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_sur_elems(RmParams P, RmBuffers B) {
+  const RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  const int ne = P.ncx * P.ncy * (P.fnz - 1);
+  const int nxy = P.fnx * P.fny;
+  const int stride = gridDim.x * blockDim.x;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
+    const int ei = e % P.ncx, ej = (e / P.ncx) % P.ncy, ek = e / (P.ncx * P.ncy);
+    const int n0 = nxy * ek + P.fnx * ej + ei;
+    const unsigned char* b = B.node_bad;
+    const int bad = b[n0] | b[n0 + 1] | b[n0 + P.fnx] | b[n0 + P.fnx + 1] | b[n0 + nxy] |
+                    b[n0 + nxy + 1] | b[n0 + nxy + P.fnx] | b[n0 + nxy + P.fnx + 1];
+    B.elem_fb[e] = bad ? 1 : 0;
+  }
+}
+
+// Error statistics of the surrogate against the FE path at the current FE
+// time over cells with material: [0] sum sq (all), [1] count, [2] sum sq of
+// the cells the surrogate would serve, [3] their count, [4] their max |err|
+// (as bits of a non-negative double), [5] fallback cells.
+__global__ void __launch_bounds__(kRmBlock) k_rm_sur_error(RmParams P, RmBuffers B) {
+  const double t = B.ctr->time_fe;
+  double s_all = 0.0, n_all = 0.0, s_use = 0.0, n_use = 0.0, mx = 0.0, n_fb = 0.0;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
+    const int c = (int)cc;
+    double Tfe = -1.0;
+    if (!rm_eval_T(P, B, c, Tfe)) continue;
+    int i, j, k;
+    rm_ijk(P, c, i, j, k);
+    const double Ts = rm_sur_T(P, B, i, j, k, t);
+    const double d = Ts - Tfe;
+    s_all += d * d;
+    n_all += 1.0;
+    const bool fb = B.elem_fb[B.tag[c]] || fabs(Ts - P.Tm) < P.sur_band;
+    if (fb) {
+      n_fb += 1.0;
+    } else {
+      s_use += d * d;
+      n_use += 1.0;
+      mx = fmax(mx, fabs(d));
+    }
+  }
+  atomicAdd(&B.sur_stats[0], s_all);
+  atomicAdd(&B.sur_stats[1], n_all);
+  atomicAdd(&B.sur_stats[2], s_use);
+  atomicAdd(&B.sur_stats[3], n_use);
+  atomicMax((unsigned long long*)&B.sur_stats[4], (unsigned long long)__double_as_longlong(mx));
+  atomicAdd(&B.sur_stats[5], n_fb);
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B) {
@@ -496,9 +619,16 @@ __global__ void k_rm_finish(RmParams P, RmBuffers B) {
     row.n_captured = ctr->step_cap;
     row.n_fresh = ctr->step_fresh;
     row.n_flagged = ctr->n_flagged;
+    row.n_fallback = 0;
+    if (B.fb_slots)
+      for (int q = 0; q < 256; q++) row.n_fallback += B.fb_slots[q];
+    row.pad = 0;
     B.log[st] = row;
   }
+  if (B.fb_slots)
+    for (int q = 0; q < 256; q++) B.fb_slots[q] = 0;
   ctr->n_flagged = 0;
+  ctr->n_fallback = 0;
   ctr->min_t_key = ~0ull;
   // FiniteElementManager::pseudo_single_integration_step
   double tf = ctr->time_fe;
@@ -553,8 +683,9 @@ hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s) 
 }
 
 const char* const kRmKernelNames[kRmKernels] = {
-    "k_rm_phase", "k_rm_liquid", "k_rm_nucleate", "k_rm_mint",   "k_rm_dt",   "k_rm_claim",
-    "k_rm_resolve", "k_rm_rowscan", "k_rm_emit", "k_rm_finish", "k_rm_nodal"};
+    "k_rm_sur_nodes", "k_rm_sur_elems", "k_rm_phase",  "k_rm_liquid", "k_rm_nucleate",
+    "k_rm_mint",      "k_rm_dt",        "k_rm_claim",  "k_rm_resolve", "k_rm_rowscan",
+    "k_rm_emit",      "k_rm_finish",    "k_rm_nodal"};
 
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s) {
@@ -566,19 +697,34 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
   if (nbk > 8192) nbk = 8192;
   if (nbk < 1) nbk = 1;
   switch (q) {
-    case 0: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 1: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 2: hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B); break;
-    case 3: hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 4: hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B); break;
-    case 5: hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
-    case 6: hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
-    case 7: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
-    case 8: hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
-    case 9: hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B); break;
-    case 10: hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B); break;
+    case 0:
+      if (!P.sur) return hipSuccess;
+      hipLaunchKernelGGL(k_rm_sur_nodes, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+      break;
+    case 1:
+      if (!P.sur) return hipSuccess;
+      hipLaunchKernelGGL(k_rm_sur_elems, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+      break;
+    case 2: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 3: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 4: hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B); break;
+    case 5: hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 6: hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B); break;
+    case 7: hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 8: hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 9: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
+    case 10: hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 11: hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B); break;
+    case 12: hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B); break;
     default: return hipErrorInvalidValue;
   }
+  return hipGetLastError();
+}
+
+hipError_t rm_launch_sur_error(const RmParams& P, const RmBuffers& B, hipStream_t s) {
+  long long cb = (P.NG + kRmBlock - 1) / kRmBlock;
+  if (cb > 4096) cb = 4096;
+  hipLaunchKernelGGL(k_rm_sur_error, dim3((unsigned)cb), dim3(kRmBlock), 0, s, P, B);
   return hipGetLastError();
 }
 

@@ -49,7 +49,71 @@ def track(nx=1000, ny=400, nz=200, h=2.5e-6, node_spacing=10e-6, n_frames=100, f
                 final_time=final_time, void_temperature=void_temperature, bin=(5, 5, 5),
                 tolerance=1e-16, max_iterations=20, initial_melted=0)
     return dict(grid=grid, fe=fe, desc=desc, state=substrate(grid, n_grains, substrate_seed),
-                nuc_seed=nuc_seed, bulk=bulk, surface=surface)
+                nuc_seed=nuc_seed, bulk=bulk, surface=surface,
+                physics=dict(P=P, eta=eta, v=v, k=k, alpha=alpha, T0=T0, x_start=x_start))
+
+
+def surrogate_desc(w, xi_scale=1e-4, T_clip=2600.0, log_target=1, T_scale=100.0):
+    """Surrogate description of workload `w` (mesh box, frames, source physics,
+    normalisation)."""
+    from .surrogate import SurrogateDesc
+    fe, g, ph = w["fe"], w["grid"], w["physics"]
+    ax = fe["axes"]
+    return SurrogateDesc(fnx=fe["shape"][0], fny=fe["shape"][1], fnz=fe["shape"][2],
+                         fx0=ax[0][0], fy0=ax[1][0], fz0=ax[2][0], fdx=ax[0][1] - ax[0][0],
+                         n_frames=len(fe["times"]), frame_dt=fe["times"][1] - fe["times"][0],
+                         eta=ph["eta"], k=ph["k"], alpha=ph["alpha"], T0=ph["T0"],
+                         z_top=g["nz"] * g["h"], r_min=g["h"],
+                         void_temperature=w["desc"]["void_temperature"],
+                         xi_scale=xi_scale, y_scale=xi_scale, z_scale=xi_scale, P_scale=300.0,
+                         v_scale=1.5, T_ref=ph["T0"], T_scale=T_scale, T_clip=T_clip,
+                         log_target=log_target)
+
+
+def scan(w):
+    from .surrogate import Scan
+    ph = w["physics"]
+    return Scan(P=ph["P"], v=ph["v"], x_start=ph["x_start"], y_c=0.5 * w["grid"]["ny"] * w["grid"]["h"])
+
+
+def train_surrogate(w, n_samples=20_000_000, seed=3, schedule=((8192, 6000, 1e-3),
+                                                               (8192, 3000, 2e-4),
+                                                               (16384, 2000, 5e-5))):
+    """A surrogate trained on the GPU on samples of the FE field over the six
+    held-in scans; returns (network, description, per-stage final losses)."""
+    from .surrogate import MLP
+    desc = surrogate_desc(w)
+    m = MLP(seed=seed)
+    m.sample(desc, held_in_scans(w), int(n_samples), near_fraction=0.6, seed=seed + 2)
+    losses = []
+    for q, (batch, steps, lr) in enumerate(schedule):
+        tr = m.train(batch=batch, steps=steps, lr=lr, seed=seed + 10 + q)
+        losses.append(float(tr[-50:].mean()))
+    return m, desc, losses
+
+
+def held_out_scans(w):
+    """The workload's own track and eight variants (power, speed, hatch offset)."""
+    from .surrogate import Scan
+    g = w["grid"]
+    yc = 0.5 * g["ny"] * g["h"]
+    ph = w["physics"]
+    v = [(ph["P"], ph["v"], 0.0)] + [(P, s, dy) for P, s, dy in
+                                     [(160.0, 0.7, 0.0), (200.0, 0.5, 3e-5), (210.0, 1.4, -3e-5),
+                                      (240.0, 1.0, 6e-5), (280.0, 0.9, 0.0), (300.0, 1.2, -6e-5),
+                                      (175.0, 1.1, 3e-5), (290.0, 0.6, 0.0)]]
+    return [Scan(P=P, v=s, x_start=ph["x_start"], y_c=yc + dy) for P, s, dy in v]
+
+
+def held_in_scans(w):
+    """Six training scans around the workload's track (power, speed, hatch offset)."""
+    from .surrogate import Scan
+    g = w["grid"]
+    yc = 0.5 * g["ny"] * g["h"]
+    x0 = w["physics"]["x_start"]
+    return [Scan(P=P, v=v, x_start=x0, y_c=yc + dy) for P, v, dy in
+            [(150.0, 0.6, 0.0), (180.0, 1.3, 2e-5), (230.0, 0.8, -2e-5), (270.0, 1.1, 0.0),
+             (300.0, 1.5, 1e-5), (250.0, 0.5, -1e-5)]]
 
 
 def domain(w, orient_capacity=1 << 20):

@@ -35,7 +35,8 @@ class RemeltCounters(C.Structure):
 
 class RemeltStepRecord(C.Structure):
     _fields_ = [("time_before", C.c_double), ("dt", C.c_double), ("n_nucleated", C.c_int32),
-                ("n_captured", C.c_int32), ("n_fresh", C.c_int32), ("n_flagged", C.c_int32)]
+                ("n_captured", C.c_int32), ("n_fresh", C.c_int32), ("n_flagged", C.c_int32),
+                ("n_fallback", C.c_int32)]
 
 
 ORIENT_SOURCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, F64P, F64P)
@@ -65,6 +66,9 @@ SIGNATURES = {
                                          C.c_double, C.c_double, C.c_double, C.c_double, I32P]),
     "amca_remelt_euler": (C.c_int, [C.c_void_p, C.c_int32, F64P, I32P]),
     "amca_remelt_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_remelt_use_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                            C.c_double, C.c_double]),
+    "amca_remelt_surrogate_error": (C.c_int, [C.c_void_p, F64P]),
     "amca_remelt_kernel_count": (C.c_int32, []),
     "amca_remelt_kernel_name": (C.c_char_p, [C.c_int32]),
     "amca_remelt_kernel_times": (C.c_int, [C.c_void_p, F64P, C.POINTER(C.c_int64)]),
@@ -235,10 +239,11 @@ class RemeltDomain:
         m = int(n.value)
         rows = (RemeltStepRecord * max(m, 1))()
         _check(lib().amca_remelt_read_log(self.h, m, rows, C.byref(n)), "read_log")
-        return np.array([(r.time_before, r.dt, r.n_nucleated, r.n_captured, r.n_fresh, r.n_flagged)
-                         for r in rows[:m]],
+        return np.array([(r.time_before, r.dt, r.n_nucleated, r.n_captured, r.n_fresh, r.n_flagged,
+                          r.n_fallback) for r in rows[:m]],
                         dtype=[("time_before", "f8"), ("dt", "f8"), ("n_nucleated", "i4"),
-                               ("n_captured", "i4"), ("n_fresh", "i4"), ("n_flagged", "i4")])
+                               ("n_captured", "i4"), ("n_fresh", "i4"), ("n_flagged", "i4"),
+                               ("n_fallback", "i4")])
 
     def run(self, max_steps=-1):
         c = RemeltCounters()
@@ -270,6 +275,18 @@ class RemeltDomain:
         out.update(grain_env=env[:4 * m].reshape(-1, 4), grain_cell=gc[:m], grain_orient=go[:m],
                    n_grains=int(n.value))
         return out
+
+    def use_surrogate(self, mlp, desc, scan, band=50.0, resid=2.0):
+        """Temperatures from the trained network `mlp` (surrogate.MLP) with the
+        FE fallback rule; `desc` and `scan` are surrogate.SurrogateDesc / Scan."""
+        _check(lib().amca_remelt_use_surrogate(self.h, mlp.h, C.byref(desc), C.byref(scan),
+                                               float(band), float(resid)), "use_surrogate")
+
+    def surrogate_error(self):
+        o = np.zeros(6, np.float64)
+        _check(lib().amca_remelt_surrogate_error(self.h, _p(o, C.c_double)), "surrogate_error")
+        return dict(rms_all=o[0], n_all=int(o[1]), rms_served=o[2], n_served=int(o[3]),
+                    max_served=o[4], n_fallback=int(o[5]))
 
     def enable_timing(self, on=True):
         _check(lib().amca_remelt_enable_timing(self.h, 1 if on else 0), "enable_timing")

@@ -7,6 +7,23 @@ import numpy as np
 from ._lib import AmcaError, load
 
 F32P = C.POINTER(C.c_float)
+F64P = C.POINTER(C.c_double)
+
+
+class SurrogateDesc(C.Structure):
+    _fields_ = [("fnx", C.c_int32), ("fny", C.c_int32), ("fnz", C.c_int32),
+                ("fx0", C.c_double), ("fy0", C.c_double), ("fz0", C.c_double), ("fdx", C.c_double),
+                ("n_frames", C.c_int32), ("frame_dt", C.c_double),
+                ("eta", C.c_double), ("k", C.c_double), ("alpha", C.c_double), ("T0", C.c_double),
+                ("z_top", C.c_double), ("r_min", C.c_double), ("void_temperature", C.c_double),
+                ("xi_scale", C.c_double), ("y_scale", C.c_double), ("z_scale", C.c_double),
+                ("P_scale", C.c_double), ("v_scale", C.c_double),
+                ("T_ref", C.c_double), ("T_scale", C.c_double), ("T_clip", C.c_double),
+                ("log_target", C.c_int32)]
+
+
+class Scan(C.Structure):
+    _fields_ = [("P", C.c_double), ("v", C.c_double), ("x_start", C.c_double), ("y_c", C.c_double)]
 
 SIGNATURES = {
     "amca_mlp_param_count": (C.c_int32, []),
@@ -22,6 +39,12 @@ SIGNATURES = {
     "amca_mlp_gemm_selftest": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          F32P, F32P, F32P]),
     "amca_mlp_last_error": (C.c_char_p, []),
+    "amca_surrogate_sample": (C.c_int, [C.c_void_p, C.POINTER(SurrogateDesc), C.POINTER(Scan),
+                                        C.c_int32, C.c_int64, C.c_double, C.c_uint64]),
+    "amca_surrogate_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
+                                       F32P]),
+    "amca_surrogate_eval": (C.c_int, [C.c_void_p, C.POINTER(SurrogateDesc), C.POINTER(Scan),
+                                      C.c_int32, F64P, F64P, F64P]),
 }
 
 _bound = False
@@ -92,6 +115,28 @@ class MLP:
         _check(lib().amca_mlp_gradients(self.h, len(X), _p(X), _p(Y), _p(g), C.byref(loss)),
                "gradients")
         return g, float(loss.value)
+
+    def sample(self, desc, scans, n, near_fraction=0.7, seed=1):
+        arr = (Scan * len(scans))(*scans)
+        _check(lib().amca_surrogate_sample(self.h, C.byref(desc), arr, len(scans), int(n),
+                                           float(near_fraction), seed), "sample")
+
+    def train(self, batch=8192, steps=2000, lr=1e-3, seed=2, trace=True):
+        tr = np.zeros(max(steps, 1), np.float32)
+        _check(lib().amca_surrogate_train(self.h, batch, steps, lr, seed,
+                                          _p(tr) if trace else None), "train")
+        return tr[:steps]
+
+    def eval(self, desc, scan, xyzt):
+        """(surrogate, FE path) temperatures in kelvin at rows (x, y, z, t)."""
+        p = np.ascontiguousarray(xyzt, np.float64)
+        n = len(p)
+        ts = np.empty(n, np.float64)
+        tf = np.empty(n, np.float64)
+        _check(lib().amca_surrogate_eval(self.h, C.byref(desc), C.byref(scan), n,
+                                         p.ctypes.data_as(F64P), ts.ctypes.data_as(F64P),
+                                         tf.ctypes.data_as(F64P)), "eval")
+        return ts, tf
 
     def close(self):
         if self.h:

@@ -219,6 +219,40 @@ amca_status amca_mlp_train_device(amca_mlp* m, int64_t n, const float* dX, const
                                   float* loss_trace);
 /* Device rows dX[n][5] -> dT[n]. */
 amca_status amca_mlp_forward_device(amca_mlp* m, int64_t n, const float* dX, float* dT);
+/* The temperature field the surrogate learns: a moving point source on a
+ * structured FE mesh (uniform node spacing), Rosenthal nodal values at frames
+ * frame_dt apart, interpolated linearly in time and by hex8 in space (the FE
+ * path of the coupled domain).  Inputs are normalised as
+ * ((x - x_start - v t)/xi_scale, (y - y_c)/y_scale, (z - z_top)/z_scale,
+ * P/P_scale, v/v_scale), the target as (min(T, T_clip) - T_ref)/T_scale, or
+ * its log1p (log_target), which weighs the far field's small rises evenly. */
+typedef struct {
+  int32_t fnx, fny, fnz;
+  double fx0, fy0, fz0, fdx;
+  int32_t n_frames;
+  double frame_dt;
+  double eta, k, alpha, T0, z_top, r_min, void_temperature;
+  double xi_scale, y_scale, z_scale, P_scale, v_scale;
+  double T_ref, T_scale, T_clip;
+  int32_t log_target;  /* 1: target log1p(max(min(T, T_clip) - T_ref, 0) / T_scale) */
+} amca_surrogate_desc;
+
+typedef struct {
+  double P, v, x_start, y_c;    /* power, speed, source x at t = 0, track line y */
+} amca_scan;
+
+/* Draws n training samples of the FE field over the scans (a fraction
+ * `near_fraction` close behind the source, the rest uniform in the mesh) into
+ * the network's device-resident training set. */
+amca_status amca_surrogate_sample(amca_mlp* m, const amca_surrogate_desc* d, const amca_scan* scans,
+                                  int32_t n_scans, int64_t n, double near_fraction, uint64_t seed);
+/* Adam steps over the training set (amca_mlp_train_device on it). */
+amca_status amca_surrogate_train(amca_mlp* m, int32_t batch, int32_t steps, float lr, uint64_t seed,
+                                 float* loss_trace);
+/* Surrogate and FE-path temperatures (kelvin) at n host points xyzt[n][4] of one scan. */
+amca_status amca_surrogate_eval(amca_mlp* m, const amca_surrogate_desc* d, const amca_scan* s,
+                                int32_t n, const double* xyzt, double* T_surrogate, double* T_fe);
+
 /* The tiled BF16 GEMM alone: C[M][N] = opA(A) opB(B) with FP32 host matrices
  * (row-major; trans_a: A is K x M, trans_b: B is N x K). */
 amca_status amca_mlp_gemm_selftest(int32_t M, int32_t N, int32_t K, int32_t trans_a,
@@ -276,6 +310,7 @@ typedef struct {
 typedef struct {
   double time_before, dt;
   int32_t n_nucleated, n_captured, n_fresh, n_flagged;
+  int32_t n_fallback;   /* with a surrogate: cells that took the FE path */
 } amca_remelt_step_record;
 
 /* Supplies n more orientation rows (Euler angles and row-major local-to-global
@@ -342,6 +377,19 @@ amca_status amca_remelt_download(amca_remelt* d, int32_t* orient, int32_t* grain
                                  int32_t* n_grains);
 /* thetaArray_: the nodal field the next step interpolates from. */
 amca_status amca_remelt_theta(amca_remelt* d, double* theta);
+/* Temperatures from the trained surrogate instead of the FE path: the network
+ * is evaluated once on moving-frame lattices (cell y, z and node y, z; xi
+ * spacing h) for the scan; each step a cell takes the table value unless it is
+ * within `band` K of Tm or in an element one of whose nodes differs from the
+ * FE nodal field by more than `resid` K, where it takes the FE path.  The mesh
+ * must be the uniform lattice `sd` describes.  Results then differ from the
+ * FE-driven run (no counterpart in the reference). */
+amca_status amca_remelt_use_surrogate(amca_remelt* d, amca_mlp* m, const amca_surrogate_desc* sd,
+                                      const amca_scan* scan, double band, double resid);
+/* Surrogate error against the FE path at the current FE time: out6 = RMS over
+ * cells with material, their count, RMS over the cells the surrogate serves,
+ * their count, their max |error|, cells on the FE path. */
+amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6);
 /* Per-kernel device time from HIP events around every launch (off by default);
  * enabling resets the totals.  Kernel i is named amca_remelt_kernel_name(i). */
 amca_status amca_remelt_enable_timing(amca_remelt* d, int32_t on);
