@@ -238,12 +238,31 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
   }
   const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
   if (lm == 0 || e.w == 0) return lm;  // no liquid neighbour, or an envelope that captures nothing
+  // The L1 norm in the grain frame is at least the Euclidean distance, so a
+  // neighbour farther than L (with a margin far above rounding) cannot be
+  // inside; the orientation row is read only when some neighbour can be.
+  const double reach = e.w * e.w * (1.0 + 1e-9);
+  unsigned cand = 0;
+  {
+    unsigned rest = lm;
+    while (rest) {
+      const int s = __builtin_ctz(rest);
+      rest &= rest - 1;
+      int di, dj, dk;
+      slot_delta(s, di, dj, dk);
+      const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
+      const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
+      const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
+      if (gx * gx + gy * gy + gz * gz <= reach) cand |= 1u << s;
+    }
+  }
+  if (cand == 0) return lm;
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
 #pragma unroll
   for (int q = 0; q < 9; q++) r[q] = rp[q];
   unsigned im = 0;
-  unsigned rest = lm;
+  unsigned rest = cand;
   while (rest) {
     const int s = __builtin_ctz(rest);
     rest &= rest - 1;
