@@ -49,7 +49,8 @@ class SimInfo(C.Structure):
         ("nx", C.c_int32), ("ny", C.c_int32), ("nz", C.c_int32), ("h", C.c_double),
         ("termination_time", C.c_double), ("time_step", C.c_double),
         ("n_sites", C.c_int32), ("n_surface_sites", C.c_int32), ("n_bulk_sites", C.c_int32),
-        ("n_site_draws", C.c_int64),
+        ("n_site_draws", C.c_int64), ("coupled", C.c_int32), ("n_frames", C.c_int32),
+        ("fe_nodes", C.c_int32), ("n_init_grains", C.c_int32),
     ]
 
 
@@ -88,6 +89,8 @@ SIGNATURES = {
     "amca_sim_initialize": (C.c_int, [C.c_void_p]),
     "amca_sim_run": (C.c_int, [C.c_void_p, C.c_int64, C.POINTER(Counters)]),
     "amca_sim_domain": (C.c_void_p, [C.c_void_p]),
+    "amca_sim_remelt": (C.c_void_p, [C.c_void_p]),
+    "amca_sim_finalize": (C.c_int, [C.c_void_p]),
     "amca_sim_get_info": (C.c_int, [C.c_void_p, C.POINTER(SimInfo)]),
     "amca_sim_initial_sites": (C.c_int, [C.c_void_p, C.c_int32, I32P, F64P, F64P, I32P]),
     "amca_sim_destroy": (C.c_int, [C.c_void_p]),

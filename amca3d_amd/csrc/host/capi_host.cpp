@@ -7,11 +7,13 @@
 #include <string>
 
 #include "../../../include/amca3d.h"
+#include "cafe.h"
 #include "realm.h"
 
 struct amca_sim {
   amca_host::SimulationConfig cfg;
   std::unique_ptr<amca_host::PlainRealm> plain;
+  std::unique_ptr<amca_host::CafeRealm> cafe;  // finite_element + cellular_automata_remelting
   bool initialized = false;
   bool prepared = false;
 };
@@ -33,17 +35,32 @@ amca_status guarded(F&& f) {
   }
 }
 
-amca_status create_from_doc(const amca_host::yaml::Node& doc, amca_sim** out) {
+// Simulation::load picks the solver from `solvers` and the realm types
+// (CellularAutomataSolver.cpp:53-79, CafeSolver.cpp:54-78).
+amca_status create_from_doc(const amca_host::yaml::Node& doc, const std::string& base,
+                            amca_sim** out) {
   auto s = std::make_unique<amca_sim>();
   s->cfg = amca_host::load_simulation(doc);
   if (s->cfg.realms.empty()) return host_fail(AMCA_ERR_INVALID, "no cellular automata realm");
   const auto& r = s->cfg.realms[0];
-  if (s->cfg.finite_element_method || r.type != "cellular_automata")
-    return host_fail(AMCA_ERR_INVALID,
-                     "realm type '" + r.type + "' with these solvers is not available in this build");
-  s->plain = std::make_unique<amca_host::PlainRealm>(r, s->cfg.ti);
+  if (s->cfg.finite_element_method) {
+    if (r.type != "cellular_automata_remelting")
+      return host_fail(AMCA_ERR_INVALID, "a finite_element_method run needs a "
+                                         "cellular_automata_remelting realm");
+    s->cafe = std::make_unique<amca_host::CafeRealm>(s->cfg, base);
+  } else {
+    if (r.type != "cellular_automata")
+      return host_fail(AMCA_ERR_INVALID, "realm type '" + r.type + "' needs the "
+                                         "finite_element_method solver and an FE realm");
+    s->plain = std::make_unique<amca_host::PlainRealm>(r, s->cfg.ti);
+  }
   *out = s.release();
   return AMCA_OK;
+}
+
+std::string dir_of(const std::string& p) {
+  const size_t k = p.find_last_of('/');
+  return k == std::string::npos ? std::string() : p.substr(0, k);
 }
 }  // namespace
 
@@ -53,18 +70,22 @@ const char* amca_sim_last_error(void) { return g_host_error.c_str(); }
 
 amca_status amca_sim_create(const char* yaml_text, amca_sim** out) {
   if (!yaml_text || !out) return host_fail(AMCA_ERR_INVALID, "null argument");
-  return guarded([&] { return create_from_doc(amca_host::yaml::parse(yaml_text), out); });
+  return guarded([&] { return create_from_doc(amca_host::yaml::parse(yaml_text), "", out); });
 }
 
 amca_status amca_sim_create_from_file(const char* path, amca_sim** out) {
   if (!path || !out) return host_fail(AMCA_ERR_INVALID, "null argument");
-  return guarded([&] { return create_from_doc(amca_host::yaml::parse_file(path), out); });
+  return guarded(
+      [&] { return create_from_doc(amca_host::yaml::parse_file(path), dir_of(path), out); });
 }
 
 amca_status amca_sim_initialize(amca_sim* s) {
   if (!s) return host_fail(AMCA_ERR_INVALID, "null simulation");
   return guarded([&] {
-    s->plain->initialize();
+    if (s->cafe)
+      s->cafe->initialize();
+    else
+      s->plain->initialize();
     s->initialized = true;
     s->prepared = true;
     return AMCA_OK;
@@ -74,7 +95,10 @@ amca_status amca_sim_initialize(amca_sim* s) {
 amca_status amca_sim_prepare(amca_sim* s) {
   if (!s) return host_fail(AMCA_ERR_INVALID, "null simulation");
   return guarded([&] {
-    s->plain->prepare();
+    if (s->cafe)
+      s->cafe->prepare();
+    else
+      s->plain->prepare();
     s->prepared = true;
     return AMCA_OK;
   });
@@ -83,13 +107,36 @@ amca_status amca_sim_prepare(amca_sim* s) {
 amca_status amca_sim_run(amca_sim* s, int64_t max_steps, amca_counters* out) {
   if (!s || !s->initialized) return host_fail(AMCA_ERR_INVALID, "simulation not initialized");
   return guarded([&] {
+    if (s->cafe) {
+      const amca_remelt_counters r = s->cafe->run(max_steps);
+      if (out) {
+        std::memset(out, 0, sizeof(*out));
+        out->step = r.step;
+        out->time = r.time;
+        out->dt = r.dt;
+        out->n_grains = r.n_grains;
+        out->n_orient = r.n_orient;
+        out->done = r.done;
+      }
+      return AMCA_OK;
+    }
     amca_counters c = s->plain->run(max_steps);
     if (out) *out = c;
     return AMCA_OK;
   });
 }
 
+amca_status amca_sim_finalize(amca_sim* s) {
+  if (!s || !s->initialized) return host_fail(AMCA_ERR_INVALID, "simulation not initialized");
+  return guarded([&] {
+    if (s->cafe) s->cafe->finalize();
+    return AMCA_OK;
+  });
+}
+
 amca_domain* amca_sim_domain(amca_sim* s) { return s && s->plain ? s->plain->domain() : nullptr; }
+
+amca_remelt* amca_sim_remelt(amca_sim* s) { return s && s->cafe ? s->cafe->domain() : nullptr; }
 
 amca_status amca_sim_get_info(amca_sim* s, amca_sim_info* info) {
   if (!s || !info) return host_fail(AMCA_ERR_INVALID, "null argument");
@@ -101,6 +148,19 @@ amca_status amca_sim_get_info(amca_sim* s, amca_sim_info* info) {
   info->h = r.h;
   info->termination_time = s->cfg.ti.termination_time;
   info->time_step = s->cfg.ti.time_step;
+  if (s->cafe && s->prepared) {
+    info->n_frames = s->cafe->frames().frames();
+    info->fe_nodes = s->cafe->frames().nodes();
+    info->n_init_grains = (int32_t)s->cafe->initial_state().grain_id.size();
+  }
+  if (s->cafe && s->initialized) {
+    int32_t ns = 0;
+    amca_remelt_sites(s->cafe->domain(), 0, nullptr, nullptr, &ns);
+    info->n_sites = ns;
+    info->n_surface_sites = s->cafe->n_surface_new();
+    info->n_bulk_sites = s->cafe->n_bulk_new();
+  }
+  info->coupled = s->cafe ? 1 : 0;
   if (s->plain && s->prepared) {
     info->n_sites = (int32_t)s->plain->initial_sites().size();
     info->n_surface_sites = s->plain->n_surface_sites();
@@ -113,6 +173,7 @@ amca_status amca_sim_get_info(amca_sim* s, amca_sim_info* info) {
 amca_status amca_sim_initial_sites(amca_sim* s, int32_t cap, int32_t* cell, double* dT_crit,
                                    double* euler3, int32_t* n) {
   if (!s || !s->prepared || !n) return host_fail(AMCA_ERR_INVALID, "simulation not prepared");
+  if (!s->plain) return host_fail(AMCA_ERR_INVALID, "coupled runs: use amca_remelt_sites");
   const auto& sc = s->plain->initial_sites();
   const auto& sd = s->plain->initial_site_dt();
   const auto& eu = s->plain->euler_rows();

@@ -55,7 +55,15 @@ void load_solution_options(const Node& realm, RealmConfig& c) {
       get_required(f, "file_name", c.micro_file);
       c.output_micro = true;
     } else if (expect_map(o, "random_type", true)) {
-      // shuffled VTR ids only change output labels; not used on this path
+      const Node& p = *o.FindValue("random_type");
+      get_if_present(p, "random_microstructure_information", c.micro_random, c.micro_random);
+      // shuffled output ids (CellularAutomataManager_3D.cpp:1967-1983) take
+      // std::random_shuffle draws from the rand() stream that the sites and
+      // nucleus orientations use; output labels are not produced here
+      bool shuffle = false;
+      get_if_present(p, "shuffer_vtr_grain_ID", shuffle, shuffle);
+      if (shuffle)
+        throw std::runtime_error("random_type-shuffer_vtr_grain_ID: yes is not supported");
     } else if (expect_map(o, "phase_state", true)) {
       const Node& p = *o.FindValue("phase_state");
       get_if_present(p, "has_been_melted", c.initial_melted, c.initial_melted);
@@ -72,8 +80,22 @@ void load_solution_options(const Node& realm, RealmConfig& c) {
     } else if (expect_map(o, "criterion_to_activate_grain", true)) {
       const Node& p = *o.FindValue("criterion_to_activate_grain");
       get_required(p, "using_nucleation_seed", c.using_nucleation_seed);
+    } else if (expect_map(o, "load_microstructure_information", true)) {
+      if (c.micro_random) continue;
+      const Node& p = *o.FindValue("load_microstructure_information");
+      std::string f;
+      get_required(p, "file_name", f);
+      if (f == "random") {
+        c.micro_random = true;
+        c.micro_load.clear();
+      } else {
+        c.micro_load = f;
+        get_if_present(p, "load_seeds", c.load_seeds, c.load_seeds);
+      }
     }
   }
+  // neither a file nor random: random (CellularAutomataManager_3D.cpp:343-357)
+  if (c.micro_load.empty()) c.micro_random = true;
 }
 
 void load_physics(const Node& realm, RealmConfig& c) {
@@ -131,6 +153,15 @@ RealmConfig load_ca_realm(const Node& realm) {
   // CellularAutomataManager_3D.cpp:79-190
   const Node* domain = realm.FindValue("domain");
   if (!domain) throw std::runtime_error("parser error: realm-domain");
+  // domain-load_microstructure_information (:84-131) takes the grid from the
+  // file header, whose counts include the ghost layer, so the loaded file
+  // never fits the grid it builds; the solution option form is the one used
+  std::string domain_file;
+  get_if_present(*domain, "load_microstructure_information", domain_file, domain_file);
+  if (!domain_file.empty())
+    throw std::runtime_error(
+        "domain-load_microstructure_information is not supported; use "
+        "solution_options-load_microstructure_information-file_name");
   get_required3(*domain, "original_point", c.origin);
   get_if_present_no_default3(*domain, "lateral_sizes", c.lateral);
   const Node* disc = realm.FindValue("discretization");
@@ -171,8 +202,43 @@ SimulationConfig load_simulation(const Node& doc) {
     const Node& r = (*realms)[i];
     std::string type;
     get_required(r, "type", type);
-    if (type == "finite_element") continue;  // FE realms belong to the FE provider
+    if (type == "finite_element") {
+      // FiniteElementManager::load (FiniteElementManager.cpp:66-135)
+      FeRealmConfig f;
+      get_if_present(r, "name", f.name, f.name);
+      if (const Node* so = r.FindValue("solution_options")) {
+        if (const Node* opts = expect_sequence(*so, "options", true)) {
+          for (size_t q = 0; q < opts->size(); q++) {
+            const Node& o = (*opts)[q];
+            if (!expect_map(o, "load_data_from_file", true)) continue;
+            const Node& l = *o.FindValue("load_data_from_file");
+            get_if_present(l, "for_whole_time", f.whole_time, f.whole_time);
+            get_if_present(l, "lines_for_title", f.title_lines, f.title_lines);
+            get_if_present(l, "lines_for_subtitle", f.subtitle_lines, f.subtitle_lines);
+            get_if_present(l, "length_scale", f.length_scale, f.length_scale);
+            get_if_present(l, "time_scale", f.time_scale, f.time_scale);
+            const char* keys[3] = {"x_offset", "y_offset", "z_offset"};
+            for (int a = 0; a < 3; a++) get_if_present(l, keys[a], f.offset[a], f.offset[a]);
+          }
+        }
+      }
+      get_required(r, "mesh", f.mesh);
+      s.fe_realms.push_back(f);
+      continue;
+    }
     s.realms.push_back(load_ca_realm(r));
+  }
+  // MapVoxelManager::load (MapVoxelManager.cpp:60-90)
+  if (const Node* tr = doc.FindValue("transfers")) {
+    for (size_t i = 0; i < tr->size(); i++) {
+      const Node& t = (*tr)[i];
+      if (const Node* b = t.FindValue("bin_size")) {
+        for (int a = 0; a < 3 && a < (int)b->size(); a++) convert((*b)[a], s.transfer.bin[a]);
+      }
+      get_if_present(t, "tolerance", s.transfer.tolerance, s.transfer.tolerance);
+      get_if_present(t, "max_iterations", s.transfer.max_iterations, s.transfer.max_iterations);
+      get_if_present(t, "void_temperature", s.transfer.void_temperature, s.transfer.void_temperature);
+    }
   }
   const Node* tis = doc.FindValue("time_integrators");
   if (!tis) throw std::runtime_error("parser error: tiem_integrations");

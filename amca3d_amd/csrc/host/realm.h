@@ -46,6 +46,11 @@ struct RealmConfig {
   std::string micro_file;
   bool initial_melted = true;
   bool using_nucleation_seed = false;  // remelting: criterion_to_activate_grain
+  // initial microstructure (CellularAutomataManager_3D.cpp:273-357): a file, or
+  // random (one grain per interior cell, the default when nothing is named)
+  std::string micro_load;
+  bool micro_random = false;
+  bool load_seeds = false;
   // output (VtrManager::load)
   int output_frequency = 1;
   std::string output_base = "./Results/Grains";
@@ -62,10 +67,31 @@ struct TimeIntegratorConfig {
   double time_step_factor = -1.0;
 };
 
+// A `finite_element` realm that reads its temperatures from a text file
+// (FiniteElementManager::load, FiniteElementManager.cpp:66-135).
+struct FeRealmConfig {
+  std::string name;
+  std::string mesh;          // the text file of frames
+  bool whole_time = false;   // load_data_from_file.for_whole_time
+  int title_lines = 4, subtitle_lines = 5;
+  double length_scale = 1.0, time_scale = 1.0;
+  double offset[3] = {0, 0, 0};
+};
+
+// transfers[] (MapVoxelManager::load, MapVoxelManager.cpp:60-90)
+struct TransferConfig {
+  int bin[3] = {10, 10, 10};
+  double tolerance = 1.0e-16;
+  int max_iterations = 20;
+  double void_temperature = -1.0e8;
+};
+
 struct SimulationConfig {
   bool cellular_automata = false;
   bool finite_element_method = false;
   std::vector<RealmConfig> realms;
+  std::vector<FeRealmConfig> fe_realms;
+  TransferConfig transfer;
   TimeIntegratorConfig ti;
 };
 

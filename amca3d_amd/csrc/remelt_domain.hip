@@ -61,6 +61,8 @@ struct amca_remelt {
   amca_orientation_source source = nullptr;
   void* source_user = nullptr;
   std::unique_ptr<amca_host::GlibcRand> rng;  // the rand() stream after amca_remelt_draw_sites
+  std::vector<int32_t> site_cells;            // nucleation sites as last set
+  std::vector<double> site_dts;
   std::vector<double> euler;                  // Euler angles of every row, 3 per row
   size_t grain_cap = 0;  // records genv/gco can hold
   std::vector<int32_t> host_tag;
@@ -540,31 +542,62 @@ amca_status amca_remelt_set_orientation_source(amca_remelt* d, amca_orientation_
   return AMCA_OK;
 }
 
-amca_status amca_remelt_draw_sites(amca_remelt* d, uint32_t seed, double ns, double dTs_mean,
-                                   double dTs_sd, double nv, double dTv_mean, double dTv_sd,
-                                   int32_t* n_sites) {
-  if (!d) return rm_fail(AMCA_ERR_INVALID, "null domain");
+amca_status amca_remelt_draw_sites_after(amca_remelt* d, uint32_t seed, int64_t skip_draws,
+                                         double ns, double dTs_mean, double dTs_sd, double nv,
+                                         double dTv_mean, double dTv_sd, int32_t* n_surface_new,
+                                         int32_t* n_bulk_new) {
+  if (!d || skip_draws < 0) return rm_fail(AMCA_ERR_INVALID, "bad arguments");
   amca_host::Grid g{d->P.nx, d->P.ny, d->P.nz};
   auto rng = std::make_unique<amca_host::GlibcRand>(seed);
+  for (int64_t q = 0; q < skip_draws; q++) (void)rng->next();
+  // create_nucleation_sites merges into the list already held (the seeds of a
+  // loaded microstructure file): a draw on a listed cell only lowers its value
   std::unordered_map<int, double> cell_dt;
   std::vector<int> sites;
+  for (size_t q = 0; q < d->site_cells.size(); q++) {
+    if (!cell_dt.emplace(d->site_cells[q], d->site_dts[q]).second || d->site_dts[q] < 0)
+      return rm_fail(AMCA_ERR_INVALID, "sites to merge with must be distinct cells with undercooling >= 0");
+    sites.push_back(d->site_cells[q]);
+  }
   const double h = d->P.h;
+  amca_host::SiteDraw ds, db;
   {
     std::vector<int> ids;
     amca_host::surface_cells(g, amca_host::SurfaceFaces::kReferenceSix, ids);
-    amca_host::create_nucleation_sites((int)ids.size(), [&](int i) { return ids[i]; }, h * h, ns,
-                                       dTs_mean, dTs_sd, *rng, cell_dt, sites);
+    ds = amca_host::create_nucleation_sites((int)ids.size(), [&](int i) { return ids[i]; }, h * h,
+                                            ns, dTs_mean, dTs_sd, *rng, cell_dt, sites);
   }
-  amca_host::create_nucleation_sites((int)(g.nx * (long long)g.ny * g.nz),
-                                     amca_host::BulkCellAt{g}, h * h * h, nv, dTv_mean, dTv_sd,
-                                     *rng, cell_dt, sites);
+  db = amca_host::create_nucleation_sites((int)(g.nx * (long long)g.ny * g.nz),
+                                          amca_host::BulkCellAt{g}, h * h * h, nv, dTv_mean,
+                                          dTv_sd, *rng, cell_dt, sites);
   std::vector<double> dts(sites.size());
   for (size_t q = 0; q < sites.size(); q++) dts[q] = cell_dt[sites[q]];
   RM_TRY(amca_remelt_set_sites(d, (int32_t)sites.size(), sites.data(), dts.data()));
   d->source = nullptr;
   d->source_user = nullptr;
   d->rng = std::move(rng);
-  if (n_sites) *n_sites = (int32_t)sites.size();
+  if (n_surface_new) *n_surface_new = ds.n_new;
+  if (n_bulk_new) *n_bulk_new = db.n_new;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_draw_sites(amca_remelt* d, uint32_t seed, double ns, double dTs_mean,
+                                   double dTs_sd, double nv, double dTv_mean, double dTv_sd,
+                                   int32_t* n_sites) {
+  int32_t a = 0, b = 0;
+  RM_TRY(amca_remelt_draw_sites_after(d, seed, 0, ns, dTs_mean, dTs_sd, nv, dTv_mean, dTv_sd, &a,
+                                      &b));
+  if (n_sites) *n_sites = (int32_t)d->site_cells.size();
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_sites(amca_remelt* d, int32_t cap, int32_t* cell, double* dT_crit,
+                              int32_t* n) {
+  if (!d || !n) return rm_fail(AMCA_ERR_INVALID, "null argument");
+  *n = (int32_t)d->site_cells.size();
+  const int m = std::max(0, std::min(cap, *n));
+  if (m > 0 && cell) std::memcpy(cell, d->site_cells.data(), sizeof(int32_t) * m);
+  if (m > 0 && dT_crit) std::memcpy(dT_crit, d->site_dts.data(), sizeof(double) * m);
   return AMCA_OK;
 }
 
@@ -605,6 +638,8 @@ amca_status amca_remelt_set_sites(amca_remelt* d, int32_t n, const int32_t* cell
     if (cell[i] < 0 || cell[i] >= d->P.NG) return rm_fail(AMCA_ERR_INVALID, "site out of range");
     if (i > 0 && cell[i] <= cell[i - 1]) return rm_fail(AMCA_ERR_INVALID, "sites must ascend");
   }
+  d->site_cells.assign(cell, cell + n);
+  d->site_dts.assign(dT_crit, dT_crit + n);
   d->release(d->B.site_cell);
   d->release(d->B.site_dt);
   RM_TRY(d->alloc(&d->B.site_cell, (size_t)std::max(n, 1)));

@@ -65,6 +65,10 @@ SIGNATURES = {
     "amca_remelt_draw_sites": (C.c_int, [C.c_void_p, C.c_uint32, C.c_double, C.c_double,
                                          C.c_double, C.c_double, C.c_double, C.c_double, I32P]),
     "amca_remelt_euler": (C.c_int, [C.c_void_p, C.c_int32, F64P, I32P]),
+    "amca_remelt_draw_sites_after": (C.c_int, [C.c_void_p, C.c_uint32, C.c_int64, C.c_double,
+                                               C.c_double, C.c_double, C.c_double, C.c_double,
+                                               C.c_double, I32P, I32P]),
+    "amca_remelt_sites": (C.c_int, [C.c_void_p, C.c_int32, I32P, F64P, I32P]),
     "amca_remelt_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_remelt_use_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_double, C.c_double]),
@@ -276,6 +280,28 @@ class RemeltDomain:
                    n_grains=int(n.value))
         return out
 
+    @classmethod
+    def view(cls, handle, shape, nodes):
+        """A handle owned elsewhere (a coupled simulation's domain)."""
+        d = cls.__new__(cls)
+        d.h = C.c_void_p(handle)
+        d.grid = tuple(shape)
+        d.ng = (shape[0] + 2) * (shape[1] + 2) * (shape[2] + 2)
+        d.nodes = int(nodes)
+        d._keep = []
+        d._owned = False
+        return d
+
+    def sites(self):
+        n = C.c_int32(0)
+        _check(lib().amca_remelt_sites(self.h, 0, None, None, C.byref(n)), "sites")
+        m = int(n.value)
+        c = np.empty(max(m, 1), np.int32)
+        t = np.empty(max(m, 1), np.float64)
+        _check(lib().amca_remelt_sites(self.h, m, _p(c, C.c_int32), _p(t, C.c_double), C.byref(n)),
+               "sites")
+        return c[:m], t[:m]
+
     def use_surrogate(self, mlp, desc, scan, band=50.0, resid=2.0):
         """Temperatures from the trained network `mlp` (surrogate.MLP) with the
         FE fallback rule; `desc` and `scan` are surrogate.SurrogateDesc / Scan."""
@@ -308,9 +334,9 @@ class RemeltDomain:
         return t
 
     def close(self):
-        if self.h:
+        if self.h and getattr(self, "_owned", True):
             lib().amca_remelt_destroy(self.h)
-            self.h = C.c_void_p()
+        self.h = C.c_void_p()
 
     def __del__(self):
         try:

@@ -4,7 +4,7 @@ import ctypes as C
 import numpy as np
 
 from . import _lib
-from ._lib import Counters, SimInfo, StepRecord, check, load
+from ._lib import AmcaError, Counters, SimInfo, StepRecord, check, load
 
 
 def _ptr(a, t):
@@ -138,10 +138,19 @@ class OwnedDomain(Domain):
 class Simulation:
     """A YAML input deck driven through the realm layer of libamca3d."""
 
-    def __init__(self, yaml_text):
+    def __init__(self, yaml_text=None, path=None):
         self._h = C.c_void_p()
-        check(load().amca_sim_create(yaml_text.encode(), C.byref(self._h)), "sim_create", sim=True)
+        if path is not None:
+            check(load().amca_sim_create_from_file(str(path).encode(), C.byref(self._h)),
+                  "sim_create", sim=True)
+        else:
+            check(load().amca_sim_create(yaml_text.encode(), C.byref(self._h)), "sim_create",
+                  sim=True)
         self._initialized = False
+
+    @classmethod
+    def from_file(cls, path):
+        return cls(path=path)
 
     def prepare(self):
         """Host RNG draws only (no GPU needed)."""
@@ -167,6 +176,19 @@ class Simulation:
         c = Counters()
         check(load().amca_sim_run(self._h, int(max_steps), C.byref(c)), "sim_run", sim=True)
         return c.as_dict()
+
+    @property
+    def remelt(self):
+        """Non-owning view of the coupled domain (finite_element_method runs)."""
+        from .remelt import RemeltDomain
+        inf = self.info()
+        h = load().amca_sim_remelt(self._h)
+        if not h:
+            raise AmcaError("not a coupled simulation, or not initialized")
+        return RemeltDomain.view(h, (inf["nx"], inf["ny"], inf["nz"]), inf["fe_nodes"])
+
+    def finalize(self):
+        check(load().amca_sim_finalize(self._h), "sim_finalize", sim=True)
 
     def initial_sites(self):
         n = C.c_int32(0)

@@ -356,10 +356,22 @@ amca_status amca_remelt_set_sites(amca_remelt* d, int32_t n, const int32_t* cell
  * bulk (h^3 per cell) populations (CellularAutomataManager.cpp:153-213), and
  * sets the orientation source to the continuing rand() stream
  * (get_random_orientation_angle, CellularAutomataManager_3D.cpp:838-850).
- * Replaces any sites and source set before. */
+ * The draws merge into the sites already set (amca_remelt_set_sites: the seeds
+ * of a loaded microstructure file, CellularAutomataManager_3DRemelting.cpp:244-249):
+ * a draw on a listed cell only lowers its undercooling.  Replaces the source. */
 amca_status amca_remelt_draw_sites(amca_remelt* d, uint32_t seed, double ns, double dTs_mean,
                                    double dTs_sd, double nv, double dTv_mean, double dTv_sd,
                                    int32_t* n_sites);
+/* Same, after discarding `skip_draws` rand() values (the draws a random initial
+ * microstructure takes first); returns the new-site counts of each population
+ * (numSitesSurface_ / numSitesBulk_). */
+amca_status amca_remelt_draw_sites_after(amca_remelt* d, uint32_t seed, int64_t skip_draws,
+                                         double ns, double dTs_mean, double dTs_sd, double nv,
+                                         double dTv_mean, double dTv_sd, int32_t* n_surface_new,
+                                         int32_t* n_bulk_new);
+/* The nucleation sites as set (cell, critical undercooling); copies min(cap, n). */
+amca_status amca_remelt_sites(amca_remelt* d, int32_t cap, int32_t* cell, double* dT_crit,
+                              int32_t* n);
 /* Euler angles of every orientation row pushed or drawn so far. */
 amca_status amca_remelt_euler(amca_remelt* d, int32_t cap, double* euler3, int32_t* n);
 /* CafeSolver::time_integration_weak_coupling (CafeSolver.cpp:178-219) until
@@ -427,9 +439,15 @@ typedef struct {
   double termination_time, time_step;
   int32_t n_sites, n_surface_sites, n_bulk_sites;
   int64_t n_site_draws;
+  int32_t coupled;          /* 1: finite_element + cellular_automata_remelting (CafeSolver) */
+  int32_t n_frames, fe_nodes, n_init_grains;
 } amca_sim_info;
 
-/* Simulation::load on a YAML document held in memory / in a file. */
+/* Simulation::load on a YAML document held in memory / in a file.  With the
+ * `finite_element_method` solver, a `finite_element` realm (frames from its
+ * `mesh` text file, load_data_from_file keys) and a
+ * `cellular_automata_remelting` realm make a coupled run (CafeSolver); file
+ * names are taken relative to the deck's directory. */
 amca_status amca_sim_create(const char* yaml_text, amca_sim** out);
 amca_status amca_sim_create_from_file(const char* path, amca_sim** out);
 /* The host half of CellularAutomataManager_3D::initialize (:366-379): the
@@ -440,6 +458,11 @@ amca_status amca_sim_initialize(amca_sim* s);
 /* Simulation::run -> time_integration; max_steps < 0 runs to termination. */
 amca_status amca_sim_run(amca_sim* s, int64_t max_steps, amca_counters* out);
 amca_domain* amca_sim_domain(amca_sim* s);
+/* The coupled domain of a `finite_element_method` run (NULL for plain runs). */
+amca_remelt* amca_sim_remelt(amca_sim* s);
+/* CafeSolver::finalize: writes the output_microstructure_information file
+ * (rank suffix "0.txt") when the deck asks for it. */
+amca_status amca_sim_finalize(amca_sim* s);
 amca_status amca_sim_get_info(amca_sim* s, amca_sim_info* info);
 /* The sites and orientation rows drawn at initialisation (for checking). */
 amca_status amca_sim_initial_sites(amca_sim* s, int32_t cap, int32_t* cell, double* dT_crit,

@@ -95,6 +95,11 @@ typedef struct {
   int32_t n_init_grains;
   const int32_t* init_grain_cell;
   const int32_t* init_grain_orient;
+  /* 1: random_microstructure_information instead of a loaded state */
+  int32_t random_micro;
+  /* seeds of a loaded microstructure file (load_seeds): per-cell critical
+     undercooling, > -1 marks a site; NULL: none */
+  const double* init_site_dt;    /* [ng] */
 } oracle_remelt_params;
 
 typedef struct {

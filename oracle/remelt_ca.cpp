@@ -482,8 +482,58 @@ struct RemeltCA {
         continue;
       }
       if (ori[c] > -1) continue;
+      if (gid[c] < 0 || gid[c] >= (int)grains.size())
+        throw std::runtime_error("a cell without orientation has no grain record");
       ori[c] = grains[gid[c]].orient;
     }
+  }
+
+  // random_microstructure_information (CellularAutomataManager_3D.cpp:1928-1984):
+  // every interior cell, in cell order, gets a new orientation (three rand()
+  // draws, get_random_orientation_angle :838-850) and its own grain at the
+  // cell centre; ids in that order
+  void random_micro() {
+    for (int c = 0; c < NG; c++) {
+      if (ghost(c)) {
+        gid[c] = -1;
+        ori[c] = -1;
+        continue;
+      }
+      Orient o;
+      double U = static_cast<double>(rand()) / RAND_MAX;
+      double V = static_cast<double>(rand()) / RAND_MAX;
+      o.euler[0] = U * 2 * kPi;
+      o.euler[1] = acos(2 * V - 1);
+      o.euler[2] = static_cast<double>(rand()) / RAND_MAX * 0.5 * kPi;
+      rotation_from_euler(o.euler, o.l2g);
+      const int oid = (int)orients.size();
+      orients.push_back(o);
+      Grain g;
+      g.id = (int)grains.size();
+      g.cell = c;
+      double x[3];
+      centre(c, x);
+      g.xc = x[0];
+      g.yc = x[1];
+      g.zc = x[2];
+      g.orient = oid;
+      g.length = 0.0;
+      g.active = false;
+      g.wave = 0;
+      grains.push_back(g);
+      gid[c] = g.id;
+      ori[c] = oid;
+    }
+  }
+
+  // load_microstructure_information with load_seeds (CellularAutomataManager_3DRemelting.cpp:244-249):
+  // every cell whose seed undercooling is > -1 joins the site list, in cell order
+  void load_seeds(const double* dt) {
+    for (int c = 0; c < NG; c++)
+      if (dt[c] > -1) {
+        nucDT[c] = dt[c];
+        sites.push_back(c);
+      }
   }
 
   void create_sites(double rho, double mean, double sd, bool surface) {
@@ -723,7 +773,11 @@ extern "C" int oracle_remelt_run(const oracle_remelt_params* prm, oracle_remelt_
   try {
     ca.fe_setup();
     ca.map_voxels();
-    if (prm->init_ori) ca.load_state();
+    if (prm->random_micro)
+      ca.random_micro();
+    else if (prm->init_ori)
+      ca.load_state();
+    if (prm->init_site_dt) ca.load_seeds(prm->init_site_dt);
     if (prm->use_sites) {
       ca.create_sites(prm->ns, prm->dTs_mean, prm->dTs_sd, true);
       ca.create_sites(prm->nv, prm->dTv_mean, prm->dTv_sd, false);

@@ -219,6 +219,7 @@ class RemeltParams(C.Structure):
         ("init_ori", I32P), ("init_gid", I32P), ("init_melted", U8P),
         ("n_init_orient", C.c_int32), ("init_euler", F64P),
         ("n_init_grains", C.c_int32), ("init_grain_cell", I32P), ("init_grain_orient", I32P),
+        ("random_micro", C.c_int32), ("init_site_dt", C.POINTER(C.c_double)),
     ]
 
 
@@ -282,6 +283,9 @@ def run_remelt(case, steps_cap=100000, grain_cap=None):
         p.ns, p.dTs_mean, p.dTs_sd = nuc["surface"]
         p.nv, p.dTv_mean, p.dTv_sd = nuc["bulk"]
     p.seed = case.get("seed", 0)
+    p.random_micro = 1 if case.get("random_micro") else 0
+    if case.get("init_site_dt") is not None:
+        p.init_site_dt = arr(case["init_site_dt"], C.c_double, np.float64)
     st = case.get("state")
     if st is not None:
         p.init_ori = arr(st["ori"], C.c_int32, np.int32)
@@ -293,7 +297,7 @@ def run_remelt(case, steps_cap=100000, grain_cap=None):
         p.init_grain_cell = arr(st["grain_cell"], C.c_int32, np.int32)
         p.init_grain_orient = arr(st["grain_orient"], C.c_int32, np.int32)
     if grain_cap is None:
-        n_loaded = len(st["grain_cell"]) if st is not None else 0
+        n_loaded = len(st["grain_cell"]) if st is not None else nx * ny * nz
         grain_cap = n_loaded + nx * ny * nz + 16
     nnodes = int(np.prod(fe["shape"]))
     out_arrays = dict(ori=np.empty(ng, np.int32), gid=np.empty(ng, np.int32),

@@ -124,3 +124,109 @@ def single_cell_case(nx=7, ny=6, nz=5, hot=(4, 3, 3), fresh=False, h=1e-6, n_fra
                 trunc=float(np.sqrt(np.float32(3.0))), tsf=0.2, cutoff_factor=4.0,
                 final_time=1.0, max_steps=-1, initial_melted=0, nucleation=None, seed=0,
                 state=state, hot_cell=hot_cell)
+
+
+def write_frames_text(path, fe, *, unit=1e-3, offset=(0.5, -0.25, 0.0), time_unit=1e-3,
+                      t0=5.0, title_lines=4):
+    """Frames in the reference's text format (README, "PBF_x"): coordinates in
+    `unit` with `offset` removed, times in `time_unit` starting at t0.  Returns
+    the node coordinates and relative frame times a reader must recover,
+    computed the way the reader does ((c + offset) * unit, (t - t_first) * scale)."""
+    fnx, fny, fnz = fe["shape"]
+    xyz = np.asarray(fe["xyz"], np.float64)
+    raw = xyz / unit - np.asarray(offset)
+    t_raw = t0 + np.asarray(fe["times"], np.float64) / time_unit
+    theta = np.asarray(fe["theta"], np.float64)
+    lines = ["Title:", "frames for a test", "2026/12/02", "End of title"][:title_lines]
+    for f in range(len(t_raw)):
+        lines += ["" if f else "Subtitle (the second line)", "frames of a structured mesh",
+                  "No. step   No. step    time   time   x0    x1   y0   y1    z0   z1",
+                  f"{f + 1} {f + 1} {float(t_raw[f])!r} {float(t_raw[f])!r} 0 {fnx - 1} 0 {fny - 1} 0 {fnz - 1}",
+                  "x  y  z  tn"]
+        lines += [f"{float(a)!r} {float(b)!r} {float(c)!r} {float(v)!r}" for (a, b, c), v in zip(raw, theta[f])]
+    with open(path, "w") as fh:
+        fh.write("\n".join(lines) + "\n")
+    read_xyz = (raw + np.asarray(offset)) * unit
+    rel = np.concatenate([[0.0], (t_raw[1:] - t_raw[0]) * time_unit])
+    return read_xyz, rel
+
+
+def cafe_deck(case, *, mesh, unit=1e-3, offset=(0.5, -0.25, 0.0), time_unit=1e-3, micro_load=None,
+              micro_out=None, seeds=True, load_seeds=False, title_lines=4):
+    """A CafeSolver input deck (finite_element + cellular_automata_remelting)
+    for `case` (see track_case)."""
+    g, ph = case["grid"], case["physics"]
+    nuc = case.get("nucleation") or dict(surface=(0.0, 1.0, 0.5), bulk=(0.0, 5.0, 1.0))
+    opts = ["         - phase_state:",
+            f"             has_been_melted: {'yes' if case['initial_melted'] else 'no'}",
+            "         - cell_grain_length_cutoff_factor:",
+            f"             factor_value: {float(case['cutoff_factor'])!r}",
+            "         - criterion_to_activate_grain:",
+            f"             using_nucleation_seed: {'yes' if seeds else 'no'}",
+            "         - load_microstructure_information:",
+            f"             file_name: {micro_load or 'random'}"]
+    if micro_load and load_seeds:
+        opts += ["             load_seeds: yes"]
+    if micro_out:
+        opts += ["         - output_microstructure_information:",
+                 f"             file_name: {micro_out}"]
+    return "\n".join([
+        "simulations:", "  - name: test", "    time_integrator: ti_1", "",
+        "solvers:", "   - cellular_automata", "   - finite_element_method", "",
+        "transfers:", "   - name: fe_ca",
+        f"     bin_size: [{case['bin'][0]}, {case['bin'][1]}, {case['bin'][2]}]",
+        f"     tolerance: {float(case['tol'])!r}", f"     max_iterations: {case['max_iter']}",
+        f"     void_temperature: {float(case['void_temp'])!r}", "",
+        "realms:", "  - name: realm0", "    type: finite_element", "    dimension: 3",
+        f"    mesh: {mesh}", "    solution_options:", "       name: my_options",
+        "       options:", "         - load_data_from_file:", "             theta: theta",
+        "             for_whole_time: yes", f"             length_scale: {float(unit)!r}",
+        f"             time_scale: {float(time_unit)!r}", f"             lines_for_title: {title_lines}",
+        "             lines_for_subtitle: 5", f"             x_offset: {float(offset[0])!r}",
+        f"             y_offset: {float(offset[1])!r}", f"             z_offset: {float(offset[2])!r}",
+        "    output:", "      output_data_base_name: ./Results/FEM", "      output_frequency: 200000",
+        "",
+        "  - name: realm1", "    type: cellular_automata_remelting", "    dimension: 3",
+        f"    random_seed: {case.get('seed', 0)}",
+        "    domain:", "      type: cubic",
+        f"      original_point: [{float(g['x0'])!r}, {float(g['y0'])!r}, {float(g['z0'])!r}]",
+        "    discretization:", f"      cell_size: {float(g['h'])!r}",
+        f"      number_cells: [{g['nx']}, {g['ny']}, {g['nz']}]",
+        "    nucleation_rules:", "      - surface:", "          type: Gaussian",
+        f"          site_density: {float(nuc['surface'][0])!r}", f"          mean: {float(nuc['surface'][1])!r}",
+        f"          standard_deviation: {float(nuc['surface'][2])!r}",
+        "      - bulk:", "          type: Gaussian", f"          site_density: {float(nuc['bulk'][0])!r}",
+        f"          mean: {float(nuc['bulk'][1])!r}", f"          standard_deviation: {float(nuc['bulk'][2])!r}",
+        "    problem_physics:", "        type: RappazGandin",
+        f"        melting_temperature: {float(ph['Tm'])!r}", f"        solidus_temperature: {float(ph['Ts'])!r}",
+        f"        minimum_temperature_for_nucleation: {float(ph['Tmin_nuc'])!r}",
+        f"        a1: {float(ph['a1'])!r}", f"        a2: {float(ph['a2'])!r}", f"        a3: {float(ph['a3'])!r}",
+        "    solution_options:", "       name: my_options", "       options:"] + opts + [
+        "    output:", "      output_data_base_name: ./Results/Grains", "      output_frequency: -1",
+        "      output_variables:", "        - grain_orientation", "",
+        "time_integrators:", "  - standard_time_integrator:", "      name: ti_1",
+        f"      termination_time: {float(case['final_time'])!r}", "      termination_step_count: 10",
+        "      time_step: 0.001", f"      time_step_factor_ca: {float(case['tsf'])!r}",
+        "      realms:", "        - realm0", "        - realm1", ""])
+
+
+def read_micro_file(path):
+    """Parse a microstructure information file (the reference's text layout)."""
+    with open(path) as fh:
+        rows = fh.read().split("\n")
+    head = rows[1].split()
+    n_cells, n_or, n_gr, n_exp = (int(head[0]), int(head[1]), int(head[2]), int(head[3]))
+    cells = np.array([r.split() for r in rows[3:3 + n_cells]], dtype=object)
+    o0 = 4 + n_cells
+    ors = [r.split() for r in rows[o0:o0 + n_or]]
+    g0 = o0 + n_or + 1
+    grs = [r.split() for r in rows[g0:g0 + n_gr]]
+    return dict(header=head, n_expected=n_exp,
+                gid=cells[:, 1].astype(np.int32), ori=cells[:, 2].astype(np.int32),
+                melted=cells[:, 3].astype(np.uint8), nuc_dt=cells[:, 4].astype(np.float64),
+                orient_id=np.array([int(o[0]) for o in ors], np.int32),
+                euler=np.array([[float(v) for v in o[1:4]] for o in ors]),
+                grain_id=np.array([int(g[0]) for g in grs], np.int32),
+                grain_cell=np.array([int(g[1]) for g in grs], np.int32),
+                grain_centre=np.array([[float(v) for v in g[2:5]] for g in grs]),
+                grain_orient=np.array([int(g[5]) for g in grs], np.int32))

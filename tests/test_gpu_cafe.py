@@ -1,0 +1,177 @@
+"""Coupled CA-FE input decks (finite_element + cellular_automata_remelting
+realms) run through the library's realm layer on the GPU.
+
+The deck names a temperature file in the reference's text frame format; the
+realm reads it, draws a random substrate (one grain per interior cell) and the
+nucleation sites from its rand() stream, runs the domain to the end and writes
+the microstructure information file.  The oracle runs the same case from the
+frames parsed here independently, and the two must agree bit for bit; the
+written file must hold the device state and, loaded by a second deck, must
+give the same run as the oracle started from that state."""
+import numpy as np
+import pytest
+
+import oracle_lib as ol
+import remelt_cases as rc
+
+pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("oracle")]
+
+UNIT, TUNIT, OFFSET = 1e-3, 1e-3, (0.5, -0.25, 0.0)
+
+
+def parse_frames_text(path, title_lines=4, subtitle_lines=5, unit=UNIT, time_unit=TUNIT,
+                      offset=OFFSET):
+    """Reader of the text frame format: per frame `subtitle_lines` lines whose
+    fourth holds "step step time time x0 x1 y0 y1 z0 z1", then one line
+    "x y z T" per node (x fastest)."""
+    with open(path) as fh:
+        rows = fh.read().split("\n")
+    r = title_lines
+    xyz, times, theta, shape = None, [], [], None
+    while r + subtitle_lines <= len(rows):
+        head = rows[r + 3].split()
+        if len(head) < 10:
+            break
+        x0, x1, y0, y1, z0, z1 = (int(v) for v in head[4:10])
+        shape = (x1 - x0 + 1, y1 - y0 + 1, z1 - z0 + 1)
+        n = shape[0] * shape[1] * shape[2]
+        body = rows[r + subtitle_lines:r + subtitle_lines + n]
+        if len(body) < n or not body[-1].strip():
+            break
+        a = np.array([[float(v) for v in b.split()[:4]] for b in body])
+        if xyz is None:
+            xyz = (a[:, :3] + np.asarray(offset)) * unit
+        times.append(float(head[2]))
+        theta.append(a[:, 3])
+        r += subtitle_lines + n
+    t = np.asarray(times)
+    return dict(shape=shape, xyz=xyz, times=(t - t[0]) * time_unit, theta=np.stack(theta))
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view(np.uint64) if a.dtype == np.float64 else a
+
+
+def run_deck(deck_path):
+    from amca3d_amd.sim import Simulation
+    s = Simulation.from_file(deck_path)
+    s.initialize()
+    d = s.remelt
+    d.enable_log(100000)
+    c = s.run()
+    rc_ = d.counters()
+    out = d.download()
+    out.update(theta=d.theta(), log=d.read_log(), counters=rc_, run=c, euler=d.euler(),
+               sites=d.sites(), info=s.info())
+    s.finalize()
+    s.close()
+    return out
+
+
+def compare(orc, dev):
+    c = dev["counters"]
+    assert c["step"] == orc["n_steps"]
+    assert c["n_grains"] == orc["n_grains"]
+    assert c["n_orient"] == orc["n_orient"]
+    assert bits(np.float64(c["time"])) == bits(np.float64(orc["time"]))
+    np.testing.assert_array_equal(dev["ori"], orc["ori"])
+    np.testing.assert_array_equal(dev["gid"], orc["gid"])
+    np.testing.assert_array_equal(bits(dev["T"]), bits(orc["T"]))
+    np.testing.assert_array_equal(dev["melted"], orc["melted"])
+    np.testing.assert_array_equal(bits(dev["theta"]), bits(orc["theta"]))
+    n = orc["n_grains"]
+    np.testing.assert_array_equal(dev["grain_cell"], orc["grain_cell"][:n])
+    np.testing.assert_array_equal(dev["grain_orient"], orc["grain_orient"][:n])
+    np.testing.assert_array_equal(bits(dev["grain_env"]), bits(orc["grain_env"][:n]))
+    np.testing.assert_array_equal(bits(dev["euler"]), bits(orc["euler"][:orc["n_orient"]]))
+    lg, so = dev["log"], orc["steps"]
+    assert len(lg) == len(so)
+    for k in ("n_nucleated", "n_captured", "n_fresh", "n_flagged"):
+        np.testing.assert_array_equal(lg[k], so[k], err_msg=k)
+    for k in ("time_before", "dt"):
+        np.testing.assert_array_equal(bits(lg[k]), bits(so[k]), err_msg=k)
+
+
+@pytest.fixture(scope="module")
+def cafe_run(tmp_path_factory):
+    """Deck 1: random substrate and sites; deck 2: loads deck 1's file."""
+    tmp = tmp_path_factory.mktemp("cafe")
+    nuc = dict(surface=(2e10, 2.0, 0.5), bulk=(2e15, 5.0, 1.0))
+    case = rc.track_case(nx=56, ny=20, nz=16, n_frames=16, P=30.0, nucleation=nuc, seed=17)
+    rc.write_frames_text(tmp / "frames.txt", case["fe"], unit=UNIT, offset=OFFSET,
+                         time_unit=TUNIT)
+    (tmp / "deck1.yaml").write_text(rc.cafe_deck(case, mesh="frames.txt", micro_out="grainInfo"))
+    dev1 = run_deck(str(tmp / "deck1.yaml"))
+    fe = parse_frames_text(tmp / "frames.txt")
+    ocase = dict(case, fe=fe, state=None, random_micro=True)
+    orc1 = ol.run_remelt(ocase)
+    return dict(tmp=tmp, case=case, ocase=ocase, dev1=dev1, orc1=orc1)
+
+
+def test_deck_with_random_substrate_matches_the_oracle(amca, cafe_run):
+    dev, orc = cafe_run["dev1"], cafe_run["orc1"]
+    nint = 56 * 20 * 16
+    assert dev["info"]["coupled"] == 1 and dev["info"]["n_init_grains"] == nint
+    assert orc["steps"]["n_captured"].sum() > 1000
+    assert orc["steps"]["n_nucleated"].sum() > 20
+    cell, dt = dev["sites"]
+    np.testing.assert_array_equal(cell, orc["site_cell"])
+    np.testing.assert_array_equal(bits(dt), bits(orc["site_dt"]))
+    compare(orc, dev)
+
+
+def test_microstructure_file_holds_the_device_state(amca, cafe_run):
+    dev, orc = cafe_run["dev1"], cafe_run["orc1"]
+    m = rc.read_micro_file(cafe_run["tmp"] / "grainInfo0.txt")
+    n, no = orc["n_grains"], orc["n_orient"]
+    np.testing.assert_array_equal(m["gid"], dev["gid"])
+    np.testing.assert_array_equal(m["ori"], dev["ori"])
+    np.testing.assert_array_equal(m["melted"], dev["melted"])
+    sdt = np.full(len(m["gid"]), -1.0)
+    sdt[orc["site_cell"]] = orc["site_dt"]
+    np.testing.assert_allclose(m["nuc_dt"], sdt, rtol=1e-5, atol=1e-12)
+    np.testing.assert_array_equal(m["orient_id"], np.arange(no))
+    np.testing.assert_allclose(m["euler"], orc["euler"][:no].reshape(-1, 3), rtol=1e-5, atol=1e-9)
+    np.testing.assert_array_equal(m["grain_id"], np.arange(n))
+    np.testing.assert_array_equal(m["grain_cell"], orc["grain_cell"][:n])
+    np.testing.assert_array_equal(m["grain_orient"], orc["grain_orient"][:n])
+    env = orc["grain_env"][:n]
+    np.testing.assert_allclose(m["grain_centre"], env[:, :3], rtol=1e-5, atol=1e-12)
+    assert m["n_expected"] == len(orc["site_cell"])
+
+
+@pytest.mark.parametrize("seeds", ["none", "loaded_and_drawn"])
+def test_deck_loading_the_written_file(amca, cafe_run, seeds):
+    """The written file as the next deck's initial state (default ostream
+    precision: the loaded Euler angles and seed undercoolings are those
+    six-digit values).  "none": no nucleation; "loaded_and_drawn": the file's
+    seeds (load_seeds) with new surface and bulk draws merged into them."""
+    tmp, case = cafe_run["tmp"], cafe_run["case"]
+    drawn = seeds != "none"
+    deck = rc.cafe_deck(case, mesh="frames.txt", micro_load="grainInfo",
+                        micro_out=f"grainInfo_{seeds}", seeds=drawn, load_seeds=drawn)
+    (tmp / f"deck_{seeds}.yaml").write_text(deck)
+    dev = run_deck(str(tmp / f"deck_{seeds}.yaml"))
+    m = rc.read_micro_file(tmp / "grainInfo0.txt")
+    state = dict(ori=m["ori"], gid=m["gid"], melted=m["melted"], euler=m["euler"],
+                 grain_cell=m["grain_cell"], grain_orient=m["grain_orient"])
+    ocase = dict(cafe_run["ocase"], random_micro=False, state=state,
+                 nucleation=case["nucleation"] if drawn else None,
+                 init_site_dt=m["nuc_dt"] if drawn else None)
+    orc = ol.run_remelt(ocase)
+    assert orc["steps"]["n_captured"].sum() > 0
+    compare(orc, dev)
+    out = rc.read_micro_file(tmp / f"grainInfo_{seeds}0.txt")
+    if drawn:
+        n_loaded = int((m["nuc_dt"] > -1).sum())
+        assert len(orc["site_cell"]) > n_loaded
+        cell, dt = dev["sites"]
+        np.testing.assert_array_equal(cell, orc["site_cell"])
+        np.testing.assert_array_equal(bits(dt), bits(orc["site_dt"]))
+        assert out["n_expected"] == m["n_expected"] + dev["info"]["n_surface_sites"] + \
+            dev["info"]["n_bulk_sites"]
+    else:
+        assert orc["steps"]["n_nucleated"].sum() == 0
+        np.testing.assert_array_equal(out["nuc_dt"], np.full(len(m["gid"]), -1.0))
+        assert out["n_expected"] == m["n_expected"]
