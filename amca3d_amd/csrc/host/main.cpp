@@ -52,6 +52,12 @@ int main(int argc, char** argv) {
     amca_sim_destroy(sim);
     return 1;
   }
+  // output_microstructure_information of a coupled deck
+  if (amca_sim_finalize(sim) != AMCA_OK) {
+    std::fprintf(stderr, "error: %s\n", amca_sim_last_error());
+    amca_sim_destroy(sim);
+    return 1;
+  }
   double sec = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   std::printf("steps %lld time %.9g remaining liquid cells %lld grains %d orientations %d wall %.3f s\n",
               (long long)c.step, c.time, (long long)c.n_liquid, c.n_grains, c.n_orient, sec);
