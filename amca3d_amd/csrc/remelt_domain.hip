@@ -118,6 +118,18 @@ amca_status rm_put_counters(amca_remelt* d, const amca::RmCounters& c) {
   return AMCA_OK;
 }
 
+// the next step runs over the whole grid and restarts the box bookkeeping
+void rm_restart_box(amca::RmCounters& c) {
+  for (int q = 0; q < 2; q++) {
+    for (int r = 0; r < 3; r++)
+      for (int a = 0; a < 3; a++) {
+        c.acc[q][r].lo[a] = INT_MAX;
+        c.acc[q][r].hi[a] = INT_MIN;
+      }
+  }
+  c.force_full = 1;
+}
+
 void rm_fill(const amca_remelt* d, const amca::RmCounters& c, amca_remelt_counters* o) {
   if (!o) return;
   o->step = c.step;
@@ -271,6 +283,10 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
   P.final_time = desc->final_time;
   P.void_temp = desc->void_temperature;
   P.orient_cap = orient_capacity;
+  P.window = 1;
+  P.hot_margin = 1.0;
+  P.sur_hot[0] = 1;
+  P.sur_hot[1] = 0;
   {
     int s = 0;
     for (int dk = -1; dk <= 1; dk++)
@@ -323,6 +339,14 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
   (void)hipMemsetAsync(B.lb_flag, 0, sizeof(unsigned) * amca::kMaxBlocks, d->stream);
   amca::RmCounters c{};
   c.min_t_key = ~0ull;
+  c.last_step = -1;
+  for (int q = 0; q < 2; q++) {
+    c.box[q].lo[0] = c.box[q].lo[1] = c.box[q].lo[2] = 0;
+    c.box[q].hi[0] = P.nxG - 1;
+    c.box[q].hi[1] = P.nyG - 1;
+    c.box[q].hi[2] = P.nzG - 1;
+  }
+  rm_restart_box(c);
   if (hipMemcpyAsync(B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream) != hipSuccess ||
       amca::rm_launch_init(P, B, d->stream) != hipSuccess ||
       hipStreamSynchronize(d->stream) != hipSuccess) {
@@ -370,10 +394,46 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   d->release((void*)d->B.frames);
   d->release((void*)d->B.frame_time);
   d->release(d->B.theta);
+  d->release(d->B.elem_box);
   double *fr = nullptr, *ft = nullptr, *th = nullptr;
+  int* eb = nullptr;
+  const size_t ne = (size_t)(fnx - 1) * (fny - 1) * (fnz - 1);
   RM_TRY(d->alloc(&fr, nodes * (size_t)n_frames));
   RM_TRY(d->alloc(&ft, (size_t)n_frames));
-  RM_TRY(d->alloc(&th, nodes));
+  RM_TRY(d->alloc(&th, 3 * nodes));
+  RM_TRY(d->alloc(&eb, 6 * ne));
+  // cell box of each element's cells, for the active box
+  {
+    std::vector<int32_t> box(6 * ne);
+    for (size_t e = 0; e < ne; e++) {
+      box[6 * e + 0] = box[6 * e + 1] = box[6 * e + 2] = INT_MAX;
+      box[6 * e + 3] = box[6 * e + 4] = box[6 * e + 5] = INT_MIN;
+    }
+    for (int c = 0; c < P.NG; c++) {
+      const int e = d->host_tag[c];
+      if (e < 0) continue;
+      const int k = c / P.nxyG, j = (c % P.nxyG) / P.nxG, i = (c % P.nxyG) % P.nxG;
+      int32_t* b = &box[6 * (size_t)e];
+      b[0] = std::min(b[0], i);
+      b[1] = std::min(b[1], j);
+      b[2] = std::min(b[2], k);
+      b[3] = std::max(b[3], i);
+      b[4] = std::max(b[4], j);
+      b[5] = std::max(b[5], k);
+    }
+    RM_HIP(hipMemcpyAsync(eb, box.data(), sizeof(int32_t) * box.size(), hipMemcpyHostToDevice,
+                          d->stream));
+    RM_HIP(hipStreamSynchronize(d->stream));
+  }
+  // no void node in any frame (and so in no interpolated field): no cell's
+  // material changes
+  {
+    const double nm = P.void_temp + 0.00001;
+    bool any_void = false;
+    for (size_t q = 0; q < nodes * (size_t)n_frames && !any_void; q++)
+      any_void = !(frame_theta[q] > nm + 1e-6);
+    P.no_void = any_void ? 0 : 1;
+  }
   std::vector<double> rel(n_frames);
   for (int f = 0; f < n_frames; f++) rel[f] = frame_time[f] - frame_time[0];  // inTimeOffset_
   RM_HIP(hipMemcpyAsync(fr, frame_theta, sizeof(double) * nodes * n_frames, hipMemcpyHostToDevice, d->stream));
@@ -386,8 +446,12 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   d->B.frames = fr;
   d->B.frame_time = ft;
   d->B.theta = th;
+  d->B.elem_box = eb;
   amca::RmCounters c;
   RM_TRY(rm_counters(d, &c));
+  c.th_cur = 0;
+  c.th_prev = 0;
+  rm_restart_box(c);
   c.frame = 1;
   c.in_t1 = rel[0];
   c.in_t0 = rel[1];
@@ -513,6 +577,7 @@ amca_status amca_remelt_set_state(amca_remelt* d, const int32_t* orient, const i
   RM_TRY(rm_counters(d, &c));
   c.n_grains = n_grains;
   c.n_orient = n_orient;
+  rm_restart_box(c);
   RM_TRY(rm_put_counters(d, c));
   d->state_ready = true;
   return AMCA_OK;
@@ -687,6 +752,7 @@ amca_status amca_remelt_read_log(amca_remelt* d, int32_t cap, amca_remelt_step_r
       rows[i].n_fresh = tmp[i].n_fresh;
       rows[i].n_flagged = tmp[i].n_flagged;
       rows[i].n_fallback = tmp[i].n_fallback;
+      rows[i].n_box = tmp[i].n_box;
     }
   }
   return AMCA_OK;
@@ -714,7 +780,35 @@ amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_count
     RM_TRY(rm_counters(d, &c));
     batch = std::min<long long>(batch * 2, 64);
   }
+  // cells outside the last step's box take that step's T and cooling
+  if (c.step > start) {
+    RM_HIP(amca::rm_launch_complete(d->P, d->B, d->stream));
+    RM_HIP(hipStreamSynchronize(d->stream));
+  }
   rm_fill(d, c, out);
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_set_window(amca_remelt* d, int32_t on) {
+  if (!d) return rm_fail(AMCA_ERR_INVALID, "null domain");
+  d->P.window = on ? 1 : 0;
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  rm_restart_box(c);
+  RM_TRY(rm_put_counters(d, c));
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_last_box(amca_remelt* d, int32_t* box6) {
+  if (!d || !box6) return rm_fail(AMCA_ERR_INVALID, "null argument");
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  if (c.last_step < 0) return rm_fail(AMCA_ERR_INVALID, "no step has run");
+  const amca::RmBox& b = c.box[c.last_step & 1];
+  for (int a = 0; a < 3; a++) {
+    box6[a] = b.lo[a];
+    box6[3 + a] = b.hi[a];
+  }
   return AMCA_OK;
 }
 
@@ -792,7 +886,7 @@ amca_status amca_remelt_use_surrogate(amca_remelt* d, amca_mlp* m, const amca_su
   RM_TRY(d->alloc(&tc, (size_t)nxi * P.nyG * P.nzG));
   RM_TRY(d->alloc(&tn, (size_t)nxi * P.fny * P.fnz));
   RM_TRY(d->alloc(&nb, (size_t)P.nodes));
-  RM_TRY(d->alloc(&eb, (size_t)(P.fnx - 1) * (P.fny - 1) * (P.fnz - 1)));
+  RM_TRY(d->alloc(&eb, 2 * (size_t)(P.fnx - 1) * (P.fny - 1) * (P.fnz - 1)));
   RM_TRY(d->alloc(&stats, 8));
   int* slots = nullptr;
   RM_TRY(d->alloc(&slots, 256));
@@ -822,6 +916,28 @@ amca_status amca_remelt_use_surrogate(amca_remelt* d, amca_mlp* m, const amca_su
   P.fy0 = sd->fy0;
   P.fz0 = sd->fz0;
   P.fdx = sd->fdx;
+  // hot region of the cell table (xi index, j, k), for the active box
+  {
+    int* hb = nullptr;
+    RM_TRY(d->alloc(&hb, 6));
+    const int init[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    RM_HIP(hipMemcpy(hb, init, sizeof(init), hipMemcpyHostToDevice));
+    RM_HIP(amca::rm_launch_sur_hot(P, d->B, hb, d->stream));
+    int got[6];
+    RM_HIP(hipMemcpyAsync(got, hb, sizeof(got), hipMemcpyDeviceToHost, d->stream));
+    RM_HIP(hipStreamSynchronize(d->stream));
+    d->release(hb);
+    P.sur_hot[0] = got[0];
+    P.sur_hot[1] = got[3];
+    P.sur_hot[2] = got[1];
+    P.sur_hot[3] = got[4];
+    P.sur_hot[4] = got[2];
+    P.sur_hot[5] = got[5];
+  }
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  rm_restart_box(c);
+  RM_TRY(rm_put_counters(d, c));
   return AMCA_OK;
 }
 
@@ -846,8 +962,10 @@ amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6) {
 
 amca_status amca_remelt_theta(amca_remelt* d, double* theta) {
   if (!d || !d->fe_ready || !theta) return rm_fail(AMCA_ERR_INVALID, "FE mesh not set");
-  RM_HIP(hipStreamSynchronize(d->stream));
-  RM_HIP(hipMemcpy(theta, d->B.theta, sizeof(double) * d->P.nodes, hipMemcpyDeviceToHost));
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  RM_HIP(hipMemcpy(theta, d->B.theta + (size_t)c.th_cur * d->P.nodes,
+                   sizeof(double) * d->P.nodes, hipMemcpyDeviceToHost));
   return AMCA_OK;
 }
 

@@ -25,6 +25,16 @@ struct RmParams {
   double sur_xi0, sur_dxi, sur_x_start, sur_v, sur_band, sur_resid;
   int sur_nxi, sur_nxi_nodes;
   double fx0, fy0, fz0, fdx;             // uniform node lattice of the structured mesh
+  int sur_hot[6];        // xi-index, j, k ranges of the cell table at or above Tm - sur_band
+  // active box: each step runs over the box of cells that can change (see k_rm_window)
+  int window;
+  int no_void;           // no frame holds a void node: the material of a cell never changes
+  double hot_margin;     // K below Tm at which an element counts as hot
+};
+
+// inclusive cell-index box (ghost layer included); empty when lo > hi
+struct RmBox {
+  int lo[3], hi[3];
 };
 
 struct RmCounters {
@@ -45,12 +55,25 @@ struct RmCounters {
   int log_cap;
   int n_fallback;      // cells of the running step that took the FE path
   unsigned tickets[4];
+  // active box of step n: box[n & 1], from acc[n & 1] = {hot elements at step
+  // n, liquid cells with material after the phase of step n-1, cells
+  // captured in step n-1}, each widened by two cells
+  RmBox box[2];
+  RmBox acc[2][3];
+  int force_full;      // the next step runs over the whole grid
+  int sur_used[2];     // step n used the surrogate: sur_used[n & 1]
+  int th_cur, th_prev; // slots of the nodal-field ring: this step's field, the last step's
+  double time_fe_prev; // FE time of the last step
+  // the last phase, for the completion pass: field slots, step, FE times and the dt of its rate
+  int last_cur, last_prev;
+  long long last_step;
+  double last_time_fe, last_time_fe_prev, last_dt;
 };
 
 struct RmLogRow {
   double time_before, dt;
   int n_nucleated, n_captured, n_fresh, n_flagged;
-  int n_fallback, pad;
+  int n_fallback, n_box;
 };
 
 struct RmBuffers {
@@ -68,7 +91,8 @@ struct RmBuffers {
   int2* gco;              // grain -> (cell, orientation) [N]
   const double* frames;   // nodal frames [n_frames * nodes]
   const double* frame_time;  // [n_frames], relative to the first
-  double* theta;          // thetaArray_ [nodes]
+  double* theta;          // thetaArray_: ring of three nodal fields [3 * nodes], slot ctr->th_cur
+  int* elem_box;          // cell box of each element's cells (i0 j0 k0 i1 j1 k1) [6 * elements]
   int* site_cell;         // nucleation sites, ascending cell (never erased here)
   double* site_dt;
   double* l2g;            // orientation rows, 9 doubles each
@@ -82,19 +106,23 @@ struct RmBuffers {
   const float* sur_cell;       // [nzG][nyG][sur_nxi] surrogate temperature at cell (y, z)
   const float* sur_node;       // [fnz][fny][sur_nxi_nodes] at node (y, z)
   unsigned char* node_bad;     // [nodes]
-  unsigned char* elem_fb;      // [elements]
+  unsigned char* elem_fb;      // [2][elements], step parity
   double* sur_stats;           // [8] error statistics
   int* fb_slots;               // [256] per-step fallback cell counts by block slot
 };
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s);
 // One step: kernels 0..kRmKernels-1 in order on stream s.
-constexpr int kRmKernels = 13;
+constexpr int kRmKernels = 14;
 extern const char* const kRmKernelNames[kRmKernels];
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s);
 hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s);
 // surrogate error statistics into B.sur_stats (zeroed by the caller)
 hipError_t rm_launch_sur_error(const RmParams& P, const RmBuffers& B, hipStream_t s);
+// T and cooling of the last step for every cell outside its active box
+hipError_t rm_launch_complete(const RmParams& P, const RmBuffers& B, hipStream_t s);
+// P.sur_hot from the cell table (B.sur_cell), written to *out (6 ints, device)
+hipError_t rm_launch_sur_hot(const RmParams& P, const RmBuffers& B, int* out, hipStream_t s);
 
 }  // namespace amca

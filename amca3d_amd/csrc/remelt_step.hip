@@ -1,7 +1,9 @@
 // Kernels of the coupled CA-FE step (realm type `cellular_automata_remelting`
 // driven by CafeSolver::time_integration_weak_coupling, CafeSolver.cpp:178-219).
 //
-// One step, in the reference's order:
+// One step, in the reference's order (each grid kernel over the step's active
+// box, see k_rm_window):
+//   k_rm_window   the active box: hot elements, liquid and captured cells
 //   k_rm_phase    temperature of every cell from the FE nodal field (hex8
 //                 interpolation, MapVoxelManager::evaluate_temperature), cooling
 //                 flag, melting (checkup_current_temperature_to_update_phase_state,
@@ -73,7 +75,8 @@ __device__ __forceinline__ double dkey_inv(unsigned long long k) {
 
 // MapVoxelManager::evaluate_temperature (MapVoxelManager.cpp:746-799) on a
 // structured mesh: element e has corner node ((ek*fny + ej)*fnx + ei).
-__device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B, int c, double& out) {
+__device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B, const double* theta,
+                                          int c, double& out) {
   const double nm = P.void_temp + 0.00001;
   const int e = B.tag[c];
   if (e < 0) return false;
@@ -92,7 +95,7 @@ __device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B,
   nid[7] = nid[3] + nxy;
   double th[8];
 #pragma unroll
-  for (int m = 0; m < 8; m++) th[m] = B.theta[nid[m]];
+  for (int m = 0; m < 8; m++) th[m] = theta[nid[m]];
 #pragma unroll
   for (int m = 0; m < 8; m++)
     if (th[m] < nm) return false;
@@ -130,6 +133,141 @@ __device__ __forceinline__ double rm_sur_T(const RmParams& P, const RmBuffers& B
   return (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
 }
 
+// Temperature of cell c at one step: the FE path on nodal field `theta`, or
+// with the surrogate (sur) the table value at FE time tfe unless the cell's
+// element falls back (efb) or the value is within sur_band of Tm.  Returns
+// whether the cell has material; n_fb counts cells of an element that took
+// the FE path.
+__device__ __forceinline__ bool rm_cell_T(const RmParams& P, const RmBuffers& B, int c,
+                                          const double* theta, const unsigned char* efb,
+                                          double tfe, int sur, double& Tn, int& n_fb) {
+  Tn = -1.0;
+  if (!sur) return rm_eval_T(P, B, theta, c, Tn);
+  const int e = B.tag[c];
+  bool fe = e < 0 || efb[e];
+  if (!fe) {
+    int si, sj, sk;
+    rm_ijk(P, c, si, sj, sk);
+    Tn = rm_sur_T(P, B, si, sj, sk, tfe);
+    fe = fabs(Tn - P.Tm) < P.sur_band;
+  }
+  if (!fe) return true;
+  const bool mat = rm_eval_T(P, B, theta, c, Tn);
+  if (!mat) Tn = -1.0;
+  if (e >= 0) n_fb++;
+  return mat;
+}
+
+__device__ __forceinline__ const double* rm_field(const RmParams& P, const RmBuffers& B, int slot) {
+  return B.theta + (size_t)slot * P.nodes;
+}
+__device__ __forceinline__ const unsigned char* rm_fb(const RmParams& P, const RmBuffers& B, int par) {
+  return B.elem_fb ? B.elem_fb + (size_t)par * (P.ncx * P.ncy * (P.fnz - 1)) : nullptr;
+}
+
+// ---------------------------------------------------------------------------
+// active box
+//
+// A cell outside the box of step n changes nothing in step n but its
+// temperature and cooling flag, which nothing reads that step: it is solid
+// and has material, its elements are below Tm - hot_margin at every node (the
+// hex8 value is a convex combination of them), it has no undercooled liquid
+// neighbour (so it is not flagged) and no flagged neighbour (so it is not
+// captured).  The box of step n is the union of
+//   - the cells of elements with a node at or above Tm - hot_margin, or whose
+//     nodes change between void and material, at step n (k_rm_window), and
+//     with the surrogate the table's hot region at step n;
+//   - the liquid cells with material after the phase of step n-1 (k_rm_liquid);
+//   - the cells captured in step n-1 (a capture of a cell without material is
+//     undone by the next phase) (k_rm_emit),
+// widened by two cells (flags reach one cell, captures one more).  The first
+// step, and any step after the state or the temperature source changed, runs
+// over the whole grid.  A cell that enters the box takes its last-step
+// temperature for the cooling rate from the last step's field, which is what
+// it would have stored; k_rm_complete writes T and cooling of the cells
+// outside the last box when a run returns.
+
+__device__ __forceinline__ void box_empty(RmBox& b) {
+  for (int a = 0; a < 3; a++) {
+    b.lo[a] = INT_MAX;
+    b.hi[a] = INT_MIN;
+  }
+}
+__device__ __forceinline__ void box_add(RmBox& b, int i, int j, int k) {
+  b.lo[0] = min(b.lo[0], i);
+  b.lo[1] = min(b.lo[1], j);
+  b.lo[2] = min(b.lo[2], k);
+  b.hi[0] = max(b.hi[0], i);
+  b.hi[1] = max(b.hi[1], j);
+  b.hi[2] = max(b.hi[2], k);
+}
+__device__ __forceinline__ bool box_has(const RmBox& b, int i, int j, int k) {
+  return i >= b.lo[0] && i <= b.hi[0] && j >= b.lo[1] && j <= b.hi[1] && k >= b.lo[2] &&
+         k <= b.hi[2];
+}
+__device__ __forceinline__ void box_full(const RmParams& P, RmBox& b) {
+  b.lo[0] = b.lo[1] = b.lo[2] = 0;
+  b.hi[0] = P.nxG - 1;
+  b.hi[1] = P.nyG - 1;
+  b.hi[2] = P.nzG - 1;
+}
+// extents of b clipped to [lo, hi] on every axis; all 0 when the clipped box
+// is empty (an empty box holds lo = INT_MAX, hi = INT_MIN: no difference of
+// the raw bounds is taken, it would overflow)
+__device__ __forceinline__ void box_clip(const RmBox& b, const int* lo, const int* hi, int* o0,
+                                         int* ext) {
+  bool empty = false;
+  for (int a = 0; a < 3; a++) {
+    const int l = max(b.lo[a], lo[a]);
+    const int h = min(b.hi[a], hi[a]);
+    o0[a] = l;
+    empty |= h < l;
+    ext[a] = h < l ? 0 : h - l + 1;
+  }
+  if (empty)
+    for (int a = 0; a < 3; a++) {
+      o0[a] = lo[a];
+      ext[a] = 0;
+    }
+}
+
+// Merge every thread's box into *dst: a min/max reduction over the block,
+// then one atomic per bound.
+__device__ __forceinline__ void block_box_merge(RmBox b, RmBox* dst) {
+  __shared__ int s_b[kRmBlock / 64][6];
+  int v[6] = {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]};
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      const int w = __shfl_xor(v[q], o, 64);
+      v[q] = q < 3 ? min(v[q], w) : max(v[q], w);
+    }
+  if ((threadIdx.x & 63) == 0)
+#pragma unroll
+    for (int q = 0; q < 6; q++) s_b[threadIdx.x >> 6][q] = v[q];
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    for (int w = 1; w < kRmBlock / 64; w++)
+      for (int q = 0; q < 6; q++) v[q] = q < 3 ? min(v[q], s_b[w][q]) : max(v[q], s_b[w][q]);
+    if (v[0] <= v[3]) {
+      for (int q = 0; q < 3; q++) {
+        atomicMin(&dst->lo[q], v[q]);
+        atomicMax(&dst->hi[q], v[q + 3]);
+      }
+    }
+  }
+}
+
+// the box of the running step (written by k_rm_phase)
+__device__ __forceinline__ RmBox rm_step_box(const RmCounters* ctr) {
+  return ctr->box[ctr->step & 1];
+}
+
+__device__ __forceinline__ bool rm_skip_now(const RmParams& P, const RmCounters* ctr) {
+  return !(ctr->tcafe < P.final_time && !ctr->exhausted);
+}
+
 // ---------------------------------------------------------------------------
 
 __global__ void k_rm_init(RmParams P, RmBuffers B) {
@@ -143,49 +281,148 @@ __global__ void k_rm_init(RmParams P, RmBuffers B) {
   }
 }
 
-__global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) {
+// Hot elements of the running step, and the surrogate's hot region, into
+// acc[step & 1][0].
+__global__ void __launch_bounds__(kRmBlock) k_rm_window(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
-  const int skip = (ctr->tcafe < P.final_time && !ctr->exhausted) ? 0 : 1;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->skip = skip;
-  if (skip) return;
-  int n_fb = 0;
-  const bool later = ctr->step > 0;
-  const double dt_prev = ctr->dt;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
-    const int c = (int)cc;
-    double Tn = -1.0;
-    bool mat;
-    if (!P.sur) {
-      mat = rm_eval_T(P, B, c, Tn);
-    } else {
-      const int e = B.tag[c];
-      bool fe = e < 0 || B.elem_fb[e];
-      if (!fe) {
-        int si, sj, sk;
-        rm_ijk(P, c, si, sj, sk);
-        Tn = rm_sur_T(P, B, si, sj, sk, ctr->time_fe);
-        fe = fabs(Tn - P.Tm) < P.sur_band;
-      }
-      if (fe) {
-        mat = rm_eval_T(P, B, c, Tn);
-        if (!mat) Tn = -1.0;
-        if (e >= 0) n_fb++;
-      } else {
-        mat = true;
+  if (rm_skip_now(P, ctr)) return;
+  const long long n = ctr->step;
+  if (!P.window || ctr->force_full || n == 0) return;
+  const int par = (int)(n & 1);
+  const double* th = rm_field(P, B, ctr->th_cur);
+  const double* tp = rm_field(P, B, ctr->th_prev);
+  const double hot = P.Tm - P.hot_margin;
+  const double nm = P.void_temp + 0.00001;
+  const int ne = P.ncx * P.ncy * (P.fnz - 1);
+  const int nxy = P.fnx * P.fny;
+  RmBox b;
+  box_empty(b);
+  const int stride = gridDim.x * blockDim.x;
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
+    const int ei = e % P.ncx, ej = (e / P.ncx) % P.ncy, ek = e / (P.ncx * P.ncy);
+    const int n0 = nxy * ek + P.fnx * ej + ei;
+    const int nid[8] = {n0, n0 + 1, n0 + P.fnx, n0 + P.fnx + 1,
+                        n0 + nxy, n0 + nxy + 1, n0 + nxy + P.fnx, n0 + nxy + P.fnx + 1};
+    double mx = th[nid[0]];
+    bool vn = mx < nm, vp = false;
+#pragma unroll
+    for (int m = 1; m < 8; m++) {
+      const double v = th[nid[m]];
+      mx = fmax(mx, v);
+      vn |= v < nm;
+    }
+    bool take = mx >= hot;
+    if (!take && !P.no_void) {
+#pragma unroll
+      for (int m = 0; m < 8; m++) vp |= tp[nid[m]] < nm;
+      take = vn != vp;
+    }
+    if (take) {
+      const int* eb = B.elem_box + 6 * (size_t)e;
+      if (eb[0] <= eb[3]) {
+        box_add(b, eb[0], eb[1], eb[2]);
+        box_add(b, eb[3], eb[4], eb[5]);
       }
     }
+  }
+  if (P.sur && blockIdx.x == 0 && threadIdx.x == 0 && P.sur_hot[0] <= P.sur_hot[1]) {
+    // cell i reads table entries floor(u), floor(u) + 1 (clamped), with
+    // u = (x0 + h (i - 0.5) - x_start - v t - xi0) / dxi
+    const double t = ctr->time_fe;
+    const double base = P.sur_x_start + P.sur_v * t + P.sur_xi0 - P.x0;
+    int ilo = 0, ihi = P.nxG - 1;
+    if (P.sur_hot[0] > 0) {
+      const double x = (P.sur_hot[0] - 1) * P.sur_dxi + base;
+      ilo = max(0, (int)floor(x / P.h + 0.5) - 2);
+    }
+    if (P.sur_hot[1] < P.sur_nxi - 2) {
+      const double x = (P.sur_hot[1] + 1) * P.sur_dxi + base;
+      ihi = min(P.nxG - 1, (int)ceil(x / P.h + 0.5) + 2);
+    }
+    if (ilo <= ihi) {
+      box_add(b, ilo, P.sur_hot[2], P.sur_hot[4]);
+      box_add(b, ihi, P.sur_hot[3], P.sur_hot[5]);
+    }
+  }
+  block_box_merge(b, &ctr->acc[par][0]);
+}
+
+__global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  const int skip = rm_skip_now(P, ctr) ? 1 : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->skip = skip;
+  if (skip) return;
+  const long long n = ctr->step;
+  const int par = (int)(n & 1);
+  // the box of this step
+  RmBox R;
+  if (!P.window || ctr->force_full || n == 0) {
+    box_full(P, R);
+  } else {
+    box_empty(R);
+    for (int q = 0; q < 3; q++) {
+      const RmBox& a = ctr->acc[par][q];
+      if (a.lo[0] > a.hi[0]) continue;
+      box_add(R, a.lo[0], a.lo[1], a.lo[2]);
+      box_add(R, a.hi[0], a.hi[1], a.hi[2]);
+    }
+    if (R.lo[0] <= R.hi[0]) {
+      const int top[3] = {P.nxG - 1, P.nyG - 1, P.nzG - 1};
+      for (int a = 0; a < 3; a++) {
+        R.lo[a] = max(0, R.lo[a] - 2);
+        R.hi[a] = min(top[a], R.hi[a] + 2);
+      }
+    }
+  }
+  const bool later = n > 0;
+  const double dt_prev = ctr->dt;
+  const RmBox Rp = ctr->box[par ^ 1];
+  const double* th = rm_field(P, B, ctr->th_cur);
+  const double* thp = rm_field(P, B, ctr->th_prev);
+  const unsigned char* efb = rm_fb(P, B, par);
+  const unsigned char* efbp = rm_fb(P, B, par ^ 1);
+  const double tfe = ctr->time_fe, tfe_p = ctr->time_fe_prev;
+  const int sur_p = ctr->sur_used[par ^ 1];
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr->box[par] = R;
+    ctr->sur_used[par] = P.sur;
+    ctr->last_cur = ctr->th_cur;
+    ctr->last_prev = ctr->th_prev;
+    ctr->last_step = n;
+    ctr->last_time_fe = tfe;
+    ctr->last_time_fe_prev = tfe_p;
+    ctr->last_dt = dt_prev;
+  }
+  const int zero[3] = {0, 0, 0}, top[3] = {P.nxG - 1, P.nyG - 1, P.nzG - 1};
+  int o0[3], ext[3];
+  box_clip(R, zero, top, o0, ext);
+  const int exy = ext[0] * ext[1];
+  const int cnt = exy * ext[2];
+  int n_fb = 0;
+  const int stride = gridDim.x * blockDim.x;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += stride) {
+    const int k = o0[2] + q / exy;
+    const int rem = q - (k - o0[2]) * exy;
+    const int j = o0[1] + rem / ext[0];
+    const int i = o0[0] + rem % ext[0];
+    const int c = k * P.nxyG + j * P.nxG + i;
+    double Tn;
+    const bool mat = rm_cell_T(P, B, c, th, efb, tfe, P.sur, Tn, n_fb);
     if (later) {
+      double Tp;
+      if (box_has(Rp, i, j, k)) {
+        Tp = B.T[c];
+      } else {
+        int unused = 0;
+        (void)rm_cell_T(P, B, c, thp, efbp, tfe_p, sur_p, Tp, unused);
+      }
       double rate = 410;
-      const double Tp = B.T[c];
       if (Tp > 0) rate = (Tn - Tp) / dt_prev;
       if (Tn <= 0) rate = 0;
       B.cooling[c] = mat ? (rate < 0) : 0;
     }
     B.T[c] = Tn;
     B.flag[c] = 0;
-    int i, j, k;
-    rm_ijk(P, c, i, j, k);
     if (rm_ghost_ijk(P, i, j, k)) continue;
     if (mat && Tn < P.Tm) continue;
     B.oid[c] = -1;
@@ -208,6 +445,58 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
   }
 }
 
+// T and cooling of the last step for the cells outside its box, computed as
+// that step's phase would have (k_rm_phase).
+__global__ void __launch_bounds__(kRmBlock) k_rm_complete(RmParams P, RmBuffers B) {
+  const RmCounters* ctr = B.ctr;
+  const long long n = ctr->last_step;
+  if (n < 0) return;
+  const int par = (int)(n & 1);
+  const RmBox R = ctr->box[par];
+  const double* th = rm_field(P, B, ctr->last_cur);
+  const double* thp = rm_field(P, B, ctr->last_prev);
+  const unsigned char* efb = rm_fb(P, B, par);
+  const unsigned char* efbp = rm_fb(P, B, par ^ 1);
+  const double tfe = ctr->last_time_fe, tfe_p = ctr->last_time_fe_prev, dt_prev = ctr->last_dt;
+  const int sur = ctr->sur_used[par], sur_p = ctr->sur_used[par ^ 1];
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
+    const int c = (int)cc;
+    int i, j, k;
+    rm_ijk(P, c, i, j, k);
+    if (box_has(R, i, j, k)) continue;
+    int unused = 0;
+    double Tn;
+    const bool mat = rm_cell_T(P, B, c, th, efb, tfe, sur, Tn, unused);
+    if (n > 0) {
+      double Tp;
+      (void)rm_cell_T(P, B, c, thp, efbp, tfe_p, sur_p, Tp, unused);
+      double rate = 410;
+      if (Tp > 0) rate = (Tn - Tp) / dt_prev;
+      if (Tn <= 0) rate = 0;
+      B.cooling[c] = mat ? (rate < 0) : 0;
+    }
+    B.T[c] = Tn;
+  }
+}
+
+// Hot region of the surrogate's cell table: entries at or above thr.
+__global__ void __launch_bounds__(kRmBlock) k_rm_sur_hot(RmParams P, RmBuffers B, double thr,
+                                                          RmBox* out) {
+  const long long total = (long long)P.nzG * P.nyG * P.sur_nxi;
+  RmBox b;
+  box_empty(b);
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    if ((double)B.sur_cell[q] < thr) continue;
+    const int x = (int)(q % P.sur_nxi);
+    const int j = (int)((q / P.sur_nxi) % P.nyG);
+    const int k = (int)(q / ((long long)P.sur_nxi * P.nyG));
+    box_add(b, x, j, k);
+  }
+  block_box_merge(b, out);
+}
+
 // Surrogate fallback, per step: a node is bad when the surrogate at the node
 // differs from the FE nodal field by more than sur_resid (or the node is
 // void); an element falls back to the FE path when any of its nodes is bad.
@@ -215,6 +504,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_nodes(RmParams P, RmBuffers
   const RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
   const double t = ctr->time_fe;
+  const double* theta = rm_field(P, B, ctr->th_cur);
   const int stride = gridDim.x * blockDim.x;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.nodes; n += stride) {
     const int ni = n % P.fnx, nj = (n / P.fnx) % P.fny, nk = n / (P.fnx * P.fny);
@@ -226,7 +516,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_nodes(RmParams P, RmBuffers
     f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
     const float* row = B.sur_node + ((size_t)nk * P.fny + nj) * P.sur_nxi_nodes;
     const double Ts = (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
-    const double th = B.theta[n];
+    const double th = theta[n];
     B.node_bad[n] = (fabs(Ts - th) > P.sur_resid || th < P.void_temp + 0.00001) ? 1 : 0;
   }
 }
@@ -236,6 +526,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_elems(RmParams P, RmBuffers
   if (ctr->skip) return;
   const int ne = P.ncx * P.ncy * (P.fnz - 1);
   const int nxy = P.fnx * P.fny;
+  unsigned char* fb = B.elem_fb + (size_t)(ctr->step & 1) * ne;
   const int stride = gridDim.x * blockDim.x;
   for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
     const int ei = e % P.ncx, ej = (e / P.ncx) % P.ncy, ek = e / (P.ncx * P.ncy);
@@ -243,7 +534,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_elems(RmParams P, RmBuffers
     const unsigned char* b = B.node_bad;
     const int bad = b[n0] | b[n0 + 1] | b[n0 + P.fnx] | b[n0 + P.fnx + 1] | b[n0 + nxy] |
                     b[n0 + nxy + 1] | b[n0 + nxy + P.fnx] | b[n0 + nxy + P.fnx + 1];
-    B.elem_fb[e] = bad ? 1 : 0;
+    fb[e] = bad ? 1 : 0;
   }
 }
 
@@ -253,19 +544,21 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_elems(RmParams P, RmBuffers
 // (as bits of a non-negative double), [5] fallback cells.
 __global__ void __launch_bounds__(kRmBlock) k_rm_sur_error(RmParams P, RmBuffers B) {
   const double t = B.ctr->time_fe;
+  const double* theta = rm_field(P, B, B.ctr->th_cur);
+  const unsigned char* efb = rm_fb(P, B, (int)(B.ctr->step & 1));
   double s_all = 0.0, n_all = 0.0, s_use = 0.0, n_use = 0.0, mx = 0.0, n_fb = 0.0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
     const int c = (int)cc;
     double Tfe = -1.0;
-    if (!rm_eval_T(P, B, c, Tfe)) continue;
+    if (!rm_eval_T(P, B, theta, c, Tfe)) continue;
     int i, j, k;
     rm_ijk(P, c, i, j, k);
     const double Ts = rm_sur_T(P, B, i, j, k, t);
     const double d = Ts - Tfe;
     s_all += d * d;
     n_all += 1.0;
-    const bool fb = B.elem_fb[B.tag[c]] || fabs(Ts - P.Tm) < P.sur_band;
+    const bool fb = efb[B.tag[c]] || fabs(Ts - P.Tm) < P.sur_band;
     if (fb) {
       n_fb += 1.0;
     } else {
@@ -283,15 +576,23 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_error(RmParams P, RmBuffers
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B) {
-  const RmCounters* ctr = B.ctr;
+  RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
   const double tc = ctr->tcafe, maxdt = ctr->max_dt;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
-    const int c = (int)cc;
-    int i, j, k;
-    rm_ijk(P, c, i, j, k);
-    if (rm_ghost_ijk(P, i, j, k)) continue;
+  const int lo[3] = {1, 1, 1}, hi[3] = {P.nx, P.ny, P.nz};
+  int o0[3], ext[3];
+  box_clip(rm_step_box(ctr), lo, hi, o0, ext);
+  const int exy = ext[0] * ext[1];
+  const int cnt = exy * ext[2];
+  RmBox liq;
+  box_empty(liq);
+  const int stride = gridDim.x * blockDim.x;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < cnt; q += stride) {
+    const int k = o0[2] + q / exy;
+    const int rem = q - (k - o0[2]) * exy;
+    const int j = o0[1] + rem / ext[0];
+    const int i = o0[0] + rem % ext[0];
+    const int c = k * P.nxyG + j * P.nxG + i;
     const double Tc = B.T[c];
     const bool mat = !(Tc < -1.0 + 0.01);
     const int o = B.oid[c];
@@ -301,6 +602,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B)
       continue;
     }
     if (o != -1) continue;
+    box_add(liq, i, j, k);  // liquid with material: part of the next step's box
     if (Tc >= P.Tm) continue;
 #pragma unroll
     for (int s = 0; s < 26; s++) {
@@ -308,6 +610,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B)
       if (B.oid[nb] > -1) B.flag[nb] = 1;
     }
   }
+  block_box_merge(liq, &ctr->acc[(ctr->step + 1) & 1][1]);
 }
 
 // nucleate_at_nucleation_sites, remelting variant: sites stay in the list;
@@ -407,13 +710,20 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
   __syncthreads();
   unsigned long long mk = ~0ull;
   int cnt = 0;
-  const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
-    const int c = (int)cc;
+  const int zero[3] = {0, 0, 0}, top[3] = {P.nxG - 1, P.nyG - 1, P.nzG - 1};
+  int o0[3], ext[3];
+  box_clip(rm_step_box(ctr), zero, top, o0, ext);
+  const int exy = ext[0] * ext[1];
+  const int total = exy * ext[2];
+  const int stride = gridDim.x * blockDim.x;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    const int k = o0[2] + q / exy;
+    const int rem = q - (k - o0[2]) * exy;
+    const int j = o0[1] + rem / ext[0];
+    const int i = o0[0] + rem % ext[0];
+    const int c = k * P.nxyG + j * P.nxG + i;
     if (!B.flag[c]) continue;
     cnt++;
-    int i, j, k;
-    rm_ijk(P, c, i, j, k);
     if (rm_ghost_ijk(P, i, j, k)) continue;
     const int g = B.gid[c];
     if (g < 0) continue;
@@ -451,23 +761,36 @@ __global__ void k_rm_dt(RmParams P, RmBuffers B) {
   ctr->step_fresh = 0;
 }
 
-// one thread per (i, j) row of interior cells, k innermost
-__device__ __forceinline__ bool rm_row(const RmParams& P, int& i, int& j) {
+// Rows of the sweep: one thread per (i, j) row of the box's interior cells, k
+// innermost.  Row r = (i - i0) * ny_box + (j - j0) keeps the x-major order.
+struct RmRows {
+  int i0, j0, k0, ex, ey, ez;
+};
+__device__ __forceinline__ RmRows rm_rows(const RmParams& P, const RmCounters* ctr) {
+  const int lo[3] = {1, 1, 1}, hi[3] = {P.nx, P.ny, P.nz};
+  int o0[3], ext[3];
+  box_clip(rm_step_box(ctr), lo, hi, o0, ext);
+  return RmRows{o0[0], o0[1], o0[2], ext[0], ext[1], ext[2]};
+}
+__device__ __forceinline__ bool rm_row(const RmRows& w, int& i, int& j, int& r) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= P.nx * P.ny) return false;
-  i = 1 + t % P.nx;
-  j = 1 + t / P.nx;
+  if (t >= w.ex * w.ey) return false;
+  const int di = t % w.ex, dj = t / w.ex;
+  i = w.i0 + di;
+  j = w.j0 + dj;
+  r = di * w.ey + dj;
   return true;
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
-  int i, j;
-  if (!rm_row(P, i, j)) return;
+  const RmRows w = rm_rows(P, ctr);
+  int i, j, r;
+  if (!rm_row(w, i, j, r)) return;
   const double dt = ctr->dt;
   const double vts = rm_velocity(P, P.Ts + 0.01);
-  for (int k = 1; k <= P.nz; k++) {
+  for (int k = w.k0; k < w.k0 + w.ez; k++) {
     const int c = k * P.nxyG + j * P.nxG + i;
     if (!B.flag[c]) continue;
     const int g = B.gid[c];
@@ -508,10 +831,11 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) 
 __global__ void __launch_bounds__(kRmBlock) k_rm_resolve(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
-  int i, j;
-  if (!rm_row(P, i, j)) return;
+  const RmRows w = rm_rows(P, ctr);
+  int i, j, r;
+  if (!rm_row(w, i, j, r)) return;
   int fresh = 0, caps = 0;
-  for (int k = 1; k <= P.nz; k++) {
+  for (int k = w.k0; k < w.k0 + w.ez; k++) {
     const int c = k * P.nxyG + j * P.nxG + i;
     if (!B.flag[c]) continue;
     if (B.gid[c] < 0) continue;
@@ -531,7 +855,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_resolve(RmParams P, RmBuffers B
     B.msk[c] = cap;
     caps += __builtin_popcount(cap);
   }
-  B.rowcnt[(i - 1) * P.ny + (j - 1)] = fresh;
+  B.rowcnt[r] = fresh;
   if (fresh) atomicAdd(&ctr->step_fresh, fresh);
   if (caps) atomicAdd(&ctr->step_cap, caps);
 }
@@ -542,7 +866,8 @@ __global__ void __launch_bounds__(kRmScan) k_rm_rowscan(RmParams P, RmBuffers B)
   RmCounters* ctr = B.ctr;
   if (ctr->skip || ctr->step_fresh == 0) return;
   __shared__ unsigned long long s_sm[kRmScan / 64];
-  const int nrow = P.nx * P.ny;
+  const RmRows w = rm_rows(P, ctr);
+  const int nrow = w.ex * w.ey;
   const int per = (nrow + kRmScan - 1) / kRmScan;
   const int lo = min(nrow, threadIdx.x * per), hi = min(nrow, lo + per);
   unsigned long long mine = 0;
@@ -560,10 +885,13 @@ __global__ void __launch_bounds__(kRmScan) k_rm_rowscan(RmParams P, RmBuffers B)
 __global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
   if (ctr->skip || ctr->step_cap == 0) return;
-  int i, j;
-  if (!rm_row(P, i, j)) return;
-  int next = ctr->step_fresh ? B.rowoff[(i - 1) * P.ny + (j - 1)] : 0;
-  for (int k = 1; k <= P.nz; k++) {
+  const RmRows w = rm_rows(P, ctr);
+  int i, j, r;
+  RmBox cap_box;
+  box_empty(cap_box);
+  if (rm_row(w, i, j, r)) {
+  int next = ctr->step_fresh ? B.rowoff[r] : 0;
+  for (int k = w.k0; k < w.k0 + w.ez; k++) {
     const int c = k * P.nxyG + j * P.nxG + i;
     if (!B.flag[c]) continue;
     const int g = B.gid[c];
@@ -598,8 +926,11 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
       B.genv[id] = ce;
       B.oid[nb] = o;
       B.claim[nb] = INT_MAX;
+      box_add(cap_box, i + di, j + dj, k + dk);
     }
   }
+  }
+  block_box_merge(cap_box, &B.ctr->acc[(ctr->step + 1) & 1][2]);
 }
 
 __global__ void k_rm_finish(RmParams P, RmBuffers B) {
@@ -622,7 +953,10 @@ __global__ void k_rm_finish(RmParams P, RmBuffers B) {
     row.n_fallback = 0;
     if (B.fb_slots)
       for (int q = 0; q < 256; q++) row.n_fallback += B.fb_slots[q];
-    row.pad = 0;
+    const RmBox& bx = ctr->box[st & 1];
+    row.n_box = (bx.lo[0] <= bx.hi[0]) ? (bx.hi[0] - bx.lo[0] + 1) * (bx.hi[1] - bx.lo[1] + 1) *
+                                             (bx.hi[2] - bx.lo[2] + 1)
+                                       : 0;
     B.log[st] = row;
   }
   if (B.fb_slots)
@@ -630,6 +964,16 @@ __global__ void k_rm_finish(RmParams P, RmBuffers B) {
   ctr->n_flagged = 0;
   ctr->n_fallback = 0;
   ctr->min_t_key = ~0ull;
+  // this step's box inputs are spent; the slot serves step + 2
+  for (int q = 0; q < 3; q++) {
+    RmBox& a = ctr->acc[st & 1][q];
+    for (int x = 0; x < 3; x++) {
+      a.lo[x] = INT_MAX;
+      a.hi[x] = INT_MIN;
+    }
+  }
+  ctr->force_full = 0;
+  ctr->time_fe_prev = ctr->time_fe;
   // FiniteElementManager::pseudo_single_integration_step
   double tf = ctr->time_fe;
   if (tf == 0) tf = ctr->in_t1;
@@ -653,6 +997,9 @@ __global__ void k_rm_finish(RmParams P, RmBuffers B) {
     ctr->ratio = (tf - in1) / (in0 - in1);
     if (ctr->ratio < 0.0) ctr->err = 2;
     ctr->max_dt = in0 - tf;  // get_max_time_step
+    // k_rm_nodal writes the next field into a fresh slot of the ring
+    ctr->th_prev = ctr->th_cur;
+    ctr->th_cur = (ctr->th_cur + 1) % 3;
   }
 }
 
@@ -662,6 +1009,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_nodal(RmParams P, RmBuffers B) 
   const double r = ctr->ratio;
   const double* th0 = B.frames + (size_t)ctr->frame * P.nodes;
   const double* th1 = th0 - P.nodes;
+  double* out = B.theta + (size_t)ctr->th_cur * P.nodes;
   const int stride = gridDim.x * blockDim.x;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.nodes; n += stride) {
     const double a = th0[n], b = th1[n];
@@ -669,7 +1017,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_nodal(RmParams P, RmBuffers B) 
     if (a <= P.void_temp) v = P.void_temp;
     else if (b <= P.void_temp) v = a;
     else v = r * (a - b) + b;
-    B.theta[n] = v;
+    out[n] = v;
   }
 }
 
@@ -683,9 +1031,9 @@ hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s) 
 }
 
 const char* const kRmKernelNames[kRmKernels] = {
-    "k_rm_sur_nodes", "k_rm_sur_elems", "k_rm_phase",  "k_rm_liquid", "k_rm_nucleate",
-    "k_rm_mint",      "k_rm_dt",        "k_rm_claim",  "k_rm_resolve", "k_rm_rowscan",
-    "k_rm_emit",      "k_rm_finish",    "k_rm_nodal"};
+    "k_rm_sur_nodes", "k_rm_sur_elems", "k_rm_window", "k_rm_phase",  "k_rm_liquid",
+    "k_rm_nucleate",  "k_rm_mint",      "k_rm_dt",     "k_rm_claim",  "k_rm_resolve",
+    "k_rm_rowscan",   "k_rm_emit",      "k_rm_finish", "k_rm_nodal"};
 
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s) {
@@ -696,6 +1044,9 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
   long long nbk = (P.nodes + kRmBlock - 1) / kRmBlock;
   if (nbk > 8192) nbk = 8192;
   if (nbk < 1) nbk = 1;
+  long long nbe = ((long long)P.ncx * P.ncy * (P.fnz - 1) + kRmBlock - 1) / kRmBlock;
+  if (nbe > 8192) nbe = 8192;
+  if (nbe < 1) nbe = 1;
   switch (q) {
     case 0:
       if (!P.sur) return hipSuccess;
@@ -705,17 +1056,21 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
       if (!P.sur) return hipSuccess;
       hipLaunchKernelGGL(k_rm_sur_elems, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
       break;
-    case 2: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 3: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 4: hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B); break;
-    case 5: hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 6: hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B); break;
-    case 7: hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
-    case 8: hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
-    case 9: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
-    case 10: hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
-    case 11: hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B); break;
-    case 12: hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B); break;
+    case 2:
+      if (!P.window) return hipSuccess;
+      hipLaunchKernelGGL(k_rm_window, dim3((unsigned)nbe), dim3(kRmBlock), 0, s, P, B);
+      break;
+    case 3: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 4: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 5: hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B); break;
+    case 6: hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 7: hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B); break;
+    case 8: hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 9: hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 10: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
+    case 11: hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 12: hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B); break;
+    case 13: hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
@@ -725,6 +1080,22 @@ hipError_t rm_launch_sur_error(const RmParams& P, const RmBuffers& B, hipStream_
   long long cb = (P.NG + kRmBlock - 1) / kRmBlock;
   if (cb > 4096) cb = 4096;
   hipLaunchKernelGGL(k_rm_sur_error, dim3((unsigned)cb), dim3(kRmBlock), 0, s, P, B);
+  return hipGetLastError();
+}
+
+hipError_t rm_launch_complete(const RmParams& P, const RmBuffers& B, hipStream_t s) {
+  long long cb = (P.NG + kRmBlock - 1) / kRmBlock;
+  if (cb > 16384) cb = 16384;
+  hipLaunchKernelGGL(k_rm_complete, dim3((unsigned)cb), dim3(kRmBlock), 0, s, P, B);
+  return hipGetLastError();
+}
+
+hipError_t rm_launch_sur_hot(const RmParams& P, const RmBuffers& B, int* out, hipStream_t s) {
+  long long cb = ((long long)P.nzG * P.nyG * P.sur_nxi + kRmBlock - 1) / kRmBlock;
+  if (cb > 8192) cb = 8192;
+  const double thr = P.Tm - P.sur_band - P.hot_margin;
+  hipLaunchKernelGGL(k_rm_sur_hot, dim3((unsigned)cb), dim3(kRmBlock), 0, s, P, B, thr,
+                     reinterpret_cast<RmBox*>(out));
   return hipGetLastError();
 }
 

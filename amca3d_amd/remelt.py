@@ -36,7 +36,7 @@ class RemeltCounters(C.Structure):
 class RemeltStepRecord(C.Structure):
     _fields_ = [("time_before", C.c_double), ("dt", C.c_double), ("n_nucleated", C.c_int32),
                 ("n_captured", C.c_int32), ("n_fresh", C.c_int32), ("n_flagged", C.c_int32),
-                ("n_fallback", C.c_int32)]
+                ("n_fallback", C.c_int32), ("n_box", C.c_int32)]
 
 
 ORIENT_SOURCE = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int32, F64P, F64P)
@@ -70,6 +70,8 @@ SIGNATURES = {
                                                C.c_double, I32P, I32P]),
     "amca_remelt_sites": (C.c_int, [C.c_void_p, C.c_int32, I32P, F64P, I32P]),
     "amca_remelt_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_remelt_set_window": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_remelt_last_box": (C.c_int, [C.c_void_p, I32P]),
     "amca_remelt_use_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_double, C.c_double]),
     "amca_remelt_surrogate_error": (C.c_int, [C.c_void_p, F64P]),
@@ -244,10 +246,10 @@ class RemeltDomain:
         rows = (RemeltStepRecord * max(m, 1))()
         _check(lib().amca_remelt_read_log(self.h, m, rows, C.byref(n)), "read_log")
         return np.array([(r.time_before, r.dt, r.n_nucleated, r.n_captured, r.n_fresh, r.n_flagged,
-                          r.n_fallback) for r in rows[:m]],
+                          r.n_fallback, r.n_box) for r in rows[:m]],
                         dtype=[("time_before", "f8"), ("dt", "f8"), ("n_nucleated", "i4"),
                                ("n_captured", "i4"), ("n_fresh", "i4"), ("n_flagged", "i4"),
-                               ("n_fallback", "i4")])
+                               ("n_fallback", "i4"), ("n_box", "i4")])
 
     def run(self, max_steps=-1):
         c = RemeltCounters()
@@ -316,6 +318,15 @@ class RemeltDomain:
 
     def enable_timing(self, on=True):
         _check(lib().amca_remelt_enable_timing(self.h, 1 if on else 0), "enable_timing")
+
+    def set_window(self, on=True):
+        """Active box on (default) or every step over the whole grid."""
+        _check(lib().amca_remelt_set_window(self.h, 1 if on else 0), "set_window")
+
+    def last_box(self):
+        b = np.zeros(6, np.int32)
+        _check(lib().amca_remelt_last_box(self.h, _p(b, C.c_int32)), "last_box")
+        return b
 
     def kernel_times(self):
         """{kernel name: (total ms, launches)} since timing was enabled."""

@@ -311,6 +311,7 @@ typedef struct {
   double time_before, dt;
   int32_t n_nucleated, n_captured, n_fresh, n_flagged;
   int32_t n_fallback;   /* with a surrogate: cells that took the FE path */
+  int32_t n_box;        /* cells of the step's active box (ghost layer included) */
 } amca_remelt_step_record;
 
 /* Supplies n more orientation rows (Euler angles and row-major local-to-global
@@ -402,6 +403,15 @@ amca_status amca_remelt_use_surrogate(amca_remelt* d, amca_mlp* m, const amca_su
  * cells with material, their count, RMS over the cells the surrogate serves,
  * their count, their max |error|, cells on the FE path. */
 amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6);
+/* Active box (on by default): each step's grid kernels run over the box of
+ * cells that can change in that step -- elements at or above Tm - 1 K or
+ * changing between void and material, liquid cells with material, last
+ * step's captures, widened by two cells -- and a run ends by writing the last
+ * step's T and cooling to the cells outside it.  Results are identical with
+ * it off (every step over the whole grid, as the reference's loops run). */
+amca_status amca_remelt_set_window(amca_remelt* d, int32_t on);
+/* The last step's box (i0, j0, k0, i1, j1, k1, ghost layer included). */
+amca_status amca_remelt_last_box(amca_remelt* d, int32_t* box6);
 /* Per-kernel device time from HIP events around every launch (off by default);
  * enabling resets the totals.  Kernel i is named amca_remelt_kernel_name(i). */
 amca_status amca_remelt_enable_timing(amca_remelt* d, int32_t on);

@@ -47,7 +47,7 @@ def run_device_own_sampler(case, orc):
     return out
 
 
-def run_device(case, orc, chunks=None):
+def run_device(case, orc, chunks=None, window=True):
     """Run `case` on the device.  Nucleation sites and the orientation rows drawn
     after the loaded ones are the host-side inputs of the path; they come from
     the oracle run so both sides consume the same rand() stream."""
@@ -75,6 +75,8 @@ def run_device(case, orc, chunks=None):
     d.set_state(st)
     if n_sites:
         d.set_sites(orc["site_cell"], orc["site_dt"])
+    if not window:
+        d.set_window(False)
     d.enable_log(100000)
     if chunks:
         for k in chunks:
@@ -171,6 +173,25 @@ def test_track_in_chunks_equals_one_run(amca, track_nuc):
     case, orc = track_nuc
     dev = run_device(case, orc, chunks=[1, 2, 5, 13, 40, 1000])
     compare(case, orc, dev)
+
+
+@pytest.mark.parametrize("chunks", [None, [1, 3, 7, 50, 1000]])
+def test_active_box_equals_whole_grid(amca, chunks):
+    """A track long and wide enough that each step's active box holds a small
+    part of the grid: the boxed run, in one go or in chunks, equals the
+    whole-grid run and the oracle bit for bit."""
+    nuc = dict(surface=(2e10, 2.0, 0.5), bulk=(2e15, 5.0, 1.0))
+    case = rc.track_case(nx=120, ny=32, nz=24, n_frames=40, P=30.0, nucleation=nuc, seed=5)
+    orc = ol.run_remelt(case)
+    assert orc["steps"]["n_captured"].sum() > 1000
+    boxed = run_device(case, orc, chunks=chunks)
+    compare(case, orc, boxed)
+    whole = run_device(case, orc, window=False)
+    compare(case, orc, whole)
+    ng = 122 * 34 * 26
+    nb = boxed["log"]["n_box"]
+    assert nb[0] == ng and (whole["log"]["n_box"] == ng).all()
+    assert np.median(nb[1:]) < 0.25 * ng, np.median(nb[1:])
 
 
 def test_track_with_fresh_captures(amca):
