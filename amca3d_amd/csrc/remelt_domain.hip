@@ -96,8 +96,30 @@ struct amca_remelt {
       }
     allocs.erase(std::remove(allocs.begin(), allocs.end(), nullptr), allocs.end());
   }
+  // kGraphSteps steps captured as one HIP graph, valid while the kernels'
+  // arguments (parameters, buffers, site blocks) are the ones captured
+  static constexpr int kGraphSteps = 8;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t graph_exec = nullptr;
+  std::vector<unsigned char> graph_sig;
+
+  void drop_graph() {
+    if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    graph_exec = nullptr;
+    graph = nullptr;
+    graph_sig.clear();
+  }
+  std::vector<unsigned char> launch_sig() const {
+    std::vector<unsigned char> v(sizeof(P) + sizeof(B) + sizeof(int));
+    std::memcpy(v.data(), &P, sizeof(P));
+    std::memcpy(v.data() + sizeof(P), &B, sizeof(B));
+    std::memcpy(v.data() + sizeof(P) + sizeof(B), &site_blocks, sizeof(int));
+    return v;
+  }
   ~amca_remelt() {
     if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
     for (hipEvent_t e : ev_pool) (void)hipEventDestroy(e);
     for (void* p : allocs) (void)hipFree(p);
     if (stream) (void)hipStreamDestroy(stream);
@@ -185,6 +207,36 @@ amca_status rm_resolve_events(amca_remelt* d) {
   }
   d->ev_pending.clear();
   d->ev_used = 0;
+  return AMCA_OK;
+}
+
+// k steps without per-kernel timing: whole graphs of kGraphSteps steps, then
+// single steps for the rest
+amca_status rm_enqueue_steps(amca_remelt* d, long long k) {
+  const int G = amca_remelt::kGraphSteps;
+  if (k >= G) {
+    std::vector<unsigned char> sig = d->launch_sig();
+    if (!d->graph_exec || sig != d->graph_sig) {
+      d->drop_graph();
+      RM_HIP(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+      hipError_t e = hipSuccess;
+      for (int q = 0; q < G && e == hipSuccess; q++)
+        e = amca::rm_launch_step(d->P, d->B, d->site_blocks, d->stream);
+      hipGraph_t g = nullptr;
+      const hipError_t e2 = hipStreamEndCapture(d->stream, &g);
+      if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rm_fail(AMCA_ERR_HIP, std::string("graph capture: ") +
+                                         hipGetErrorString(e != hipSuccess ? e : e2));
+      }
+      d->graph = g;
+      RM_HIP(hipGraphInstantiate(&d->graph_exec, g, nullptr, nullptr, 0));
+      d->graph_sig = std::move(sig);
+    }
+    for (long long q = 0; q < k / G; q++) RM_HIP(hipGraphLaunch(d->graph_exec, d->stream));
+  }
+  for (long long q = 0; q < k % G; q++)
+    RM_HIP(amca::rm_launch_step(d->P, d->B, d->site_blocks, d->stream));
   return AMCA_OK;
 }
 
@@ -394,15 +446,15 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   d->release((void*)d->B.frames);
   d->release((void*)d->B.frame_time);
   d->release(d->B.theta);
-  d->release(d->B.elem_box);
+  d->release(d->B.node_box);
   double *fr = nullptr, *ft = nullptr, *th = nullptr;
   int* eb = nullptr;
   const size_t ne = (size_t)(fnx - 1) * (fny - 1) * (fnz - 1);
   RM_TRY(d->alloc(&fr, nodes * (size_t)n_frames));
   RM_TRY(d->alloc(&ft, (size_t)n_frames));
   RM_TRY(d->alloc(&th, 3 * nodes));
-  RM_TRY(d->alloc(&eb, 6 * ne));
-  // cell box of each element's cells, for the active box
+  RM_TRY(d->alloc(&eb, 6 * nodes));
+  // cell box of the cells in the elements around each node, for the active box
   {
     std::vector<int32_t> box(6 * ne);
     for (size_t e = 0; e < ne; e++) {
@@ -421,7 +473,27 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
       b[4] = std::max(b[4], j);
       b[5] = std::max(b[5], k);
     }
-    RM_HIP(hipMemcpyAsync(eb, box.data(), sizeof(int32_t) * box.size(), hipMemcpyHostToDevice,
+    std::vector<int32_t> nbox(6 * nodes);
+    for (size_t n = 0; n < nodes; n++) {
+      nbox[6 * n + 0] = nbox[6 * n + 1] = nbox[6 * n + 2] = INT_MAX;
+      nbox[6 * n + 3] = nbox[6 * n + 4] = nbox[6 * n + 5] = INT_MIN;
+    }
+    const int ncx = fnx - 1, ncy = fny - 1;
+    for (size_t e = 0; e < ne; e++) {
+      const int32_t* b = &box[6 * e];
+      if (b[0] > b[3]) continue;
+      const int ei = (int)(e % ncx), ej = (int)((e / ncx) % ncy), ek = (int)(e / ((size_t)ncx * ncy));
+      for (int dk = 0; dk < 2; dk++)
+        for (int dj = 0; dj < 2; dj++)
+          for (int di = 0; di < 2; di++) {
+            int32_t* nb = &nbox[6 * (((size_t)(ek + dk) * fny + (ej + dj)) * fnx + (ei + di))];
+            for (int a = 0; a < 3; a++) {
+              nb[a] = std::min(nb[a], b[a]);
+              nb[3 + a] = std::max(nb[3 + a], b[3 + a]);
+            }
+          }
+    }
+    RM_HIP(hipMemcpyAsync(eb, nbox.data(), sizeof(int32_t) * nbox.size(), hipMemcpyHostToDevice,
                           d->stream));
     RM_HIP(hipStreamSynchronize(d->stream));
   }
@@ -446,7 +518,7 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   d->B.frames = fr;
   d->B.frame_time = ft;
   d->B.theta = th;
-  d->B.elem_box = eb;
+  d->B.node_box = eb;
   amca::RmCounters c;
   RM_TRY(rm_counters(d, &c));
   c.th_cur = 0;
@@ -776,7 +848,11 @@ amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_count
     const long long room = d->P.orient_cap - (long long)c.n_orient;
     if (c.n_sites > 0) k = std::max(1LL, std::min<long long>(k, room / c.n_sites));
     RM_TRY(ensure_rows(d, (long long)c.n_orient + k * c.n_sites));
-    for (long long q = 0; q < k; q++) RM_TRY(rm_enqueue_step(d));
+    if (d->timing) {
+      for (long long q = 0; q < k; q++) RM_TRY(rm_enqueue_step(d));
+    } else {
+      RM_TRY(rm_enqueue_steps(d, k));
+    }
     RM_TRY(rm_counters(d, &c));
     batch = std::min<long long>(batch * 2, 64);
   }

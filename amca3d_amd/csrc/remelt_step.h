@@ -92,7 +92,8 @@ struct RmBuffers {
   const double* frames;   // nodal frames [n_frames * nodes]
   const double* frame_time;  // [n_frames], relative to the first
   double* theta;          // thetaArray_: ring of three nodal fields [3 * nodes], slot ctr->th_cur
-  int* elem_box;          // cell box of each element's cells (i0 j0 k0 i1 j1 k1) [6 * elements]
+  int* node_box;          // cell box of the cells in the elements around each node
+                          // (i0 j0 k0 i1 j1 k1, empty: i0 > i1) [6 * nodes]
   int* site_cell;         // nucleation sites, ascending cell (never erased here)
   double* site_dt;
   double* l2g;            // orientation rows, 9 doubles each

@@ -250,10 +250,13 @@ __device__ __forceinline__ void block_box_merge(RmBox b, RmBox* dst) {
   if (threadIdx.x == 0) {
     for (int w = 1; w < kRmBlock / 64; w++)
       for (int q = 0; q < 6; q++) v[q] = q < 3 ? min(v[q], s_b[w][q]) : max(v[q], s_b[w][q]);
+    // only bounds that widen the current box go to the atomic unit: the box
+    // settles after a few blocks, and same-address atomics from every block
+    // would serialise
     if (v[0] <= v[3]) {
       for (int q = 0; q < 3; q++) {
-        atomicMin(&dst->lo[q], v[q]);
-        atomicMax(&dst->hi[q], v[q + 3]);
+        if (v[q] < ld_coherent(&dst->lo[q])) atomicMin(&dst->lo[q], v[q]);
+        if (v[q + 3] > ld_coherent(&dst->hi[q])) atomicMax(&dst->hi[q], v[q + 3]);
       }
     }
   }
@@ -293,36 +296,21 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_window(RmParams P, RmBuffers B)
   const double* tp = rm_field(P, B, ctr->th_prev);
   const double hot = P.Tm - P.hot_margin;
   const double nm = P.void_temp + 0.00001;
-  const int ne = P.ncx * P.ncy * (P.fnz - 1);
   const int nxy = P.fnx * P.fny;
   RmBox b;
   box_empty(b);
+  // An element is hot, or changes material, when one of its nodes does: such
+  // a node adds the cell box of the elements around it.
   const int stride = gridDim.x * blockDim.x;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
-    const int ei = e % P.ncx, ej = (e / P.ncx) % P.ncy, ek = e / (P.ncx * P.ncy);
-    const int n0 = nxy * ek + P.fnx * ej + ei;
-    const int nid[8] = {n0, n0 + 1, n0 + P.fnx, n0 + P.fnx + 1,
-                        n0 + nxy, n0 + nxy + 1, n0 + nxy + P.fnx, n0 + nxy + P.fnx + 1};
-    double mx = th[nid[0]];
-    bool vn = mx < nm, vp = false;
-#pragma unroll
-    for (int m = 1; m < 8; m++) {
-      const double v = th[nid[m]];
-      mx = fmax(mx, v);
-      vn |= v < nm;
-    }
-    bool take = mx >= hot;
-    if (!take && !P.no_void) {
-#pragma unroll
-      for (int m = 0; m < 8; m++) vp |= tp[nid[m]] < nm;
-      take = vn != vp;
-    }
-    if (take) {
-      const int* eb = B.elem_box + 6 * (size_t)e;
-      if (eb[0] <= eb[3]) {
-        box_add(b, eb[0], eb[1], eb[2]);
-        box_add(b, eb[3], eb[4], eb[5]);
-      }
+  for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += stride) {
+    const double v = th[nd];
+    bool take = v >= hot;
+    if (!take && !P.no_void) take = (v < nm) != (tp[nd] < nm);
+    if (!take) continue;
+    const int* nb = B.node_box + 6 * (size_t)nd;
+    if (nb[0] <= nb[3]) {
+      box_add(b, nb[0], nb[1], nb[2]);
+      box_add(b, nb[3], nb[4], nb[5]);
     }
   }
   if (P.sur && blockIdx.x == 0 && threadIdx.x == 0 && P.sur_hot[0] <= P.sur_hot[1]) {
@@ -429,7 +417,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
     B.melted[c] = mat ? 1 : 0;
   }
   if (P.sur) {
-    // block sum, one atomic per block into one of 256 slots (summed by k_rm_finish)
+    // block sum, one atomic per block into one of 256 slots (summed at the end of the step)
     __shared__ int s_fb[kRmBlock / 64];
     int v = n_fb;
 #pragma unroll
@@ -604,11 +592,13 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B)
     if (o != -1) continue;
     box_add(liq, i, j, k);  // liquid with material: part of the next step's box
     if (Tc >= P.Tm) continue;
+    // all 26 loads before any store (a byte store may alias the loads)
+    int no[26];
 #pragma unroll
-    for (int s = 0; s < 26; s++) {
-      const int nb = c + P.off[s];
-      if (B.oid[nb] > -1) B.flag[nb] = 1;
-    }
+    for (int s = 0; s < 26; s++) no[s] = B.oid[c + P.off[s]];
+#pragma unroll
+    for (int s = 0; s < 26; s++)
+      if (no[s] > -1) B.flag[c + P.off[s]] = 1;
   }
   block_box_merge(liq, &ctr->acc[(ctr->step + 1) & 1][1]);
 }
@@ -698,6 +688,25 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_nucleate(RmParams P, RmBuffers 
   }
 }
 
+// dt = min(dT_FE, min(factor*h/v(minT), dT_FE))
+__global__ void k_rm_dt(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  double minT = P.Tm;
+  if (ctr->min_t_key != ~0ull) {
+    const double m = dkey_inv(ctr->min_t_key);
+    if (minT > m) minT = m;
+  }
+  const double v = rm_velocity(P, minT);
+  const double maxV = (v > 0 ? v : 0);
+  const double a = P.tsf * P.h / maxV;
+  const double dtca = (ctr->max_dt < a) ? ctr->max_dt : a;  // std::min(a, maxDt)
+  const double dtfe = ctr->max_dt;
+  ctr->dt = ((dtfe) < (dtca) ? (dtfe) : (dtca));  // __min(dT_FE, dT_CA)
+  ctr->step_cap = 0;
+  ctr->step_fresh = 0;
+}
+
 __global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
@@ -722,14 +731,15 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
     const int j = o0[1] + rem / ext[0];
     const int i = o0[0] + rem % ext[0];
     const int c = k * P.nxyG + j * P.nxG + i;
-    if (!B.flag[c]) continue;
+    const unsigned char fl = B.flag[c];
+    const int g = B.gid[c];
+    const double Tc = B.T[c];
+    if (!fl) continue;
     cnt++;
     if (rm_ghost_ijk(P, i, j, k)) continue;
-    const int g = B.gid[c];
     if (g < 0) continue;
     const double L = B.genv[g].w;
     if (!(L < P.cutoff)) B.genv[g].w = P.h;  // length reset (:688-694)
-    const double Tc = B.T[c];
     if (Tc <= P.Ts) continue;
     const unsigned long long k2 = dkey(Tc);
     if (k2 < mk) mk = k2;
@@ -743,26 +753,9 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
   }
 }
 
-__global__ void k_rm_dt(RmParams P, RmBuffers B) {
-  RmCounters* ctr = B.ctr;
-  if (ctr->skip) return;
-  double minT = P.Tm;
-  if (ctr->min_t_key != ~0ull) {
-    const double m = dkey_inv(ctr->min_t_key);
-    if (minT > m) minT = m;
-  }
-  const double v = rm_velocity(P, minT);
-  const double maxV = (v > 0 ? v : 0);
-  const double a = P.tsf * P.h / maxV;
-  const double dtca = (ctr->max_dt < a) ? ctr->max_dt : a;  // std::min(a, maxDt)
-  const double dtfe = ctr->max_dt;
-  ctr->dt = ((dtfe) < (dtca) ? (dtfe) : (dtca));  // __min(dT_FE, dT_CA)
-  ctr->step_cap = 0;
-  ctr->step_fresh = 0;
-}
-
-// Rows of the sweep: one thread per (i, j) row of the box's interior cells, k
-// innermost.  Row r = (i - i0) * ny_box + (j - j0) keeps the x-major order.
+// The sweep kernels run one thread per interior cell of the box (i fastest).
+// Rows (i, j) keep the x-major sweep order: row r = (i - i0) * ny_box + (j - j0),
+// and per-row counts of fresh captures give each row its first new grain id.
 struct RmRows {
   int i0, j0, k0, ex, ey, ez;
 };
@@ -772,38 +765,49 @@ __device__ __forceinline__ RmRows rm_rows(const RmParams& P, const RmCounters* c
   box_clip(rm_step_box(ctr), lo, hi, o0, ext);
   return RmRows{o0[0], o0[1], o0[2], ext[0], ext[1], ext[2]};
 }
-__device__ __forceinline__ bool rm_row(const RmRows& w, int& i, int& j, int& r) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= w.ex * w.ey) return false;
-  const int di = t % w.ex, dj = t / w.ex;
-  i = w.i0 + di;
-  j = w.j0 + dj;
-  r = di * w.ey + dj;
-  return true;
+__device__ __forceinline__ void rm_cell_of(const RmRows& w, int q, int& i, int& j, int& k) {
+  const int exy = w.ex * w.ey;
+  k = w.k0 + q / exy;
+  const int rem = q - (k - w.k0) * exy;
+  j = w.j0 + rem / w.ex;
+  i = w.i0 + rem % w.ex;
 }
+
+constexpr unsigned kSlotBits = (1u << 26) - 1u;  // msk: captured slots; bits 26.. fresh count
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
   const RmRows w = rm_rows(P, ctr);
-  int i, j, r;
-  if (!rm_row(w, i, j, r)) return;
+  const int total = w.ex * w.ey * w.ez;
   const double dt = ctr->dt;
   const double vts = rm_velocity(P, P.Ts + 0.01);
-  for (int k = w.k0; k < w.k0 + w.ez; k++) {
+  const int stride = gridDim.x * blockDim.x;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    int i, j, k;
+    rm_cell_of(w, q, i, j, k);
+    if (k == w.k0) B.rowcnt[(i - w.i0) * w.ey + (j - w.j0)] = 0;
     const int c = k * P.nxyG + j * P.nxG + i;
-    if (!B.flag[c]) continue;
+    const unsigned char fl = B.flag[c];
     const int g = B.gid[c];
+    if (!fl) continue;
     if (g < 0) continue;
     double4 e = B.genv[g];
     const double Tc = B.T[c];
+    const int o = B.gco[g].y;
+    // neighbour state, loaded before any store or atomic
+    int no[26];
+#pragma unroll
+    for (int s = 0; s < 26; s++) no[s] = B.oid[c + P.off[s]];
+    double tn[26];
+#pragma unroll
+    for (int s = 0; s < 26; s++) tn[s] = no[s] > -1 ? 0.0 : B.T[c + P.off[s]];
     const double vel = (Tc < P.Ts) ? vts : rm_velocity(P, Tc);
     e.w = e.w + dt * vel;  // Grain::grow
     B.genv[g].w = e.w;
-    const int o = B.gco[g].y;
     double r[9];
 #pragma unroll
-    for (int q = 0; q < 9; q++) r[q] = B.l2g[9 * (size_t)o + q];
+    for (int x = 0; x < 9; x++) r[x] = B.l2g[9 * (size_t)o + x];
     const int key = (i * P.nyG + j) * P.nzG + k;
     unsigned m = 0;
 #pragma unroll
@@ -811,8 +815,8 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) 
       int di, dj, dk;
       slot_delta(s, di, dj, dk);
       const int nb = c + P.off[s];
-      if (B.oid[nb] > -1) continue;
-      const double Tn = B.T[nb];
+      if (no[s] > -1) continue;
+      const double Tn = tn[s];
       if (Tn > P.Tm) continue;
       if (!(Tn > -1.0)) continue;
       if (rm_ghost_ijk(P, i + di, j + dj, k + dk)) continue;
@@ -832,17 +836,21 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_resolve(RmParams P, RmBuffers B
   RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
   const RmRows w = rm_rows(P, ctr);
-  int i, j, r;
-  if (!rm_row(w, i, j, r)) return;
-  int fresh = 0, caps = 0;
-  for (int k = w.k0; k < w.k0 + w.ez; k++) {
+  const int total = w.ex * w.ey * w.ez;
+  const int stride = gridDim.x * blockDim.x;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    int i, j, k;
+    rm_cell_of(w, q, i, j, k);
     const int c = k * P.nxyG + j * P.nxG + i;
-    if (!B.flag[c]) continue;
-    if (B.gid[c] < 0) continue;
+    const unsigned char fl = B.flag[c];
+    const int g = B.gid[c];
     unsigned m = B.msk[c];
+    if (!fl) continue;
+    if (g < 0) continue;
     if (!m) continue;
     const int key = (i * P.nyG + j) * P.nzG + k;
     unsigned cap = 0;
+    int fresh = 0;
     while (m) {
       const int s = __builtin_ctz(m);
       m &= m - 1;
@@ -852,12 +860,13 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_resolve(RmParams P, RmBuffers B
         if (B.gid[nb] < 0) fresh++;
       }
     }
-    B.msk[c] = cap;
-    caps += __builtin_popcount(cap);
+    B.msk[c] = cap | ((unsigned)fresh << 26);
+    if (fresh) {
+      atomicAdd(&B.rowcnt[(i - w.i0) * w.ey + (j - w.j0)], fresh);
+      atomicAdd(&ctr->step_fresh, fresh);
+    }
+    if (cap) atomicAdd(&ctr->step_cap, __builtin_popcount(cap));
   }
-  B.rowcnt[r] = fresh;
-  if (fresh) atomicAdd(&ctr->step_fresh, fresh);
-  if (caps) atomicAdd(&ctr->step_cap, caps);
 }
 
 constexpr int kRmScan = 1024;
@@ -882,57 +891,7 @@ __global__ void __launch_bounds__(kRmScan) k_rm_rowscan(RmParams P, RmBuffers B)
   if (threadIdx.x == 0) ctr->n_grains += (int)tot;
 }
 
-__global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
-  const RmCounters* ctr = B.ctr;
-  if (ctr->skip || ctr->step_cap == 0) return;
-  const RmRows w = rm_rows(P, ctr);
-  int i, j, r;
-  RmBox cap_box;
-  box_empty(cap_box);
-  if (rm_row(w, i, j, r)) {
-  int next = ctr->step_fresh ? B.rowoff[r] : 0;
-  for (int k = w.k0; k < w.k0 + w.ez; k++) {
-    const int c = k * P.nxyG + j * P.nxG + i;
-    if (!B.flag[c]) continue;
-    const int g = B.gid[c];
-    if (g < 0) continue;
-    unsigned cap = B.msk[c];
-    if (!cap) continue;
-    const double4 e = B.genv[g];
-    const int o = B.gco[g].y;
-    double r[9];
-#pragma unroll
-    for (int q = 0; q < 9; q++) r[q] = B.l2g[9 * (size_t)o + q];
-    while (cap) {
-      const int s = __builtin_ctz(cap);
-      cap &= cap - 1;
-      int di, dj, dk;
-      slot_delta(s, di, dj, dk);
-      const int nb = c + P.off[s];
-      const double gx = centre_coord(P.x0, P.h, i + di) - e.x;
-      const double gy = centre_coord(P.y0, P.h, j + dj) - e.y;
-      const double gz = centre_coord(P.z0, P.h, k + dk) - e.z;
-      double out[4];
-      child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
-      const double4 ce = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
-      int id = B.gid[nb];
-      if (id < 0) {
-        id = next++;
-        B.gid[nb] = id;
-        B.gco[id] = make_int2(nb, o);
-      } else {
-        B.gco[id].y = o;
-      }
-      B.genv[id] = ce;
-      B.oid[nb] = o;
-      B.claim[nb] = INT_MAX;
-      box_add(cap_box, i + di, j + dj, k + dk);
-    }
-  }
-  }
-  block_box_merge(cap_box, &B.ctr->acc[(ctr->step + 1) & 1][2]);
-}
-
+// time advance and FE frame selection
 __global__ void k_rm_finish(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
@@ -1003,6 +962,70 @@ __global__ void k_rm_finish(RmParams P, RmBuffers B) {
   }
 }
 
+__global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip || ctr->step_cap == 0) return;
+  const RmRows w = rm_rows(P, ctr);
+  const int total = w.ex * w.ey * w.ez;
+  RmBox cap_box;
+  box_empty(cap_box);
+  const int stride = gridDim.x * blockDim.x;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    int i, j, k;
+    rm_cell_of(w, q, i, j, k);
+    const int c = k * P.nxyG + j * P.nxG + i;
+    const unsigned char fl = B.flag[c];
+    const int g = B.gid[c];
+    const unsigned mk = B.msk[c];
+    if (!fl) continue;
+    if (g < 0) continue;
+    unsigned cap = mk & kSlotBits;
+    if (!cap) continue;
+    // first new id of this cell: the row's first id plus the fresh captures
+    // of the row's flagged cells below it (cells of the row are never
+    // captured in this step, so their flags, ids and masks are settled)
+    int next = 0;
+    if (mk >> 26) {
+      next = B.rowoff[(i - w.i0) * w.ey + (j - w.j0)];
+      for (int kk = w.k0; kk < k; kk++) {
+        const int cc = kk * P.nxyG + j * P.nxG + i;
+        if (B.flag[cc] && B.gid[cc] >= 0) next += (int)(B.msk[cc] >> 26);
+      }
+    }
+    const double4 e = B.genv[g];
+    const int o = B.gco[g].y;
+    double r[9];
+#pragma unroll
+    for (int x = 0; x < 9; x++) r[x] = B.l2g[9 * (size_t)o + x];
+    while (cap) {
+      const int s = __builtin_ctz(cap);
+      cap &= cap - 1;
+      int di, dj, dk;
+      slot_delta(s, di, dj, dk);
+      const int nb = c + P.off[s];
+      const double gx = centre_coord(P.x0, P.h, i + di) - e.x;
+      const double gy = centre_coord(P.y0, P.h, j + dj) - e.y;
+      const double gz = centre_coord(P.z0, P.h, k + dk) - e.z;
+      double out[4];
+      child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
+      const double4 ce = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
+      int id = B.gid[nb];
+      if (id < 0) {
+        id = next++;
+        B.gid[nb] = id;
+        B.gco[id] = make_int2(nb, o);
+      } else {
+        B.gco[id].y = o;
+      }
+      B.genv[id] = ce;
+      B.oid[nb] = o;
+      B.claim[nb] = INT_MAX;
+      box_add(cap_box, i + di, j + dj, k + dk);
+    }
+  }
+  block_box_merge(cap_box, &ctr->acc[(ctr->step + 1) & 1][2]);
+}
+
 __global__ void __launch_bounds__(kRmBlock) k_rm_nodal(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
   if (ctr->skip || ctr->exhausted) return;
@@ -1037,16 +1060,14 @@ const char* const kRmKernelNames[kRmKernels] = {
 
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s) {
+  // grid-stride kernels: with the active box, 2048 blocks of 256 (eight per
+  // CU) cost little when the box is small; over the whole grid, up to 16384
   long long cb = (P.NG + kRmBlock - 1) / kRmBlock;
-  if (cb > 16384) cb = 16384;
+  if (cb > (P.window ? 2048 : 16384)) cb = P.window ? 2048 : 16384;
   const unsigned cells = (unsigned)cb;
-  const unsigned rows = (unsigned)((P.nx * P.ny + kRmBlock - 1) / kRmBlock);
   long long nbk = (P.nodes + kRmBlock - 1) / kRmBlock;
   if (nbk > 8192) nbk = 8192;
   if (nbk < 1) nbk = 1;
-  long long nbe = ((long long)P.ncx * P.ncy * (P.fnz - 1) + kRmBlock - 1) / kRmBlock;
-  if (nbe > 8192) nbe = 8192;
-  if (nbe < 1) nbe = 1;
   switch (q) {
     case 0:
       if (!P.sur) return hipSuccess;
@@ -1058,17 +1079,17 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
       break;
     case 2:
       if (!P.window) return hipSuccess;
-      hipLaunchKernelGGL(k_rm_window, dim3((unsigned)nbe), dim3(kRmBlock), 0, s, P, B);
+      hipLaunchKernelGGL(k_rm_window, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
       break;
     case 3: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 4: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 5: hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B); break;
     case 6: hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 7: hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B); break;
-    case 8: hipLaunchKernelGGL(k_rm_claim, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
-    case 9: hipLaunchKernelGGL(k_rm_resolve, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 8: hipLaunchKernelGGL(k_rm_claim, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 9: hipLaunchKernelGGL(k_rm_resolve, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 10: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
-    case 11: hipLaunchKernelGGL(k_rm_emit, dim3(rows), dim3(kRmBlock), 0, s, P, B); break;
+    case 11: hipLaunchKernelGGL(k_rm_emit, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 12: hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B); break;
     case 13: hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B); break;
     default: return hipErrorInvalidValue;

@@ -90,31 +90,33 @@ def measured_traffic(kernel, summary="pmc_traffic_plain_uniform512.json"):
 
 
 def remelt_algorithmic_bytes(P, log):
-    """Algorithmic bytes per kernel of the coupled step over a step log.
+    """Algorithmic bytes per kernel of the coupled step over a step log; X is
+    the step's active box in cells (the whole grid without it).
 
-    k_rm_phase   46 B per cell (element tag 4, parametric coordinates 24, T read
-                 8 and written 8, cooling and liquid-neighbour flags 1 + 1) and
-                 8 B per FE node (the nodal field)
-    k_rm_liquid  13 B per interior cell (T 8, orientation 4, melted flag 1)
-    k_rm_mint    1 B per cell (flag) + 20 B per flagged cell (grain id 4,
+    k_rm_window  8 B per FE node (the nodal field, once per element corner set)
+    k_rm_phase   46 B per box cell (element tag 4, parametric coordinates 24, T
+                 read 8 and written 8, cooling and liquid-neighbour flags 1 + 1)
+                 and 8 B per FE node (the nodal field)
+    k_rm_liquid  13 B per box cell (T 8, orientation 4, melted flag 1)
+    k_rm_mint    1 B per box cell (flag) + 20 B per flagged cell (grain id 4,
                  length 8, T 8)
-    k_rm_claim   1 B per interior cell + 56 B per flagged cell (grain id 4,
-                 envelope read 32 and length written 8, T 8, orientation 4)
-    k_rm_emit    1 B per interior cell + 44 B per capture
+    k_rm_claim   1 B per box cell + 56 B per flagged cell (grain id 4, envelope
+                 read 32 and length written 8, T 8, orientation 4)
+    k_rm_emit    1 B per box cell + 44 B per capture
     k_rm_nodal   24 B per node (two frames read, nodal field written)
     """
-    ng = float((P["nx"] + 2) * (P["ny"] + 2) * (P["nz"] + 2))
-    n = float(P["nx"] * P["ny"] * P["nz"])
     nodes = float(P["nodes"])
+    X = log["n_box"].astype(np.float64)
     F = log["n_flagged"].astype(np.float64)
     C = log["n_captured"].astype(np.float64)
     one = np.ones_like(F)
     return {
-        "k_rm_phase": (46.0 * ng + 8.0 * nodes) * one,
-        "k_rm_liquid": 13.0 * n * one,
-        "k_rm_mint": 1.0 * ng + 20.0 * F,
-        "k_rm_claim": 1.0 * n + 56.0 * F,
-        "k_rm_emit": 1.0 * n + 44.0 * C,
+        "k_rm_window": 8.0 * nodes * one,
+        "k_rm_phase": 46.0 * X + 8.0 * nodes,
+        "k_rm_liquid": 13.0 * X,
+        "k_rm_mint": 1.0 * X + 20.0 * F,
+        "k_rm_claim": 1.0 * X + 56.0 * F,
+        "k_rm_emit": 1.0 * X + 44.0 * C,
         "k_rm_nodal": 24.0 * nodes * one,
     }
 
@@ -144,26 +146,42 @@ def pbf_cpu_baseline(steps=300):
 
 
 def pbf_main(args):
-    """Coupled CA-FE single track: 1000 x 400 x 200 cells of 2.5e-6 m."""
+    """Coupled CA-FE single track: 1000 x 400 x 200 cells of 2.5e-6 m.
+
+    The timed region runs the steps as the library does by default (HIP graphs
+    of eight steps, no per-kernel events).  A second domain built from the same
+    inputs then runs the same steps with HIP events around every kernel, for
+    the per-kernel times and the roofline; its grain-ID field must equal the
+    first domain's."""
     from amca3d_amd import pbf
     t_setup = time.perf_counter()
     w = pbf.track(n_frames=args.frames)
-    d = pbf.domain(w)
     setup_s = time.perf_counter() - t_setup
-    d.enable_log(args.warmup + args.steps + 8)
-    if args.warmup > 0:
-        d.run(args.warmup)
-    if not args.no_timing:
-        d.enable_timing(True)
-    c0 = d.counters()
-    t0 = time.perf_counter()
-    c = d.run(args.steps)
-    wall = time.perf_counter() - t0
-    executed = int(c["step"] - c0["step"])
     g = w["grid"]
     cells = g["nx"] * g["ny"] * g["nz"]
+
+    def run(timing):
+        d = pbf.domain(w)
+        if args.whole_grid:
+            d.set_window(False)
+        d.enable_log(args.warmup + args.steps + 8)
+        if args.warmup > 0:
+            d.run(args.warmup)
+        if timing:
+            d.enable_timing(True)
+        c0 = d.counters()
+        t0 = time.perf_counter()
+        c = d.run(args.steps)
+        wall = time.perf_counter() - t0
+        log = d.read_log()[int(c0["step"]):int(c["step"])]
+        kt = d.kernel_times() if timing else None
+        ori = d.download()["ori"]
+        d.close()
+        return c0, c, wall, log, kt, ori
+
+    c0, c, wall, log, _, ori = run(False)
+    executed = int(c["step"] - c0["step"])
     value = cells * executed / wall
-    log = d.read_log()[int(c0["step"]):int(c["step"])]
     result = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup,
@@ -177,19 +195,27 @@ def pbf_main(args):
                    "cells": cells, "fe_nodes": int(np.prod(w["fe"]["shape"])),
                    "frames": args.frames, "executed_steps": executed,
                    "sim_time_s": c["time"], "captures": int(log["n_captured"].sum()),
-                   "nucleated": int(log["n_nucleated"].sum()), "setup_s": setup_s},
+                   "nucleated": int(log["n_nucleated"].sum()), "setup_s": setup_s,
+                   "value_counts": "every grid cell per step (the step's result equals "
+                                   "the whole-grid step's)",
+                   "active_box": "off (every step over the whole grid)" if args.whole_grid else
+                   {"mean_cells": float(log["n_box"].mean()),
+                    "mean_fraction_of_grid": float(log["n_box"].mean()) /
+                    ((g["nx"] + 2) * (g["ny"] + 2) * (g["nz"] + 2))}},
     }
     if not args.no_timing:
-        kt = d.kernel_times()
+        _, c2, wall2, log2, kt, ori2 = run(True)
         total = {k: v[0] for k, v in kt.items()}
         dom_k = max(total, key=total.get)
         Pn = dict(nx=g["nx"], ny=g["ny"], nz=g["nz"], nodes=int(np.prod(w["fe"]["shape"])))
-        ab = remelt_algorithmic_bytes(Pn, log)
+        ab = remelt_algorithmic_bytes(Pn, log2)
         roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": dom_k,
                 "kernel_ms_total": total[dom_k], "launches": kt[dom_k][1],
                 "avg_launch_us": 1e3 * total[dom_k] / max(kt[dom_k][1], 1),
-                "kernel_share_of_wall": total[dom_k] / (1e3 * wall)}
-        if dom_k in ab and kt[dom_k][1] == len(log):
+                "events_run_wall_ms": 1e3 * wall2,
+                "timed_run_equals_events_run": bool(np.array_equal(ori, ori2)) and
+                int(c2["step"]) == int(c["step"])}
+        if dom_k in ab and kt[dom_k][1] == len(log2):
             bytes_total = float(ab[dom_k].sum())
             achieved = bytes_total / (total[dom_k] * 1e-3) / 1e9
             roof.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS,
@@ -199,7 +225,6 @@ def pbf_main(args):
         roof["traffic"] = measured_traffic(dom_k, "pmc_traffic_pbf_track.json")
         result["roofline"] = roof
         result["kernel_ms"] = {k: round(v, 3) for k, v in total.items()}
-    d.close()
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = pbf_cpu_baseline()
     print(json.dumps(result))
@@ -306,8 +331,10 @@ def surrogate_main(args):
                      "rms_outside_band_K": float(np.sqrt(np.mean(e[far] ** 2)))})
     runs = {}
     lattice = {}
-    for mode in ("fe", "surrogate"):
+    for mode in ("fe_whole_grid", "fe", "surrogate"):
         d = pbf.domain(w)
+        if mode == "fe_whole_grid":
+            d.set_window(False)
         d.enable_log(args.steps + 8)
         if mode == "surrogate":
             t1 = time.perf_counter()
@@ -342,7 +369,12 @@ def surrogate_main(args):
                                "5-128x4-1 SiLU, BF16 MFMA), FE fallback within 50 K of Tm or "
                                "where node residual > 2 K",
                    "fe_driven_value": runs["fe"]["value"],
+                   "fe_driven_whole_grid_value": runs["fe_whole_grid"]["value"],
                    "surrogate_over_fe": runs["surrogate"]["value"] / runs["fe"]["value"],
+                   "surrogate_over_fe_whole_grid":
+                       runs["surrogate"]["value"] / runs["fe_whole_grid"]["value"],
+                   "fe_active_box_equals_whole_grid":
+                       bool(np.array_equal(runs["fe"]["ori"], runs["fe_whole_grid"]["ori"])),
                    "grain_id_mismatch_fraction": mismatch,
                    "temperature_rms_served_K": err["rms_served"],
                    "temperature_max_served_K": err["max_served"],
@@ -377,6 +409,8 @@ def main():
     ap.add_argument("--domains", type=int, default=64, help="batch: number of domains")
     ap.add_argument("--no-autotune", action="store_true", help="batch: keep the default schedule")
     ap.add_argument("--frames", type=int, default=100, help="pbf: FE frames (10 us apart)")
+    ap.add_argument("--whole-grid", action="store_true",
+                    help="pbf: every step over the whole grid (no active box)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs the CA step on "
