@@ -98,7 +98,8 @@ struct amca_remelt {
   }
   // kGraphSteps steps captured as one HIP graph, valid while the kernels'
   // arguments (parameters, buffers, site blocks) are the ones captured
-  static constexpr int kGraphSteps = 8;
+  int graph_steps = 8;  // 0: every kernel launched on the stream
+  int row_reserve = 4096;  // orientation rows drawn ahead of the rows in use
   hipGraph_t graph = nullptr;
   hipGraphExec_t graph_exec = nullptr;
   std::vector<unsigned char> graph_sig;
@@ -172,6 +173,11 @@ amca_status push_rows(amca_remelt* d, int n, const double* l2g9) {
     RM_HIP(hipMemcpyAsync(d->B.l2g + 9 * (size_t)d->rows_pushed, l2g9, sizeof(double) * 9 * n,
                           hipMemcpyHostToDevice, d->stream));
   d->rows_pushed += n;
+  // the device runs a step only when its armed sites fit in the pushed rows
+  amca::RmCounters c;
+  RM_TRY(rm_counters(d, &c));
+  c.rows_avail = d->rows_pushed;
+  RM_TRY(rm_put_counters(d, c));
   RM_HIP(hipStreamSynchronize(d->stream));
   return AMCA_OK;
 }
@@ -210,11 +216,11 @@ amca_status rm_resolve_events(amca_remelt* d) {
   return AMCA_OK;
 }
 
-// k steps without per-kernel timing: whole graphs of kGraphSteps steps, then
+// k steps without per-kernel timing: whole graphs of graph_steps steps, then
 // single steps for the rest
 amca_status rm_enqueue_steps(amca_remelt* d, long long k) {
-  const int G = amca_remelt::kGraphSteps;
-  if (k >= G) {
+  const int G = d->graph_steps > 0 ? d->graph_steps : 1;
+  if (d->graph_steps > 0 && k >= G) {
     std::vector<unsigned char> sig = d->launch_sig();
     if (!d->graph_exec || sig != d->graph_sig) {
       d->drop_graph();
@@ -235,7 +241,8 @@ amca_status rm_enqueue_steps(amca_remelt* d, long long k) {
     }
     for (long long q = 0; q < k / G; q++) RM_HIP(hipGraphLaunch(d->graph_exec, d->stream));
   }
-  for (long long q = 0; q < k % G; q++)
+  const long long rest = d->graph_steps > 0 ? k % G : k;
+  for (long long q = 0; q < rest; q++)
     RM_HIP(amca::rm_launch_step(d->P, d->B, d->site_blocks, d->stream));
   return AMCA_OK;
 }
@@ -842,12 +849,22 @@ amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_count
     if (c.exhausted || !(c.tcafe < d->desc.final_time)) break;
     const long long taken = c.step - start;
     if (max_steps >= 0 && taken >= max_steps) break;
+    if (c.need_rows) {
+      // a step stopped: its armed sites could need more rows than were pushed
+      const long long want = (long long)c.n_orient + c.armed[c.step & 1];
+      if (want > d->P.orient_cap) return rm_fail(AMCA_ERR_CAPACITY, "orientation table exhausted");
+      RM_TRY(ensure_rows(d, want + d->row_reserve));
+      if (d->rows_pushed < want)
+        return rm_fail(AMCA_ERR_CAPACITY, "orientation rows exhausted and no orientation source set");
+      RM_TRY(rm_counters(d, &c));
+      c.need_rows = 0;
+      c.armed[c.step & 1] = 0;
+      RM_TRY(rm_put_counters(d, c));
+    }
     long long k = batch;
     if (max_steps >= 0) k = std::min<long long>(k, max_steps - taken);
-    // each live site nucleates at most once per step
-    const long long room = d->P.orient_cap - (long long)c.n_orient;
-    if (c.n_sites > 0) k = std::max(1LL, std::min<long long>(k, room / c.n_sites));
-    RM_TRY(ensure_rows(d, (long long)c.n_orient + k * c.n_sites));
+    if (c.n_sites > 0 && (d->source || d->rng))
+      RM_TRY(ensure_rows(d, (long long)c.n_orient + d->row_reserve));
     if (d->timing) {
       for (long long q = 0; q < k; q++) RM_TRY(rm_enqueue_step(d));
     } else {
@@ -859,7 +876,9 @@ amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_count
   // cells outside the last step's box take that step's T and cooling
   if (c.step > start) {
     RM_HIP(amca::rm_launch_complete(d->P, d->B, d->stream));
-    RM_HIP(hipStreamSynchronize(d->stream));
+    RM_TRY(rm_counters(d, &c));
+    c.field_due = 0;  // written by the completion kernel
+    RM_TRY(rm_put_counters(d, c));
   }
   rm_fill(d, c, out);
   return AMCA_OK;
@@ -872,6 +891,20 @@ amca_status amca_remelt_set_window(amca_remelt* d, int32_t on) {
   RM_TRY(rm_counters(d, &c));
   rm_restart_box(c);
   RM_TRY(rm_put_counters(d, c));
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_set_row_reserve(amca_remelt* d, int32_t rows) {
+  if (!d || rows < 0) return rm_fail(AMCA_ERR_INVALID, "bad row reserve");
+  d->row_reserve = rows;
+  return AMCA_OK;
+}
+
+amca_status amca_remelt_set_graph_steps(amca_remelt* d, int32_t steps) {
+  if (!d || steps < 0 || steps > 1024) return rm_fail(AMCA_ERR_INVALID, "bad graph step count");
+  RM_HIP(hipStreamSynchronize(d->stream));
+  d->drop_graph();
+  d->graph_steps = steps;
   return AMCA_OK;
 }
 
@@ -1021,8 +1054,8 @@ amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6) {
   if (!d || !out6 || !d->P.sur) return rm_fail(AMCA_ERR_INVALID, "no surrogate in use");
   RM_HIP(hipMemsetAsync(d->B.sur_stats, 0, sizeof(double) * 8, d->stream));
   // fallback flags for the current time, as the next step's phase would see them
-  RM_HIP(amca::rm_launch_kernel(0, d->P, d->B, d->site_blocks, d->stream));
   RM_HIP(amca::rm_launch_kernel(1, d->P, d->B, d->site_blocks, d->stream));
+  RM_HIP(amca::rm_launch_kernel(2, d->P, d->B, d->site_blocks, d->stream));
   RM_HIP(amca::rm_launch_sur_error(d->P, d->B, d->stream));
   double s[8];
   RM_HIP(hipMemcpyAsync(s, d->B.sur_stats, sizeof(s), hipMemcpyDeviceToHost, d->stream));

@@ -26,7 +26,7 @@ struct RmParams {
   int sur_nxi, sur_nxi_nodes;
   double fx0, fy0, fz0, fdx;             // uniform node lattice of the structured mesh
   int sur_hot[6];        // xi-index, j, k ranges of the cell table at or above Tm - sur_band
-  // active box: each step runs over the box of cells that can change (see k_rm_window)
+  // active box: each step runs over the box of cells that can change (see k_rm_field)
   int window;
   int no_void;           // no frame holds a void node: the material of a cell never changes
   double hot_margin;     // K below Tm at which an element counts as hot
@@ -63,6 +63,13 @@ struct RmCounters {
   int force_full;      // the next step runs over the whole grid
   int sur_used[2];     // step n used the surrogate: sur_used[n & 1]
   int th_cur, th_prev; // slots of the nodal-field ring: this step's field, the last step's
+  int field_due;       // this step's field is not yet interpolated into slot th_cur
+  // Orientation rows: a step can nucleate at most at the sites whose cell is
+  // liquid when it starts (armed[step & 1], counted by k_rm_field); a step
+  // that could need more rows than the host has pushed (rows_avail) is not
+  // run: need_rows stops it and every later step until the host pushes rows.
+  int rows_avail, need_rows;
+  int armed[2];
   double time_fe_prev; // FE time of the last step
   // the last phase, for the completion pass: field slots, step, FE times and the dt of its rate
   int last_cur, last_prev;
@@ -114,14 +121,15 @@ struct RmBuffers {
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s);
 // One step: kernels 0..kRmKernels-1 in order on stream s.
-constexpr int kRmKernels = 14;
+constexpr int kRmKernels = 10;
 extern const char* const kRmKernelNames[kRmKernels];
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s);
 hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks, hipStream_t s);
 // surrogate error statistics into B.sur_stats (zeroed by the caller)
 hipError_t rm_launch_sur_error(const RmParams& P, const RmBuffers& B, hipStream_t s);
-// T and cooling of the last step for every cell outside its active box
+// T and cooling of the last step for every cell outside its active box, and
+// the next step's nodal field when it is due
 hipError_t rm_launch_complete(const RmParams& P, const RmBuffers& B, hipStream_t s);
 // P.sur_hot from the cell table (B.sur_cell), written to *out (6 ints, device)
 hipError_t rm_launch_sur_hot(const RmParams& P, const RmBuffers& B, int* out, hipStream_t s);

@@ -2,31 +2,34 @@
 // driven by CafeSolver::time_integration_weak_coupling, CafeSolver.cpp:178-219).
 //
 // One step, in the reference's order (each grid kernel over the step's active
-// box, see k_rm_window):
-//   k_rm_window   the active box: hot elements, liquid and captured cells
-//   k_rm_phase    temperature of every cell from the FE nodal field (hex8
-//                 interpolation, MapVoxelManager::evaluate_temperature), cooling
-//                 flag, melting (checkup_current_temperature_to_update_phase_state,
-//                 CellularAutomataManager_3DRemelting.cpp:338-404)
-//   k_rm_liquid   flags the solid neighbours of undercooled liquid cells
-//                 (checkup_liquid_cell_neigh_to_growth, :485-542)
-//   k_rm_nucleate nucleation sites, remelting variant (:936-999)
-//   k_rm_mint     length cutoff and the minimum flagged temperature (:656-727)
-//   k_rm_dt       dt = min(factor*h/v(minT), dT_FE)
-//   k_rm_claim    growth of every flagged grain and claims on enclosed liquid
-//                 neighbours (single_integration_step, :544-652)
-//   k_rm_resolve  captures: the claim with the smallest x-major sweep key wins
-//   k_rm_rowscan  grain ids for cells that never had a grain, in sweep order
-//   k_rm_emit     child envelopes (capture_cell, :807-847)
-//   k_rm_finish   time advance and FE frame selection
-//                 (FiniteElementManager::pseudo_single_integration_step,
-//                 LoadDataFromFile.cpp:207-313)
-//   k_rm_nodal    nodal time interpolation of the FE field
+// box, see the active-box notes below):
+//   k_rm_field          the step's nodal field, interpolated in time between
+//                       the frames the last step selected
+//                       (FiniteElementManager::pseudo_single_integration_step,
+//                       LoadDataFromFile.cpp:207-313), and the hot nodes that
+//                       seed the active box
+//   k_rm_sur_nodes/elems  (surrogate only) element fallback flags
+//   k_rm_phase          temperature of every cell from the FE nodal field (hex8
+//                       interpolation, MapVoxelManager::evaluate_temperature),
+//                       cooling flag, melting
+//                       (checkup_current_temperature_to_update_phase_state,
+//                       CellularAutomataManager_3DRemelting.cpp:338-404)
+//   k_rm_liquid         flags the solid neighbours of undercooled liquid cells
+//                       (checkup_liquid_cell_neigh_to_growth, :485-542)
+//   k_rm_nucleate_mint  nucleation sites, remelting variant (:936-999), and in
+//                       the same launch the length cutoff and the minimum
+//                       flagged temperature (:656-727)
+//   k_rm_claim          dt = min(factor*h/v(minT), dT_FE); growth of every
+//                       flagged grain and claims on enclosed liquid neighbours
+//                       (single_integration_step, :544-652)
+//   k_rm_resolve        captures: the claim with the smallest x-major sweep key
+//   k_rm_rowscan        grain ids for cells that never had a grain, in sweep
+//                       order; then the time advance and FE frame selection
+//   k_rm_emit           child envelopes (capture_cell, :807-847)
 // The reference sweeps cells with i outermost and k innermost, and a capture in
 // the sweep is final, so a liquid cell goes to the first flagged cell in that
-// order whose envelope holds it: key (i*nyG + j)*nzG + k.  The three sweep
-// kernels walk one thread per (i, j) row over k, which keeps loads coalesced
-// and makes per-row counts come out in sweep order.
+// order whose envelope holds it: key (i*nyG + j)*nzG + k.  Per-row counts of
+// fresh captures, scanned in x-major row order, give the new grain ids.
 #include <hip/hip_runtime.h>
 #include <climits>
 
@@ -175,7 +178,7 @@ __device__ __forceinline__ const unsigned char* rm_fb(const RmParams& P, const R
 // neighbour (so it is not flagged) and no flagged neighbour (so it is not
 // captured).  The box of step n is the union of
 //   - the cells of elements with a node at or above Tm - hot_margin, or whose
-//     nodes change between void and material, at step n (k_rm_window), and
+//     nodes change between void and material, at step n (k_rm_field), and
 //     with the surrogate the table's hot region at step n;
 //   - the liquid cells with material after the phase of step n-1 (k_rm_liquid);
 //   - the cells captured in step n-1 (a capture of a cell without material is
@@ -254,9 +257,14 @@ __device__ __forceinline__ void block_box_merge(RmBox b, RmBox* dst) {
     // settles after a few blocks, and same-address atomics from every block
     // would serialise
     if (v[0] <= v[3]) {
+      int cur[6];
       for (int q = 0; q < 3; q++) {
-        if (v[q] < ld_coherent(&dst->lo[q])) atomicMin(&dst->lo[q], v[q]);
-        if (v[q + 3] > ld_coherent(&dst->hi[q])) atomicMax(&dst->hi[q], v[q + 3]);
+        cur[q] = ld_coherent(&dst->lo[q]);
+        cur[q + 3] = ld_coherent(&dst->hi[q]);
+      }
+      for (int q = 0; q < 3; q++) {
+        if (v[q] < cur[q]) atomicMin(&dst->lo[q], v[q]);
+        if (v[q + 3] > cur[q + 3]) atomicMax(&dst->hi[q], v[q + 3]);
       }
     }
   }
@@ -268,7 +276,19 @@ __device__ __forceinline__ RmBox rm_step_box(const RmCounters* ctr) {
 }
 
 __device__ __forceinline__ bool rm_skip_now(const RmParams& P, const RmCounters* ctr) {
-  return !(ctr->tcafe < P.final_time && !ctr->exhausted);
+  return !(ctr->tcafe < P.final_time && !ctr->exhausted) || ctr->need_rows;
+}
+
+__device__ __forceinline__ int block_sum_int(int v) {
+  __shared__ int s_v[kRmBlock / 64];
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) s_v[threadIdx.x >> 6] = v;
+  __syncthreads();
+  int t = 0;
+#pragma unroll
+  for (int q = 0; q < kRmBlock / 64; q++) t += s_v[q];
+  return t;
 }
 
 // ---------------------------------------------------------------------------
@@ -284,15 +304,41 @@ __global__ void k_rm_init(RmParams P, RmBuffers B) {
   }
 }
 
-// Hot elements of the running step, and the surrogate's hot region, into
-// acc[step & 1][0].
-__global__ void __launch_bounds__(kRmBlock) k_rm_window(RmParams P, RmBuffers B) {
+// FiniteElementManager::pseudo_single_integration_step's nodal field
+// (LoadDataFromFile.cpp:207-313): void where the newer frame is void, the
+// newer frame where the older is void, else linear in time.
+__device__ __forceinline__ double rm_nodal(const RmParams& P, const RmBuffers& B,
+                                           const RmCounters* ctr, int nd) {
+  const double* th0 = B.frames + (size_t)ctr->frame * P.nodes;
+  const double a = th0[nd], b = th0[nd - P.nodes];
+  if (a <= P.void_temp) return P.void_temp;
+  if (b <= P.void_temp) return a;
+  return ctr->ratio * (a - b) + b;
+}
+
+// The step's nodal field (interpolated here when the last step selected new
+// frames), then the hot nodes of the active box and the surrogate's hot
+// region into acc[step & 1][0].
+__global__ void __launch_bounds__(kRmBlock) k_rm_field(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
   if (rm_skip_now(P, ctr)) return;
   const long long n = ctr->step;
-  if (!P.window || ctr->force_full || n == 0) return;
   const int par = (int)(n & 1);
-  const double* th = rm_field(P, B, ctr->th_cur);
+  {
+    // sites whose cell is liquid: the most nuclei this step can make
+    const int ns = ctr->n_sites;
+    int armed = 0;
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x)
+      armed += B.oid[B.site_cell[i]] == -1 ? 1 : 0;
+    if (ns > 0) {
+      armed = block_sum_int(armed);
+      if (threadIdx.x == 0 && armed) atomicAdd(&ctr->armed[par], armed);
+    }
+  }
+  const bool due = ctr->field_due != 0;
+  const bool hot_test = P.window && !ctr->force_full && n > 0;
+  if (!due && !hot_test) return;
+  double* th = B.theta + (size_t)ctr->th_cur * P.nodes;
   const double* tp = rm_field(P, B, ctr->th_prev);
   const double hot = P.Tm - P.hot_margin;
   const double nm = P.void_temp + 0.00001;
@@ -303,7 +349,14 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_window(RmParams P, RmBuffers B)
   // a node adds the cell box of the elements around it.
   const int stride = gridDim.x * blockDim.x;
   for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += stride) {
-    const double v = th[nd];
+    double v;
+    if (due) {
+      v = rm_nodal(P, B, ctr, nd);
+      th[nd] = v;
+    } else {
+      v = th[nd];
+    }
+    if (!hot_test) continue;
     bool take = v >= hot;
     if (!take && !P.no_void) take = (v < nm) != (tp[nd] < nm);
     if (!take) continue;
@@ -313,7 +366,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_window(RmParams P, RmBuffers B)
       box_add(b, nb[3], nb[4], nb[5]);
     }
   }
-  if (P.sur && blockIdx.x == 0 && threadIdx.x == 0 && P.sur_hot[0] <= P.sur_hot[1]) {
+  if (hot_test && P.sur && blockIdx.x == 0 && threadIdx.x == 0 && P.sur_hot[0] <= P.sur_hot[1]) {
     // cell i reads table entries floor(u), floor(u) + 1 (clamped), with
     // u = (x0 + h (i - 0.5) - x_start - v t - xi0) / dxi
     const double t = ctr->time_fe;
@@ -332,16 +385,20 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_window(RmParams P, RmBuffers B)
       box_add(b, ihi, P.sur_hot[3], P.sur_hot[5]);
     }
   }
-  block_box_merge(b, &ctr->acc[par][0]);
+  if (hot_test) block_box_merge(b, &ctr->acc[par][0]);
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
-  const int skip = rm_skip_now(P, ctr) ? 1 : 0;
-  if (blockIdx.x == 0 && threadIdx.x == 0) ctr->skip = skip;
-  if (skip) return;
   const long long n = ctr->step;
   const int par = (int)(n & 1);
+  const bool short_rows = ctr->n_sites > 0 && ctr->n_orient + ctr->armed[par] > ctr->rows_avail;
+  const int skip = (rm_skip_now(P, ctr) || short_rows) ? 1 : 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr->skip = skip;
+    if (short_rows && !rm_skip_now(P, ctr)) ctr->need_rows = 1;
+  }
+  if (skip) return;
   // the box of this step
   RmBox R;
   if (!P.window || ctr->force_full || n == 0) {
@@ -372,6 +429,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
   const double tfe = ctr->time_fe, tfe_p = ctr->time_fe_prev;
   const int sur_p = ctr->sur_used[par ^ 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ctr->field_due = 0;  // k_rm_field has written this step's field
     ctr->box[par] = R;
     ctr->sur_used[par] = P.sur;
     ctr->last_cur = ctr->th_cur;
@@ -437,6 +495,14 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
 // that step's phase would have (k_rm_phase).
 __global__ void __launch_bounds__(kRmBlock) k_rm_complete(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
+  if (ctr->field_due) {
+    // the next step's field, as its k_rm_field would write it (the host
+    // clears field_due after this kernel)
+    double* th = B.theta + (size_t)ctr->th_cur * P.nodes;
+    const int s = gridDim.x * blockDim.x;
+    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += s)
+      th[nd] = rm_nodal(P, B, ctr, nd);
+  }
   const long long n = ctr->last_step;
   if (n < 0) return;
   const int par = (int)(n & 1);
@@ -605,9 +671,8 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B)
 
 // nucleate_at_nucleation_sites, remelting variant: sites stay in the list;
 // a nucleus reuses the cell's grain record when it has one
-__global__ void __launch_bounds__(kRmBlock) k_rm_nucleate(RmParams P, RmBuffers B) {
+__device__ __forceinline__ void rm_nucleate_body(const RmParams& P, const RmBuffers& B, int nblk) {
   RmCounters* ctr = B.ctr;
-  if (ctr->skip) return;
   __shared__ int s_bid;
   __shared__ unsigned long long s_sm[kRmBlock / 64];
   __shared__ unsigned long long s_excl;
@@ -630,7 +695,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_nucleate(RmParams P, RmBuffers 
     }
   };
   int lo, hi;
-  chunk_of(n, bid, gridDim.x, kRmBlock, lo, hi);
+  chunk_of(n, bid, nblk, kRmBlock, lo, hi);
   unsigned long long mine = 0;
   for (int i = lo + threadIdx.x; i < hi; i += kRmBlock) {
     int c;
@@ -677,7 +742,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_nucleate(RmParams P, RmBuffers 
     }
     run += tot;
   }
-  if (threadIdx.x == 0 && bid == (int)gridDim.x - 1) {
+  if (threadIdx.x == 0 && bid == nblk - 1) {
     const int nn = (int)(run >> 32);
     const int nf = (int)(run & 0xffffffffull);
     if (no0 + nn > P.orient_cap) ctr->err = 1;
@@ -688,10 +753,9 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_nucleate(RmParams P, RmBuffers 
   }
 }
 
-// dt = min(dT_FE, min(factor*h/v(minT), dT_FE))
-__global__ void k_rm_dt(RmParams P, RmBuffers B) {
-  RmCounters* ctr = B.ctr;
-  if (ctr->skip) return;
+// dt = min(dT_FE, min(factor*h/v(minT), dT_FE)) from the minimum flagged
+// temperature (every thread of k_rm_claim computes it the same way)
+__device__ __forceinline__ double rm_step_dt(const RmParams& P, const RmCounters* ctr) {
   double minT = P.Tm;
   if (ctr->min_t_key != ~0ull) {
     const double m = dkey_inv(ctr->min_t_key);
@@ -702,14 +766,12 @@ __global__ void k_rm_dt(RmParams P, RmBuffers B) {
   const double a = P.tsf * P.h / maxV;
   const double dtca = (ctr->max_dt < a) ? ctr->max_dt : a;  // std::min(a, maxDt)
   const double dtfe = ctr->max_dt;
-  ctr->dt = ((dtfe) < (dtca) ? (dtfe) : (dtca));  // __min(dT_FE, dT_CA)
-  ctr->step_cap = 0;
-  ctr->step_fresh = 0;
+  return ((dtfe) < (dtca) ? (dtfe) : (dtca));  // __min(dT_FE, dT_CA)
 }
 
-__global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
+__device__ __forceinline__ void rm_mint_body(const RmParams& P, const RmBuffers& B, int blk,
+                                             int nblk) {
   RmCounters* ctr = B.ctr;
-  if (ctr->skip) return;
   __shared__ unsigned long long s_min;
   __shared__ int s_cnt;
   if (threadIdx.x == 0) {
@@ -724,8 +786,8 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
   box_clip(rm_step_box(ctr), zero, top, o0, ext);
   const int exy = ext[0] * ext[1];
   const int total = exy * ext[2];
-  const int stride = gridDim.x * blockDim.x;
-  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+  const int stride = nblk * blockDim.x;
+  for (int q = blk * blockDim.x + threadIdx.x; q < total; q += stride) {
     const int k = o0[2] + q / exy;
     const int rem = q - (k - o0[2]) * exy;
     const int j = o0[1] + rem / ext[0];
@@ -753,17 +815,33 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_mint(RmParams P, RmBuffers B) {
   }
 }
 
+// Nucleation at the sites (blocks 0 .. site_blocks-1) and the minimum flagged
+// temperature (the other blocks) in one launch: nucleation touches only liquid
+// site cells and their own grain records, the minimum only flagged (solid)
+// cells and theirs.
+__global__ void __launch_bounds__(kRmBlock) k_rm_nucleate_mint(RmParams P, RmBuffers B,
+                                                                int site_blocks) {
+  if (B.ctr->skip) return;
+  if ((int)blockIdx.x < site_blocks)
+    rm_nucleate_body(P, B, site_blocks);
+  else
+    rm_mint_body(P, B, (int)blockIdx.x - site_blocks, (int)gridDim.x - site_blocks);
+}
+
 // The sweep kernels run one thread per interior cell of the box (i fastest).
 // Rows (i, j) keep the x-major sweep order: row r = (i - i0) * ny_box + (j - j0),
 // and per-row counts of fresh captures give each row its first new grain id.
 struct RmRows {
   int i0, j0, k0, ex, ey, ez;
 };
-__device__ __forceinline__ RmRows rm_rows(const RmParams& P, const RmCounters* ctr) {
+__device__ __forceinline__ RmRows rm_rows_of(const RmParams& P, const RmBox& b) {
   const int lo[3] = {1, 1, 1}, hi[3] = {P.nx, P.ny, P.nz};
   int o0[3], ext[3];
-  box_clip(rm_step_box(ctr), lo, hi, o0, ext);
+  box_clip(b, lo, hi, o0, ext);
   return RmRows{o0[0], o0[1], o0[2], ext[0], ext[1], ext[2]};
+}
+__device__ __forceinline__ RmRows rm_rows(const RmParams& P, const RmCounters* ctr) {
+  return rm_rows_of(P, rm_step_box(ctr));
 }
 __device__ __forceinline__ void rm_cell_of(const RmRows& w, int q, int& i, int& j, int& k) {
   const int exy = w.ex * w.ey;
@@ -780,7 +858,13 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) 
   if (ctr->skip) return;
   const RmRows w = rm_rows(P, ctr);
   const int total = w.ex * w.ey * w.ez;
-  const double dt = ctr->dt;
+  const double dt = rm_step_dt(P, ctr);
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    // read by later kernels only
+    B.ctr->dt = dt;
+    B.ctr->step_cap = 0;
+    B.ctr->step_fresh = 0;
+  }
   const double vts = rm_velocity(P, P.Ts + 0.01);
   const int stride = gridDim.x * blockDim.x;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
@@ -871,30 +955,9 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_resolve(RmParams P, RmBuffers B
 
 constexpr int kRmScan = 1024;
 
-__global__ void __launch_bounds__(kRmScan) k_rm_rowscan(RmParams P, RmBuffers B) {
+// time advance and FE frame selection (one thread)
+__device__ void rm_finish(const RmParams& P, const RmBuffers& B) {
   RmCounters* ctr = B.ctr;
-  if (ctr->skip || ctr->step_fresh == 0) return;
-  __shared__ unsigned long long s_sm[kRmScan / 64];
-  const RmRows w = rm_rows(P, ctr);
-  const int nrow = w.ex * w.ey;
-  const int per = (nrow + kRmScan - 1) / kRmScan;
-  const int lo = min(nrow, threadIdx.x * per), hi = min(nrow, lo + per);
-  unsigned long long mine = 0;
-  for (int r = lo; r < hi; r++) mine += (unsigned long long)B.rowcnt[r];
-  unsigned long long tot;
-  unsigned long long off = block_scan_excl(mine, &tot, s_sm) + (unsigned long long)ctr->n_grains;
-  for (int r = lo; r < hi; r++) {
-    B.rowoff[r] = (int)off;
-    off += (unsigned long long)B.rowcnt[r];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) ctr->n_grains += (int)tot;
-}
-
-// time advance and FE frame selection
-__global__ void k_rm_finish(RmParams P, RmBuffers B) {
-  RmCounters* ctr = B.ctr;
-  if (ctr->skip) return;
   const double t0 = ctr->t;
   const double dt = ctr->dt;
   ctr->t = t0 + dt;
@@ -920,6 +983,7 @@ __global__ void k_rm_finish(RmParams P, RmBuffers B) {
   }
   if (B.fb_slots)
     for (int q = 0; q < 256; q++) B.fb_slots[q] = 0;
+  ctr->armed[st & 1] = 0;
   ctr->n_flagged = 0;
   ctr->n_fallback = 0;
   ctr->min_t_key = ~0ull;
@@ -956,16 +1020,46 @@ __global__ void k_rm_finish(RmParams P, RmBuffers B) {
     ctr->ratio = (tf - in1) / (in0 - in1);
     if (ctr->ratio < 0.0) ctr->err = 2;
     ctr->max_dt = in0 - tf;  // get_max_time_step
-    // k_rm_nodal writes the next field into a fresh slot of the ring
+    // the next step's k_rm_field writes its field into a fresh slot of the ring
     ctr->th_prev = ctr->th_cur;
     ctr->th_cur = (ctr->th_cur + 1) % 3;
+    ctr->field_due = 1;
   }
+}
+
+// Row offsets of the fresh grain ids (when there are fresh captures), then the
+// step's end: time advance and frame selection.  Nothing later in the step
+// reads what rm_finish changes except k_rm_emit, which takes the step number
+// as it was.
+__global__ void __launch_bounds__(kRmScan) k_rm_rowscan(RmParams P, RmBuffers B) {
+  RmCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  if (ctr->step_fresh) {
+    __shared__ unsigned long long s_sm[kRmScan / 64];
+    const RmRows w = rm_rows(P, ctr);
+    const int nrow = w.ex * w.ey;
+    const int per = (nrow + kRmScan - 1) / kRmScan;
+    const int lo = min(nrow, threadIdx.x * per), hi = min(nrow, lo + per);
+    unsigned long long mine = 0;
+    for (int r = lo; r < hi; r++) mine += (unsigned long long)B.rowcnt[r];
+    unsigned long long tot;
+    unsigned long long off =
+        block_scan_excl(mine, &tot, s_sm) + (unsigned long long)ctr->n_grains;
+    for (int r = lo; r < hi; r++) {
+      B.rowoff[r] = (int)off;
+      off += (unsigned long long)B.rowcnt[r];
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) ctr->n_grains += (int)tot;
+  }
+  if (threadIdx.x == 0) rm_finish(P, B);
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
   if (ctr->skip || ctr->step_cap == 0) return;
-  const RmRows w = rm_rows(P, ctr);
+  const long long st = ctr->step - 1;  // k_rm_rowscan has advanced the step
+  const RmRows w = rm_rows_of(P, ctr->box[st & 1]);
   const int total = w.ex * w.ey * w.ez;
   RmBox cap_box;
   box_empty(cap_box);
@@ -1023,25 +1117,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
       box_add(cap_box, i + di, j + dj, k + dk);
     }
   }
-  block_box_merge(cap_box, &ctr->acc[(ctr->step + 1) & 1][2]);
-}
-
-__global__ void __launch_bounds__(kRmBlock) k_rm_nodal(RmParams P, RmBuffers B) {
-  const RmCounters* ctr = B.ctr;
-  if (ctr->skip || ctr->exhausted) return;
-  const double r = ctr->ratio;
-  const double* th0 = B.frames + (size_t)ctr->frame * P.nodes;
-  const double* th1 = th0 - P.nodes;
-  double* out = B.theta + (size_t)ctr->th_cur * P.nodes;
-  const int stride = gridDim.x * blockDim.x;
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.nodes; n += stride) {
-    const double a = th0[n], b = th1[n];
-    double v;
-    if (a <= P.void_temp) v = P.void_temp;
-    else if (b <= P.void_temp) v = a;
-    else v = r * (a - b) + b;
-    out[n] = v;
-  }
+  block_box_merge(cap_box, &ctr->acc[(st + 1) & 1][2]);
 }
 
 // ---------------------------------------------------------------------------
@@ -1054,9 +1130,8 @@ hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s) 
 }
 
 const char* const kRmKernelNames[kRmKernels] = {
-    "k_rm_sur_nodes", "k_rm_sur_elems", "k_rm_window", "k_rm_phase",  "k_rm_liquid",
-    "k_rm_nucleate",  "k_rm_mint",      "k_rm_dt",     "k_rm_claim",  "k_rm_resolve",
-    "k_rm_rowscan",   "k_rm_emit",      "k_rm_finish", "k_rm_nodal"};
+    "k_rm_field", "k_rm_sur_nodes", "k_rm_sur_elems", "k_rm_phase",   "k_rm_liquid",
+    "k_rm_nucleate_mint", "k_rm_claim", "k_rm_resolve", "k_rm_rowscan", "k_rm_emit"};
 
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s) {
@@ -1070,28 +1145,27 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
   if (nbk < 1) nbk = 1;
   switch (q) {
     case 0:
-      if (!P.sur) return hipSuccess;
-      hipLaunchKernelGGL(k_rm_sur_nodes, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+      hipLaunchKernelGGL(k_rm_field, dim3((unsigned)(nbk < 1024 ? nbk : 1024)), dim3(kRmBlock), 0,
+                         s, P, B);
       break;
     case 1:
       if (!P.sur) return hipSuccess;
-      hipLaunchKernelGGL(k_rm_sur_elems, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+      hipLaunchKernelGGL(k_rm_sur_nodes, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
       break;
     case 2:
-      if (!P.window) return hipSuccess;
-      hipLaunchKernelGGL(k_rm_window, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+      if (!P.sur) return hipSuccess;
+      hipLaunchKernelGGL(k_rm_sur_elems, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
       break;
     case 3: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 4: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 5: hipLaunchKernelGGL(k_rm_nucleate, dim3(site_blocks), dim3(kRmBlock), 0, s, P, B); break;
-    case 6: hipLaunchKernelGGL(k_rm_mint, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 7: hipLaunchKernelGGL(k_rm_dt, dim3(1), dim3(1), 0, s, P, B); break;
-    case 8: hipLaunchKernelGGL(k_rm_claim, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 9: hipLaunchKernelGGL(k_rm_resolve, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 10: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
-    case 11: hipLaunchKernelGGL(k_rm_emit, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
-    case 12: hipLaunchKernelGGL(k_rm_finish, dim3(1), dim3(1), 0, s, P, B); break;
-    case 13: hipLaunchKernelGGL(k_rm_nodal, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B); break;
+    case 5:
+      hipLaunchKernelGGL(k_rm_nucleate_mint, dim3(cells + (unsigned)site_blocks), dim3(kRmBlock), 0,
+                         s, P, B, site_blocks);
+      break;
+    case 6: hipLaunchKernelGGL(k_rm_claim, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 7: hipLaunchKernelGGL(k_rm_resolve, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 8: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
+    case 9: hipLaunchKernelGGL(k_rm_emit, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();

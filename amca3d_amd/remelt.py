@@ -72,6 +72,8 @@ SIGNATURES = {
     "amca_remelt_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_remelt_set_window": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_remelt_last_box": (C.c_int, [C.c_void_p, I32P]),
+    "amca_remelt_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_remelt_set_row_reserve": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_remelt_use_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_double, C.c_double]),
     "amca_remelt_surrogate_error": (C.c_int, [C.c_void_p, F64P]),
@@ -322,6 +324,14 @@ class RemeltDomain:
     def set_window(self, on=True):
         """Active box on (default) or every step over the whole grid."""
         _check(lib().amca_remelt_set_window(self.h, 1 if on else 0), "set_window")
+
+    def set_graph_steps(self, n):
+        """Steps per HIP graph in untimed runs (0: plain stream launches)."""
+        _check(lib().amca_remelt_set_graph_steps(self.h, int(n)), "set_graph_steps")
+
+    def set_row_reserve(self, n):
+        """Orientation rows drawn ahead of the rows in use."""
+        _check(lib().amca_remelt_set_row_reserve(self.h, int(n)), "set_row_reserve")
 
     def last_box(self):
         b = np.zeros(6, np.int32)

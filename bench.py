@@ -93,17 +93,17 @@ def remelt_algorithmic_bytes(P, log):
     """Algorithmic bytes per kernel of the coupled step over a step log; X is
     the step's active box in cells (the whole grid without it).
 
-    k_rm_window  8 B per FE node (the nodal field, once per element corner set)
-    k_rm_phase   46 B per box cell (element tag 4, parametric coordinates 24, T
-                 read 8 and written 8, cooling and liquid-neighbour flags 1 + 1)
-                 and 8 B per FE node (the nodal field)
-    k_rm_liquid  13 B per box cell (T 8, orientation 4, melted flag 1)
-    k_rm_mint    1 B per box cell (flag) + 20 B per flagged cell (grain id 4,
-                 length 8, T 8)
-    k_rm_claim   1 B per box cell + 56 B per flagged cell (grain id 4, envelope
-                 read 32 and length written 8, T 8, orientation 4)
-    k_rm_emit    1 B per box cell + 44 B per capture
-    k_rm_nodal   24 B per node (two frames read, nodal field written)
+    k_rm_field          24 B per FE node (two frames read, nodal field written;
+                        the hot-node test uses the value in registers)
+    k_rm_phase          46 B per box cell (element tag 4, parametric coordinates
+                        24, T read 8 and written 8, cooling and liquid-neighbour
+                        flags 1 + 1) and 8 B per FE node (the nodal field)
+    k_rm_liquid         13 B per box cell (T 8, orientation 4, melted flag 1)
+    k_rm_nucleate_mint  1 B per box cell (flag) + 20 B per flagged cell (grain
+                        id 4, length 8, T 8)
+    k_rm_claim          1 B per box cell + 56 B per flagged cell (grain id 4,
+                        envelope read 32 and length written 8, T 8, orientation 4)
+    k_rm_emit           1 B per box cell + 44 B per capture
     """
     nodes = float(P["nodes"])
     X = log["n_box"].astype(np.float64)
@@ -111,13 +111,12 @@ def remelt_algorithmic_bytes(P, log):
     C = log["n_captured"].astype(np.float64)
     one = np.ones_like(F)
     return {
-        "k_rm_window": 8.0 * nodes * one,
+        "k_rm_field": 24.0 * nodes * one,
         "k_rm_phase": 46.0 * X + 8.0 * nodes,
         "k_rm_liquid": 13.0 * X,
-        "k_rm_mint": 1.0 * X + 20.0 * F,
+        "k_rm_nucleate_mint": 1.0 * X + 20.0 * F,
         "k_rm_claim": 1.0 * X + 56.0 * F,
         "k_rm_emit": 1.0 * X + 44.0 * C,
-        "k_rm_nodal": 24.0 * nodes * one,
     }
 
 
@@ -164,6 +163,7 @@ def pbf_main(args):
         d = pbf.domain(w)
         if args.whole_grid:
             d.set_window(False)
+        d.set_graph_steps(args.graph_steps)
         d.enable_log(args.warmup + args.steps + 8)
         if args.warmup > 0:
             d.run(args.warmup)
@@ -409,6 +409,8 @@ def main():
     ap.add_argument("--domains", type=int, default=64, help="batch: number of domains")
     ap.add_argument("--no-autotune", action="store_true", help="batch: keep the default schedule")
     ap.add_argument("--frames", type=int, default=100, help="pbf: FE frames (10 us apart)")
+    ap.add_argument("--graph-steps", type=int, default=8,
+                    help="pbf/surrogate: steps per HIP graph (0: stream launches)")
     ap.add_argument("--whole-grid", action="store_true",
                     help="pbf: every step over the whole grid (no active box)")
     args = ap.parse_args()

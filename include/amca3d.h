@@ -410,6 +410,14 @@ amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6);
  * step's T and cooling to the cells outside it.  Results are identical with
  * it off (every step over the whole grid, as the reference's loops run). */
 amca_status amca_remelt_set_window(amca_remelt* d, int32_t on);
+/* Orientation rows drawn ahead of the rows in use (default 4096).  A step
+ * runs only when every site whose cell is liquid at its start could nucleate
+ * with the rows pushed; otherwise the run draws more and repeats it.  Results
+ * do not depend on it. */
+amca_status amca_remelt_set_row_reserve(amca_remelt* d, int32_t rows);
+/* Steps per HIP graph when a run is not timed (default 8; 0: every kernel is
+ * launched on the domain's stream).  Results do not depend on it. */
+amca_status amca_remelt_set_graph_steps(amca_remelt* d, int32_t steps);
 /* The last step's box (i0, j0, k0, i1, j1, k1, ghost layer included). */
 amca_status amca_remelt_last_box(amca_remelt* d, int32_t* box6);
 /* Per-kernel device time from HIP events around every launch (off by default);

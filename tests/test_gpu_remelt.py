@@ -26,7 +26,7 @@ def _desc(case):
     return d
 
 
-def run_device_own_sampler(case, orc):
+def run_device_own_sampler(case, orc, reserve=None):
     """Run `case` with the library's own host sampler and rand() stream
     (amca_remelt_draw_sites) instead of inputs taken from the oracle run."""
     from amca3d_amd.remelt import RemeltDomain
@@ -38,6 +38,8 @@ def run_device_own_sampler(case, orc):
     d.set_fe(case["fe"])
     d.set_state(st)
     n_sites = d.draw_sites(case["seed"], nuc["surface"], nuc["bulk"])
+    if reserve is not None:
+        d.set_row_reserve(reserve)
     d.enable_log(100000)
     c = d.run(case["max_steps"])
     out = d.download()
@@ -165,6 +167,17 @@ def test_track_with_the_library_sampler(amca, track_nuc):
     case, orc = track_nuc
     dev = run_device_own_sampler(case, orc)
     assert dev["n_sites"] == len(orc["site_cell"])
+    compare(case, orc, dev)
+    np.testing.assert_array_equal(bits(dev["euler"]), bits(orc["euler"]))
+
+
+@pytest.mark.parametrize("reserve", [0, 3])
+def test_track_with_rows_drawn_on_demand(amca, track_nuc, reserve):
+    """With (almost) no orientation rows drawn ahead, steps whose liquid sites
+    could outrun the pushed rows stop and are repeated after the host draws
+    more: the run is the oracle's."""
+    case, orc = track_nuc
+    dev = run_device_own_sampler(case, orc, reserve=reserve)
     compare(case, orc, dev)
     np.testing.assert_array_equal(bits(dev["euler"]), bits(orc["euler"]))
 
