@@ -381,6 +381,7 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
   A(&B.lb_agg, (size_t)amca::kMaxBlocks);
   A(&B.lb_pref, (size_t)amca::kMaxBlocks);
   A(&B.ctr, 1);
+  A(&B.blk_box, 6 * 16384);
   A(&B.site_cell, 1);
   A(&B.site_dt, 1);
   if (st != AMCA_OK) {
@@ -397,7 +398,7 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
   (void)hipMemsetAsync(B.melted, desc->initial_melted ? 1 : 0, ng, d->stream);
   (void)hipMemsetAsync(B.lb_flag, 0, sizeof(unsigned) * amca::kMaxBlocks, d->stream);
   amca::RmCounters c{};
-  c.min_t_key = ~0ull;
+  for (int q = 0; q < 64; q++) c.min_t_keys[q] = ~0ull;
   c.last_step = -1;
   for (int q = 0; q < 2; q++) {
     c.box[q].lo[0] = c.box[q].lo[1] = c.box[q].lo[2] = 0;
@@ -453,15 +454,17 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   d->release((void*)d->B.frames);
   d->release((void*)d->B.frame_time);
   d->release(d->B.theta);
-  d->release(d->B.node_box);
+  d->release(const_cast<int*>(d->B.frame_hot));
   double *fr = nullptr, *ft = nullptr, *th = nullptr;
-  int* eb = nullptr;
+  int* fh = nullptr;
   const size_t ne = (size_t)(fnx - 1) * (fny - 1) * (fnz - 1);
   RM_TRY(d->alloc(&fr, nodes * (size_t)n_frames));
   RM_TRY(d->alloc(&ft, (size_t)n_frames));
-  RM_TRY(d->alloc(&th, 3 * nodes));
-  RM_TRY(d->alloc(&eb, 6 * nodes));
-  // cell box of the cells in the elements around each node, for the active box
+  RM_TRY(d->alloc(&th, nodes));
+  RM_TRY(d->alloc(&fh, 6 * (size_t)n_frames));
+  // For the active box: the cell box of the elements around each node, then
+  // per frame the union over its hot nodes (at or above Tm - hot_margin), and
+  // the union over nodes whose void state can change between frames.
   {
     std::vector<int32_t> box(6 * ne);
     for (size_t e = 0; e < ne; e++) {
@@ -500,18 +503,38 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
             }
           }
     }
-    RM_HIP(hipMemcpyAsync(eb, nbox.data(), sizeof(int32_t) * nbox.size(), hipMemcpyHostToDevice,
+    auto add = [](int32_t* dst, const int32_t* b) {
+      if (b[0] > b[3]) return;
+      for (int a = 0; a < 3; a++) {
+        dst[a] = std::min(dst[a], b[a]);
+        dst[3 + a] = std::max(dst[3 + a], b[3 + a]);
+      }
+    };
+    const double hot = P.Tm - P.hot_margin;
+    const double nm = P.void_temp + 0.00001;
+    std::vector<int32_t> fhot(6 * (size_t)n_frames);
+    std::vector<unsigned char> maybe_void(nodes, 0), always_void(nodes, 1);
+    for (int f = 0; f < n_frames; f++) {
+      int32_t* fb = &fhot[6 * (size_t)f];
+      fb[0] = fb[1] = fb[2] = INT_MAX;
+      fb[3] = fb[4] = fb[5] = INT_MIN;
+      const double* v = frame_theta + (size_t)f * nodes;
+      for (size_t n = 0; n < nodes; n++) {
+        if (v[n] >= hot) add(fb, &nbox[6 * n]);
+        // the field is void_temp where a frame is at or below it, and is
+        // compared with void_temp + 1e-5: a node near that value in some
+        // frame may change between void and material
+        if (!(v[n] > nm + 1e-6)) maybe_void[n] = 1;
+        if (!(v[n] <= P.void_temp)) always_void[n] = 0;
+      }
+    }
+    int32_t vb[6] = {INT_MAX, INT_MAX, INT_MAX, INT_MIN, INT_MIN, INT_MIN};
+    for (size_t n = 0; n < nodes; n++)
+      if (maybe_void[n] && !always_void[n]) add(vb, &nbox[6 * n]);
+    for (int a = 0; a < 6; a++) P.void_box[a] = vb[a];
+    RM_HIP(hipMemcpyAsync(fh, fhot.data(), sizeof(int32_t) * fhot.size(), hipMemcpyHostToDevice,
                           d->stream));
     RM_HIP(hipStreamSynchronize(d->stream));
-  }
-  // no void node in any frame (and so in no interpolated field): no cell's
-  // material changes
-  {
-    const double nm = P.void_temp + 0.00001;
-    bool any_void = false;
-    for (size_t q = 0; q < nodes * (size_t)n_frames && !any_void; q++)
-      any_void = !(frame_theta[q] > nm + 1e-6);
-    P.no_void = any_void ? 0 : 1;
   }
   std::vector<double> rel(n_frames);
   for (int f = 0; f < n_frames; f++) rel[f] = frame_time[f] - frame_time[0];  // inTimeOffset_
@@ -525,11 +548,11 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   d->B.frames = fr;
   d->B.frame_time = ft;
   d->B.theta = th;
-  d->B.node_box = eb;
+  d->B.frame_hot = fh;
   amca::RmCounters c;
   RM_TRY(rm_counters(d, &c));
-  c.th_cur = 0;
-  c.th_prev = 0;
+  // the coming step's field is the first frame as loaded
+  c.fs[c.step % 3] = amca::RmField{0, 1, 0.0};
   rm_restart_box(c);
   c.frame = 1;
   c.in_t1 = rel[0];
@@ -876,9 +899,7 @@ amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_count
   // cells outside the last step's box take that step's T and cooling
   if (c.step > start) {
     RM_HIP(amca::rm_launch_complete(d->P, d->B, d->stream));
-    RM_TRY(rm_counters(d, &c));
-    c.field_due = 0;  // written by the completion kernel
-    RM_TRY(rm_put_counters(d, c));
+    RM_HIP(hipStreamSynchronize(d->stream));
   }
   rm_fill(d, c, out);
   return AMCA_OK;
@@ -1071,10 +1092,8 @@ amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6) {
 
 amca_status amca_remelt_theta(amca_remelt* d, double* theta) {
   if (!d || !d->fe_ready || !theta) return rm_fail(AMCA_ERR_INVALID, "FE mesh not set");
-  amca::RmCounters c;
-  RM_TRY(rm_counters(d, &c));
-  RM_HIP(hipMemcpy(theta, d->B.theta + (size_t)c.th_cur * d->P.nodes,
-                   sizeof(double) * d->P.nodes, hipMemcpyDeviceToHost));
+  RM_HIP(hipStreamSynchronize(d->stream));
+  RM_HIP(hipMemcpy(theta, d->B.theta, sizeof(double) * d->P.nodes, hipMemcpyDeviceToHost));
   return AMCA_OK;
 }
 

@@ -28,8 +28,16 @@ struct RmParams {
   int sur_hot[6];        // xi-index, j, k ranges of the cell table at or above Tm - sur_band
   // active box: each step runs over the box of cells that can change (see k_rm_field)
   int window;
-  int no_void;           // no frame holds a void node: the material of a cell never changes
-  double hot_margin;     // K below Tm at which an element counts as hot
+  int void_box[6];       // cells of elements whose nodes are void in some frame and not in another
+  double hot_margin;     // K below Tm at which a node counts as hot
+};
+
+// The nodal field of one step (thetaArray_), as a function of the resident
+// frames: frame 0 itself at the start, else the time interpolation between
+// frames `frame` - 1 and `frame` with weight `ratio`.
+struct RmField {
+  int frame, init;
+  double ratio;
 };
 
 // inclusive cell-index box (ghost layer included); empty when lo > hi
@@ -46,11 +54,14 @@ struct RmCounters {
   double in_t0, in_t1;  // inTimeSeries_[0], [1]
   double ratio;      // nodal time-interpolation ratio
   long long step;
-  unsigned long long min_t_key;  // ordered key of the minimum flagged temperature
+  // ordered keys of the minimum flagged temperature and flagged-cell counts,
+  // spread over 64 words by block so that blocks rarely meet at one address
+  unsigned long long min_t_keys[64];
+  int flagged_parts[64];
   int frame;         // frame held as theta_[0]; frame-1 is theta_[1]
   int skip, exhausted;
   int n_grains, n_orient, n_sites;
-  int step_nuc, step_cap, step_fresh, n_flagged;
+  int step_nuc, step_cap, step_fresh;
   int err;
   int log_cap;
   int n_fallback;      // cells of the running step that took the FE path
@@ -62,8 +73,7 @@ struct RmCounters {
   RmBox acc[2][3];
   int force_full;      // the next step runs over the whole grid
   int sur_used[2];     // step n used the surrogate: sur_used[n & 1]
-  int th_cur, th_prev; // slots of the nodal-field ring: this step's field, the last step's
-  int field_due;       // this step's field is not yet interpolated into slot th_cur
+  RmField fs[3];       // the field of step n: fs[n % 3] (set by the end of step n - 1)
   // Orientation rows: a step can nucleate at most at the sites whose cell is
   // liquid when it starts (armed[step & 1], counted by k_rm_field); a step
   // that could need more rows than the host has pushed (rows_avail) is not
@@ -71,8 +81,7 @@ struct RmCounters {
   int rows_avail, need_rows;
   int armed[2];
   double time_fe_prev; // FE time of the last step
-  // the last phase, for the completion pass: field slots, step, FE times and the dt of its rate
-  int last_cur, last_prev;
+  // the last phase, for the completion pass: step, FE times and the dt of its rate
   long long last_step;
   double last_time_fe, last_time_fe_prev, last_dt;
 };
@@ -98,9 +107,9 @@ struct RmBuffers {
   int2* gco;              // grain -> (cell, orientation) [N]
   const double* frames;   // nodal frames [n_frames * nodes]
   const double* frame_time;  // [n_frames], relative to the first
-  double* theta;          // thetaArray_: ring of three nodal fields [3 * nodes], slot ctr->th_cur
-  int* node_box;          // cell box of the cells in the elements around each node
-                          // (i0 j0 k0 i1 j1 k1, empty: i0 > i1) [6 * nodes]
+  double* theta;          // thetaArray_ of the coming step [nodes] (written with the surrogate
+                          // each step, else at the end of a run)
+  const int* frame_hot;   // cell box of the hot nodes of each frame [6 * n_frames]
   int* site_cell;         // nucleation sites, ascending cell (never erased here)
   double* site_dt;
   double* l2g;            // orientation rows, 9 doubles each
@@ -117,6 +126,7 @@ struct RmBuffers {
   unsigned char* elem_fb;      // [2][elements], step parity
   double* sur_stats;           // [8] error statistics
   int* fb_slots;               // [256] per-step fallback cell counts by block slot
+  int* blk_box;                // [6 * 16384] per-block boxes of k_rm_liquid
 };
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s);
@@ -129,7 +139,7 @@ hipError_t rm_launch_step(const RmParams& P, const RmBuffers& B, int site_blocks
 // surrogate error statistics into B.sur_stats (zeroed by the caller)
 hipError_t rm_launch_sur_error(const RmParams& P, const RmBuffers& B, hipStream_t s);
 // T and cooling of the last step for every cell outside its active box, and
-// the next step's nodal field when it is due
+// the coming step's nodal field into B.theta
 hipError_t rm_launch_complete(const RmParams& P, const RmBuffers& B, hipStream_t s);
 // P.sur_hot from the cell table (B.sur_cell), written to *out (6 ints, device)
 hipError_t rm_launch_sur_hot(const RmParams& P, const RmBuffers& B, int* out, hipStream_t s);

@@ -76,9 +76,23 @@ __device__ __forceinline__ double dkey_inv(unsigned long long k) {
   return __longlong_as_double((long long)b);
 }
 
+// thetaArray_ at node nd for field f (FiniteElementManager::
+// pseudo_single_integration_step, LoadDataFromFile.cpp:207-313): the first
+// frame as loaded, else void where the newer frame is void, the newer frame
+// where the older is void, otherwise linear in time.
+__device__ __forceinline__ double rm_theta(const RmParams& P, const RmBuffers& B, const RmField& f,
+                                           int nd) {
+  if (f.init) return B.frames[nd];
+  const double* th0 = B.frames + (size_t)f.frame * P.nodes;
+  const double a = th0[nd], b = th0[nd - P.nodes];
+  if (a <= P.void_temp) return P.void_temp;
+  if (b <= P.void_temp) return a;
+  return f.ratio * (a - b) + b;
+}
+
 // MapVoxelManager::evaluate_temperature (MapVoxelManager.cpp:746-799) on a
 // structured mesh: element e has corner node ((ek*fny + ej)*fnx + ei).
-__device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B, const double* theta,
+__device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B, const RmField& f,
                                           int c, double& out) {
   const double nm = P.void_temp + 0.00001;
   const int e = B.tag[c];
@@ -98,7 +112,7 @@ __device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B,
   nid[7] = nid[3] + nxy;
   double th[8];
 #pragma unroll
-  for (int m = 0; m < 8; m++) th[m] = theta[nid[m]];
+  for (int m = 0; m < 8; m++) th[m] = rm_theta(P, B, f, nid[m]);
 #pragma unroll
   for (int m = 0; m < 8; m++)
     if (th[m] < nm) return false;
@@ -136,16 +150,15 @@ __device__ __forceinline__ double rm_sur_T(const RmParams& P, const RmBuffers& B
   return (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
 }
 
-// Temperature of cell c at one step: the FE path on nodal field `theta`, or
-// with the surrogate (sur) the table value at FE time tfe unless the cell's
-// element falls back (efb) or the value is within sur_band of Tm.  Returns
-// whether the cell has material; n_fb counts cells of an element that took
-// the FE path.
+// Temperature of cell c at one step: the FE path on nodal field f, or with
+// the surrogate (sur) the table value at FE time tfe unless the cell's element
+// falls back (efb) or the value is within sur_band of Tm.  Returns whether the
+// cell has material; n_fb counts cells of an element that took the FE path.
 __device__ __forceinline__ bool rm_cell_T(const RmParams& P, const RmBuffers& B, int c,
-                                          const double* theta, const unsigned char* efb,
+                                          const RmField& f, const unsigned char* efb,
                                           double tfe, int sur, double& Tn, int& n_fb) {
   Tn = -1.0;
-  if (!sur) return rm_eval_T(P, B, theta, c, Tn);
+  if (!sur) return rm_eval_T(P, B, f, c, Tn);
   const int e = B.tag[c];
   bool fe = e < 0 || efb[e];
   if (!fe) {
@@ -155,14 +168,10 @@ __device__ __forceinline__ bool rm_cell_T(const RmParams& P, const RmBuffers& B,
     fe = fabs(Tn - P.Tm) < P.sur_band;
   }
   if (!fe) return true;
-  const bool mat = rm_eval_T(P, B, theta, c, Tn);
+  const bool mat = rm_eval_T(P, B, f, c, Tn);
   if (!mat) Tn = -1.0;
   if (e >= 0) n_fb++;
   return mat;
-}
-
-__device__ __forceinline__ const double* rm_field(const RmParams& P, const RmBuffers& B, int slot) {
-  return B.theta + (size_t)slot * P.nodes;
 }
 __device__ __forceinline__ const unsigned char* rm_fb(const RmParams& P, const RmBuffers& B, int par) {
   return B.elem_fb ? B.elem_fb + (size_t)par * (P.ncx * P.ncy * (P.fnz - 1)) : nullptr;
@@ -236,8 +245,10 @@ __device__ __forceinline__ void box_clip(const RmBox& b, const int* lo, const in
 
 // Merge every thread's box into *dst: a min/max reduction over the block,
 // then one atomic per bound.
-__device__ __forceinline__ void block_box_merge(RmBox b, RmBox* dst) {
-  __shared__ int s_b[kRmBlock / 64][6];
+// Box of the whole block, valid in thread 0 (a min/max reduction over the block).
+template <int kThreads>
+__device__ __forceinline__ RmBox block_box_reduce(RmBox b) {
+  __shared__ int s_b[kThreads / 64][6];
   int v[6] = {b.lo[0], b.lo[1], b.lo[2], b.hi[0], b.hi[1], b.hi[2]};
 #pragma unroll
   for (int q = 0; q < 6; q++)
@@ -250,24 +261,38 @@ __device__ __forceinline__ void block_box_merge(RmBox b, RmBox* dst) {
 #pragma unroll
     for (int q = 0; q < 6; q++) s_b[threadIdx.x >> 6][q] = v[q];
   __syncthreads();
-  if (threadIdx.x == 0) {
-    for (int w = 1; w < kRmBlock / 64; w++)
+  if (threadIdx.x == 0)
+    for (int w = 1; w < kThreads / 64; w++)
       for (int q = 0; q < 6; q++) v[q] = q < 3 ? min(v[q], s_b[w][q]) : max(v[q], s_b[w][q]);
-    // only bounds that widen the current box go to the atomic unit: the box
-    // settles after a few blocks, and same-address atomics from every block
-    // would serialise
-    if (v[0] <= v[3]) {
-      int cur[6];
-      for (int q = 0; q < 3; q++) {
-        cur[q] = ld_coherent(&dst->lo[q]);
-        cur[q + 3] = ld_coherent(&dst->hi[q]);
-      }
-      for (int q = 0; q < 3; q++) {
-        if (v[q] < cur[q]) atomicMin(&dst->lo[q], v[q]);
-        if (v[q + 3] > cur[q + 3]) atomicMax(&dst->hi[q], v[q + 3]);
-      }
+  RmBox r;
+  for (int q = 0; q < 3; q++) {
+    r.lo[q] = v[q];
+    r.hi[q] = v[q + 3];
+  }
+  return r;
+}
+
+// Merge every thread's box into *dst: a block reduction, then one atomic per
+// bound.  For boxes that few blocks contribute to (hot nodes, captures); a box
+// that most blocks contribute to goes through per-block slots instead, since
+// same-address atomics from every block serialise.
+__device__ __forceinline__ void block_box_merge(RmBox b, RmBox* dst) {
+  const RmBox v = block_box_reduce<kRmBlock>(b);
+  if (threadIdx.x == 0 && v.lo[0] <= v.hi[0]) {
+    for (int q = 0; q < 3; q++) {
+      atomicMin(&dst->lo[q], v.lo[q]);
+      atomicMax(&dst->hi[q], v.hi[q]);
     }
   }
+}
+
+// blocks of the grid-stride cell kernels: with the active box, 2048 blocks of
+// 256 (eight per CU) cost little when the box is small; over the whole grid,
+// up to 16384
+__host__ __device__ __forceinline__ int rm_cell_blocks(const RmParams& P) {
+  const long long cb = ((long long)P.NG + kRmBlock - 1) / kRmBlock;
+  const int cap = P.window ? 2048 : 16384;
+  return (int)(cb < cap ? cb : cap);
 }
 
 // the box of the running step (written by k_rm_phase)
@@ -304,28 +329,19 @@ __global__ void k_rm_init(RmParams P, RmBuffers B) {
   }
 }
 
-// FiniteElementManager::pseudo_single_integration_step's nodal field
-// (LoadDataFromFile.cpp:207-313): void where the newer frame is void, the
-// newer frame where the older is void, else linear in time.
-__device__ __forceinline__ double rm_nodal(const RmParams& P, const RmBuffers& B,
-                                           const RmCounters* ctr, int nd) {
-  const double* th0 = B.frames + (size_t)ctr->frame * P.nodes;
-  const double a = th0[nd], b = th0[nd - P.nodes];
-  if (a <= P.void_temp) return P.void_temp;
-  if (b <= P.void_temp) return a;
-  return ctr->ratio * (a - b) + b;
-}
-
-// The step's nodal field (interpolated here when the last step selected new
-// frames), then the hot nodes of the active box and the surrogate's hot
-// region into acc[step & 1][0].
+// Start of a step: the sites whose cell is liquid (the most nuclei the step
+// can make), the hot region of the active box into acc[step & 1][0], and with
+// the surrogate the step's nodal field into B.theta for the fallback test.
+// A node is hot in the step's field only if it is hot in one of the two frames
+// the field interpolates (the interpolation is a convex combination, and void
+// nodes are cold), so the per-frame boxes of hot nodes bound the hot region.
 __global__ void __launch_bounds__(kRmBlock) k_rm_field(RmParams P, RmBuffers B) {
   RmCounters* ctr = B.ctr;
   if (rm_skip_now(P, ctr)) return;
   const long long n = ctr->step;
   const int par = (int)(n & 1);
+  const RmField f = ctr->fs[n % 3];
   {
-    // sites whose cell is liquid: the most nuclei this step can make
     const int ns = ctr->n_sites;
     int armed = 0;
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < ns; i += gridDim.x * blockDim.x)
@@ -335,38 +351,25 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_field(RmParams P, RmBuffers B) 
       if (threadIdx.x == 0 && armed) atomicAdd(&ctr->armed[par], armed);
     }
   }
-  const bool due = ctr->field_due != 0;
+  if (P.sur) {
+    const int stride = gridDim.x * blockDim.x;
+    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += stride)
+      B.theta[nd] = rm_theta(P, B, f, nd);
+  }
   const bool hot_test = P.window && !ctr->force_full && n > 0;
-  if (!due && !hot_test) return;
-  double* th = B.theta + (size_t)ctr->th_cur * P.nodes;
-  const double* tp = rm_field(P, B, ctr->th_prev);
-  const double hot = P.Tm - P.hot_margin;
-  const double nm = P.void_temp + 0.00001;
-  const int nxy = P.fnx * P.fny;
+  if (!hot_test || blockIdx.x != 0 || threadIdx.x != 0) return;
   RmBox b;
   box_empty(b);
-  // An element is hot, or changes material, when one of its nodes does: such
-  // a node adds the cell box of the elements around it.
-  const int stride = gridDim.x * blockDim.x;
-  for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += stride) {
-    double v;
-    if (due) {
-      v = rm_nodal(P, B, ctr, nd);
-      th[nd] = v;
-    } else {
-      v = th[nd];
+  auto add = [&](const int* q) {
+    if (q[0] <= q[3]) {
+      box_add(b, q[0], q[1], q[2]);
+      box_add(b, q[3], q[4], q[5]);
     }
-    if (!hot_test) continue;
-    bool take = v >= hot;
-    if (!take && !P.no_void) take = (v < nm) != (tp[nd] < nm);
-    if (!take) continue;
-    const int* nb = B.node_box + 6 * (size_t)nd;
-    if (nb[0] <= nb[3]) {
-      box_add(b, nb[0], nb[1], nb[2]);
-      box_add(b, nb[3], nb[4], nb[5]);
-    }
-  }
-  if (hot_test && P.sur && blockIdx.x == 0 && threadIdx.x == 0 && P.sur_hot[0] <= P.sur_hot[1]) {
+  };
+  add(B.frame_hot + 6 * (size_t)(f.init ? 0 : f.frame));
+  if (!f.init) add(B.frame_hot + 6 * (size_t)(f.frame - 1));
+  add(P.void_box);
+  if (P.sur && P.sur_hot[0] <= P.sur_hot[1]) {
     // cell i reads table entries floor(u), floor(u) + 1 (clamped), with
     // u = (x0 + h (i - 0.5) - x_start - v t - xi0) / dxi
     const double t = ctr->time_fe;
@@ -385,7 +388,12 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_field(RmParams P, RmBuffers B) 
       box_add(b, ihi, P.sur_hot[3], P.sur_hot[5]);
     }
   }
-  if (hot_test) block_box_merge(b, &ctr->acc[par][0]);
+  RmBox& a = ctr->acc[par][0];
+  if (b.lo[0] <= b.hi[0])
+    for (int q = 0; q < 3; q++) {
+      a.lo[q] = min(a.lo[q], b.lo[q]);
+      a.hi[q] = max(a.hi[q], b.hi[q]);
+    }
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) {
@@ -422,18 +430,15 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
   const bool later = n > 0;
   const double dt_prev = ctr->dt;
   const RmBox Rp = ctr->box[par ^ 1];
-  const double* th = rm_field(P, B, ctr->th_cur);
-  const double* thp = rm_field(P, B, ctr->th_prev);
+  const RmField th = ctr->fs[n % 3];
+  const RmField thp = ctr->fs[(n + 2) % 3];  // step n - 1
   const unsigned char* efb = rm_fb(P, B, par);
   const unsigned char* efbp = rm_fb(P, B, par ^ 1);
   const double tfe = ctr->time_fe, tfe_p = ctr->time_fe_prev;
   const int sur_p = ctr->sur_used[par ^ 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctr->field_due = 0;  // k_rm_field has written this step's field
     ctr->box[par] = R;
     ctr->sur_used[par] = P.sur;
-    ctr->last_cur = ctr->th_cur;
-    ctr->last_prev = ctr->th_prev;
     ctr->last_step = n;
     ctr->last_time_fe = tfe;
     ctr->last_time_fe_prev = tfe_p;
@@ -495,20 +500,19 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
 // that step's phase would have (k_rm_phase).
 __global__ void __launch_bounds__(kRmBlock) k_rm_complete(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
-  if (ctr->field_due) {
-    // the next step's field, as its k_rm_field would write it (the host
-    // clears field_due after this kernel)
-    double* th = B.theta + (size_t)ctr->th_cur * P.nodes;
+  {
+    // thetaArray_ of the coming step
+    const RmField f = ctr->fs[ctr->step % 3];
     const int s = gridDim.x * blockDim.x;
     for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += s)
-      th[nd] = rm_nodal(P, B, ctr, nd);
+      B.theta[nd] = rm_theta(P, B, f, nd);
   }
   const long long n = ctr->last_step;
   if (n < 0) return;
   const int par = (int)(n & 1);
   const RmBox R = ctr->box[par];
-  const double* th = rm_field(P, B, ctr->last_cur);
-  const double* thp = rm_field(P, B, ctr->last_prev);
+  const RmField th = ctr->fs[n % 3];
+  const RmField thp = ctr->fs[(n + 2) % 3];
   const unsigned char* efb = rm_fb(P, B, par);
   const unsigned char* efbp = rm_fb(P, B, par ^ 1);
   const double tfe = ctr->last_time_fe, tfe_p = ctr->last_time_fe_prev, dt_prev = ctr->last_dt;
@@ -558,7 +562,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_nodes(RmParams P, RmBuffers
   const RmCounters* ctr = B.ctr;
   if (ctr->skip) return;
   const double t = ctr->time_fe;
-  const double* theta = rm_field(P, B, ctr->th_cur);
+  const double* theta = B.theta;
   const int stride = gridDim.x * blockDim.x;
   for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.nodes; n += stride) {
     const int ni = n % P.fnx, nj = (n / P.fnx) % P.fny, nk = n / (P.fnx * P.fny);
@@ -598,14 +602,14 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_elems(RmParams P, RmBuffers
 // (as bits of a non-negative double), [5] fallback cells.
 __global__ void __launch_bounds__(kRmBlock) k_rm_sur_error(RmParams P, RmBuffers B) {
   const double t = B.ctr->time_fe;
-  const double* theta = rm_field(P, B, B.ctr->th_cur);
+  const RmField fcur = B.ctr->fs[B.ctr->step % 3];
   const unsigned char* efb = rm_fb(P, B, (int)(B.ctr->step & 1));
   double s_all = 0.0, n_all = 0.0, s_use = 0.0, n_use = 0.0, mx = 0.0, n_fb = 0.0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
     const int c = (int)cc;
     double Tfe = -1.0;
-    if (!rm_eval_T(P, B, theta, c, Tfe)) continue;
+    if (!rm_eval_T(P, B, fcur, c, Tfe)) continue;
     int i, j, k;
     rm_ijk(P, c, i, j, k);
     const double Ts = rm_sur_T(P, B, i, j, k, t);
@@ -666,7 +670,15 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_liquid(RmParams P, RmBuffers B)
     for (int s = 0; s < 26; s++)
       if (no[s] > -1) B.flag[c + P.off[s]] = 1;
   }
-  block_box_merge(liq, &ctr->acc[(ctr->step + 1) & 1][1]);
+  // per-block slot, reduced by k_rm_rowscan into the next step's box inputs
+  const RmBox v = block_box_reduce<kRmBlock>(liq);
+  if (threadIdx.x == 0) {
+    int* slot = B.blk_box + 6 * (size_t)blockIdx.x;
+    for (int q = 0; q < 3; q++) {
+      slot[q] = v.lo[q];
+      slot[q + 3] = v.hi[q];
+    }
+  }
 }
 
 // nucleate_at_nucleation_sites, remelting variant: sites stay in the list;
@@ -757,8 +769,11 @@ __device__ __forceinline__ void rm_nucleate_body(const RmParams& P, const RmBuff
 // temperature (every thread of k_rm_claim computes it the same way)
 __device__ __forceinline__ double rm_step_dt(const RmParams& P, const RmCounters* ctr) {
   double minT = P.Tm;
-  if (ctr->min_t_key != ~0ull) {
-    const double m = dkey_inv(ctr->min_t_key);
+  unsigned long long key = ~0ull;
+#pragma unroll
+  for (int q = 0; q < 64; q++) key = min(key, ctr->min_t_keys[q]);
+  if (key != ~0ull) {
+    const double m = dkey_inv(key);
     if (minT > m) minT = m;
   }
   const double v = rm_velocity(P, minT);
@@ -810,8 +825,8 @@ __device__ __forceinline__ void rm_mint_body(const RmParams& P, const RmBuffers&
   if (cnt) atomicAdd(&s_cnt, cnt);
   __syncthreads();
   if (threadIdx.x == 0) {
-    if (s_min != ~0ull) atomicMin(&ctr->min_t_key, s_min);
-    if (s_cnt) atomicAdd(&ctr->n_flagged, s_cnt);
+    if (s_min != ~0ull) atomicMin(&ctr->min_t_keys[blk & 63], s_min);
+    if (s_cnt) atomicAdd(&ctr->flagged_parts[blk & 63], s_cnt);
   }
 }
 
@@ -971,7 +986,8 @@ __device__ void rm_finish(const RmParams& P, const RmBuffers& B) {
     row.n_nucleated = ctr->step_nuc;
     row.n_captured = ctr->step_cap;
     row.n_fresh = ctr->step_fresh;
-    row.n_flagged = ctr->n_flagged;
+    row.n_flagged = 0;
+    for (int q = 0; q < 64; q++) row.n_flagged += ctr->flagged_parts[q];
     row.n_fallback = 0;
     if (B.fb_slots)
       for (int q = 0; q < 256; q++) row.n_fallback += B.fb_slots[q];
@@ -984,9 +1000,11 @@ __device__ void rm_finish(const RmParams& P, const RmBuffers& B) {
   if (B.fb_slots)
     for (int q = 0; q < 256; q++) B.fb_slots[q] = 0;
   ctr->armed[st & 1] = 0;
-  ctr->n_flagged = 0;
+  for (int q = 0; q < 64; q++) {
+    ctr->flagged_parts[q] = 0;
+    ctr->min_t_keys[q] = ~0ull;
+  }
   ctr->n_fallback = 0;
-  ctr->min_t_key = ~0ull;
   // this step's box inputs are spent; the slot serves step + 2
   for (int q = 0; q < 3; q++) {
     RmBox& a = ctr->acc[st & 1][q];
@@ -1020,11 +1038,16 @@ __device__ void rm_finish(const RmParams& P, const RmBuffers& B) {
     ctr->ratio = (tf - in1) / (in0 - in1);
     if (ctr->ratio < 0.0) ctr->err = 2;
     ctr->max_dt = in0 - tf;  // get_max_time_step
-    // the next step's k_rm_field writes its field into a fresh slot of the ring
-    ctr->th_prev = ctr->th_cur;
-    ctr->th_cur = (ctr->th_cur + 1) % 3;
-    ctr->field_due = 1;
   }
+  // the next step's field: the frames just selected, or this step's when the
+  // frames ran out (the run ends there)
+  RmField nf = ctr->fs[st % 3];
+  if (!ctr->exhausted) {
+    nf.frame = f;
+    nf.init = 0;
+    nf.ratio = ctr->ratio;
+  }
+  ctr->fs[(st + 1) % 3] = nf;
 }
 
 // Row offsets of the fresh grain ids (when there are fresh captures), then the
@@ -1051,6 +1074,26 @@ __global__ void __launch_bounds__(kRmScan) k_rm_rowscan(RmParams P, RmBuffers B)
     }
     __syncthreads();
     if (threadIdx.x == 0) ctr->n_grains += (int)tot;
+  }
+  // liquid cells with material after this step's phase: the blocks' boxes
+  {
+    RmBox liq;
+    box_empty(liq);
+    const int nb = rm_cell_blocks(P);
+    for (int b = threadIdx.x; b < nb; b += blockDim.x) {
+      const int* slot = B.blk_box + 6 * (size_t)b;
+      if (slot[0] > slot[3]) continue;
+      box_add(liq, slot[0], slot[1], slot[2]);
+      box_add(liq, slot[3], slot[4], slot[5]);
+    }
+    const RmBox v = block_box_reduce<kRmScan>(liq);
+    if (threadIdx.x == 0 && v.lo[0] <= v.hi[0]) {
+      RmBox& a = ctr->acc[(ctr->step + 1) & 1][1];
+      for (int q = 0; q < 3; q++) {
+        a.lo[q] = min(a.lo[q], v.lo[q]);
+        a.hi[q] = max(a.hi[q], v.hi[q]);
+      }
+    }
   }
   if (threadIdx.x == 0) rm_finish(P, B);
 }
@@ -1135,18 +1178,15 @@ const char* const kRmKernelNames[kRmKernels] = {
 
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s) {
-  // grid-stride kernels: with the active box, 2048 blocks of 256 (eight per
-  // CU) cost little when the box is small; over the whole grid, up to 16384
-  long long cb = (P.NG + kRmBlock - 1) / kRmBlock;
-  if (cb > (P.window ? 2048 : 16384)) cb = P.window ? 2048 : 16384;
-  const unsigned cells = (unsigned)cb;
+  const unsigned cells = (unsigned)rm_cell_blocks(P);
   long long nbk = (P.nodes + kRmBlock - 1) / kRmBlock;
   if (nbk > 8192) nbk = 8192;
   if (nbk < 1) nbk = 1;
   switch (q) {
     case 0:
-      hipLaunchKernelGGL(k_rm_field, dim3((unsigned)(nbk < 1024 ? nbk : 1024)), dim3(kRmBlock), 0,
-                         s, P, B);
+      // the site count (and with the surrogate the nodal field), grid-stride
+      hipLaunchKernelGGL(k_rm_field, dim3(P.sur ? (unsigned)(nbk < 1024 ? nbk : 1024) : 64u),
+                         dim3(kRmBlock), 0, s, P, B);
       break;
     case 1:
       if (!P.sur) return hipSuccess;
