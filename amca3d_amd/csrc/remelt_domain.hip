@@ -860,48 +860,120 @@ amca_status amca_remelt_read_log(amca_remelt* d, int32_t cap, amca_remelt_step_r
   return AMCA_OK;
 }
 
-amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_counters* out) {
+}  // extern "C"
+
+namespace {
+
+// One domain's run, in rounds: plan (the next batch of steps, after any rows
+// a stopped step asked for), enqueue, collect the counters.
+struct RunState {
+  long long start = 0, batch = 4, k = 0;
+  bool done = false;
+  amca::RmCounters c{};
+};
+
+amca_status run_begin(amca_remelt* d, RunState& st) {
   if (!d || !d->fe_ready) return rm_fail(AMCA_ERR_INVALID, "FE mesh not set");
-  amca::RmCounters c;
-  RM_TRY(rm_counters(d, &c));
-  const long long start = c.step;
-  long long batch = 4;
-  while (true) {
-    if (c.err == 1) return rm_fail(AMCA_ERR_CAPACITY, "orientation table exhausted");
-    if (c.err == 2) return rm_fail(AMCA_ERR_INVALID, "FE time ratio below zero");
-    if (c.exhausted || !(c.tcafe < d->desc.final_time)) break;
-    const long long taken = c.step - start;
-    if (max_steps >= 0 && taken >= max_steps) break;
-    if (c.need_rows) {
-      // a step stopped: its armed sites could need more rows than were pushed
-      const long long want = (long long)c.n_orient + c.armed[c.step & 1];
-      if (want > d->P.orient_cap) return rm_fail(AMCA_ERR_CAPACITY, "orientation table exhausted");
-      RM_TRY(ensure_rows(d, want + d->row_reserve));
-      if (d->rows_pushed < want)
-        return rm_fail(AMCA_ERR_CAPACITY, "orientation rows exhausted and no orientation source set");
-      RM_TRY(rm_counters(d, &c));
-      c.need_rows = 0;
-      c.armed[c.step & 1] = 0;
-      RM_TRY(rm_put_counters(d, c));
-    }
-    long long k = batch;
-    if (max_steps >= 0) k = std::min<long long>(k, max_steps - taken);
-    if (c.n_sites > 0 && (d->source || d->rng))
-      RM_TRY(ensure_rows(d, (long long)c.n_orient + d->row_reserve));
-    if (d->timing) {
-      for (long long q = 0; q < k; q++) RM_TRY(rm_enqueue_step(d));
-    } else {
-      RM_TRY(rm_enqueue_steps(d, k));
-    }
-    RM_TRY(rm_counters(d, &c));
-    batch = std::min<long long>(batch * 2, 64);
+  RM_TRY(rm_counters(d, &st.c));
+  st.start = st.c.step;
+  return AMCA_OK;
+}
+
+amca_status run_plan(amca_remelt* d, RunState& st, int64_t max_steps) {
+  amca::RmCounters& c = st.c;
+  st.k = 0;
+  if (c.err == 1) return rm_fail(AMCA_ERR_CAPACITY, "orientation table exhausted");
+  if (c.err == 2) return rm_fail(AMCA_ERR_INVALID, "FE time ratio below zero");
+  const long long taken = c.step - st.start;
+  if (c.exhausted || !(c.tcafe < d->desc.final_time) || (max_steps >= 0 && taken >= max_steps)) {
+    st.done = true;
+    return AMCA_OK;
   }
+  if (c.need_rows) {
+    // a step stopped: its armed sites could need more rows than were pushed
+    const long long want = (long long)c.n_orient + c.armed[c.step & 1];
+    if (want > d->P.orient_cap) return rm_fail(AMCA_ERR_CAPACITY, "orientation table exhausted");
+    RM_TRY(ensure_rows(d, want + d->row_reserve));
+    if (d->rows_pushed < want)
+      return rm_fail(AMCA_ERR_CAPACITY, "orientation rows exhausted and no orientation source set");
+    RM_TRY(rm_counters(d, &c));
+    c.need_rows = 0;
+    c.armed[c.step & 1] = 0;
+    RM_TRY(rm_put_counters(d, c));
+  }
+  st.k = st.batch;
+  if (max_steps >= 0) st.k = std::min<long long>(st.k, max_steps - taken);
+  if (c.n_sites > 0 && (d->source || d->rng))
+    RM_TRY(ensure_rows(d, (long long)c.n_orient + d->row_reserve));
+  return AMCA_OK;
+}
+
+amca_status run_enqueue(amca_remelt* d, RunState& st) {
+  if (st.k <= 0) return AMCA_OK;
+  if (d->timing) {
+    for (long long q = 0; q < st.k; q++) RM_TRY(rm_enqueue_step(d));
+  } else {
+    RM_TRY(rm_enqueue_steps(d, st.k));
+  }
+  return AMCA_OK;
+}
+
+amca_status run_collect(amca_remelt* d, RunState& st) {
+  if (st.k <= 0) return AMCA_OK;
+  RM_TRY(rm_counters(d, &st.c));
+  st.batch = std::min<long long>(st.batch * 2, 64);
+  return AMCA_OK;
+}
+
+amca_status run_end(amca_remelt* d, RunState& st, amca_remelt_counters* out) {
   // cells outside the last step's box take that step's T and cooling
-  if (c.step > start) {
+  if (st.c.step > st.start) {
     RM_HIP(amca::rm_launch_complete(d->P, d->B, d->stream));
     RM_HIP(hipStreamSynchronize(d->stream));
   }
-  rm_fill(d, c, out);
+  rm_fill(d, st.c, out);
+  return AMCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_counters* out) {
+  RunState st;
+  RM_TRY(run_begin(d, st));
+  while (true) {
+    RM_TRY(run_plan(d, st, max_steps));
+    if (st.done) break;
+    RM_TRY(run_enqueue(d, st));
+    RM_TRY(run_collect(d, st));
+  }
+  return run_end(d, st, out);
+}
+
+amca_status amca_remelt_run_many(amca_remelt* const* ds, int32_t n, int64_t max_steps,
+                                 amca_remelt_counters* outs) {
+  if (!ds || n < 1) return rm_fail(AMCA_ERR_INVALID, "no domains");
+  for (int i = 0; i < n; i++)
+    for (int j = 0; j < i; j++)
+      if (ds[i] == ds[j]) return rm_fail(AMCA_ERR_INVALID, "a domain is listed twice");
+  std::vector<RunState> st((size_t)n);
+  for (int i = 0; i < n; i++) RM_TRY(run_begin(ds[i], st[i]));
+  while (true) {
+    int live = 0;
+    // every domain's next batch is queued on its own stream before any is waited for
+    for (int i = 0; i < n; i++) {
+      if (st[i].done) continue;
+      RM_TRY(run_plan(ds[i], st[i], max_steps));
+      if (st[i].done) continue;
+      RM_TRY(run_enqueue(ds[i], st[i]));
+      live++;
+    }
+    if (!live) break;
+    for (int i = 0; i < n; i++)
+      if (!st[i].done) RM_TRY(run_collect(ds[i], st[i]));
+  }
+  for (int i = 0; i < n; i++) RM_TRY(run_end(ds[i], st[i], outs ? &outs[i] : nullptr));
   return AMCA_OK;
 }
 

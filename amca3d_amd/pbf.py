@@ -73,7 +73,22 @@ def surrogate_desc(w, xi_scale=1e-4, T_clip=2600.0, log_target=1, T_scale=100.0)
 def scan(w):
     from .surrogate import Scan
     ph = w["physics"]
-    return Scan(P=ph["P"], v=ph["v"], x_start=ph["x_start"], y_c=0.5 * w["grid"]["ny"] * w["grid"]["h"])
+    return Scan(P=ph["P"], v=ph["v"], x_start=ph["x_start"],
+                y_c=ph.get("y_c", 0.5 * w["grid"]["ny"] * w["grid"]["h"]))
+
+
+def scan_track(w, s):
+    """Workload `w` with the frames of scan `s` (power, speed, start, hatch
+    position): same grid, mesh, substrate and nucleation."""
+    g, ph = w["grid"], w["physics"]
+    fe = dict(w["fe"])
+    fe["theta"] = remelt.rosenthal_frames(fe["xyz"], fe["times"], P=s.P, eta=ph["eta"], v=s.v,
+                                          k=ph["k"], alpha=ph["alpha"], T0=ph["T0"],
+                                          x_start=s.x_start, y_c=s.y_c, z_top=g["nz"] * g["h"],
+                                          r_min=g["h"])
+    out = dict(w, fe=fe)
+    out["physics"] = dict(ph, P=s.P, v=s.v, x_start=s.x_start, y_c=s.y_c)
+    return out
 
 
 def train_surrogate(w, n_samples=20_000_000, seed=3, schedule=((8192, 6000, 1e-3),

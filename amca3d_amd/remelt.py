@@ -74,6 +74,8 @@ SIGNATURES = {
     "amca_remelt_last_box": (C.c_int, [C.c_void_p, I32P]),
     "amca_remelt_set_graph_steps": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_remelt_set_row_reserve": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_remelt_run_many": (C.c_int, [C.POINTER(C.c_void_p), C.c_int32, C.c_int64,
+                                       C.POINTER(RemeltCounters)]),
     "amca_remelt_use_surrogate": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                             C.c_double, C.c_double]),
     "amca_remelt_surrogate_error": (C.c_int, [C.c_void_p, F64P]),
@@ -112,6 +114,16 @@ def _check(status, where):
 
 def _p(a, t):
     return a.ctypes.data_as(C.POINTER(t))
+
+
+def run_many(domains, max_steps=-1):
+    """Run several RemeltDomain objects at once (amca_remelt_run_many); returns
+    each domain's counters."""
+    n = len(domains)
+    hs = (C.c_void_p * n)(*[d.h.value for d in domains])
+    outs = (RemeltCounters * n)()
+    _check(lib().amca_remelt_run_many(hs, n, int(max_steps), outs), "run_many")
+    return [o.as_dict() for o in outs]
 
 
 def map_voxels(grid, fe, bin=(5, 5, 5), tol=1e-16, max_iter=20):

@@ -378,6 +378,13 @@ amca_status amca_remelt_euler(amca_remelt* d, int32_t cap, double* euler3, int32
 /* CafeSolver::time_integration_weak_coupling (CafeSolver.cpp:178-219) until
  * termination_time, the end of the frames, or max_steps steps (< 0: no limit). */
 amca_status amca_remelt_run(amca_remelt* d, int64_t max_steps, amca_remelt_counters* out);
+/* Batched form: runs n independent domains (scan variants, seeds) at once,
+ * each as amca_remelt_run would, queuing every domain's next steps on its own
+ * stream before waiting for any, so the GPU runs the domains' small kernels
+ * side by side.  No reference counterpart (the reference runs one track per
+ * job).  outs (n entries) may be NULL. */
+amca_status amca_remelt_run_many(amca_remelt* const* ds, int32_t n, int64_t max_steps,
+                                 amca_remelt_counters* outs);
 amca_status amca_remelt_get_counters(amca_remelt* d, amca_remelt_counters* out);
 amca_status amca_remelt_enable_log(amca_remelt* d, int32_t capacity);
 amca_status amca_remelt_read_log(amca_remelt* d, int32_t cap, amca_remelt_step_record* rows,

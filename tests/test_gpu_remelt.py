@@ -26,9 +26,10 @@ def _desc(case):
     return d
 
 
-def run_device_own_sampler(case, orc, reserve=None):
-    """Run `case` with the library's own host sampler and rand() stream
-    (amca_remelt_draw_sites) instead of inputs taken from the oracle run."""
+def build_own_sampler(case, reserve=None):
+    """A domain for `case` with the library's own host sampler and rand()
+    stream (amca_remelt_draw_sites) instead of inputs taken from the oracle
+    run; returns (domain, site count)."""
     from amca3d_amd.remelt import RemeltDomain
     st = case["state"]
     n0 = len(st["euler"])
@@ -41,12 +42,21 @@ def run_device_own_sampler(case, orc, reserve=None):
     if reserve is not None:
         d.set_row_reserve(reserve)
     d.enable_log(100000)
-    c = d.run(case["max_steps"])
+    return d, n_sites
+
+
+def collect(d, c, n_sites=None):
     out = d.download()
     out.update(theta=d.theta(), log=d.read_log(), counters=c, tag_par=d.voxel_map(),
                euler=d.euler(), n_sites=n_sites)
     d.close()
     return out
+
+
+def run_device_own_sampler(case, orc, reserve=None):
+    """Run `case` with the library's own sampler (see build_own_sampler)."""
+    d, n_sites = build_own_sampler(case, reserve)
+    return collect(d, d.run(case["max_steps"]), n_sites)
 
 
 def run_device(case, orc, chunks=None, window=True):
@@ -180,6 +190,23 @@ def test_track_with_rows_drawn_on_demand(amca, track_nuc, reserve):
     dev = run_device_own_sampler(case, orc, reserve=reserve)
     compare(case, orc, dev)
     np.testing.assert_array_equal(bits(dev["euler"]), bits(orc["euler"]))
+
+
+def test_run_many_equals_the_oracle(amca):
+    """Three tracks of different size, power and seed run side by side
+    (amca_remelt_run_many, one stream each): each equals its oracle run."""
+    from amca3d_amd import remelt
+    nuc = dict(surface=(2e10, 2.0, 0.5), bulk=(2e15, 5.0, 1.0))
+    cases = [rc.track_case(nx=48 + 16 * q, ny=18 + 2 * q, nz=14, n_frames=14 + 4 * q,
+                           P=30.0 + 5.0 * q, nucleation=nuc, seed=21 + q) for q in range(3)]
+    orcs = [ol.run_remelt(c) for c in cases]
+    built = [build_own_sampler(c) for c in cases]
+    counters = remelt.run_many([d for d, _ in built])
+    for case, orc, (d, ns), c in zip(cases, orcs, built, counters):
+        dev = collect(d, c, ns)
+        assert dev["n_sites"] == len(orc["site_cell"])
+        compare(case, orc, dev)
+        np.testing.assert_array_equal(bits(dev["euler"]), bits(orc["euler"]))
 
 
 def test_track_in_chunks_equals_one_run(amca, track_nuc):
