@@ -1146,9 +1146,6 @@ amca_status amca_remelt_use_surrogate(amca_remelt* d, amca_mlp* m, const amca_su
 amca_status amca_remelt_surrogate_error(amca_remelt* d, double* out6) {
   if (!d || !out6 || !d->P.sur) return rm_fail(AMCA_ERR_INVALID, "no surrogate in use");
   RM_HIP(hipMemsetAsync(d->B.sur_stats, 0, sizeof(double) * 8, d->stream));
-  // fallback flags for the current time, as the next step's phase would see them
-  RM_HIP(amca::rm_launch_kernel(1, d->P, d->B, d->site_blocks, d->stream));
-  RM_HIP(amca::rm_launch_kernel(2, d->P, d->B, d->site_blocks, d->stream));
   RM_HIP(amca::rm_launch_sur_error(d->P, d->B, d->stream));
   double s[8];
   RM_HIP(hipMemcpyAsync(s, d->B.sur_stats, sizeof(s), hipMemcpyDeviceToHost, d->stream));

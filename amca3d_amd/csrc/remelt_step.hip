@@ -150,17 +150,55 @@ __device__ __forceinline__ double rm_sur_T(const RmParams& P, const RmBuffers& B
   return (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
 }
 
+// surrogate temperature of FE node (ni, nj, nk) at FE time t (node table)
+__device__ __forceinline__ double rm_sur_node_T(const RmParams& P, const RmBuffers& B, int ni,
+                                                int nj, int nk, double t) {
+  const double xi = P.fx0 + P.fdx * ni - P.sur_x_start - P.sur_v * t;
+  double u = (xi - P.sur_xi0) / P.sur_dxi;
+  int i0 = (int)floor(u);
+  i0 = i0 < 0 ? 0 : (i0 > P.sur_nxi_nodes - 2 ? P.sur_nxi_nodes - 2 : i0);
+  double f = u - i0;
+  f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
+  const float* row = B.sur_node + ((size_t)nk * P.fny + nj) * P.sur_nxi_nodes;
+  return (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
+}
+
+// Surrogate fallback rule of one FE node at one step: the surrogate differs
+// from the FE nodal field by more than sur_resid, or the node is void.
+__device__ __forceinline__ bool rm_node_bad(const RmParams& P, const RmBuffers& B, int nd,
+                                            const RmField& f, double t) {
+  const int ni = nd % P.fnx, nj = (nd / P.fnx) % P.fny, nk = nd / (P.fnx * P.fny);
+  const double Ts = rm_sur_node_T(P, B, ni, nj, nk, t);
+  const double th = rm_theta(P, B, f, nd);
+  return fabs(Ts - th) > P.sur_resid || th < P.void_temp + 0.00001;
+}
+
+// An element falls back to the FE path when any of its nodes is bad.
+__device__ __forceinline__ bool rm_elem_bad(const RmParams& P, const RmBuffers& B, int e,
+                                            const RmField& f, double t) {
+  const int ei = e % P.ncx, ej = (e / P.ncx) % P.ncy, ek = e / (P.ncx * P.ncy);
+  const int nxy = P.fnx * P.fny;
+  const int n0 = nxy * ek + P.fnx * ej + ei;
+  const int nid[8] = {n0, n0 + 1, n0 + P.fnx, n0 + P.fnx + 1,
+                      n0 + nxy, n0 + nxy + 1, n0 + nxy + P.fnx, n0 + nxy + P.fnx + 1};
+  bool bad = false;
+#pragma unroll
+  for (int m = 0; m < 8; m++) bad |= rm_node_bad(P, B, nid[m], f, t);
+  return bad;
+}
+
 // Temperature of cell c at one step: the FE path on nodal field f, or with
 // the surrogate (sur) the table value at FE time tfe unless the cell's element
-// falls back (efb) or the value is within sur_band of Tm.  Returns whether the
-// cell has material; n_fb counts cells of an element that took the FE path.
+// falls back (efb, the step's flags; NULL: the rule evaluated here) or the
+// value is within sur_band of Tm.  Returns whether the cell has material;
+// n_fb counts cells of an element that took the FE path.
 __device__ __forceinline__ bool rm_cell_T(const RmParams& P, const RmBuffers& B, int c,
                                           const RmField& f, const unsigned char* efb,
                                           double tfe, int sur, double& Tn, int& n_fb) {
   Tn = -1.0;
   if (!sur) return rm_eval_T(P, B, f, c, Tn);
   const int e = B.tag[c];
-  bool fe = e < 0 || efb[e];
+  bool fe = e < 0 || (efb ? efb[e] != 0 : rm_elem_bad(P, B, e, f, tfe));
   if (!fe) {
     int si, sj, sk;
     rm_ijk(P, c, si, sj, sk);
@@ -351,13 +389,12 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_field(RmParams P, RmBuffers B) 
       if (threadIdx.x == 0 && armed) atomicAdd(&ctr->armed[par], armed);
     }
   }
-  if (P.sur) {
-    const int stride = gridDim.x * blockDim.x;
-    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += stride)
-      B.theta[nd] = rm_theta(P, B, f, nd);
-  }
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const bool hot_test = P.window && !ctr->force_full && n > 0;
-  if (!hot_test || blockIdx.x != 0 || threadIdx.x != 0) return;
+  if (!hot_test) {
+    box_full(P, ctr->box[par]);
+    return;
+  }
   RmBox b;
   box_empty(b);
   auto add = [&](const int* q) {
@@ -394,6 +431,23 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_field(RmParams P, RmBuffers B) 
       a.lo[q] = min(a.lo[q], b.lo[q]);
       a.hi[q] = max(a.hi[q], b.hi[q]);
     }
+  // the step's box: its three inputs, widened by two cells
+  RmBox R;
+  box_empty(R);
+  for (int q = 0; q < 3; q++) {
+    const RmBox& x = ctr->acc[par][q];
+    if (x.lo[0] > x.hi[0]) continue;
+    box_add(R, x.lo[0], x.lo[1], x.lo[2]);
+    box_add(R, x.hi[0], x.hi[1], x.hi[2]);
+  }
+  if (R.lo[0] <= R.hi[0]) {
+    const int top[3] = {P.nxG - 1, P.nyG - 1, P.nzG - 1};
+    for (int q = 0; q < 3; q++) {
+      R.lo[q] = max(0, R.lo[q] - 2);
+      R.hi[q] = min(top[q], R.hi[q] + 2);
+    }
+  }
+  ctr->box[par] = R;
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) {
@@ -407,37 +461,16 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
     if (short_rows && !rm_skip_now(P, ctr)) ctr->need_rows = 1;
   }
   if (skip) return;
-  // the box of this step
-  RmBox R;
-  if (!P.window || ctr->force_full || n == 0) {
-    box_full(P, R);
-  } else {
-    box_empty(R);
-    for (int q = 0; q < 3; q++) {
-      const RmBox& a = ctr->acc[par][q];
-      if (a.lo[0] > a.hi[0]) continue;
-      box_add(R, a.lo[0], a.lo[1], a.lo[2]);
-      box_add(R, a.hi[0], a.hi[1], a.hi[2]);
-    }
-    if (R.lo[0] <= R.hi[0]) {
-      const int top[3] = {P.nxG - 1, P.nyG - 1, P.nzG - 1};
-      for (int a = 0; a < 3; a++) {
-        R.lo[a] = max(0, R.lo[a] - 2);
-        R.hi[a] = min(top[a], R.hi[a] + 2);
-      }
-    }
-  }
+  const RmBox R = ctr->box[par];  // written by k_rm_field
   const bool later = n > 0;
   const double dt_prev = ctr->dt;
   const RmBox Rp = ctr->box[par ^ 1];
   const RmField th = ctr->fs[n % 3];
   const RmField thp = ctr->fs[(n + 2) % 3];  // step n - 1
   const unsigned char* efb = rm_fb(P, B, par);
-  const unsigned char* efbp = rm_fb(P, B, par ^ 1);
   const double tfe = ctr->time_fe, tfe_p = ctr->time_fe_prev;
   const int sur_p = ctr->sur_used[par ^ 1];
   if (blockIdx.x == 0 && threadIdx.x == 0) {
-    ctr->box[par] = R;
     ctr->sur_used[par] = P.sur;
     ctr->last_step = n;
     ctr->last_time_fe = tfe;
@@ -464,8 +497,10 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
       if (box_has(Rp, i, j, k)) {
         Tp = B.T[c];
       } else {
+        // a cell entering the box: its last-step temperature, evaluated
+        // here (fallback rule included) as that step would have
         int unused = 0;
-        (void)rm_cell_T(P, B, c, thp, efbp, tfe_p, sur_p, Tp, unused);
+        (void)rm_cell_T(P, B, c, thp, nullptr, tfe_p, sur_p, Tp, unused);
       }
       double rate = 410;
       if (Tp > 0) rate = (Tn - Tp) / dt_prev;
@@ -513,8 +548,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_complete(RmParams P, RmBuffers 
   const RmBox R = ctr->box[par];
   const RmField th = ctr->fs[n % 3];
   const RmField thp = ctr->fs[(n + 2) % 3];
-  const unsigned char* efb = rm_fb(P, B, par);
-  const unsigned char* efbp = rm_fb(P, B, par ^ 1);
+  // the fallback flags are kept for the box only: the rule is evaluated here
   const double tfe = ctr->last_time_fe, tfe_p = ctr->last_time_fe_prev, dt_prev = ctr->last_dt;
   const int sur = ctr->sur_used[par], sur_p = ctr->sur_used[par ^ 1];
   const long long stride = (long long)gridDim.x * blockDim.x;
@@ -525,10 +559,10 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_complete(RmParams P, RmBuffers 
     if (box_has(R, i, j, k)) continue;
     int unused = 0;
     double Tn;
-    const bool mat = rm_cell_T(P, B, c, th, efb, tfe, sur, Tn, unused);
+    const bool mat = rm_cell_T(P, B, c, th, nullptr, tfe, sur, Tn, unused);
     if (n > 0) {
       double Tp;
-      (void)rm_cell_T(P, B, c, thp, efbp, tfe_p, sur_p, Tp, unused);
+      (void)rm_cell_T(P, B, c, thp, nullptr, tfe_p, sur_p, Tp, unused);
       double rate = 410;
       if (Tp > 0) rate = (Tn - Tp) / dt_prev;
       if (Tn <= 0) rate = 0;
@@ -555,39 +589,60 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_hot(RmParams P, RmBuffers B
   block_box_merge(b, out);
 }
 
-// Surrogate fallback, per step: a node is bad when the surrogate at the node
-// differs from the FE nodal field by more than sur_resid (or the node is
-// void); an element falls back to the FE path when any of its nodes is bad.
+// Surrogate fallback flags of the step, for the elements under the step's box
+// (on the uniform node lattice the surrogate requires, with one element of
+// margin for cells on element faces): a node is bad when the surrogate
+// differs from the FE nodal field by more than sur_resid or the node is void;
+// an element falls back to the FE path when any of its nodes is bad.
+__device__ __forceinline__ void rm_box_nodes(const RmParams& P, const RmBox& R, int* lo, int* hi) {
+  const double c0[3] = {P.x0, P.y0, P.z0}, f0[3] = {P.fx0, P.fy0, P.fz0};
+  const int top[3] = {P.fnx - 1, P.fny - 1, P.fnz - 1};
+  for (int a = 0; a < 3; a++) {
+    if (R.lo[0] > R.hi[0]) {
+      lo[a] = 1;
+      hi[a] = 0;
+      continue;
+    }
+    const double xl = c0[a] + P.h * (R.lo[a] - 0.5), xh = c0[a] + P.h * (R.hi[a] - 0.5);
+    lo[a] = max(0, (int)floor((xl - f0[a]) / P.fdx) - 1);
+    hi[a] = min(top[a], (int)floor((xh - f0[a]) / P.fdx) + 2);
+  }
+}
+
 __global__ void __launch_bounds__(kRmBlock) k_rm_sur_nodes(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
-  if (ctr->skip) return;
+  if (rm_skip_now(P, ctr)) return;
+  const long long n = ctr->step;
+  const RmField f = ctr->fs[n % 3];
   const double t = ctr->time_fe;
-  const double* theta = B.theta;
+  int lo[3], hi[3];
+  rm_box_nodes(P, ctr->box[n & 1], lo, hi);
+  const int ex = hi[0] - lo[0] + 1, ey = hi[1] - lo[1] + 1, ez = hi[2] - lo[2] + 1;
+  const int total = (ex > 0 && ey > 0 && ez > 0) ? ex * ey * ez : 0;
   const int stride = gridDim.x * blockDim.x;
-  for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < P.nodes; n += stride) {
-    const int ni = n % P.fnx, nj = (n / P.fnx) % P.fny, nk = n / (P.fnx * P.fny);
-    const double xi = P.fx0 + P.fdx * ni - P.sur_x_start - P.sur_v * t;
-    double u = (xi - P.sur_xi0) / P.sur_dxi;
-    int i0 = (int)floor(u);
-    i0 = i0 < 0 ? 0 : (i0 > P.sur_nxi_nodes - 2 ? P.sur_nxi_nodes - 2 : i0);
-    double f = u - i0;
-    f = f < 0.0 ? 0.0 : (f > 1.0 ? 1.0 : f);
-    const float* row = B.sur_node + ((size_t)nk * P.fny + nj) * P.sur_nxi_nodes;
-    const double Ts = (1.0 - f) * (double)row[i0] + f * (double)row[i0 + 1];
-    const double th = theta[n];
-    B.node_bad[n] = (fabs(Ts - th) > P.sur_resid || th < P.void_temp + 0.00001) ? 1 : 0;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    const int ni = lo[0] + q % ex, nj = lo[1] + (q / ex) % ey, nk = lo[2] + q / (ex * ey);
+    const int nd = (nk * P.fny + nj) * P.fnx + ni;
+    B.node_bad[nd] = rm_node_bad(P, B, nd, f, t) ? 1 : 0;
   }
 }
 
 __global__ void __launch_bounds__(kRmBlock) k_rm_sur_elems(RmParams P, RmBuffers B) {
   const RmCounters* ctr = B.ctr;
-  if (ctr->skip) return;
+  if (rm_skip_now(P, ctr)) return;
+  const long long n = ctr->step;
   const int ne = P.ncx * P.ncy * (P.fnz - 1);
   const int nxy = P.fnx * P.fny;
-  unsigned char* fb = B.elem_fb + (size_t)(ctr->step & 1) * ne;
+  unsigned char* fb = B.elem_fb + (size_t)(n & 1) * ne;
+  int lo[3], hi[3];
+  rm_box_nodes(P, ctr->box[n & 1], lo, hi);
+  // elements whose 8 nodes all lie in the node box
+  const int ex = hi[0] - lo[0], ey = hi[1] - lo[1], ez = hi[2] - lo[2];
+  const int total = (ex > 0 && ey > 0 && ez > 0) ? ex * ey * ez : 0;
   const int stride = gridDim.x * blockDim.x;
-  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < ne; e += stride) {
-    const int ei = e % P.ncx, ej = (e / P.ncx) % P.ncy, ek = e / (P.ncx * P.ncy);
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
+    const int ei = lo[0] + q % ex, ej = lo[1] + (q / ex) % ey, ek = lo[2] + q / (ex * ey);
+    const int e = (ek * P.ncy + ej) * P.ncx + ei;
     const int n0 = nxy * ek + P.fnx * ej + ei;
     const unsigned char* b = B.node_bad;
     const int bad = b[n0] | b[n0 + 1] | b[n0 + P.fnx] | b[n0 + P.fnx + 1] | b[n0 + nxy] |
@@ -603,7 +658,6 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_elems(RmParams P, RmBuffers
 __global__ void __launch_bounds__(kRmBlock) k_rm_sur_error(RmParams P, RmBuffers B) {
   const double t = B.ctr->time_fe;
   const RmField fcur = B.ctr->fs[B.ctr->step % 3];
-  const unsigned char* efb = rm_fb(P, B, (int)(B.ctr->step & 1));
   double s_all = 0.0, n_all = 0.0, s_use = 0.0, n_use = 0.0, mx = 0.0, n_fb = 0.0;
   const long long stride = (long long)gridDim.x * blockDim.x;
   for (long long cc = (long long)blockIdx.x * blockDim.x + threadIdx.x; cc < P.NG; cc += stride) {
@@ -616,7 +670,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_sur_error(RmParams P, RmBuffers
     const double d = Ts - Tfe;
     s_all += d * d;
     n_all += 1.0;
-    const bool fb = efb[B.tag[c]] || fabs(Ts - P.Tm) < P.sur_band;
+    const bool fb = rm_elem_bad(P, B, B.tag[c], fcur, t) || fabs(Ts - P.Tm) < P.sur_band;
     if (fb) {
       n_fb += 1.0;
     } else {
@@ -1184,17 +1238,18 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
   if (nbk < 1) nbk = 1;
   switch (q) {
     case 0:
-      // the site count (and with the surrogate the nodal field), grid-stride
-      hipLaunchKernelGGL(k_rm_field, dim3(P.sur ? (unsigned)(nbk < 1024 ? nbk : 1024) : 64u),
-                         dim3(kRmBlock), 0, s, P, B);
+      // the site count (grid-stride) and the step's box
+      hipLaunchKernelGGL(k_rm_field, dim3(64u), dim3(kRmBlock), 0, s, P, B);
       break;
     case 1:
       if (!P.sur) return hipSuccess;
-      hipLaunchKernelGGL(k_rm_sur_nodes, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+      hipLaunchKernelGGL(k_rm_sur_nodes, dim3((unsigned)(nbk < 1024 ? nbk : 1024)), dim3(kRmBlock),
+                         0, s, P, B);
       break;
     case 2:
       if (!P.sur) return hipSuccess;
-      hipLaunchKernelGGL(k_rm_sur_elems, dim3((unsigned)nbk), dim3(kRmBlock), 0, s, P, B);
+      hipLaunchKernelGGL(k_rm_sur_elems, dim3((unsigned)(nbk < 1024 ? nbk : 1024)), dim3(kRmBlock),
+                         0, s, P, B);
       break;
     case 3: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 4: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;

@@ -353,10 +353,12 @@ def surrogate_main(args):
                 lat_s += time.perf_counter() - t1
             d.counters()
             doms.append(d)
+        c0 = remelt.run_many(doms, args.warmup) if args.warmup > 0 else None
         t1 = time.perf_counter()
         cs = remelt.run_many(doms, args.steps)
         wall = time.perf_counter() - t1
-        steps = [int(c["step"]) for c in cs]
+        steps = [int(c["step"]) - (int(a["step"]) if c0 else 0) for c, a in
+                 zip(cs, c0 if c0 else cs)]
         out = dict(value=cells * sum(steps) / wall, wall=wall, steps=steps,
                    ori=[d.download()["ori"] for d in doms], logs=[d.read_log() for d in doms],
                    err=[d.surrogate_error() for d in doms] if mode == "surrogate" else None,
@@ -371,12 +373,12 @@ def surrogate_main(args):
         if not window:
             d.set_window(False)
         d.set_graph_steps(args.graph_steps)
-        d.counters()
+        c0 = d.run(args.warmup) if args.warmup > 0 else d.counters()
         t1 = time.perf_counter()
         c = d.run(args.steps)
         wall = time.perf_counter() - t1
         d.close()
-        return cells * int(c["step"]) / wall
+        return cells * (int(c["step"]) - int(c0["step"])) / wall
 
     plain_whole = single(False)
     plain_box = single(True)
@@ -395,7 +397,7 @@ def surrogate_main(args):
     value = sur_b["value"]
     result = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s",
-        "n_gpus": 1, "steps": args.steps, "warmup": 0,
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
         "ms_per_step": 1e3 * sur_b["wall"] / max(max(sur_b["steps"]), 1),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic (Rosenthal FE frames; surrogate trained on 2e7 FE samples of six "
