@@ -92,8 +92,9 @@ __device__ __forceinline__ double rm_theta(const RmParams& P, const RmBuffers& B
 
 // MapVoxelManager::evaluate_temperature (MapVoxelManager.cpp:746-799) on a
 // structured mesh: element e has corner node ((ek*fny + ej)*fnx + ei).
+// `field` (optional): field f already evaluated at every node.
 __device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B, const RmField& f,
-                                          int c, double& out) {
+                                          int c, double& out, const double* field = nullptr) {
   const double nm = P.void_temp + 0.00001;
   const int e = B.tag[c];
   if (e < 0) return false;
@@ -112,7 +113,7 @@ __device__ __forceinline__ bool rm_eval_T(const RmParams& P, const RmBuffers& B,
   nid[7] = nid[3] + nxy;
   double th[8];
 #pragma unroll
-  for (int m = 0; m < 8; m++) th[m] = rm_theta(P, B, f, nid[m]);
+  for (int m = 0; m < 8; m++) th[m] = field ? field[nid[m]] : rm_theta(P, B, f, nid[m]);
 #pragma unroll
   for (int m = 0; m < 8; m++)
     if (th[m] < nm) return false;
@@ -194,9 +195,10 @@ __device__ __forceinline__ bool rm_elem_bad(const RmParams& P, const RmBuffers& 
 // n_fb counts cells of an element that took the FE path.
 __device__ __forceinline__ bool rm_cell_T(const RmParams& P, const RmBuffers& B, int c,
                                           const RmField& f, const unsigned char* efb,
-                                          double tfe, int sur, double& Tn, int& n_fb) {
+                                          double tfe, int sur, double& Tn, int& n_fb,
+                                          const double* field = nullptr) {
   Tn = -1.0;
-  if (!sur) return rm_eval_T(P, B, f, c, Tn);
+  if (!sur) return rm_eval_T(P, B, f, c, Tn, field);
   const int e = B.tag[c];
   bool fe = e < 0 || (efb ? efb[e] != 0 : rm_elem_bad(P, B, e, f, tfe));
   if (!fe) {
@@ -206,7 +208,7 @@ __device__ __forceinline__ bool rm_cell_T(const RmParams& P, const RmBuffers& B,
     fe = fabs(Tn - P.Tm) < P.sur_band;
   }
   if (!fe) return true;
-  const bool mat = rm_eval_T(P, B, f, c, Tn);
+  const bool mat = rm_eval_T(P, B, f, c, Tn, field);
   if (!mat) Tn = -1.0;
   if (e >= 0) n_fb++;
   return mat;
@@ -389,6 +391,12 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_field(RmParams P, RmBuffers B) 
       if (threadIdx.x == 0 && armed) atomicAdd(&ctr->armed[par], armed);
     }
   }
+  if (!P.window) {
+    // every cell each step: the field at every node once, read by k_rm_phase
+    const int stride = gridDim.x * blockDim.x;
+    for (int nd = blockIdx.x * blockDim.x + threadIdx.x; nd < P.nodes; nd += stride)
+      B.theta[nd] = rm_theta(P, B, f, nd);
+  }
   if (blockIdx.x != 0 || threadIdx.x != 0) return;
   const bool hot_test = P.window && !ctr->force_full && n > 0;
   if (!hot_test) {
@@ -491,7 +499,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_phase(RmParams P, RmBuffers B) 
     const int i = o0[0] + rem % ext[0];
     const int c = k * P.nxyG + j * P.nxG + i;
     double Tn;
-    const bool mat = rm_cell_T(P, B, c, th, efb, tfe, P.sur, Tn, n_fb);
+    const bool mat = rm_cell_T(P, B, c, th, efb, tfe, P.sur, Tn, n_fb, P.window ? nullptr : B.theta);
     if (later) {
       double Tp;
       if (box_has(Rp, i, j, k)) {
@@ -1238,8 +1246,10 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
   if (nbk < 1) nbk = 1;
   switch (q) {
     case 0:
-      // the site count (grid-stride) and the step's box
-      hipLaunchKernelGGL(k_rm_field, dim3(64u), dim3(kRmBlock), 0, s, P, B);
+      // the site count (grid-stride), the step's box, and without the box the
+      // field at every node
+      hipLaunchKernelGGL(k_rm_field, dim3(P.window ? 64u : (unsigned)(nbk < 1024 ? nbk : 1024)),
+                         dim3(kRmBlock), 0, s, P, B);
       break;
     case 1:
       if (!P.sur) return hipSuccess;
