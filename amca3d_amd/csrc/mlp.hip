@@ -17,6 +17,7 @@
 #include <vector>
 
 #include "../../include/amca3d.h"
+#include "surrogate.h"
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float floatx16 __attribute__((ext_vector_type(16)));
@@ -35,7 +36,7 @@ constexpr int kFanOut[kLayers] = {kHid, kHid, kHid, kHid, 1};
 struct Layout {
   int w[kLayers], b[kLayers];
   int total;
-  Layout() {
+  __host__ __device__ Layout() {
     int o = 0;
     for (int l = 0; l < kLayers; l++) {
       w[l] = o;
@@ -213,6 +214,14 @@ __global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float
   p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
 }
 
+}  // namespace mlpk
+
+namespace mlpk {
+// fused inference (defined with the sampler below): mode 0 rows of X -> network
+// output, mode 1 lattice points of (d, s, g) -> temperatures
+hipError_t launch_fused(int mode, const float* p, long long n, const float* X, float* out,
+                        const amca_surrogate_desc& d, const amca_scan& s, const SurLattice& g,
+                        hipStream_t st);
 }  // namespace mlpk
 
 using namespace mlpk;
@@ -527,6 +536,17 @@ amca_status amca_mlp_train_device(amca_mlp* h, int64_t n, const float* dX, const
 }
 
 // Forward over device-resident rows (no copies): T[n] = mlp(X[n][5]).
+amca_status amca_mlp_forward_fused(amca_mlp* h, int32_t n, const float* x5, float* t) {
+  if (!h || n < 1 || !x5 || !t) return mlp_fail(AMCA_ERR_INVALID, "bad arguments");
+  MLP_TRY(ensure_cap(h, n));
+  MLP_HIP(hipMemcpyAsync(h->X, x5, sizeof(float) * kIn * n, hipMemcpyHostToDevice, h->stream));
+  MLP_HIP(launch_fused(0, h->params, n, h->X, h->T, amca_surrogate_desc{}, amca_scan{},
+                       SurLattice{}, h->stream));
+  MLP_HIP(hipMemcpyAsync(t, h->T, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
+  MLP_HIP(hipStreamSynchronize(h->stream));
+  return AMCA_OK;
+}
+
 amca_status amca_mlp_forward_device(amca_mlp* h, int64_t n, const float* dX, float* dT) {
   if (!h || n < 1 || !dX || !dT) return mlp_fail(AMCA_ERR_INVALID, "bad arguments");
   const int chunk = 1 << 20;
@@ -583,7 +603,6 @@ extern "C" amca_status amca_mlp_gemm_selftest(int32_t M, int32_t N, int32_t K, i
 // ===========================================================================
 // Training data and lattice evaluation of the temperature surrogate.
 
-#include "surrogate.h"
 
 namespace mlpk {
 
@@ -718,6 +737,176 @@ __global__ void k_denorm(const float* __restrict__ t, int m, amca_surrogate_desc
   out[r] = (float)from_target(d, (double)t[r]);
 }
 
+// ---------------------------------------------------------------------------
+// Fused inference: the whole network per 32 rows in one wave, weights in LDS,
+// activations in registers.  Layer l computes D'[out][row] = W_l^T act^T with
+// v_mfma_f32_32x32x16_bf16 (A = weights, rows = output features; B =
+// activations, columns = batch rows).  The accumulator of a 32 x 32 tile holds,
+// in lane L and register i, output feature (i/4)*8 + (L/32)*4 + i%4 of batch
+// row L%32: after bias, SiLU and BF16 rounding, registers 8s..8s+7 of tile t
+// are exactly the B operand of the next layer's k-step 2t+s, provided that
+// k-step's k order is the one those registers carry.  The weights are laid out
+// in LDS in that order, so no activation moves between lanes.
+//   k-step ks, lane group g (= L/32), element ii  <->  input feature
+//   16*ks + 8*(ii/4) + 4*g + ii%4
+// Rounding points are those of the layered path (inputs, weights and
+// activations to BF16, FP32 accumulation, bias and SiLU in FP32); only the
+// accumulation order differs.
+
+constexpr int kFusedWaves = 8;
+constexpr int kWtStride = kHid + 8;  // bf16 per LDS row (padding spreads the banks)
+
+struct FusedLds {
+  __bf16 wt[3][kHid][kWtStride];  // layers 2..4: wt[l][out][16*ks + 8*g + ii]
+  __bf16 w1t[kHid][8];            // layer 1: W1[k][out] for k < 5, zero above
+  float b[4][kHid];               // biases of layers 1..4
+  float w5[kHid];                 // W5, rounded to BF16
+  float b5;
+};
+
+__device__ __forceinline__ int fused_feature(int ks, int g, int ii) {
+  return 16 * ks + 8 * (ii >> 2) + 4 * g + (ii & 3);
+}
+
+// parameter offsets of the layered layout: W1 at 0, b1 at 640; layer l >= 2
+// (0-based 1..3) at 768 + (l-1) * 16512, its bias 16384 further; W5, b5 last
+__device__ __forceinline__ int fused_w(int l) { return l == 0 ? 0 : 768 + (l - 1) * 16512; }
+__device__ __forceinline__ int fused_b(int l) { return l == 0 ? 640 : fused_w(l) + kHid * kHid; }
+
+__device__ __forceinline__ void fused_load(const float* __restrict__ p, FusedLds& L) {
+  for (int q = threadIdx.x; q < 3 * kHid * kHid; q += blockDim.x) {
+    const int l = q / (kHid * kHid), out = (q / kHid) % kHid, kk = q % kHid;
+    const int in = fused_feature(kk >> 4, (kk >> 3) & 1, kk & 7);
+    L.wt[l][out][kk] = (__bf16)p[fused_w(l + 1) + in * kHid + out];
+  }
+  for (int q = threadIdx.x; q < kHid * 8; q += blockDim.x) {
+    const int out = q / 8, k = q % 8;
+    L.w1t[out][k] = (__bf16)(k < kIn ? p[k * kHid + out] : 0.0f);
+  }
+  for (int q = threadIdx.x; q < 4 * kHid; q += blockDim.x)
+    L.b[q / kHid][q % kHid] = p[fused_b(q / kHid) + q % kHid];
+  const int w5 = fused_b(3) + kHid;
+  for (int q = threadIdx.x; q < kHid; q += blockDim.x) L.w5[q] = (float)(__bf16)p[w5 + q];
+  if (threadIdx.x == 0) L.b5 = p[w5 + kHid];
+  __syncthreads();
+}
+
+// bias + SiLU + BF16 of the four accumulator tiles into the next B operands
+__device__ __forceinline__ void fused_epilogue(const floatx16* acc, const float* bias, int g,
+                                               bf16x8* bop) {
+#pragma unroll
+  for (int ks = 0; ks < 8; ks++) {
+    const int t = ks >> 1, s = ks & 1;
+#pragma unroll
+    for (int ii = 0; ii < 8; ii++) {
+      const float z = acc[t][8 * s + ii] + bias[fused_feature(ks, g, ii)];
+      bop[ks][ii] = (__bf16)silu(z);
+    }
+  }
+}
+
+// Network output (the training target scale) of the 32 rows whose inputs the
+// lanes of group 0 hold in x[0..4]; valid in lanes 0..31.
+__device__ __forceinline__ float fused_rows(const FusedLds& L, const float* x) {
+  const int lane = threadIdx.x & 63, g = lane >> 5, r = lane & 31;
+  floatx16 acc[4];
+  bf16x8 bop[8];
+  // layer 1: K = 5, padded to one k-step of 16 (lane group 1 carries zeros)
+  bf16x8 bx;
+#pragma unroll
+  for (int ii = 0; ii < 8; ii++) bx[ii] = (__bf16)((g == 0 && ii < kIn) ? x[ii] : 0.0f);
+#pragma unroll
+  for (int ot = 0; ot < 4; ot++) {
+    bf16x8 a;
+#pragma unroll
+    for (int ii = 0; ii < 8; ii++) a[ii] = g == 0 ? L.w1t[32 * ot + r][ii] : (__bf16)0.0f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) acc[ot][i] = 0.0f;
+    acc[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bx, acc[ot], 0, 0, 0);
+  }
+  fused_epilogue(acc, L.b[0], g, bop);
+  // layers 2..4
+#pragma unroll 1
+  for (int l = 0; l < 3; l++) {
+#pragma unroll
+    for (int ot = 0; ot < 4; ot++) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) acc[ot][i] = 0.0f;
+    }
+#pragma unroll
+    for (int ks = 0; ks < 8; ks++) {
+#pragma unroll
+      for (int ot = 0; ot < 4; ot++) {
+        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&L.wt[l][32 * ot + r][16 * ks + 8 * g]);
+        acc[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bop[ks], acc[ot], 0, 0, 0);
+      }
+    }
+    fused_epilogue(acc, L.b[l + 1], g, bop);
+  }
+  // layer 5: a dot product over the 128 features, half in each lane group
+  float t = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < 8; ks++)
+#pragma unroll
+    for (int ii = 0; ii < 8; ii++) t += (float)bop[ks][ii] * L.w5[fused_feature(ks, g, ii)];
+  t += __shfl_xor(t, 32, 64);
+  return t + L.b5;
+}
+
+// mode 0: rows of X (n x 5) -> T (network output); mode 1: lattice points ->
+// temperatures (from_target) into out
+template <int kMode>
+__global__ void __launch_bounds__(64 * kFusedWaves)
+k_mlp_fused(const float* __restrict__ p, long long n, const float* __restrict__ X,
+            float* __restrict__ out, amca_surrogate_desc d, amca_scan s, SurLattice g) {
+  extern __shared__ __align__(16) unsigned char lds_raw[];
+  FusedLds& L = *reinterpret_cast<FusedLds*>(lds_raw);
+  fused_load(p, L);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long groups = (n + 31) / 32;
+  for (long long gr = (long long)blockIdx.x * kFusedWaves + wave; gr < groups;
+       gr += (long long)gridDim.x * kFusedWaves) {
+    const long long row = gr * 32 + (lane & 31);
+    float x[kIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (row < n && lane < 32) {
+      if (kMode == 0) {
+#pragma unroll
+        for (int q = 0; q < kIn; q++) x[q] = X[row * kIn + q];
+      } else {
+        const int i = (int)(row % g.nxi);
+        const int j = (int)((row / g.nxi) % g.ny);
+        const int k = (int)(row / ((long long)g.nxi * g.ny));
+        inputs(d, s, g.xi0 + i * g.dxi + s.x_start, g.y0 + j * g.dy, g.z0 + k * g.dz, 0.0, x);
+      }
+    }
+    const float t = fused_rows(L, x);
+    if (row < n && lane < 32) out[row] = kMode == 0 ? t : (float)from_target(d, (double)t);
+  }
+}
+
+hipError_t launch_fused(int mode, const float* p, long long n, const float* X, float* out,
+                        const amca_surrogate_desc& d, const amca_scan& s, const SurLattice& g,
+                        hipStream_t st) {
+  const size_t lds = sizeof(FusedLds);
+  long long blocks = (n + 32 * kFusedWaves - 1) / (32 * kFusedWaves);
+  if (blocks > 1024) blocks = 1024;
+  if (blocks < 1) blocks = 1;
+  if (mode == 0) {
+    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_fused<0>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_fused<0>, dim3((unsigned)blocks), dim3(64 * kFusedWaves), lds, st, p,
+                       n, X, out, d, s, g);
+  } else {
+    hipError_t e = hipFuncSetAttribute((const void*)k_mlp_fused<1>,
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_mlp_fused<1>, dim3((unsigned)blocks), dim3(64 * kFusedWaves), lds, st, p,
+                       n, X, out, d, s, g);
+  }
+  return hipGetLastError();
+}
+
 __global__ void k_eval_points(amca_surrogate_desc d, amca_scan s, int n, const double* __restrict__ p,
                               float* __restrict__ X, double* __restrict__ Tfe) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -734,18 +923,7 @@ const char* surrogate_error() { return g_mlp_error.c_str(); }
 amca_status surrogate_eval_lattice(amca_mlp* h, const amca_surrogate_desc& d, const amca_scan& s,
                                    const SurLattice& g, float* out) {
   const long long total = (long long)g.nxi * g.ny * g.nz;
-  const int chunk = 1 << 20;
-  MLP_TRY(ensure_cap(h, (int)std::min<long long>(total, chunk)));
-  for (long long r0 = 0; r0 < total; r0 += chunk) {
-    const int m = (int)std::min<long long>(chunk, total - r0);
-    hipLaunchKernelGGL(k_lattice_inputs, dim3((m + 255) / 256), dim3(256), 0, h->stream, d, s, g, r0,
-                       m, h->X);
-    MLP_HIP(hipGetLastError());
-    MLP_TRY(forward(h, m, false));
-    hipLaunchKernelGGL(k_denorm, dim3((m + 255) / 256), dim3(256), 0, h->stream, h->T, m, d,
-                       out + r0);
-    MLP_HIP(hipGetLastError());
-  }
+  MLP_HIP(launch_fused(1, h->params, total, nullptr, out, d, s, g, h->stream));
   MLP_HIP(hipStreamSynchronize(h->stream));
   return AMCA_OK;
 }

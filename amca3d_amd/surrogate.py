@@ -36,6 +36,7 @@ SIGNATURES = {
     "amca_mlp_train_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
                                         C.c_int32, C.c_float, C.c_uint64, F32P]),
     "amca_mlp_forward_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "amca_mlp_forward_fused": (C.c_int, [C.c_void_p, C.c_int32, F32P, F32P]),
     "amca_mlp_gemm_selftest": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          F32P, F32P, F32P]),
     "amca_mlp_last_error": (C.c_char_p, []),
@@ -106,6 +107,13 @@ class MLP:
         X = _f32(X)
         t = np.empty(len(X), np.float32)
         _check(lib().amca_mlp_forward(self.h, len(X), _p(X), _p(t)), "forward")
+        return t
+
+    def forward_fused(self, X):
+        """The forward pass as one fused kernel (amca_mlp_forward_fused)."""
+        X = _f32(X)
+        t = np.empty(len(X), np.float32)
+        _check(lib().amca_mlp_forward_fused(self.h, len(X), _p(X), _p(t)), "forward_fused")
         return t
 
     def gradients(self, X, Y):

@@ -419,7 +419,7 @@ def surrogate_main(args):
                    "per_scan": per_scan,
                    "train_seconds": train_s, "train_final_losses": losses,
                    "held_out_scans": held, "lattice_eval": lattice},
-        "roofline": {"bound": "mfma", "kernel": "k_gemm (lattice evaluation)",
+        "roofline": {"bound": "mfma", "kernel": "k_mlp_fused (lattice evaluation)",
                      "achieved": lattice.get("tflops"), "peak": MFMA_BF16_PEAK_TFLOPS,
                      "unit": "TFLOP/s",
                      "frac": (lattice.get("tflops") or 0.0) / MFMA_BF16_PEAK_TFLOPS,

@@ -54,6 +54,29 @@ def test_forward_matches_oracle(amca):
     m.close()
 
 
+@pytest.mark.parametrize("n", [1, 31, 1000, 70001])
+def test_fused_forward_matches_oracle_and_layered(amca, n):
+    """The fused kernel (weights in LDS, activations in registers, k order of
+    each layer permuted to the accumulator layout) computes the layered
+    forward: the oracle to its tolerance, the layered kernels to FP32
+    accumulation order."""
+    from amca3d_amd.surrogate import MLP
+    m = MLP(1)
+    p = _params(7)
+    m.set_params(p)
+    rng = np.random.default_rng(n)
+    X = rng.uniform(-1, 1, (n, 5)).astype(np.float32)
+    tf = m.forward_fused(X)
+    tl = m.forward(X)
+    m.close()
+    scale = np.abs(tl).max() + 1
+    np.testing.assert_allclose(tf, tl, rtol=0, atol=2e-2 * scale)
+    assert np.median(np.abs(tf - tl)) < 1e-3 * scale
+    if n <= 1000:
+        ref, _ = mo.forward(p.astype(np.float64), X)
+        np.testing.assert_allclose(tf, ref, rtol=0, atol=2e-2 * (np.abs(ref).max() + 1))
+
+
 def test_gradients_match_oracle(amca):
     from amca3d_amd.surrogate import MLP
     m = MLP(1)
