@@ -63,9 +63,11 @@ struct EpiArgs {
   __bf16* Aout;      // kEpiBiasSilu: activation out (BF16), row stride ldc
 };
 
-__device__ __forceinline__ float silu(float z) { return z / (1.0f + __expf(-z)); }
+// v_rcp_f32 (1 ulp) in place of the IEEE divide sequence: the result is
+// rounded to bf16 or feeds an fp32 gradient, far coarser than the reciprocal.
+__device__ __forceinline__ float silu(float z) { return z * __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
 __device__ __forceinline__ float dsilu(float z) {
-  const float s = 1.0f / (1.0f + __expf(-z));
+  const float s = __builtin_amdgcn_rcpf(1.0f + __expf(-z));
   return s * (1.0f + z * (1.0f - s));
 }
 
@@ -817,9 +819,10 @@ __device__ __forceinline__ float fused_rows(const FusedLds& L, const float* x) {
   for (int ii = 0; ii < 8; ii++) bx[ii] = (__bf16)((g == 0 && ii < kIn) ? x[ii] : 0.0f);
 #pragma unroll
   for (int ot = 0; ot < 4; ot++) {
+    const bf16x8 w = *reinterpret_cast<const bf16x8*>(&L.w1t[32 * ot + r][0]);
     bf16x8 a;
 #pragma unroll
-    for (int ii = 0; ii < 8; ii++) a[ii] = g == 0 ? L.w1t[32 * ot + r][ii] : (__bf16)0.0f;
+    for (int ii = 0; ii < 8; ii++) a[ii] = g == 0 ? w[ii] : (__bf16)0.0f;
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[ot][i] = 0.0f;
     acc[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bx, acc[ot], 0, 0, 0);
@@ -833,13 +836,26 @@ __device__ __forceinline__ float fused_rows(const FusedLds& L, const float* x) {
 #pragma unroll
       for (int i = 0; i < 16; i++) acc[ot][i] = 0.0f;
     }
+    // the weight fragments of k-step ks+1 are read from LDS while the MFMAs of
+    // k-step ks run (the scheduling barrier keeps the reads ahead of them)
+    const __bf16* wl = &L.wt[l][r][8 * g];
+    bf16x8 a[2][4];
+#pragma unroll
+    for (int ot = 0; ot < 4; ot++)
+      a[0][ot] = *reinterpret_cast<const bf16x8*>(wl + 32 * ot * kWtStride);
 #pragma unroll
     for (int ks = 0; ks < 8; ks++) {
+      if (ks + 1 < 8) {
 #pragma unroll
-      for (int ot = 0; ot < 4; ot++) {
-        const bf16x8 a = *reinterpret_cast<const bf16x8*>(&L.wt[l][32 * ot + r][16 * ks + 8 * g]);
-        acc[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bop[ks], acc[ot], 0, 0, 0);
+        for (int ot = 0; ot < 4; ot++)
+          a[(ks + 1) & 1][ot] =
+              *reinterpret_cast<const bf16x8*>(wl + 32 * ot * kWtStride + 16 * (ks + 1));
       }
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int ot = 0; ot < 4; ot++)
+        acc[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks & 1][ot], bop[ks], acc[ot], 0, 0, 0);
+      __builtin_amdgcn_sched_barrier(0);
     }
     fused_epilogue(acc, L.b[l + 1], g, bop);
   }
