@@ -751,9 +751,17 @@ __global__ void k_denorm(const float* __restrict__ t, int m, amca_surrogate_desc
 // in LDS in that order, so no activation moves between lanes.
 //   k-step ks, lane group g (= L/32), element ii  <->  input feature
 //   16*ks + 8*(ii/4) + 4*g + ii%4
-// Rounding points are those of the layered path (inputs, weights and
-// activations to BF16, FP32 accumulation, bias and SiLU in FP32); only the
-// accumulation order differs.
+// The hidden layers carry y = -log2(e) z instead of the pre-activation z, so
+// that sigmoid(z) = 1 / (1 + 2^y) is one v_exp_f32 and one v_rcp_f32 with no
+// scaling multiply: the stored activation is a' = y sigmoid(z) = -log2(e)
+// silu(z), whose factor cancels against the -log2(e) the next layer's bias
+// carries (y_next = W a' - log2(e) b = -log2(e) z_next).  Layer 1 scales its
+// accumulator once (one FMA with the scaled bias); layer 5 is an MFMA with W5
+// in every row of A and scales its output by -1/log2(e) before adding b5.  The biases of layers 2..4 are loaded into the
+// accumulators before the first MFMA, so those layers add no bias either.
+// Inputs and weights round to BF16 as in the layered path; the activations
+// round to BF16 as scaled values, which differ from the layered path's in the
+// last BF16 place.
 
 constexpr int kFusedWaves = 8;
 constexpr int kWtStride = kHid + 8;  // bf16 per LDS row (padding spreads the banks)
@@ -761,10 +769,13 @@ constexpr int kWtStride = kHid + 8;  // bf16 per LDS row (padding spreads the ba
 struct FusedLds {
   __bf16 wt[3][kHid][kWtStride];  // layers 2..4: wt[l][out][16*ks + 8*g + ii]
   __bf16 w1t[kHid][8];            // layer 1: W1[k][out] for k < 5, zero above
-  float b[4][kHid];               // biases of layers 1..4
-  float w5[kHid];                 // W5, rounded to BF16
+  float b[4][kHid];               // biases of layers 1..4, times -log2(e)
+  __bf16 w5[8][2][8];             // W5 in BF16, as the A fragment of k-step ks, lane group g
   float b5;
 };
+
+constexpr float kNegLog2e = -1.4426950408889634f;
+constexpr float kNegLn2 = -0.6931471805599453f;  // -1/log2(e)
 
 __device__ __forceinline__ int fused_feature(int ks, int g, int ii) {
   return 16 * ks + 8 * (ii >> 2) + 4 * g + (ii & 3);
@@ -786,87 +797,146 @@ __device__ __forceinline__ void fused_load(const float* __restrict__ p, FusedLds
     L.w1t[out][k] = (__bf16)(k < kIn ? p[k * kHid + out] : 0.0f);
   }
   for (int q = threadIdx.x; q < 4 * kHid; q += blockDim.x)
-    L.b[q / kHid][q % kHid] = p[fused_b(q / kHid) + q % kHid];
+    L.b[q / kHid][q % kHid] = kNegLog2e * p[fused_b(q / kHid) + q % kHid];
   const int w5 = fused_b(3) + kHid;
-  for (int q = threadIdx.x; q < kHid; q += blockDim.x) L.w5[q] = (float)(__bf16)p[w5 + q];
+  for (int q = threadIdx.x; q < kHid; q += blockDim.x) {
+    const int ks = q >> 4, g = (q >> 3) & 1, ii = q & 7;
+    L.w5[ks][g][ii] = (__bf16)p[w5 + fused_feature(ks, g, ii)];
+  }
   if (threadIdx.x == 0) L.b5 = p[w5 + kHid];
   __syncthreads();
 }
 
-// bias + SiLU + BF16 of the four accumulator tiles into the next B operands
-__device__ __forceinline__ void fused_epilogue(const floatx16* acc, const float* bias, int g,
-                                               bf16x8* bop) {
+// Chunk ks of the epilogue: a' = y / (1 + 2^y) of the eight accumulator
+// registers 8(ks%2).. of tile ks/2, rounded to BF16, into the next layer's B
+// operand bop[ks]; kFirst: y = -log2(e) acc + bias (layer 1), else y = acc.
+template <bool kFirst>
+__device__ __forceinline__ void fused_epi_chunk(const floatx16* acc, const float* bias, int g,
+                                                int ks, bf16x8* bop) {
+  const int t = ks >> 1, s = ks & 1;
 #pragma unroll
-  for (int ks = 0; ks < 8; ks++) {
-    const int t = ks >> 1, s = ks & 1;
-#pragma unroll
-    for (int ii = 0; ii < 8; ii++) {
-      const float z = acc[t][8 * s + ii] + bias[fused_feature(ks, g, ii)];
-      bop[ks][ii] = (__bf16)silu(z);
-    }
+  for (int ii = 0; ii < 8; ii++) {
+    float y = acc[t][8 * s + ii];
+    if (kFirst) y = fmaf(y, kNegLog2e, bias[fused_feature(ks, g, ii)]);
+    bop[ks][ii] = (__bf16)(y * __builtin_amdgcn_rcpf(1.0f + __builtin_amdgcn_exp2f(y)));
   }
 }
 
-// Network output (the training target scale) of the 32 rows whose inputs the
-// lanes of group 0 hold in x[0..4]; valid in lanes 0..31.
+// One phase of the two-half schedule: the 32 MFMAs of hidden layer l + 2
+// (l = 0..2) for half P, interleaved with the epilogue of the previous layer
+// of half Q, which needs no MFMA result of this phase.  The accumulators of P
+// start at the layer's bias (register 4q+j of tile ot holds output feature
+// 32 ot + 8 q + 4 g + j); the weight fragments of k-step ks+1 are read from
+// LDS while those of k-step ks feed the MFMAs.
+template <bool kFirstEpi>
+__device__ __forceinline__ void fused_phase(const FusedLds& L, int l, int r, int g,
+                                            floatx16* accP, const bf16x8* bopP,
+                                            const floatx16* accQ, bf16x8* bopQ) {
+#pragma unroll
+  for (int ot = 0; ot < 4; ot++) {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const float4 b = *reinterpret_cast<const float4*>(&L.b[l + 1][32 * ot + 8 * q + 4 * g]);
+      accP[ot][4 * q] = b.x;
+      accP[ot][4 * q + 1] = b.y;
+      accP[ot][4 * q + 2] = b.z;
+      accP[ot][4 * q + 3] = b.w;
+    }
+  }
+  const __bf16* wl = &L.wt[l][r][8 * g];
+  bf16x8 a[2][4];
+#pragma unroll
+  for (int ot = 0; ot < 4; ot++)
+    a[0][ot] = *reinterpret_cast<const bf16x8*>(wl + 32 * ot * kWtStride);
+  __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+  for (int ks = 0; ks < 8; ks++) {
+    if (ks + 1 < 8) {
+#pragma unroll
+      for (int ot = 0; ot < 4; ot++)
+        a[(ks + 1) & 1][ot] =
+            *reinterpret_cast<const bf16x8*>(wl + 32 * ot * kWtStride + 16 * (ks + 1));
+    }
+#pragma unroll
+    for (int ot = 0; ot < 4; ot++)
+      accP[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks & 1][ot], bopP[ks], accP[ot], 0, 0, 0);
+    fused_epi_chunk<kFirstEpi>(accQ, L.b[0], g, ks, bopQ);
+    // LDS reads first, then each MFMA followed by a share of the epilogue
+    __builtin_amdgcn_sched_group_barrier(0x100, 6, 0);
+#pragma unroll
+    for (int m = 0; m < 4; m++) {
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 10, 0);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+  }
+}
+
+// Layer 5 of half P (8 MFMAs, A = W5 in every row, so every accumulator
+// register of lane L holds the output of row L % 32), optionally interleaved
+// with the last epilogue of half Q.
+template <bool kEpi>
+__device__ __forceinline__ float fused_out(const FusedLds& L, int g, const bf16x8* bopP,
+                                           const floatx16* accQ, bf16x8* bopQ) {
+  floatx16 acc;
+#pragma unroll
+  for (int i = 0; i < 16; i++) acc[i] = 0.0f;
+#pragma unroll
+  for (int ks = 0; ks < 8; ks++) {
+    const bf16x8 a = *reinterpret_cast<const bf16x8*>(&L.w5[ks][g][0]);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bopP[ks], acc, 0, 0, 0);
+    if (kEpi) {
+      fused_epi_chunk<false>(accQ, nullptr, g, ks, bopQ);
+      __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+      __builtin_amdgcn_sched_group_barrier(0x002, 40, 0);
+      __builtin_amdgcn_sched_barrier(0);
+    }
+  }
+  return fmaf(acc[0], kNegLn2, L.b5);
+}
+
+// Network output (the training target scale) of the 64 rows whose inputs the
+// lanes hold in x[0..4] (lane L: row L of the group), returned in lane L.
+// Half X is rows 0..31, half Y rows 32..63; each runs the whole network, and
+// the MFMAs of one half overlap the SiLU epilogue of the other.
 __device__ __forceinline__ float fused_rows(const FusedLds& L, const float* x) {
   const int lane = threadIdx.x & 63, g = lane >> 5, r = lane & 31;
-  floatx16 acc[4];
-  bf16x8 bop[8];
-  // layer 1: K = 5, padded to one k-step of 16 (lane group 1 carries zeros)
-  bf16x8 bx;
+  floatx16 accX[4], accY[4];
+  bf16x8 bopX[8], bopY[8];
+  // layer 1: K = 5, padded to one k-step of 16.  Half X takes k 0..7 (lane
+  // group 0 holds its rows), half Y k 8..15 (lane group 1); the weight rows
+  // repeat in both k halves and the other half's inputs are zero.
+  bf16x8 bx, by;
 #pragma unroll
-  for (int ii = 0; ii < 8; ii++) bx[ii] = (__bf16)((g == 0 && ii < kIn) ? x[ii] : 0.0f);
+  for (int ii = 0; ii < 8; ii++) {
+    const __bf16 v = (__bf16)(ii < kIn ? x[ii] : 0.0f);
+    bx[ii] = g == 0 ? v : (__bf16)0.0f;
+    by[ii] = g == 1 ? v : (__bf16)0.0f;
+  }
 #pragma unroll
   for (int ot = 0; ot < 4; ot++) {
     const bf16x8 w = *reinterpret_cast<const bf16x8*>(&L.w1t[32 * ot + r][0]);
-    bf16x8 a;
 #pragma unroll
-    for (int ii = 0; ii < 8; ii++) a[ii] = g == 0 ? w[ii] : (__bf16)0.0f;
-#pragma unroll
-    for (int i = 0; i < 16; i++) acc[ot][i] = 0.0f;
-    acc[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, bx, acc[ot], 0, 0, 0);
-  }
-  fused_epilogue(acc, L.b[0], g, bop);
-  // layers 2..4
-#pragma unroll 1
-  for (int l = 0; l < 3; l++) {
-#pragma unroll
-    for (int ot = 0; ot < 4; ot++) {
-#pragma unroll
-      for (int i = 0; i < 16; i++) acc[ot][i] = 0.0f;
+    for (int i = 0; i < 16; i++) {
+      accX[ot][i] = 0.0f;
+      accY[ot][i] = 0.0f;
     }
-    // the weight fragments of k-step ks+1 are read from LDS while the MFMAs of
-    // k-step ks run (the scheduling barrier keeps the reads ahead of them)
-    const __bf16* wl = &L.wt[l][r][8 * g];
-    bf16x8 a[2][4];
-#pragma unroll
-    for (int ot = 0; ot < 4; ot++)
-      a[0][ot] = *reinterpret_cast<const bf16x8*>(wl + 32 * ot * kWtStride);
-#pragma unroll
-    for (int ks = 0; ks < 8; ks++) {
-      if (ks + 1 < 8) {
-#pragma unroll
-        for (int ot = 0; ot < 4; ot++)
-          a[(ks + 1) & 1][ot] =
-              *reinterpret_cast<const bf16x8*>(wl + 32 * ot * kWtStride + 16 * (ks + 1));
-      }
-      __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-      for (int ot = 0; ot < 4; ot++)
-        acc[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a[ks & 1][ot], bop[ks], acc[ot], 0, 0, 0);
-      __builtin_amdgcn_sched_barrier(0);
-    }
-    fused_epilogue(acc, L.b[l + 1], g, bop);
+    accX[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, bx, accX[ot], 0, 0, 0);
+    accY[ot] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(w, by, accY[ot], 0, 0, 0);
   }
-  // layer 5: a dot product over the 128 features, half in each lane group
-  float t = 0.0f;
 #pragma unroll
-  for (int ks = 0; ks < 8; ks++)
-#pragma unroll
-    for (int ii = 0; ii < 8; ii++) t += (float)bop[ks][ii] * L.w5[fused_feature(ks, g, ii)];
-  t += __shfl_xor(t, 32, 64);
-  return t + L.b5;
+  for (int ks = 0; ks < 8; ks++) fused_epi_chunk<true>(accX, L.b[0], g, ks, bopX);
+  // hidden layers 2..4
+  fused_phase<true>(L, 0, r, g, accX, bopX, accY, bopY);
+  fused_phase<false>(L, 0, r, g, accY, bopY, accX, bopX);
+  fused_phase<false>(L, 1, r, g, accX, bopX, accY, bopY);
+  fused_phase<false>(L, 1, r, g, accY, bopY, accX, bopX);
+  fused_phase<false>(L, 2, r, g, accX, bopX, accY, bopY);
+  fused_phase<false>(L, 2, r, g, accY, bopY, accX, bopX);
+  const float tx = fused_out<true>(L, g, bopX, accY, bopY);
+  const float ty = fused_out<false>(L, g, bopY, nullptr, nullptr);
+  return g == 0 ? tx : ty;
 }
 
 // mode 0: rows of X (n x 5) -> T (network output); mode 1: lattice points ->
@@ -879,12 +949,12 @@ k_mlp_fused(const float* __restrict__ p, long long n, const float* __restrict__ 
   FusedLds& L = *reinterpret_cast<FusedLds*>(lds_raw);
   fused_load(p, L);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long groups = (n + 31) / 32;
+  const long long groups = (n + 63) / 64;
   for (long long gr = (long long)blockIdx.x * kFusedWaves + wave; gr < groups;
        gr += (long long)gridDim.x * kFusedWaves) {
-    const long long row = gr * 32 + (lane & 31);
+    const long long row = gr * 64 + lane;
     float x[kIn] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (row < n && lane < 32) {
+    if (row < n) {
       if (kMode == 0) {
 #pragma unroll
         for (int q = 0; q < kIn; q++) x[q] = X[row * kIn + q];
@@ -896,7 +966,7 @@ k_mlp_fused(const float* __restrict__ p, long long n, const float* __restrict__ 
       }
     }
     const float t = fused_rows(L, x);
-    if (row < n && lane < 32) out[row] = kMode == 0 ? t : (float)from_target(d, (double)t);
+    if (row < n) out[row] = kMode == 0 ? t : (float)from_target(d, (double)t);
   }
 }
 
@@ -904,7 +974,7 @@ hipError_t launch_fused(int mode, const float* p, long long n, const float* X, f
                         const amca_surrogate_desc& d, const amca_scan& s, const SurLattice& g,
                         hipStream_t st) {
   const size_t lds = sizeof(FusedLds);
-  long long blocks = (n + 32 * kFusedWaves - 1) / (32 * kFusedWaves);
+  long long blocks = (n + 64 * kFusedWaves - 1) / (64 * kFusedWaves);
   if (blocks > 1024) blocks = 1024;
   if (blocks < 1) blocks = 1;
   if (mode == 0) {

@@ -48,6 +48,11 @@ def bf16(x):
     return r.astype(np.uint32).view(np.float32).astype(np.float64)
 
 
+def f32(x):
+    """Round to FP32, returned as float64."""
+    return np.asarray(x, np.float32).astype(np.float64)
+
+
 def silu(z):
     return z / (1.0 + np.exp(-z))
 
@@ -69,6 +74,25 @@ def forward(p, X):
         A.append(a)
     T = (bf16(a) @ bf16(W[4]) + B[4]).astype(np.float32).astype(np.float64)[:, 0]
     return T, (Z, A)
+
+
+def forward_scaled(p, X):
+    """Forward at the fused inference kernel's rounding points: the hidden
+    layers carry y = -log2(e) z (FP32), sigmoid(z) = 1 / (1 + 2^y), and the
+    BF16-rounded activation is y * sigmoid(z) = -log2(e) silu(z); the biases
+    are scaled by -log2(e) in FP32, and the output layer accumulates against
+    the BF16 W5 in FP32, then takes acc * (-1/log2(e)) + b5 in one FMA.
+    The scalings cancel: without rounding this is `forward64`."""
+    W, B = unpack(p)
+    c, ci = f32(-1.4426950408889634), f32(-0.6931471805599453)
+    a = bf16(np.asarray(X, np.float64))
+    for l in range(4):
+        b = f32(c * f32(B[l]))
+        acc = f32(a @ bf16(W[l]))
+        y = f32(f32(acc * c) + b) if l == 0 else f32(acc + b)
+        a = bf16(f32(y * f32(1.0 / (1.0 + np.exp2(y)))))
+    acc = f32(a @ bf16(W[4][:, 0]))
+    return f32(acc * ci + f32(B[4][0]))
 
 
 def gradients(p, X, Y):

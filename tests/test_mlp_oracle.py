@@ -68,3 +68,19 @@ def test_rounded_forms_track_exact_forms():
     g, _ = mo.gradients(p, X, Y)
     g64, _ = mo.gradients64(p, X, Y)
     assert np.linalg.norm(g - g64) < 0.05 * np.linalg.norm(g64)
+
+
+def test_scaled_form_cancels_its_scalings(monkeypatch):
+    """forward_scaled (the fused kernel's rounding points) carries -log2(e) z
+    through the hidden layers; with rounding switched off the scalings cancel
+    and it is the exact network, and with rounding on it stays within the
+    BF16 tolerance of it."""
+    rng = np.random.default_rng(5)
+    p = _params(3)
+    X = rng.uniform(-1, 1, (256, 5))
+    T64 = mo.forward64(p, X)
+    Ts = mo.forward_scaled(p, X)
+    assert np.max(np.abs(Ts - T64)) < 0.05 * (np.abs(T64).max() + 1)
+    monkeypatch.setattr(mo, "bf16", lambda x: np.asarray(x, np.float64))
+    monkeypatch.setattr(mo, "f32", lambda x: np.asarray(x, np.float64))
+    np.testing.assert_allclose(mo.forward_scaled(p, X), T64, rtol=1e-12, atol=1e-12)
