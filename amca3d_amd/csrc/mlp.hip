@@ -26,6 +26,7 @@ namespace mlpk {
 
 constexpr int kIn = 5, kHid = 128, kLayers = 5;
 constexpr int kBM = 128, kBN = 128, kBK = 32;
+constexpr int kWgradRows = 64;  // batch rows per work-group of a weight-gradient GEMM
 constexpr int kPad = 8;  // bf16 elements of padding per LDS row
 constexpr int kThreads = 256;
 
@@ -61,6 +62,8 @@ struct EpiArgs {
   const float* bias; // per column, or null
   float* Z;          // kEpiBiasSilu: pre-activation out; kEpiDSilu: pre-activation in
   __bf16* Aout;      // kEpiBiasSilu: activation out (BF16), row stride ldc
+  float* colsum;     // kEpiAtomic with opB = B: column sums of B over the block's K
+                     // range (FP32, before rounding) added atomically, or null
 };
 
 // v_rcp_f32 (1 ulp) in place of the IEEE divide sequence: the result is
@@ -93,9 +96,15 @@ k_gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B, int ldb, int
   for (int t = 0; t < 4; t++)
 #pragma unroll
     for (int i = 0; i < 16; i++) acc[t][i] = 0.0f;
+  // column sum of B (thread t stages column t % kBN of every B tile when !TrB)
+  constexpr bool kColsum = E == kEpiAtomic && !TrB;
+  float cs = 0.0f;
 
   for (int k0 = kbeg; k0 < kend; k0 += kBK) {
-    // stage A (128 x 32) and B (32 x 128): 4096 elements each, 16 per thread
+    // stage A (128 x 32) and B (32 x 128): 4096 elements each, 16 per thread.
+    // Every load is issued before any is used: an element outside the matrix
+    // (or the block's K range) loads element 0 and stages as zero.
+    float va[(kBM * kBK) / kThreads], vb[(kBN * kBK) / kThreads];
 #pragma unroll
     for (int q = 0; q < (kBM * kBK) / kThreads; q++) {
       const int idx = threadIdx.x + q * kThreads;
@@ -108,9 +117,10 @@ k_gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B, int ldb, int
         mm = idx / kBK;
       }
       const int gm = m0 + mm, gk = k0 + kk;
-      float v = 0.0f;
-      if (gm < M && gk < kend) v = to_f(TrA ? A[(size_t)gk * lda + gm] : A[(size_t)gm * lda + gk]);
-      As[mm][kk] = (__bf16)v;
+      const bool in = gm < M && gk < kend;
+      const size_t o = in ? (TrA ? (size_t)gk * lda + gm : (size_t)gm * lda + gk) : 0;
+      va[q] = to_f(A[o]);
+      if (!in) va[q] = 0.0f;
     }
 #pragma unroll
     for (int q = 0; q < (kBN * kBK) / kThreads; q++) {
@@ -124,9 +134,23 @@ k_gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B, int ldb, int
         kk = idx / kBN;
       }
       const int gn = n0 + nn, gk = k0 + kk;
-      float v = 0.0f;
-      if (gn < N && gk < kend) v = to_f(TrB ? B[(size_t)gn * ldb + gk] : B[(size_t)gk * ldb + gn]);
-      Bs[nn][kk] = (__bf16)v;
+      const bool in = gn < N && gk < kend;
+      const size_t o = in ? (TrB ? (size_t)gn * ldb + gk : (size_t)gk * ldb + gn) : 0;
+      vb[q] = to_f(B[o]);
+      if (!in) vb[q] = 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < (kBM * kBK) / kThreads; q++) {
+      const int idx = threadIdx.x + q * kThreads;
+      const int mm = TrA ? idx % kBM : idx / kBK, kk = TrA ? idx / kBM : idx % kBK;
+      As[mm][kk] = (__bf16)va[q];
+    }
+#pragma unroll
+    for (int q = 0; q < (kBN * kBK) / kThreads; q++) {
+      const int idx = threadIdx.x + q * kThreads;
+      const int nn = TrB ? idx / kBK : idx % kBN, kk = TrB ? idx % kBK : idx / kBN;
+      if (kColsum) cs += vb[q];
+      Bs[nn][kk] = (__bf16)vb[q];
     }
     __syncthreads();
     // two MFMA k-steps of 16: lane holds A[row = lane%32][k = (lane/32)*8 ..+8]
@@ -147,7 +171,26 @@ k_gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B, int ldb, int
     }
     __syncthreads();
   }
-  // D layout: element i of lane l is row (i/4)*8 + (l/32)*4 + i%4, column l%32
+  if (kColsum && e.colsum) {
+    static_assert(kThreads % kBN == 0, "a thread stages one column of B");
+    const int gn = n0 + (int)(threadIdx.x % kBN);
+    if (blockIdx.y == 0 && gn < N) atomicAdd(&e.colsum[gn], cs);
+  }
+  // D layout: element i of lane l is row (i/4)*8 + (l/32)*4 + i%4, column l%32.
+  // kEpiDSilu reads the pre-activations of the whole tile before using any.
+  float zin[4][16];
+  if (E == kEpiDSilu) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+#pragma unroll
+      for (int i = 0; i < 16; i++) {
+        const int m = m0 + wave * 32 + (i >> 2) * 8 + (lane >> 5) * 4 + (i & 3);
+        const int n = n0 + t * 32 + (lane & 31);
+        const bool in = m < M && n < N;
+        zin[t][i] = e.Z[in ? (size_t)m * e.ldc + n : 0];
+      }
+    }
+  }
 #pragma unroll
   for (int t = 0; t < 4; t++) {
 #pragma unroll
@@ -165,23 +208,12 @@ k_gemm(const TA* __restrict__ A, int lda, const TB* __restrict__ B, int ldb, int
         if (e.Z) e.Z[o] = z;
         e.Aout[o] = (__bf16)silu(z);
       } else if (E == kEpiDSilu) {
-        e.C[o] = v * dsilu(e.Z[o]);
+        e.C[o] = v * dsilu(zin[t][i]);
       } else {
         atomicAdd(&e.C[o], v);
       }
     }
   }
-}
-
-// column sums of an M x N FP32 matrix (bias gradients), atomically added
-__global__ void k_colsum(const float* __restrict__ X, int M, int N, int rows_per_block,
-                         float* __restrict__ out) {
-  const int n = blockIdx.x * blockDim.x + threadIdx.x;
-  if (n >= N) return;
-  const int r0 = blockIdx.y * rows_per_block, r1 = min(M, r0 + rows_per_block);
-  float s = 0.0f;
-  for (int r = r0; r < r1; r++) s += X[(size_t)r * N + n];
-  atomicAdd(&out[n], s);
 }
 
 // dT = 2 (T - Y) / M (the gradient of the mean squared error), loss += sum (T - Y)^2
@@ -364,16 +396,15 @@ amca_status backward(amca_mlp* h, int n) {
   hipLaunchKernelGGL(k_mse, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, s, h->T, h->Y,
                      n, h->dT, h->loss);
   MLP_HIP(hipGetLastError());
-  const int ksplit = std::max(1, std::min(64, n / 512));
-  const int rpb = 256;
+  // weight gradients: one 128 x 128 tile per layer, split over the batch in
+  // chunks of kWgradRows rows so that the split fills the chip; the bias
+  // gradient (column sums of dZ) is taken while dZ is staged
+  const int ksplit = std::max(1, (n + kWgradRows - 1) / kWgradRows);
   // layer 5: dW5 = A4^T dT, db5 = sum dT, dZ4 = (dT W5^T) * silu'(Z4)
   {
-    EpiArgs e{h->grads + L.w[4], 1, nullptr, nullptr, nullptr};
+    EpiArgs e{h->grads + L.w[4], 1, nullptr, nullptr, nullptr, h->grads + L.b[4]};
     MLP_HIP((gemm<__bf16, float, true, false, kEpiAtomic>(s, h->Aact[3], kHid, h->dT, 1, kHid, 1, n,
                                                            e, ksplit)));
-    hipLaunchKernelGGL(k_colsum, dim3(1, (n + rpb - 1) / rpb), dim3(64), 0, s, h->dT, n, 1, rpb,
-                       h->grads + L.b[4]);
-    MLP_HIP(hipGetLastError());
     EpiArgs d{h->dZ[0], kHid, nullptr, h->Z[3], nullptr};
     MLP_HIP((gemm<float, float, false, true, kEpiDSilu>(s, h->dT, 1, h->params + L.w[4], 1, n, kHid,
                                                          1, d)));
@@ -382,16 +413,13 @@ amca_status backward(amca_mlp* h, int n) {
   for (int l = 3; l >= 0; l--) {
     float* dz = h->dZ[cur];
     // dW_l = A_{l-1}^T dZ_l (A_{-1} = X), db_l = sum dZ_l
-    EpiArgs e{h->grads + L.w[l], kHid, nullptr, nullptr, nullptr};
+    EpiArgs e{h->grads + L.w[l], kHid, nullptr, nullptr, nullptr, h->grads + L.b[l]};
     if (l == 0)
       MLP_HIP((gemm<float, float, true, false, kEpiAtomic>(s, h->X, kIn, dz, kHid, kIn, kHid, n, e,
                                                             ksplit)));
     else
       MLP_HIP((gemm<__bf16, float, true, false, kEpiAtomic>(s, h->Aact[l - 1], kHid, dz, kHid, kHid,
                                                              kHid, n, e, ksplit)));
-    hipLaunchKernelGGL(k_colsum, dim3((kHid + 63) / 64, (n + rpb - 1) / rpb), dim3(64), 0, s, dz, n,
-                       kHid, rpb, h->grads + L.b[l]);
-    MLP_HIP(hipGetLastError());
     if (l > 0) {
       // dZ_{l-1} = (dZ_l W_l^T) * silu'(Z_{l-1})
       EpiArgs d{h->dZ[cur ^ 1], kHid, nullptr, h->Z[l - 1], nullptr};

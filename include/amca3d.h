@@ -219,8 +219,10 @@ amca_status amca_mlp_train_device(amca_mlp* m, int64_t n, const float* dX, const
                                   float* loss_trace);
 /* Device rows dX[n][5] -> dT[n]. */
 /* The same forward pass as one fused kernel (weights in LDS, activations in
- * registers, the whole network per 32 rows); host pointers, n rows.  Equal to
- * amca_mlp_forward up to FP32 accumulation order. */
+ * registers, the whole network per 64 rows in one wave); host pointers, n rows.
+ * Equal to amca_mlp_forward up to FP32 accumulation order and the BF16
+ * rounding of the hidden activations, which this kernel rounds scaled by
+ * -log2(e) (the scaling cancels in the next layer). */
 amca_status amca_mlp_forward_fused(amca_mlp* h, int32_t n, const float* x5, float* t);
 amca_status amca_mlp_forward_device(amca_mlp* m, int64_t n, const float* dX, float* dT);
 /* The temperature field the surrogate learns: a moving point source on a
