@@ -26,7 +26,7 @@ namespace mlpk {
 
 constexpr int kIn = 5, kHid = 128, kLayers = 5;
 constexpr int kBM = 128, kBN = 128, kBK = 32;
-constexpr int kWgradRows = 64;  // batch rows per work-group of a weight-gradient GEMM
+constexpr int kWgradRows = 128;  // batch rows per work-group of a weight-gradient GEMM
 constexpr int kPad = 8;  // bf16 elements of padding per LDS row
 constexpr int kThreads = 256;
 
