@@ -256,6 +256,9 @@ namespace mlpk {
 hipError_t launch_fused(int mode, const float* p, long long n, const float* X, float* out,
                         const amca_surrogate_desc& d, const amca_scan& s, const SurLattice& g,
                         hipStream_t st);
+hipError_t launch_fused_timed(amca_mlp* h, int mode, long long n, const float* X, float* out,
+                              const amca_surrogate_desc& d, const amca_scan& s,
+                              const SurLattice& g);
 }  // namespace mlpk
 
 using namespace mlpk;
@@ -283,9 +286,16 @@ struct amca_mlp {
   float* setY = nullptr;
   long long set_n = 0;
   std::vector<void*> allocs;
+  // HIP events around the fused kernel's launches (amca_mlp_enable_timing)
+  bool timing = false;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  double fused_ms = 0.0;
+  long long fused_launches = 0, fused_rows = 0;
   ~amca_mlp() {
     if (stream) (void)hipStreamSynchronize(stream);
     for (void* p : allocs) (void)hipFree(p);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -570,10 +580,28 @@ amca_status amca_mlp_forward_fused(amca_mlp* h, int32_t n, const float* x5, floa
   if (!h || n < 1 || !x5 || !t) return mlp_fail(AMCA_ERR_INVALID, "bad arguments");
   MLP_TRY(ensure_cap(h, n));
   MLP_HIP(hipMemcpyAsync(h->X, x5, sizeof(float) * kIn * n, hipMemcpyHostToDevice, h->stream));
-  MLP_HIP(launch_fused(0, h->params, n, h->X, h->T, amca_surrogate_desc{}, amca_scan{},
-                       SurLattice{}, h->stream));
+  MLP_HIP(launch_fused_timed(h, 0, n, h->X, h->T, amca_surrogate_desc{}, amca_scan{},
+                             SurLattice{}));
   MLP_HIP(hipMemcpyAsync(t, h->T, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream));
   MLP_HIP(hipStreamSynchronize(h->stream));
+  return AMCA_OK;
+}
+
+amca_status amca_mlp_enable_timing(amca_mlp* h, int32_t on) {
+  if (!h) return mlp_fail(AMCA_ERR_INVALID, "null handle");
+  for (hipEvent_t& e : h->ev)
+    if (!e) MLP_HIP(hipEventCreate(&e));
+  h->timing = on != 0;
+  h->fused_ms = 0.0;
+  h->fused_launches = h->fused_rows = 0;
+  return AMCA_OK;
+}
+
+amca_status amca_mlp_fused_times(amca_mlp* h, double* ms, int64_t* launches, int64_t* rows) {
+  if (!h || !ms || !launches || !rows) return mlp_fail(AMCA_ERR_INVALID, "bad arguments");
+  *ms = h->fused_ms;
+  *launches = h->fused_launches;
+  *rows = h->fused_rows;
   return AMCA_OK;
 }
 
@@ -1021,6 +1049,26 @@ hipError_t launch_fused(int mode, const float* p, long long n, const float* X, f
   return hipGetLastError();
 }
 
+// launch_fused, bracketed by HIP events on the handle's stream when timing is
+// on (the launch is waited for and its duration added to the totals)
+hipError_t launch_fused_timed(amca_mlp* h, int mode, long long n, const float* X, float* out,
+                              const amca_surrogate_desc& d, const amca_scan& s,
+                              const SurLattice& g) {
+  if (!h->timing) return launch_fused(mode, h->params, n, X, out, d, s, g, h->stream);
+  hipError_t e = hipEventRecord(h->ev[0], h->stream);
+  if (e == hipSuccess) e = launch_fused(mode, h->params, n, X, out, d, s, g, h->stream);
+  if (e == hipSuccess) e = hipEventRecord(h->ev[1], h->stream);
+  if (e == hipSuccess) e = hipEventSynchronize(h->ev[1]);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, h->ev[0], h->ev[1]);
+  if (e == hipSuccess) {
+    h->fused_ms += ms;
+    h->fused_launches++;
+    h->fused_rows += n;
+  }
+  return e;
+}
+
 __global__ void k_eval_points(amca_surrogate_desc d, amca_scan s, int n, const double* __restrict__ p,
                               float* __restrict__ X, double* __restrict__ Tfe) {
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1037,7 +1085,7 @@ const char* surrogate_error() { return g_mlp_error.c_str(); }
 amca_status surrogate_eval_lattice(amca_mlp* h, const amca_surrogate_desc& d, const amca_scan& s,
                                    const SurLattice& g, float* out) {
   const long long total = (long long)g.nxi * g.ny * g.nz;
-  MLP_HIP(launch_fused(1, h->params, total, nullptr, out, d, s, g, h->stream));
+  MLP_HIP(launch_fused_timed(h, 1, total, nullptr, out, d, s, g));
   MLP_HIP(hipStreamSynchronize(h->stream));
   return AMCA_OK;
 }

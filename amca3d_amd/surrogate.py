@@ -37,6 +37,9 @@ SIGNATURES = {
                                         C.c_int32, C.c_float, C.c_uint64, F32P]),
     "amca_mlp_forward_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
     "amca_mlp_forward_fused": (C.c_int, [C.c_void_p, C.c_int32, F32P, F32P]),
+    "amca_mlp_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
+    "amca_mlp_fused_times": (C.c_int, [C.c_void_p, F64P, C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int64)]),
     "amca_mlp_gemm_selftest": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                          F32P, F32P, F32P]),
     "amca_mlp_last_error": (C.c_char_p, []),
@@ -145,6 +148,18 @@ class MLP:
                                          p.ctypes.data_as(F64P), ts.ctypes.data_as(F64P),
                                          tf.ctypes.data_as(F64P)), "eval")
         return ts, tf
+
+    def enable_timing(self, on=True):
+        """HIP events around the fused kernel's launches (resets the totals)."""
+        _check(lib().amca_mlp_enable_timing(self.h, 1 if on else 0), "enable_timing")
+
+    def fused_times(self):
+        """(milliseconds, launches, rows) of the fused kernel since enable_timing."""
+        ms = C.c_double(0.0)
+        launches, rows = C.c_int64(0), C.c_int64(0)
+        _check(lib().amca_mlp_fused_times(self.h, C.byref(ms), C.byref(launches), C.byref(rows)),
+               "fused_times")
+        return ms.value, launches.value, rows.value
 
     def close(self):
         if self.h:

@@ -217,14 +217,20 @@ amca_status amca_mlp_gradients(amca_mlp* m, int32_t n, const float* x5, const fl
 amca_status amca_mlp_train_device(amca_mlp* m, int64_t n, const float* dX, const float* dY,
                                   int32_t batch, int32_t steps, float lr, uint64_t seed,
                                   float* loss_trace);
-/* Device rows dX[n][5] -> dT[n]. */
 /* The same forward pass as one fused kernel (weights in LDS, activations in
  * registers, the whole network per 64 rows in one wave); host pointers, n rows.
  * Equal to amca_mlp_forward up to FP32 accumulation order and the BF16
  * rounding of the hidden activations, which this kernel rounds scaled by
  * -log2(e) (the scaling cancels in the next layer). */
 amca_status amca_mlp_forward_fused(amca_mlp* h, int32_t n, const float* x5, float* t);
+/* Device rows dX[n][5] -> dT[n]. */
 amca_status amca_mlp_forward_device(amca_mlp* m, int64_t n, const float* dX, float* dT);
+/* Device time of the fused kernel's launches (amca_mlp_forward_fused and the
+ * lattice evaluations of amca_remelt_use_surrogate), from HIP events on the
+ * network's stream: off by default; enabling resets the totals (milliseconds,
+ * launches, rows evaluated). */
+amca_status amca_mlp_enable_timing(amca_mlp* m, int32_t on);
+amca_status amca_mlp_fused_times(amca_mlp* m, double* ms, int64_t* launches, int64_t* rows);
 /* The temperature field the surrogate learns: a moving point source on a
  * structured FE mesh (uniform node spacing), Rosenthal nodal values at frames
  * frame_dt apart, interpolated linearly in time and by hex8 in space (the FE

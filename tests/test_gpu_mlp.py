@@ -85,6 +85,25 @@ def test_fused_forward_matches_oracle_and_layered(amca, n):
         np.testing.assert_allclose(tf, ref, rtol=0, atol=2e-2 * (np.abs(ref).max() + 1))
 
 
+def test_fused_timing_counts_launches_and_rows(amca):
+    """HIP-event totals of the fused kernel: one launch per call, its rows,
+    a positive duration; enabling again resets them; off adds nothing."""
+    from amca3d_amd.surrogate import MLP
+    m = MLP(2)
+    X = np.random.default_rng(0).uniform(-1, 1, (5000, 5)).astype(np.float32)
+    m.enable_timing(True)
+    m.forward_fused(X)
+    m.forward_fused(X[:123])
+    ms, launches, rows = m.fused_times()
+    assert launches == 2 and rows == 5123 and ms > 0.0
+    m.enable_timing(True)
+    assert m.fused_times() == (0.0, 0, 0)
+    m.enable_timing(False)
+    m.forward_fused(X)
+    assert m.fused_times() == (0.0, 0, 0)
+    m.close()
+
+
 def test_gradients_match_oracle(amca):
     from amca3d_amd.surrogate import MLP
     m = MLP(1)
