@@ -373,6 +373,36 @@ __device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffe
   }
 }
 
+// One child of emit_children: the capture of parent g's neighbour slot s as
+// grain `id`, written to the wave list at `pos`.
+__device__ __forceinline__ void emit_child(const DevParams& P, const DevBuffers& B, int g, int s,
+                                           int id, int* __restrict__ kids, int pos) {
+  const int2 co = B.gco[g];
+  const int c = co.x;
+  const int o = co.y;
+  const double4 e = B.genv[g];
+  int ci, cj, ck;
+  cell_ijk(P, c, ci, cj, ck);
+  double r[9];
+  const double* rp = B.l2g + 9 * (size_t)o;
+#pragma unroll
+  for (int q = 0; q < 9; q++) r[q] = rp[q];
+  int di, dj, dk;
+  slot_delta(s, di, dj, dk);
+  const int nb = c + P.off[s];
+  const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
+  const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
+  const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
+  double out[4];
+  child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
+  B.genv[id] = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
+  B.gco[id] = make_int2(nb, o);
+  B.gid[nb] = id;
+  B.oid[nb] = o;
+  brick_clear(P, B.bmap, 0, ci + di, cj + dj, ck + dk);
+  kids[pos] = id;
+}
+
 // The list of wave entries is cut into `nb` contiguous chunks (nb <= grid):
 // k_resolve decides every entry and writes per-chunk totals, k_scan turns the
 // totals into absolute output positions, k_emit writes survivors and children.
@@ -454,7 +484,9 @@ __device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& 
                                           int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
+  constexpr int kEmitStage = 1024;
   __shared__ unsigned long long s_sm[kBlock / 64];
+  __shared__ int2 s_stage[kEmitStage];
   const int* list;
   int n;
   wave_input(B, wave, list, n);
@@ -485,15 +517,39 @@ __device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& 
     }
     unsigned long long tot;
     unsigned long long off = run + block_scan_excl(cnt, &tot, s_sm);
+    const int cbase = (int)(run >> 32);  // first child position of this tile
+    const int ntile = (int)(tot >> 32);  // children of this tile
+    int cpos[kItems];                    // first child of each entry, tile-relative
 #pragma unroll
     for (int q = 0; q < kItems; q++) {
       const unsigned cap = r[q] & 0x3ffffffu;
       if (r[q] >> 31) surv[(int)(off & 0xffffffffull)] = g[q];
-      if (cap) {
-        const int pos = (int)(off >> 32);
-        emit_children(P, B, g[q], cap, gbase + pos, kids, pos);
-      }
+      cpos[q] = (int)(off >> 32) - cbase;
       off += ((unsigned long long)__builtin_popcount(cap) << 32) | (r[q] >> 31);
+    }
+    // The tile's children are built one per thread: (parent, slot) pairs are
+    // staged in LDS in child order, kEmitStage at a time, so that the few
+    // capturing entries of a wave do not serialise its lanes.
+    for (int w0 = 0; w0 < ntile; w0 += kEmitStage) {
+#pragma unroll
+      for (int q = 0; q < kItems; q++) {
+        unsigned cap = r[q] & 0x3ffffffu;
+        int j = cpos[q];
+        while (cap) {
+          const int sl = __builtin_ctz(cap);
+          cap &= cap - 1;
+          if (j >= w0 && j < w0 + kEmitStage) s_stage[j - w0] = make_int2(g[q], sl);
+          j++;
+        }
+      }
+      __syncthreads();
+      const int m = min(kEmitStage, ntile - w0);
+      for (int t = threadIdx.x; t < m; t += kBlock) {
+        const int2 it = s_stage[t];
+        const int pos = cbase + w0 + t;
+        emit_child(P, B, it.x, it.y, gbase + pos, kids, pos);
+      }
+      __syncthreads();
     }
     run += tot;
   }
