@@ -796,8 +796,9 @@ __global__ void k_denorm(const float* __restrict__ t, int m, amca_surrogate_desc
 }
 
 // ---------------------------------------------------------------------------
-// Fused inference: the whole network per 32 rows in one wave, weights in LDS,
-// activations in registers.  Layer l computes D'[out][row] = W_l^T act^T with
+// Fused inference: the whole network per 64 rows in one wave (two halves of 32
+// whose MFMAs and SiLU epilogues interleave), weights in LDS, activations in
+// registers.  Layer l computes D'[out][row] = W_l^T act^T with
 // v_mfma_f32_32x32x16_bf16 (A = weights, rows = output features; B =
 // activations, columns = batch rows).  The accumulator of a 32 x 32 tile holds,
 // in lane L and register i, output feature (i/4)*8 + (L/32)*4 + i%4 of batch
@@ -813,8 +814,9 @@ __global__ void k_denorm(const float* __restrict__ t, int m, amca_surrogate_desc
 // silu(z), whose factor cancels against the -log2(e) the next layer's bias
 // carries (y_next = W a' - log2(e) b = -log2(e) z_next).  Layer 1 scales its
 // accumulator once (one FMA with the scaled bias); layer 5 is an MFMA with W5
-// in every row of A and scales its output by -1/log2(e) before adding b5.  The biases of layers 2..4 are loaded into the
-// accumulators before the first MFMA, so those layers add no bias either.
+// in every row of A and scales its output by -1/log2(e) before adding b5.  The
+// biases of layers 2..4 are loaded into the accumulators before the first
+// MFMA, so those layers add no bias either.
 // Inputs and weights round to BF16 as in the layered path; the activations
 // round to BF16 as scaled values, which differ from the layered path's in the
 // last BF16 place.
