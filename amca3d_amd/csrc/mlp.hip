@@ -1127,6 +1127,23 @@ amca_status amca_surrogate_train(amca_mlp* h, int32_t batch, int32_t steps, floa
   return amca_mlp_train_device(h, h->set_n, h->setX, h->setY, batch, steps, lr, seed, loss_trace);
 }
 
+amca_status amca_surrogate_eval_lattice(amca_mlp* h, const amca_surrogate_desc* d,
+                                        const amca_scan* s, int32_t nxi, double xi0, double dxi,
+                                        int32_t ny, double y0, double dy, int32_t nz, double z0,
+                                        double dz, float* out) {
+  if (!h || !d || !s || !out || nxi < 1 || ny < 1 || nz < 1)
+    return mlp_fail(AMCA_ERR_INVALID, "bad arguments");
+  const long long n = (long long)nxi * ny * nz;
+  float* dev = nullptr;
+  MLP_HIP(hipMalloc(&dev, sizeof(float) * n));
+  const SurLattice g{xi0, dxi, nxi, y0, dy, ny, z0, dz, nz};
+  amca_status st = surrogate_eval_lattice(h, *d, *s, g, dev);
+  if (st == AMCA_OK && hipMemcpy(out, dev, sizeof(float) * n, hipMemcpyDeviceToHost) != hipSuccess)
+    st = mlp_fail(AMCA_ERR_HIP, "readback");
+  (void)hipFree(dev);
+  return st;
+}
+
 amca_status amca_surrogate_eval(amca_mlp* h, const amca_surrogate_desc* d, const amca_scan* s,
                                 int32_t n, const double* xyzt, double* Ts, double* Tfe) {
   if (!h || !d || !s || n < 1 || !xyzt || !Ts || !Tfe) return mlp_fail(AMCA_ERR_INVALID, "bad arguments");

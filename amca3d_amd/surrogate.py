@@ -47,6 +47,10 @@ SIGNATURES = {
                                         C.c_int32, C.c_int64, C.c_double, C.c_uint64]),
     "amca_surrogate_train": (C.c_int, [C.c_void_p, C.c_int32, C.c_int32, C.c_float, C.c_uint64,
                                        F32P]),
+    "amca_surrogate_eval_lattice": (C.c_int, [C.c_void_p, C.POINTER(SurrogateDesc), C.POINTER(Scan),
+                                              C.c_int32, C.c_double, C.c_double, C.c_int32,
+                                              C.c_double, C.c_double, C.c_int32, C.c_double,
+                                              C.c_double, F32P]),
     "amca_surrogate_eval": (C.c_int, [C.c_void_p, C.POINTER(SurrogateDesc), C.POINTER(Scan),
                                       C.c_int32, F64P, F64P, F64P]),
 }
@@ -137,6 +141,16 @@ class MLP:
         _check(lib().amca_surrogate_train(self.h, batch, steps, lr, seed,
                                           _p(tr) if trace else None), "train")
         return tr[:steps]
+
+    def eval_lattice(self, desc, scan, nxi, xi0, dxi, ny, y0, dy, nz, z0, dz):
+        """Fused-kernel temperatures (kelvin) on a moving-frame lattice at t = 0,
+        shape (nz, ny, nxi); point (i, j, k) is (x_start + xi0 + i dxi, y0 + j dy, z0 + k dz)."""
+        out = np.empty(int(nxi) * int(ny) * int(nz), np.float32)
+        _check(lib().amca_surrogate_eval_lattice(self.h, C.byref(desc), C.byref(scan), int(nxi),
+                                                 float(xi0), float(dxi), int(ny), float(y0),
+                                                 float(dy), int(nz), float(z0), float(dz), _p(out)),
+               "eval_lattice")
+        return out.reshape(int(nz), int(ny), int(nxi))
 
     def eval(self, desc, scan, xyzt):
         """(surrogate, FE path) temperatures in kelvin at rows (x, y, z, t)."""

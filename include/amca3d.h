@@ -261,6 +261,13 @@ amca_status amca_surrogate_sample(amca_mlp* m, const amca_surrogate_desc* d, con
 /* Adam steps over the training set (amca_mlp_train_device on it). */
 amca_status amca_surrogate_train(amca_mlp* m, int32_t batch, int32_t steps, float lr, uint64_t seed,
                                  float* loss_trace);
+/* The fused kernel over a lattice of the moving frame, as amca_remelt_use_surrogate
+ * builds its tables: out[(k*ny + j)*nxi + i] = surrogate temperature (kelvin) at
+ * (x_start + xi0 + i*dxi, y0 + j*dy, z0 + k*dz, t = 0) of scan s.  Host output. */
+amca_status amca_surrogate_eval_lattice(amca_mlp* m, const amca_surrogate_desc* d,
+                                        const amca_scan* s, int32_t nxi, double xi0, double dxi,
+                                        int32_t ny, double y0, double dy, int32_t nz, double z0,
+                                        double dz, float* out);
 /* Surrogate and FE-path temperatures (kelvin) at n host points xyzt[n][4] of one scan. */
 amca_status amca_surrogate_eval(amca_mlp* m, const amca_surrogate_desc* d, const amca_scan* s,
                                 int32_t n, const double* xyzt, double* T_surrogate, double* T_fe);

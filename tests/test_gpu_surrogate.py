@@ -3,10 +3,16 @@ coupled path's own temperatures, training on the GPU reduces the error by
 orders of magnitude, and a surrogate-driven track stays close to the
 FE-driven one (temperature and grain IDs) while the surrogate serves most
 cells."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import oracle_lib as ol
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import mlp_oracle as mo  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
@@ -45,6 +51,48 @@ def test_samples_are_the_fe_path(amca, oracle, small_track):
     _, tfe = m.eval(pbf.surrogate_desc(w), pbf.scan(w), pts[pick])
     np.testing.assert_allclose(tfe, r["T"][c[pick]], rtol=1e-9)
     m.close()
+
+
+def test_lattice_evaluation_matches_the_fused_rows_and_the_oracle(amca, small_track):
+    """The fused kernel's lattice mode (inputs computed in the kernel from the
+    lattice indices, output in kelvin) equals the fused kernel on the same
+    inputs computed here ((x - x_start)/xi_scale, (y - y_c)/y_scale,
+    (z - z_top)/z_scale, P/P_scale, v/v_scale at t = 0), and its own
+    rounding-point oracle, compared in the network's target space.  The
+    layered kernels, which round the activations unscaled, agree within the
+    BF16 rounding of the activations.  A swapped or shifted lattice index
+    would differ by the whole field."""
+    from amca3d_amd import pbf
+    from amca3d_amd.surrogate import MLP
+    w = small_track
+    d = pbf.surrogate_desc(w)
+    s = pbf.scan(w)
+    h = w["grid"]["h"]
+    nxi, ny, nz = 61, 9, 7
+    xi0, y0, z0 = -4e-4, s.y_c - 4 * h, w["grid"]["nz"] * h - 6 * h
+    m = MLP(4)
+    tab = m.eval_lattice(d, s, nxi, xi0, h, ny, y0, h, nz, z0, h).ravel().astype(np.float64)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nxi), indexing="ij")
+    x = s.x_start + xi0 + i.ravel() * h
+    y = y0 + j.ravel() * h
+    z = z0 + k.ravel() * h
+    X = np.column_stack([(x - s.x_start) / d.xi_scale, (y - s.y_c) / d.y_scale,
+                         (z - d.z_top) / d.z_scale, np.full(x.size, s.P / d.P_scale),
+                         np.full(x.size, s.v / d.v_scale)]).astype(np.float32)
+    rows = m.forward_fused(X).astype(np.float64)
+    ref = mo.forward_scaled(m.params().astype(np.float64), X)
+    ts, _ = m.eval(d, s, np.column_stack([x, y, z, np.zeros(x.size)]))
+    m.close()
+    assert d.log_target == 1
+    lt = np.log1p((tab - d.T_ref) / d.T_scale)  # back to the network output
+    assert np.std(lt) > 0.05                    # the field varies over the lattice
+    np.testing.assert_allclose(lt, rows, rtol=0, atol=1e-5)
+    assert np.median(np.abs(lt - rows)) < 1e-6
+    sc = np.abs(ref).max() + 1
+    np.testing.assert_allclose(lt, ref, rtol=0, atol=2e-2 * sc)
+    assert np.median(np.abs(lt - ref)) < 1e-5 * sc
+    # layered kernels at the same points: within BF16 rounding of the activations
+    assert np.max(np.abs(lt - np.log1p((ts - d.T_ref) / d.T_scale))) < 5e-2 * sc
 
 
 def test_training_and_surrogate_driven_track(amca, small_track):
