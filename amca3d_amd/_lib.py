@@ -91,6 +91,10 @@ SIGNATURES = {
     "amca_sim_domain": (C.c_void_p, [C.c_void_p]),
     "amca_sim_remelt": (C.c_void_p, [C.c_void_p]),
     "amca_sim_finalize": (C.c_int, [C.c_void_p]),
+    "amca_write_final_grains": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32,
+                                          C.c_int32, C.c_double, C.c_double, C.c_double,
+                                          C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,
+                                          C.c_int32]),
     "amca_sim_get_info": (C.c_int, [C.c_void_p, C.POINTER(SimInfo)]),
     "amca_sim_initial_sites": (C.c_int, [C.c_void_p, C.c_int32, I32P, F64P, F64P, I32P]),
     "amca_sim_destroy": (C.c_int, [C.c_void_p]),

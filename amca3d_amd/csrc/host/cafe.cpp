@@ -1,5 +1,7 @@
 #include "cafe.h"
 
+#include "final_grains.h"
+
 #include <cmath>
 #include <stdexcept>
 
@@ -164,6 +166,24 @@ amca_remelt_counters CafeRealm::run(long long max_steps) {
   amca_remelt_counters c{};
   check(amca_remelt_run(dom_, max_steps, &c), "amca_remelt_run");
   return c;
+}
+
+void CafeRealm::write_final_grains(const std::string& vtk, const std::string& result) {
+  if (!dom_) return;
+  Grid g{ca_.ncell[0], ca_.ncell[1], ca_.ncell[2]};
+  std::vector<int32_t> ori((size_t)g.NG()), melted((size_t)g.NG());
+  std::vector<uint8_t> m8((size_t)g.NG());
+  int32_t ng = 0;
+  check(amca_remelt_download(dom_, ori.data(), nullptr, nullptr, m8.data(), nullptr, 0, nullptr,
+                             nullptr, nullptr, &ng),
+        "download");
+  for (size_t c = 0; c < m8.size(); c++) melted[c] = m8[c];
+  int32_t no = 0;
+  check(amca_remelt_euler(dom_, 0, nullptr, &no), "euler");
+  std::vector<double> eu(3 * (size_t)std::max(no, 1));
+  check(amca_remelt_euler(dom_, no, eu.data(), &no), "euler");
+  amca_host::write_final_grains(vtk, result, g.nx, g.ny, g.nz, ca_.origin[0], ca_.origin[1],
+                                ca_.origin[2], ca_.h, ori.data(), melted.data(), eu.data(), no);
 }
 
 void CafeRealm::finalize() {

@@ -28,6 +28,9 @@ class CafeRealm {
   amca_remelt_counters run(long long max_steps);
   // finalize: output_microstructure_information when the deck asks for it
   void finalize();
+  // finalGrains.vtk and finalResult.txt of the current state (output_orientations,
+  // CellularAutomataManager.cpp:462-465, with hasBeenMelted_); empty paths skip
+  void write_final_grains(const std::string& vtk, const std::string& result);
   amca_remelt* domain() const { return dom_; }
   const RealmConfig& ca() const { return ca_; }
   const FeFrames& frames() const { return frames_; }

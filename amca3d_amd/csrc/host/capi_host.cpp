@@ -2,16 +2,19 @@
 // layer of the reference (Simulation.cpp:36-93, CellularAutomataSolver.cpp:40-202)
 // over the device domain.
 #include <cstring>
+#include <filesystem>
 #include <memory>
 #include <stdexcept>
 #include <string>
 
 #include "../../../include/amca3d.h"
 #include "cafe.h"
+#include "final_grains.h"
 #include "realm.h"
 
 struct amca_sim {
   amca_host::SimulationConfig cfg;
+  std::string base;  // the deck's folder (empty: the working directory)
   std::unique_ptr<amca_host::PlainRealm> plain;
   std::unique_ptr<amca_host::CafeRealm> cafe;  // finite_element + cellular_automata_remelting
   bool initialized = false;
@@ -41,6 +44,7 @@ amca_status create_from_doc(const amca_host::yaml::Node& doc, const std::string&
                             amca_sim** out) {
   auto s = std::make_unique<amca_sim>();
   s->cfg = amca_host::load_simulation(doc);
+  s->base = base;
   if (s->cfg.realms.empty()) return host_fail(AMCA_ERR_INVALID, "no cellular automata realm");
   const auto& r = s->cfg.realms[0];
   if (s->cfg.finite_element_method) {
@@ -130,6 +134,35 @@ amca_status amca_sim_finalize(amca_sim* s) {
   if (!s || !s->initialized) return host_fail(AMCA_ERR_INVALID, "simulation not initialized");
   return guarded([&] {
     if (s->cafe) s->cafe->finalize();
+    // finalGrains.vtk next to the deck; finalResult.txt in the folder of the
+    // output base name (VtrManager.cpp:156-169 takes everything before the last
+    // '/', or the whole name when there is none)
+    const auto& r = s->cfg.realms[0];
+    auto join = [](const std::string& a, const std::string& b) {
+      return (a.empty() || (!b.empty() && b[0] == '/')) ? b : a + "/" + b;
+    };
+    const size_t k = r.output_base.find_last_of('/');
+    const std::string dir = join(s->base, k == std::string::npos ? r.output_base
+                                                                 : r.output_base.substr(0, k));
+    std::filesystem::create_directories(dir);
+    const std::string vtk = join(s->base, "finalGrains.vtk"), res = dir + "/finalResult.txt";
+    if (s->cafe)
+      s->cafe->write_final_grains(vtk, res);
+    else
+      s->plain->write_final_grains(vtk, res);
+    return AMCA_OK;
+  });
+}
+
+amca_status amca_write_final_grains(const char* vtk_path, const char* result_path, int32_t nx,
+                                    int32_t ny, int32_t nz, double x0, double y0, double z0,
+                                    double h, const int32_t* orient, const int32_t* melted,
+                                    const double* euler3, int32_t n_orient) {
+  if (!orient || nx < 1 || ny < 1 || nz < 1 || n_orient < 0)
+    return host_fail(AMCA_ERR_INVALID, "bad arguments");
+  return guarded([&] {
+    amca_host::write_final_grains(vtk_path ? vtk_path : "", result_path ? result_path : "", nx,
+                                  ny, nz, x0, y0, z0, h, orient, melted, euler3, n_orient);
     return AMCA_OK;
   });
 }

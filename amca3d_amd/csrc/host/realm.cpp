@@ -1,5 +1,7 @@
 #include "realm.h"
 
+#include "final_grains.h"
+
 #include <cmath>
 #include <stdexcept>
 #include <unordered_map>
@@ -64,6 +66,10 @@ void load_solution_options(const Node& realm, RealmConfig& c) {
       get_if_present(p, "shuffer_vtr_grain_ID", shuffle, shuffle);
       if (shuffle)
         throw std::runtime_error("random_type-shuffer_vtr_grain_ID: yes is not supported");
+      // the finalGrains.vtk ids shuffle the same way (:277, writeToFile :1424-1431)
+      get_if_present(p, "shuffer_vtk_grain_ID", shuffle, shuffle);
+      if (shuffle)
+        throw std::runtime_error("random_type-shuffer_vtk_grain_ID: yes is not supported");
     } else if (expect_map(o, "phase_state", true)) {
       const Node& p = *o.FindValue("phase_state");
       get_if_present(p, "has_been_melted", c.initial_melted, c.initial_melted);
@@ -337,6 +343,18 @@ amca_counters PlainRealm::run(long long max_steps) {
   if (amca_domain_run(dom_, max_steps, &c) != AMCA_OK)
     throw std::runtime_error(std::string("amca_domain_run: ") + amca_last_error());
   return c;
+}
+
+void PlainRealm::write_final_grains(const std::string& vtk, const std::string& result) {
+  if (!dom_) return;
+  std::vector<int32_t> ori((size_t)grid_.NG());
+  if (amca_domain_download(dom_, ori.data(), nullptr, nullptr, 0, nullptr, nullptr, nullptr) !=
+      AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_download: ") + amca_last_error());
+  // a realm without remelting writes every melted flag as 1 (hasBeenMelted_ empty)
+  amca_host::write_final_grains(vtk, result, grid_.nx, grid_.ny, grid_.nz, cfg_.origin[0],
+                                cfg_.origin[1], cfg_.origin[2], cfg_.h, ori.data(), nullptr,
+                                euler_.data(), (int32_t)(euler_.size() / 3));
 }
 
 }  // namespace amca_host

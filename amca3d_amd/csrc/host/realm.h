@@ -110,6 +110,9 @@ class PlainRealm {
   void initialize();
   // CellularAutomataManager::time_integration (CellularAutomataManager.cpp:134-147)
   amca_counters run(long long max_steps);
+  // finalGrains.vtk and finalResult.txt of the current state (output_orientations,
+  // CellularAutomataManager.cpp:462-465); an empty path skips that file
+  void write_final_grains(const std::string& vtk, const std::string& result);
   amca_domain* domain() const { return dom_; }
   const Grid& grid() const { return grid_; }
   const std::vector<int>& initial_sites() const { return site_cells_; }

@@ -220,6 +220,25 @@ class Simulation:
             pass
 
 
+def write_final_grains(vtk_path, result_path, nx, ny, nz, x0, y0, z0, h, orient, melted,
+                       euler3):
+    """finalGrains.vtk / finalResult.txt of a state (amca_write_final_grains):
+    orient and melted ghost-inclusive, melted None = every flag 1, euler3
+    (n_orient, 3).  A None path skips that file."""
+    ori = np.ascontiguousarray(orient, np.int32).ravel()
+    mel = None if melted is None else np.ascontiguousarray(melted, np.int32).ravel()
+    eu = np.ascontiguousarray(euler3, np.float64).reshape(-1)
+    ng = (nx + 2) * (ny + 2) * (nz + 2)
+    if ori.size != ng or (mel is not None and mel.size != ng):
+        raise ValueError("orient / melted must hold (nx+2)(ny+2)(nz+2) cells")
+    enc = (lambda p: None if p is None else str(p).encode())
+    check(load().amca_write_final_grains(enc(vtk_path), enc(result_path), int(nx), int(ny),
+                                         int(nz), float(x0), float(y0), float(z0), float(h),
+                                         ori.ctypes.data, None if mel is None else mel.ctypes.data,
+                                         eu.ctypes.data, eu.size // 3),
+          "write_final_grains", sim=True)
+
+
 def envelope_selftest(env4, l2g9, cell3, h, trunc):
     """Device capture test + child envelope on arrays (for parity checks)."""
     env4 = np.ascontiguousarray(env4, np.float64)

@@ -504,9 +504,24 @@ amca_status amca_sim_run(amca_sim* s, int64_t max_steps, amca_counters* out);
 amca_domain* amca_sim_domain(amca_sim* s);
 /* The coupled domain of a `finite_element_method` run (NULL for plain runs). */
 amca_remelt* amca_sim_remelt(amca_sim* s);
-/* CafeSolver::finalize: writes the output_microstructure_information file
- * (rank suffix "0.txt") when the deck asks for it. */
+/* The end of CellularAutomataSolver / CafeSolver::execute
+ * (CellularAutomataSolver.cpp:197-200, CafeSolver.cpp:252-260): writes the
+ * output_microstructure_information file (rank suffix "0.txt") when the deck
+ * asks for it, then finalGrains.vtk (in the deck's folder, or the working
+ * directory for a deck given as text) and finalResult.txt (in the folder of
+ * output_data_base_name, created if missing).  The VTR time series is not
+ * written. */
 amca_status amca_sim_finalize(amca_sim* s);
+/* CellularAutomataManager_3D::writeToFile (CellularAutomataManager_3D.cpp:1381-1639):
+ * finalGrains.vtk (legacy ASCII rectilinear grid: grain_id, number_cells,
+ * melted_flag, EulerAngle) and finalResult.txt (one row per cell) from host
+ * arrays.  orient[ng] and melted[ng] are ghost-inclusive (ng = (nx+2)(ny+2)(nz+2),
+ * x fastest; melted may be null: every flag 1); euler3 holds 3 angles per
+ * orientation id.  A null path skips that file.  No GPU is used. */
+amca_status amca_write_final_grains(const char* vtk_path, const char* result_path, int32_t nx,
+                                    int32_t ny, int32_t nz, double x0, double y0, double z0,
+                                    double h, const int32_t* orient, const int32_t* melted,
+                                    const double* euler3, int32_t n_orient);
 amca_status amca_sim_get_info(amca_sim* s, amca_sim_info* info);
 /* The sites and orientation rows drawn at initialisation (for checking). */
 amca_status amca_sim_initial_sites(amca_sim* s, int32_t cap, int32_t* cell, double* dT_crit,

@@ -8,11 +8,17 @@ the microstructure information file.  The oracle runs the same case from the
 frames parsed here independently, and the two must agree bit for bit; the
 written file must hold the device state and, loaded by a second deck, must
 give the same run as the oracle started from that state."""
+import os
+import sys
+
 import numpy as np
 import pytest
 
 import oracle_lib as ol
 import remelt_cases as rc
+
+sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
+import final_grains as fg  # noqa: E402
 
 pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("oracle")]
 
@@ -66,7 +72,24 @@ def run_deck(deck_path):
                sites=d.sites(), info=s.info())
     s.finalize()
     s.close()
+    # end-of-run grain files: next to the deck, and in the folder of the
+    # realm's output_data_base_name (./Results/Grains)
+    folder = os.path.dirname(deck_path)
+    with open(os.path.join(folder, "finalGrains.vtk")) as f:
+        vtk = f.read()
+    with open(os.path.join(folder, "Results", "finalResult.txt")) as f:
+        res = f.read()
+    out["final_files"] = (vtk, res)
     return out
+
+
+def check_final_files(dev, case):
+    g = case["grid"]
+    want = fg.final_grains_text(g["nx"], g["ny"], g["nz"], g["x0"], g["y0"], g["z0"], g["h"],
+                                dev["ori"], dev["melted"].astype(np.int32),
+                                np.asarray(dev["euler"]).reshape(-1, 3))
+    assert dev["final_files"][0] == want[0]
+    assert dev["final_files"][1] == want[1]
 
 
 def compare(orc, dev):
@@ -119,6 +142,7 @@ def test_deck_with_random_substrate_matches_the_oracle(amca, cafe_run):
     np.testing.assert_array_equal(cell, orc["site_cell"])
     np.testing.assert_array_equal(bits(dt), bits(orc["site_dt"]))
     compare(orc, dev)
+    check_final_files(dev, cafe_run["case"])
 
 
 def test_microstructure_file_holds_the_device_state(amca, cafe_run):
@@ -162,6 +186,7 @@ def test_deck_loading_the_written_file(amca, cafe_run, seeds):
     orc = ol.run_remelt(ocase)
     assert orc["steps"]["n_captured"].sum() > 0
     compare(orc, dev)
+    check_final_files(dev, case)
     out = rc.read_micro_file(tmp / f"grainInfo_{seeds}0.txt")
     if drawn:
         n_loaded = int((m["nuc_dt"] > -1).sum())

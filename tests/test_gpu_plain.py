@@ -105,6 +105,32 @@ def test_device_envelope_matches_oracle(amca, oracle):
     assert np.array_equal(gc, oc)
 
 
+def test_final_grain_files_of_a_deck_run(amca, oracle, tmp_path):
+    """A plain deck run from its file and finalised writes finalGrains.vtk next
+    to the deck and Results/finalResult.txt (output_data_base_name
+    ./Results/Grains): both equal the oracle's writer applied to the oracle's
+    own run of the case."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import final_grains as fg
+    from amca3d_amd.sim import Simulation
+    text, prm = case(**SMALL[1])
+    deck = tmp_path / "deck.yaml"
+    deck.write_text(text)
+    s = Simulation.from_file(str(deck))
+    s.initialize()
+    s.run(-1)
+    s.finalize()
+    s.close()
+    r = oracle.run_plain(oracle.plain_params(**prm))
+    vtk, res = fg.final_grains_text(prm["nx"], prm["ny"], prm["nz"], 0.0, 0.0, 0.0, prm["h"],
+                                    r["ori"], None, r["euler"])
+    assert (tmp_path / "finalGrains.vtk").read_text() == vtk
+    assert (tmp_path / "Results" / "finalResult.txt").read_text() == res
+
+
 def test_nucleation_example_parity(amca, oracle):
     """The reference's nucleation example deck (60^3, 270 bulk draws) to full
     solidification."""
