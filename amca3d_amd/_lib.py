@@ -91,6 +91,7 @@ SIGNATURES = {
     "amca_sim_domain": (C.c_void_p, [C.c_void_p]),
     "amca_sim_remelt": (C.c_void_p, [C.c_void_p]),
     "amca_sim_finalize": (C.c_int, [C.c_void_p]),
+    "amca_sim_enable_vtr": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_write_final_grains": (C.c_int, [C.c_char_p, C.c_char_p, C.c_int32, C.c_int32,
                                           C.c_int32, C.c_double, C.c_double, C.c_double,
                                           C.c_double, C.c_void_p, C.c_void_p, C.c_void_p,

@@ -1,7 +1,9 @@
 #include "cafe.h"
 
 #include "final_grains.h"
+#include "vtr_series.h"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 
@@ -164,8 +166,58 @@ void CafeRealm::initialize() {
 amca_remelt_counters CafeRealm::run(long long max_steps) {
   if (!dom_) initialize();
   amca_remelt_counters c{};
-  check(amca_remelt_run(dom_, max_steps, &c), "amca_remelt_run");
+  if (!vtr_) {
+    check(amca_remelt_run(dom_, max_steps, &c), "amca_remelt_run");
+    return c;
+  }
+  // VtrManager::execute after every step (CellularAutomataManager_3DRemelting.cpp:640)
+  check(amca_remelt_get_counters(dom_, &c), "counters");
+  long long taken = 0;
+  while (!c.done && (max_steps < 0 || taken < max_steps)) {
+    long long chunk = vtr_->steps_to_next(c.step);
+    if (chunk == 0) chunk = -1;  // no frame can come during this run
+    if (max_steps >= 0) chunk = chunk < 0 ? max_steps - taken : std::min(chunk, max_steps - taken);
+    const long long s0 = c.step;
+    check(amca_remelt_run(dom_, chunk, &c), "amca_remelt_run");
+    taken += c.step - s0;
+    if (c.step == s0) break;
+    if (vtr_->due(c.step, c.time)) vtr_frame(c.time);
+  }
   return c;
+}
+
+void CafeRealm::enable_vtr(const std::string& base_dir, bool on) {
+  vtr_.reset();
+  if (on) vtr_ = std::make_unique<VtrSeries>(ca_, base_dir, true);
+}
+
+void CafeRealm::vtr_frame(double time) {
+  Grid g{ca_.ncell[0], ca_.ncell[1], ca_.ncell[2]};
+  std::vector<int32_t> ori((size_t)g.NG());
+  std::vector<double> T((size_t)g.NG());
+  std::vector<uint8_t> melted((size_t)g.NG());
+  int32_t ng = 0;
+  check(amca_remelt_download(dom_, ori.data(), nullptr, T.data(), melted.data(), nullptr, 0,
+                             nullptr, nullptr, nullptr, &ng),
+        "download");
+  int32_t no = 0;
+  check(amca_remelt_euler(dom_, 0, nullptr, &no), "euler");
+  std::vector<double> eu(3 * (size_t)std::max(no, 1));
+  check(amca_remelt_euler(dom_, no, eu.data(), &no), "euler");
+  VtrFields f;
+  f.orient = ori.data();
+  f.T = T.data();
+  f.melted = melted.data();
+  f.euler3 = eu.data();
+  f.n_orient = no;
+  vtr_->write(time, f);
+}
+
+void CafeRealm::vtr_final_frame() {
+  if (!vtr_ || !dom_) return;
+  amca_remelt_counters c{};
+  check(amca_remelt_get_counters(dom_, &c), "counters");
+  if (vtr_->due(0, c.time)) vtr_frame(c.time);  // output_last_step_result
 }
 
 void CafeRealm::write_final_grains(const std::string& vtk, const std::string& result) {

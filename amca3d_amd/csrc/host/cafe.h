@@ -31,6 +31,9 @@ class CafeRealm {
   // finalGrains.vtk and finalResult.txt of the current state (output_orientations,
   // CellularAutomataManager.cpp:462-465, with hasBeenMelted_); empty paths skip
   void write_final_grains(const std::string& vtk, const std::string& result);
+  // the VTR time series of the CA realm's output block (off until enabled)
+  void enable_vtr(const std::string& base_dir, bool on);
+  void vtr_final_frame();
   amca_remelt* domain() const { return dom_; }
   const RealmConfig& ca() const { return ca_; }
   const FeFrames& frames() const { return frames_; }
@@ -51,6 +54,8 @@ class CafeRealm {
   std::vector<double> loaded_seed_dts_;
   int n_surface_new_ = 0, n_bulk_new_ = 0;
   amca_remelt* dom_ = nullptr;
+  std::unique_ptr<VtrSeries> vtr_;
+  void vtr_frame(double time);
   bool prepared_ = false;
 };
 

@@ -134,6 +134,11 @@ amca_status amca_sim_finalize(amca_sim* s) {
   if (!s || !s->initialized) return host_fail(AMCA_ERR_INVALID, "simulation not initialized");
   return guarded([&] {
     if (s->cafe) s->cafe->finalize();
+    // the last VTR frame (output_last_step_result), then the end-of-run files
+    if (s->cafe)
+      s->cafe->vtr_final_frame();
+    else
+      s->plain->vtr_final_frame();
     // finalGrains.vtk next to the deck; finalResult.txt in the folder of the
     // output base name (VtrManager.cpp:156-169 takes everything before the last
     // '/', or the whole name when there is none)
@@ -150,6 +155,17 @@ amca_status amca_sim_finalize(amca_sim* s) {
       s->cafe->write_final_grains(vtk, res);
     else
       s->plain->write_final_grains(vtk, res);
+    return AMCA_OK;
+  });
+}
+
+amca_status amca_sim_enable_vtr(amca_sim* s, int32_t on) {
+  if (!s) return host_fail(AMCA_ERR_INVALID, "null simulation");
+  return guarded([&] {
+    if (s->cafe)
+      s->cafe->enable_vtr(s->base, on != 0);
+    else
+      s->plain->enable_vtr(s->base, on != 0);
     return AMCA_OK;
   });
 }

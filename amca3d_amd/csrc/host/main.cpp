@@ -40,6 +40,12 @@ int main(int argc, char** argv) {
     std::fprintf(stderr, "error: %s\n", amca_sim_last_error());
     return 1;
   }
+  // the reference's VTR time series, as its solver writes it
+  if (amca_sim_enable_vtr(sim, 1) != AMCA_OK) {
+    std::fprintf(stderr, "error: %s\n", amca_sim_last_error());
+    amca_sim_destroy(sim);
+    return 1;
+  }
   auto t0 = std::chrono::steady_clock::now();
   if (amca_sim_initialize(sim) != AMCA_OK) {
     std::fprintf(stderr, "error: %s\n", amca_sim_last_error());

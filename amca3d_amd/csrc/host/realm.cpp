@@ -1,7 +1,9 @@
 #include "realm.h"
 
 #include "final_grains.h"
+#include "vtr_series.h"
 
+#include <algorithm>
 #include <cmath>
 #include <stdexcept>
 #include <unordered_map>
@@ -340,9 +342,55 @@ void PlainRealm::initialize() {
 
 amca_counters PlainRealm::run(long long max_steps) {
   amca_counters c{};
-  if (amca_domain_run(dom_, max_steps, &c) != AMCA_OK)
-    throw std::runtime_error(std::string("amca_domain_run: ") + amca_last_error());
+  auto steps = [&](long long n) {
+    if (amca_domain_run(dom_, n, &c) != AMCA_OK)
+      throw std::runtime_error(std::string("amca_domain_run: ") + amca_last_error());
+  };
+  if (!vtr_) {
+    steps(max_steps);
+    return c;
+  }
+  // VtrManager::execute after every step (CellularAutomataManager_3D.cpp:661):
+  // run to the next step that can take a frame, then ask
+  if (amca_domain_counters(dom_, &c) != AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_counters: ") + amca_last_error());
+  long long taken = 0;
+  while (!c.done && (max_steps < 0 || taken < max_steps)) {
+    long long chunk = vtr_->steps_to_next(c.step);
+    if (chunk == 0) chunk = -1;  // no frame can come during this run
+    if (max_steps >= 0) chunk = chunk < 0 ? max_steps - taken : std::min(chunk, max_steps - taken);
+    const long long s0 = c.step;
+    steps(chunk);
+    taken += c.step - s0;
+    if (c.step == s0) break;
+    if (vtr_->due(c.step, c.time)) vtr_frame(c.time);
+  }
   return c;
+}
+
+void PlainRealm::enable_vtr(const std::string& base_dir, bool on) {
+  vtr_.reset();
+  if (on) vtr_ = std::make_unique<VtrSeries>(cfg_, base_dir, false);
+}
+
+void PlainRealm::vtr_frame(double time) {
+  std::vector<int32_t> ori((size_t)grid_.NG());
+  if (amca_domain_download(dom_, ori.data(), nullptr, nullptr, 0, nullptr, nullptr, nullptr) !=
+      AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_download: ") + amca_last_error());
+  VtrFields f;
+  f.orient = ori.data();
+  f.euler3 = euler_.data();
+  f.n_orient = (int32_t)(euler_.size() / 3);
+  vtr_->write(time, f);
+}
+
+void PlainRealm::vtr_final_frame() {
+  if (!vtr_ || !dom_) return;
+  amca_counters c{};
+  if (amca_domain_counters(dom_, &c) != AMCA_OK)
+    throw std::runtime_error(std::string("amca_domain_counters: ") + amca_last_error());
+  if (vtr_->due(0, c.time)) vtr_frame(c.time);  // execute(0, time_) (:1990-1993)
 }
 
 void PlainRealm::write_final_grains(const std::string& vtk, const std::string& result) {

@@ -100,6 +100,8 @@ SimulationConfig load_simulation(const yaml::Node& doc);
 RealmConfig load_ca_realm(const yaml::Node& realm);
 
 // A plain `cellular_automata` realm on the GPU.
+class VtrSeries;
+
 class PlainRealm {
  public:
   explicit PlainRealm(const RealmConfig& cfg, const TimeIntegratorConfig& ti);
@@ -113,6 +115,10 @@ class PlainRealm {
   // finalGrains.vtk and finalResult.txt of the current state (output_orientations,
   // CellularAutomataManager.cpp:462-465); an empty path skips that file
   void write_final_grains(const std::string& vtk, const std::string& result);
+  // the VTR time series of the output block (off until enabled): frames after
+  // the steps it names during run(), and the last frame (output_last_step_result)
+  void enable_vtr(const std::string& base_dir, bool on);
+  void vtr_final_frame();
   amca_domain* domain() const { return dom_; }
   const Grid& grid() const { return grid_; }
   const std::vector<int>& initial_sites() const { return site_cells_; }
@@ -127,6 +133,8 @@ class PlainRealm {
   TimeIntegratorConfig ti_;
   Grid grid_{0, 0, 0};
   amca_domain* dom_ = nullptr;
+  std::unique_ptr<VtrSeries> vtr_;
+  void vtr_frame(double time);
   std::vector<int> site_cells_;
   std::vector<double> site_dt_;
   std::vector<double> euler_;

@@ -187,6 +187,12 @@ class Simulation:
             raise AmcaError("not a coupled simulation, or not initialized")
         return RemeltDomain.view(h, (inf["nx"], inf["ny"], inf["nz"]), inf["fe_nodes"])
 
+    def enable_vtr(self, on=True):
+        """The output block's VTR time series during run() and at finalize()
+        (off until enabled)."""
+        check(load().amca_sim_enable_vtr(self._h, 1 if on else 0), "sim_enable_vtr", sim=True)
+        return self
+
     def finalize(self):
         check(load().amca_sim_finalize(self._h), "sim_finalize", sim=True)
 

@@ -512,6 +512,14 @@ amca_remelt* amca_sim_remelt(amca_sim* s);
  * output_data_base_name, created if missing).  The VTR time series is not
  * written. */
 amca_status amca_sim_finalize(amca_sim* s);
+/* The VTR time series of the realm's output block (VtrManager, VtrManager.cpp:41-561;
+ * off until enabled, as a library call should not scatter folders): with it on,
+ * amca_sim_run writes a frame after every step that output_frequency (or,
+ * below 1, output_time_interval) names, and amca_sim_finalize the last one --
+ * <dir>/grainsTimeStates/timeStep_NNNNN/<name>00000.vtr and <name>NNNNN.pvtr,
+ * and <dir>/<name>.pvd, where <dir>/<name> is output_data_base_name (relative
+ * to the deck's folder).  The amca3d driver turns it on. */
+amca_status amca_sim_enable_vtr(amca_sim* s, int32_t on);
 /* CellularAutomataManager_3D::writeToFile (CellularAutomataManager_3D.cpp:1381-1639):
  * finalGrains.vtk (legacy ASCII rectilinear grid: grain_id, number_cells,
  * melted_flag, EulerAngle) and finalResult.txt (one row per cell) from host

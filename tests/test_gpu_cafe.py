@@ -19,6 +19,7 @@ import remelt_cases as rc
 
 sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle"))
 import final_grains as fg  # noqa: E402
+import vtr_series as vs  # noqa: E402
 
 pytestmark = [pytest.mark.gpu, pytest.mark.usefixtures("oracle")]
 
@@ -200,3 +201,45 @@ def test_deck_loading_the_written_file(amca, cafe_run, seeds):
         assert orc["steps"]["n_nucleated"].sum() == 0
         np.testing.assert_array_equal(out["nuc_dt"], np.full(len(m["gid"]), -1.0))
         assert out["n_expected"] == m["n_expected"]
+
+
+def test_vtr_series_of_a_coupled_deck(amca, cafe_run):
+    """The CA realm's VTR time series (enabled) on deck 1's case: every frame
+    (cell temperature truncated to int, ids, Euler angles, melted flags, and
+    the debug variable's .pvtr-only declarations) byte-equal to the oracle's
+    restatement applied to the oracle's run stopped at that step."""
+    from amca3d_amd.sim import Simulation
+    tmp, case = cafe_run["tmp"], cafe_run["case"]
+    n_steps = cafe_run["orc1"]["n_steps"]
+    freq = n_steps // 3 + 1
+    names = ["temperature", "grain_ID", "grain_orientation", "melted_flag", "debug"]
+    deck = rc.cafe_deck(case, mesh=str(tmp / "frames.txt"))
+    old = "      output_frequency: -1\n      output_variables:\n        - grain_orientation"
+    assert old in deck
+    deck = deck.replace(old, f"      output_frequency: {freq}\n      output_variables:\n" +
+                        "\n".join(f"        - {v}" for v in names))
+    sub = tmp / "vtr"
+    sub.mkdir()
+    (sub / "deck.yaml").write_text(deck)
+    s = Simulation.from_file(str(sub / "deck.yaml"))
+    s.enable_vtr(True)
+    s.initialize()
+    s.run()
+    s.finalize()
+    s.close()
+    g = case["grid"]
+    ids = vs.variable_ids(names)
+    steps = list(range(freq, n_steps + 1, freq)) + [n_steps]
+    times = []
+    for c, st in enumerate(steps):
+        orc = ol.run_remelt(dict(cafe_run["ocase"], max_steps=st))
+        assert orc["n_steps"] == st
+        times.append(orc["time"])
+        frame = sub / "Results" / "grainsTimeStates" / f"timeStep_{c:05d}"
+        want = vs.vtr_text(g["nx"], g["ny"], g["nz"], g["x0"], g["y0"], g["z0"], g["h"], ids,
+                           orc["ori"], orc["euler"][:orc["n_orient"]], T=orc["T"],
+                           melted=orc["melted"])
+        assert (frame / "Grains00000.vtr").read_text() == want, c
+        assert (frame / f"Grains{c:05d}.pvtr").read_text() == \
+            vs.pvtr_text(g["nx"], g["ny"], g["nz"], ids, "Grains")
+    assert (sub / "Results" / "Grains.pvd").read_text() == vs.pvd_text(times, "Grains")

@@ -131,6 +131,53 @@ def test_final_grain_files_of_a_deck_run(amca, oracle, tmp_path):
     assert (tmp_path / "Results" / "finalResult.txt").read_text() == res
 
 
+def test_vtr_series_of_a_deck_run(amca, oracle, tmp_path):
+    """The output block's VTR time series (enabled): a frame after every
+    output_frequency-th step and one at the end, each byte-equal to the
+    oracle's restatement of VtrManager applied to the oracle's own run
+    stopped at that step, and the .pvd listing their times.  The variable
+    list holds a name the reference does not know (skipped) and temperature,
+    which a realm without a temperature field declares in the .pvtr only."""
+    import os
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import vtr_series as vs
+    from amca3d_amd.sim import Simulation
+    text, prm = case(**SMALL[1])
+    n_steps = oracle.run_plain(oracle.plain_params(**prm))["n_steps"]
+    freq = n_steps // 4 + 1
+    names = ["grain_ID", "no_such_variable", "grain_orientation", "temperature"]
+    text = text.replace("      output_frequency: 100\n      output_variables:\n        - grain_ID",
+                        f"      output_frequency: {freq}\n      output_variables:\n" +
+                        "\n".join(f"        - {v}" for v in names))
+    assert f"output_frequency: {freq}" in text
+    deck = tmp_path / "deck.yaml"
+    deck.write_text(text)
+    s = Simulation.from_file(str(deck))
+    s.enable_vtr(True)
+    s.initialize()
+    s.run(-1)
+    s.finalize()
+    s.close()
+    steps = list(range(freq, n_steps + 1, freq)) + [n_steps]  # the last from finalize
+    ids = vs.variable_ids(names)
+    root = tmp_path / "Results"
+    times = []
+    for c, st in enumerate(steps):
+        r = oracle.run_plain(oracle.plain_params(**dict(prm, max_steps=st)))
+        assert r["n_steps"] == st
+        times.append(r["time"])
+        frame = root / "grainsTimeStates" / f"timeStep_{c:05d}"
+        want = vs.vtr_text(prm["nx"], prm["ny"], prm["nz"], 0.0, 0.0, 0.0, prm["h"], ids,
+                           r["ori"], r["euler"])
+        assert (frame / "Grains00000.vtr").read_text() == want, c
+        assert (frame / f"Grains{c:05d}.pvtr").read_text() == \
+            vs.pvtr_text(prm["nx"], prm["ny"], prm["nz"], ids, "Grains")
+    assert not (root / "grainsTimeStates" / f"timeStep_{len(steps):05d}").exists()
+    assert (root / "Grains.pvd").read_text() == vs.pvd_text(times, "Grains")
+
+
 def test_nucleation_example_parity(amca, oracle):
     """The reference's nucleation example deck (60^3, 270 bulk draws) to full
     solidification."""
