@@ -133,7 +133,10 @@ amca_status amca_sim_run(amca_sim* s, int64_t max_steps, amca_counters* out) {
 amca_status amca_sim_finalize(amca_sim* s) {
   if (!s || !s->initialized) return host_fail(AMCA_ERR_INVALID, "simulation not initialized");
   return guarded([&] {
-    if (s->cafe) s->cafe->finalize();
+    if (s->cafe)
+      s->cafe->finalize();
+    else
+      s->plain->write_microstructure(s->base);
     // the last VTR frame (output_last_step_result), then the end-of-run files
     if (s->cafe)
       s->cafe->vtr_final_frame();

@@ -1,6 +1,7 @@
 #include "realm.h"
 
 #include "final_grains.h"
+#include "micro_io.h"
 #include "vtr_series.h"
 
 #include <algorithm>
@@ -366,6 +367,63 @@ amca_counters PlainRealm::run(long long max_steps) {
     if (vtr_->due(c.step, c.time)) vtr_frame(c.time);
   }
   return c;
+}
+
+void PlainRealm::write_microstructure(const std::string& base_dir) {
+  if (!dom_ || !cfg_.output_micro) return;
+  auto fail = [](const char* what) {
+    throw std::runtime_error(std::string(what) + ": " + amca_last_error());
+  };
+  amca_counters c{};
+  if (amca_domain_counters(dom_, &c) != AMCA_OK) fail("amca_domain_counters");
+  const long long NG = grid_.NG();
+  MicroState m;
+  m.num_cells = NG;
+  // the sites drawn by this realm; a plain realm loads no file
+  m.num_global_expected = n_surface_ + n_bulk_;
+  m.x0 = cfg_.origin[0];
+  m.y0 = cfg_.origin[1];
+  m.z0 = cfg_.origin[2];
+  m.h = cfg_.h;
+  m.nxG = grid_.nxG();
+  m.nyG = grid_.nyG();
+  m.nzG = grid_.nzG();
+  m.gid.resize((size_t)NG);
+  m.ori.resize((size_t)NG);
+  m.melted.assign((size_t)NG, 0);  // hasBeenMelted_ is empty without remelting
+  const int cap = (int)site_cells_.size();
+  std::vector<int32_t> sc((size_t)std::max(cap, 1));
+  std::vector<double> sd((size_t)std::max(cap, 1));
+  int32_t ns = 0;
+  if (amca_domain_download(dom_, m.ori.data(), m.gid.data(), nullptr, cap, sc.data(), sd.data(),
+                           &ns) != AMCA_OK)
+    fail("amca_domain_download");
+  // cellNucleationDT_: -1 off the sites; a site the nucleation pass dropped
+  // (nucleated or captured) reads 0 (CellularAutomataManager.cpp:286-292); the
+  // sites still listed keep their undercooling
+  m.nuc_dt.assign((size_t)NG, -1.0);
+  for (int c0 : site_cells_) m.nuc_dt[(size_t)c0] = 0.0;
+  for (int q = 0; q < std::min(ns, cap); q++) m.nuc_dt[(size_t)sc[q]] = sd[q];
+  const int no = c.n_orient;
+  m.orient_id.resize((size_t)no);
+  m.euler.assign(euler_.begin(), euler_.begin() + 3 * (size_t)no);
+  for (int o = 0; o < no; o++) m.orient_id[(size_t)o] = o;
+  const int ng = c.n_grains;
+  std::vector<double> env(4 * (size_t)std::max(ng, 1));
+  m.grain_id.resize((size_t)ng);
+  m.grain_cell.resize((size_t)ng);
+  m.grain_orient.resize((size_t)ng);
+  int32_t got = 0;
+  if (amca_domain_download_grains(dom_, ng, env.data(), m.grain_cell.data(),
+                                  m.grain_orient.data(), &got) != AMCA_OK)
+    fail("amca_domain_download_grains");
+  m.grain_centre.resize(3 * (size_t)ng);
+  for (int g = 0; g < ng; g++) {
+    m.grain_id[(size_t)g] = g;
+    for (int a = 0; a < 3; a++) m.grain_centre[3 * (size_t)g + a] = env[4 * (size_t)g + a];
+  }
+  const std::string name = rank_file_name(cfg_.micro_file);
+  write_micro((base_dir.empty() || name[0] == '/') ? name : base_dir + "/" + name, m);
 }
 
 void PlainRealm::enable_vtr(const std::string& base_dir, bool on) {

@@ -119,6 +119,10 @@ class PlainRealm {
   // the steps it names during run(), and the last frame (output_last_step_result)
   void enable_vtr(const std::string& base_dir, bool on);
   void vtr_final_frame();
+  // output_microstructure_information (CellularAutomataManager_3D.cpp:1835-1913)
+  // at the end of time_integration (CellularAutomataManager.cpp:144-145), when
+  // the deck asks for it; relative names resolve against base_dir
+  void write_microstructure(const std::string& base_dir);
   amca_domain* domain() const { return dom_; }
   const Grid& grid() const { return grid_; }
   const std::vector<int>& initial_sites() const { return site_cells_; }

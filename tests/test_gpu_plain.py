@@ -131,6 +131,48 @@ def test_final_grain_files_of_a_deck_run(amca, oracle, tmp_path):
     assert (tmp_path / "Results" / "finalResult.txt").read_text() == res
 
 
+def test_microstructure_file_of_a_plain_deck(amca, oracle, tmp_path):
+    """output_microstructure_information on a cellular_automata realm: written
+    at the end of the run (CellularAutomataManager.cpp:144-145) with the
+    reference's rank suffix; every section equals the oracle's run of the same
+    case -- ids, orientation ids, no melted flags, cellNucleationDT_ (-1 off the
+    sites, 0 where the nucleation pass dropped a site, the undercooling of the
+    sites still listed), Euler angles and grain centres at six digits."""
+    import remelt_cases as rc
+    from amca3d_amd.sim import Simulation
+    text, prm = case(**SMALL[0])
+    old = "       options:\n"
+    assert old in text
+    text = text.replace(old, old + "         - output_microstructure_information:\n"
+                                   "             file_name: micro.txt\n", 1)
+    deck = tmp_path / "deck.yaml"
+    deck.write_text(text)
+    s = Simulation.from_file(str(deck))
+    s.initialize()
+    s.run(-1)
+    inf = s.info()
+    s.finalize()
+    s.close()
+    r = oracle.run_plain(oracle.plain_params(**prm))
+    m = rc.read_micro_file(tmp_path / "micro0.txt")
+    nG = (prm["nx"] + 2) * (prm["ny"] + 2) * (prm["nz"] + 2)
+    head = m["header"]
+    assert int(head[0]) == nG and int(head[1]) == r["n_orient"] and int(head[2]) == r["n_grains"]
+    assert m["n_expected"] == inf["n_surface_sites"] + inf["n_bulk_sites"]
+    assert [int(v) for v in head[7:10]] == [prm["nx"] + 2, prm["ny"] + 2, prm["nz"] + 2]
+    np.testing.assert_array_equal(m["gid"], r["gid"])
+    np.testing.assert_array_equal(m["ori"], r["ori"])
+    np.testing.assert_array_equal(m["melted"], np.zeros(nG, np.uint8))
+    np.testing.assert_allclose(m["nuc_dt"], r["nuc_dt"], rtol=1e-5, atol=0)
+    assert (m["nuc_dt"] == 0).sum() > 0 and (m["nuc_dt"] == -1).sum() > 0
+    np.testing.assert_array_equal(m["orient_id"], np.arange(r["n_orient"]))
+    np.testing.assert_allclose(m["euler"], r["euler"], rtol=1e-5, atol=1e-9)
+    np.testing.assert_array_equal(m["grain_id"], np.arange(r["n_grains"]))
+    np.testing.assert_array_equal(m["grain_cell"], r["grain_cell"])
+    np.testing.assert_array_equal(m["grain_orient"], r["grain_orient"])
+    np.testing.assert_allclose(m["grain_centre"], r["env"][:, :3], rtol=1e-5, atol=1e-12)
+
+
 def test_vtr_series_of_a_deck_run(amca, oracle, tmp_path):
     """The output block's VTR time series (enabled): a frame after every
     output_frequency-th step and one at the end, each byte-equal to the
