@@ -7,6 +7,8 @@
 
 #include <vector>
 
+#include "../voxel_math.h"
+
 namespace amca_host {
 
 struct FeMesh {
@@ -15,6 +17,7 @@ struct FeMesh {
   int nodes() const { return fnx * fny * fnz; }
   int elements() const { return (fnx - 1) * (fny - 1) * (fnz - 1); }
   void element_nodes(int e, int* nid) const;
+  amca_vox::MeshView view() const { return amca_vox::MeshView{xyz.data(), fnx, fny, fnz}; }
 };
 
 struct VoxelMapParams {
@@ -24,6 +27,14 @@ struct VoxelMapParams {
   double tol = 1.0e-16;            // transfers.tolerance
   int max_iter = 20;               // transfers.max_iterations
 };
+
+// The search box and its bins (CSR: bin b's elements, ascending, are
+// bin_elems[bin_start[b] .. bin_start[b+1]]), as the host and device mappers share them.
+struct VoxelPlan {
+  amca_vox::VoxelGrid grid;
+  std::vector<int> bin_start, bin_elems;
+};
+bool plan_voxels(const FeMesh& mesh, const VoxelMapParams& p, VoxelPlan& plan);
 
 // tag[NG] (element or -1) and par[3*NG] (-2 where untagged); returns false when
 // no cell found an element (the reference throws "a proc has no corresponding

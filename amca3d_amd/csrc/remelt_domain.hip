@@ -422,6 +422,78 @@ amca_status amca_remelt_destroy(amca_remelt* d) {
   return AMCA_OK;
 }
 
+namespace {
+
+// tag_voxels on the device: one thread per CA cell (ghosts included), the
+// arithmetic of the host mapper (voxel_math.h), so tags and parametric
+// coordinates equal the host's bit for bit
+__global__ void k_map_voxels(amca_vox::MeshView M, amca_vox::VoxelGrid G,
+                             const int* __restrict__ bin_start, const int* __restrict__ bin_elems,
+                             long long NG, double tol, int max_iter, int32_t* __restrict__ tag,
+                             double* __restrict__ par) {
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < NG; v += stride) {
+    double pc[3];
+    const int e = amca_vox::tag_voxel(M, G, bin_start, bin_elems, v, tol, max_iter, pc);
+    tag[v] = e;
+    for (int q = 0; q < 3; q++) par[3 * v + q] = e >= 0 ? pc[q] : -2.0;
+  }
+}
+
+// map_voxels on the device: the plan (search box, bins) on the host, the
+// per-cell search in k_map_voxels; results copied into host vectors.
+// Returns false when no cell found an element.
+amca_status map_voxels_device(const amca_host::FeMesh& mesh, const amca_host::VoxelMapParams& vp,
+                              std::vector<int32_t>& tag, std::vector<double>& par, bool& any) {
+  amca_host::VoxelPlan plan;
+  amca_host::plan_voxels(mesh, vp, plan);
+  const amca_vox::VoxelGrid& G = plan.grid;
+  const long long NG = (long long)G.nxG * G.nyG * G.nzG;
+  if (plan.bin_elems.empty()) plan.bin_elems.push_back(0);  // a valid pointer; no bin reads it
+  double* dxyz = nullptr;
+  int *dbs = nullptr, *dbe = nullptr;
+  int32_t* dtag = nullptr;
+  double* dpar = nullptr;
+  amca_status st = AMCA_OK;
+  auto hip = [&](hipError_t e, const char* what) {
+    if (e != hipSuccess && st == AMCA_OK) st = rm_fail(AMCA_ERR_HIP, std::string(what) + ": " +
+                                                              hipGetErrorString(e));
+    return e == hipSuccess;
+  };
+  if (hip(hipMalloc(&dxyz, sizeof(double) * mesh.xyz.size()), "voxel map alloc") &&
+      hip(hipMalloc(&dbs, sizeof(int) * plan.bin_start.size()), "voxel map alloc") &&
+      hip(hipMalloc(&dbe, sizeof(int) * plan.bin_elems.size()), "voxel map alloc") &&
+      hip(hipMalloc(&dtag, sizeof(int32_t) * (size_t)NG), "voxel map alloc") &&
+      hip(hipMalloc(&dpar, sizeof(double) * 3 * (size_t)NG), "voxel map alloc") &&
+      hip(hipMemcpy(dxyz, mesh.xyz.data(), sizeof(double) * mesh.xyz.size(),
+                    hipMemcpyHostToDevice), "voxel map upload") &&
+      hip(hipMemcpy(dbs, plan.bin_start.data(), sizeof(int) * plan.bin_start.size(),
+                    hipMemcpyHostToDevice), "voxel map upload") &&
+      hip(hipMemcpy(dbe, plan.bin_elems.data(), sizeof(int) * plan.bin_elems.size(),
+                    hipMemcpyHostToDevice), "voxel map upload")) {
+    const amca_vox::MeshView mv{dxyz, mesh.fnx, mesh.fny, mesh.fnz};
+    long long blocks = (NG + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(k_map_voxels, dim3((unsigned)blocks), dim3(256), 0, 0, mv, G, dbs, dbe, NG,
+                       vp.tol, vp.max_iter, dtag, dpar);
+    tag.resize((size_t)NG);
+    par.resize(3 * (size_t)NG);
+    if (hip(hipGetLastError(), "k_map_voxels") &&
+        hip(hipMemcpy(tag.data(), dtag, sizeof(int32_t) * (size_t)NG, hipMemcpyDeviceToHost),
+            "voxel map readback"))
+      hip(hipMemcpy(par.data(), dpar, sizeof(double) * 3 * (size_t)NG, hipMemcpyDeviceToHost),
+          "voxel map readback");
+  }
+  for (void* ptr : {(void*)dxyz, (void*)dbs, (void*)dbe, (void*)dtag, (void*)dpar})
+    if (ptr) (void)hipFree(ptr);
+  any = false;
+  if (st == AMCA_OK)
+    for (long long v = 0; v < NG && !any; v++) any = tag[(size_t)v] >= 0;
+  return st;
+}
+
+}  // namespace
+
 amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t fnz,
                                const double* node_xyz, int32_t n_frames, const double* frame_time,
                                const double* frame_theta) {
@@ -442,8 +514,9 @@ amca_status amca_remelt_set_fe(amca_remelt* d, int32_t fnx, int32_t fny, int32_t
   for (int q = 0; q < 3; q++) vp.bin[q] = d->desc.bin[q] > 0 ? d->desc.bin[q] : 10;
   vp.tol = d->desc.tolerance;
   vp.max_iter = d->desc.max_iterations;
-  if (!amca_host::map_voxels(mesh, vp, d->host_tag, d->host_par))
-    return rm_fail(AMCA_ERR_INVALID, "no CA cell lies in the FE mesh");
+  bool any = false;
+  RM_TRY(map_voxels_device(mesh, vp, d->host_tag, d->host_par, any));
+  if (!any) return rm_fail(AMCA_ERR_INVALID, "no CA cell lies in the FE mesh");
   P.fnx = fnx;
   P.fny = fny;
   P.fnz = fnz;
@@ -591,6 +664,33 @@ amca_status amca_fe_map_voxels(int32_t nx, int32_t ny, int32_t nz, double h, dou
   std::vector<int32_t> t;
   std::vector<double> p;
   const bool any = amca_host::map_voxels(mesh, vp, t, p);
+  std::memcpy(tag, t.data(), sizeof(int32_t) * t.size());
+  std::memcpy(par, p.data(), sizeof(double) * p.size());
+  return any ? AMCA_OK : rm_fail(AMCA_ERR_INVALID, "no CA cell lies in the FE mesh");
+}
+
+amca_status amca_fe_map_voxels_device(int32_t nx, int32_t ny, int32_t nz, double h, double x0,
+                                      double y0, double z0, const int32_t* bin, double tolerance,
+                                      int32_t max_iterations, int32_t fnx, int32_t fny,
+                                      int32_t fnz, const double* node_xyz, int32_t* tag,
+                                      double* par) {
+  if (nx < 1 || ny < 1 || nz < 1 || fnx < 2 || fny < 2 || fnz < 2 || !node_xyz || !tag || !par)
+    return rm_fail(AMCA_ERR_INVALID, "bad arguments");
+  amca_host::FeMesh mesh;
+  mesh.fnx = fnx;
+  mesh.fny = fny;
+  mesh.fnz = fnz;
+  mesh.xyz.assign(node_xyz, node_xyz + 3 * (size_t)fnx * fny * fnz);
+  amca_host::VoxelMapParams vp;
+  vp.nx = nx; vp.ny = ny; vp.nz = nz;
+  vp.h = h; vp.x0 = x0; vp.y0 = y0; vp.z0 = z0;
+  for (int q = 0; q < 3; q++) vp.bin[q] = (bin && bin[q] > 0) ? bin[q] : 10;
+  vp.tol = tolerance;
+  vp.max_iter = max_iterations;
+  std::vector<int32_t> t;
+  std::vector<double> p;
+  bool any = false;
+  RM_TRY(map_voxels_device(mesh, vp, t, p, any));
   std::memcpy(tag, t.data(), sizeof(int32_t) * t.size());
   std::memcpy(par, p.data(), sizeof(double) * p.size());
   return any ? AMCA_OK : rm_fail(AMCA_ERR_INVALID, "no CA cell lies in the FE mesh");

@@ -50,6 +50,10 @@ SIGNATURES = {
     "amca_fe_map_voxels": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                      C.c_double, C.c_double, I32P, C.c_double, C.c_int32,
                                      C.c_int32, C.c_int32, C.c_int32, F64P, I32P, F64P]),
+    "amca_fe_map_voxels_device": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double,
+                                            C.c_double, C.c_double, C.c_double, I32P, C.c_double,
+                                            C.c_int32, C.c_int32, C.c_int32, C.c_int32, F64P,
+                                            I32P, F64P]),
     "amca_remelt_set_state": (C.c_int, [C.c_void_p, I32P, I32P, U8P, C.c_int32, C.c_int32, I32P,
                                         I32P]),
     "amca_remelt_push_orientations": (C.c_int, [C.c_void_p, C.c_int32, F64P, F64P]),
@@ -126,17 +130,19 @@ def run_many(domains, max_steps=-1):
     return [o.as_dict() for o in outs]
 
 
-def map_voxels(grid, fe, bin=(5, 5, 5), tol=1e-16, max_iter=20):
-    """Host-only voxel mapping (element tag and parametric coordinates per cell)."""
+def map_voxels(grid, fe, bin=(5, 5, 5), tol=1e-16, max_iter=20, device=False):
+    """Voxel mapping (element tag and parametric coordinates per cell): on the
+    host, or with the per-cell search on the GPU (device=True)."""
     nx, ny, nz = grid["nx"], grid["ny"], grid["nz"]
     ng = (nx + 2) * (ny + 2) * (nz + 2)
     tag = np.empty(ng, np.int32)
     par = np.empty(3 * ng, np.float64)
     b = (C.c_int32 * 3)(*bin)
     xyz = np.ascontiguousarray(fe["xyz"], np.float64)
-    _check(lib().amca_fe_map_voxels(nx, ny, nz, grid["h"], grid["x0"], grid["y0"], grid["z0"], b,
-                                    tol, max_iter, *fe["shape"], _p(xyz, C.c_double),
-                                    _p(tag, C.c_int32), _p(par, C.c_double)), "map_voxels")
+    fn = lib().amca_fe_map_voxels_device if device else lib().amca_fe_map_voxels
+    _check(fn(nx, ny, nz, grid["h"], grid["x0"], grid["y0"], grid["z0"], b, tol, max_iter,
+              *fe["shape"], _p(xyz, C.c_double), _p(tag, C.c_int32), _p(par, C.c_double)),
+           "map_voxels")
     return tag, par.reshape(-1, 3)
 
 

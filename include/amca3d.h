@@ -355,6 +355,14 @@ amca_status amca_fe_map_voxels(int32_t nx, int32_t ny, int32_t nz, double h, dou
                                double z0, const int32_t* bin, double tolerance,
                                int32_t max_iterations, int32_t fnx, int32_t fny, int32_t fnz,
                                const double* node_xyz, int32_t* tag, double* par);
+/* The same mapping with the per-cell search on the GPU (one thread per cell,
+ * the host mapper's arithmetic): tags and parametric coordinates equal
+ * amca_fe_map_voxels bit for bit.  amca_remelt_set_fe maps this way. */
+amca_status amca_fe_map_voxels_device(int32_t nx, int32_t ny, int32_t nz, double h, double x0,
+                                      double y0, double z0, const int32_t* bin, double tolerance,
+                                      int32_t max_iterations, int32_t fnx, int32_t fny,
+                                      int32_t fnz, const double* node_xyz, int32_t* tag,
+                                      double* par);
 /* load_microstructure_information + retrieve_grain_information_before_simulation
  * (CellularAutomataManager_3DRemelting.cpp:165-332): per-cell orientation, grain
  * id and melted flag; grain records (cell, orientation), centres reset to the

@@ -297,3 +297,21 @@ def test_state_validation(amca):
     with pytest.raises(AmcaError, match="ascend"):
         d.set_sites([30, 20], [1.0, 1.0])
     d.close()
+
+
+@pytest.mark.parametrize("distort", [False, True])
+def test_device_voxel_map_equals_host(amca, distort):
+    """tag_voxels on the GPU (one thread per cell) equals the host mapper bit for
+    bit -- tags and parametric coordinates -- on box hexes and on distorted
+    hexes, with bins small enough that cells search several of them."""
+    from amca3d_amd import remelt
+    case = rc.track_case(nx=41, ny=17, nz=13, n_frames=2, P=30.0)
+    if distort:
+        rng = np.random.default_rng(11)
+        case["fe"]["xyz"] = case["fe"]["xyz"] + rng.uniform(-0.3, 0.3, case["fe"]["xyz"].shape) * 2.5e-6
+    kw = dict(bin=(3, 2, 4), tol=case["tol"], max_iter=case["max_iter"])
+    th, ph = remelt.map_voxels(case["grid"], case["fe"], **kw)
+    td, pd = remelt.map_voxels(case["grid"], case["fe"], device=True, **kw)
+    assert (th >= 0).sum() > 1000
+    np.testing.assert_array_equal(td, th)
+    np.testing.assert_array_equal(pd.view(np.uint64), ph.view(np.uint64))
