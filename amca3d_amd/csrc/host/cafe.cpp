@@ -47,7 +47,7 @@ void CafeRealm::prepare() {
   o.length_scale = fe_.length_scale;
   o.time_scale = fe_.time_scale;
   for (int a = 0; a < 3; a++) o.offset[a] = fe_.offset[a];
-  frames_ = load_fe_text(path(fe_.mesh), o);
+  frames_ = load_fe_frames_cached(path(fe_.mesh), o);
   if (frames_.frames() < 2) throw std::runtime_error("the temperature file holds fewer than two frames");
 
   Grid g{ca_.ncell[0], ca_.ncell[1], ca_.ncell[2]};

@@ -1,5 +1,8 @@
 #include "fe_text.h"
 
+#include <algorithm>
+#include <cstdint>
+#include <filesystem>
 #include <fstream>
 #include <iomanip>
 #include <sstream>
@@ -75,6 +78,86 @@ FeFrames load_fe_text(const std::string& path, const FeTextOptions& o) {
     f.times.push_back(tnow);
     f.theta.insert(f.theta.end(), th.begin(), th.end());
   }
+  return f;
+}
+
+namespace {
+
+// cache layout: magic, key (source bytes, source mtime, options), mesh shape,
+// frame count, then xyz, times, theta as raw doubles
+constexpr char kCacheMagic[8] = {'A', 'M', 'C', 'A', 'F', 'R', '0', '1'};
+
+struct CacheKey {
+  int64_t src_bytes, src_mtime;
+  int32_t title_lines, subtitle_lines;
+  double length_scale, time_scale, offset[3];
+};
+
+bool same_key(const CacheKey& a, const CacheKey& b) {
+  return a.src_bytes == b.src_bytes && a.src_mtime == b.src_mtime &&
+         a.title_lines == b.title_lines && a.subtitle_lines == b.subtitle_lines &&
+         a.length_scale == b.length_scale && a.time_scale == b.time_scale &&
+         a.offset[0] == b.offset[0] && a.offset[1] == b.offset[1] && a.offset[2] == b.offset[2];
+}
+
+}  // namespace
+
+FeFrames load_fe_frames_cached(const std::string& path, const FeTextOptions& o) {
+  namespace fs = std::filesystem;
+  std::error_code ec;
+  CacheKey key{};
+  key.src_bytes = (int64_t)fs::file_size(path, ec);
+  if (ec) return load_fe_text(path, o);  // reports the missing file as the loader does
+  key.src_mtime = (int64_t)fs::last_write_time(path, ec).time_since_epoch().count();
+  if (ec) return load_fe_text(path, o);
+  key.title_lines = o.title_lines;
+  key.subtitle_lines = o.subtitle_lines;
+  key.length_scale = o.length_scale;
+  key.time_scale = o.time_scale;
+  for (int q = 0; q < 3; q++) key.offset[q] = o.offset[q];
+  const std::string cache = path + ".amcacache";
+  {
+    std::ifstream in(cache, std::ios::binary);
+    char magic[8];
+    CacheKey k{};
+    int32_t shape[4];
+    if (in.read(magic, 8) && std::equal(magic, magic + 8, kCacheMagic) &&
+        in.read((char*)&k, sizeof(k)) && same_key(k, key) &&
+        in.read((char*)shape, sizeof(shape)) && shape[0] > 1 && shape[1] > 1 && shape[2] > 1 &&
+        shape[3] > 0) {
+      FeFrames f;
+      f.fnx = shape[0];
+      f.fny = shape[1];
+      f.fnz = shape[2];
+      const size_t n = (size_t)f.fnx * f.fny * f.fnz;
+      f.xyz.resize(3 * n);
+      f.times.resize((size_t)shape[3]);
+      f.theta.resize(n * (size_t)shape[3]);
+      if (in.read((char*)f.xyz.data(), sizeof(double) * f.xyz.size()) &&
+          in.read((char*)f.times.data(), sizeof(double) * f.times.size()) &&
+          in.read((char*)f.theta.data(), sizeof(double) * f.theta.size()))
+        return f;
+    }
+  }
+  FeFrames f = load_fe_text(path, o);
+  const std::string tmp = cache + ".tmp";
+  {
+    std::ofstream out(tmp, std::ios::binary);
+    const int32_t shape[4] = {f.fnx, f.fny, f.fnz, f.frames()};
+    out.write(kCacheMagic, 8);
+    out.write((const char*)&key, sizeof(key));
+    out.write((const char*)shape, sizeof(shape));
+    out.write((const char*)f.xyz.data(), sizeof(double) * f.xyz.size());
+    out.write((const char*)f.times.data(), sizeof(double) * f.times.size());
+    out.write((const char*)f.theta.data(), sizeof(double) * f.theta.size());
+    if (!out) {
+      out.close();
+      fs::remove(tmp, ec);
+      return f;
+    }
+  }
+  fs::rename(tmp, cache, ec);
+  if (ec) fs::remove(tmp, ec);
   return f;
 }
 

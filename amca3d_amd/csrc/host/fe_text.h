@@ -32,6 +32,12 @@ struct FeFrames {
 // std::runtime_error with the reference's messages on malformed files.
 FeFrames load_fe_text(const std::string& path, const FeTextOptions& o);
 
+// load_fe_text through a binary cache beside the file (<path>.amcacache): a
+// cache whose source size, modification time and parse options match is read
+// instead of the text; otherwise the text is parsed and the cache (re)written
+// (a cache that cannot be written is skipped).  The frames are the same.
+FeFrames load_fe_frames_cached(const std::string& path, const FeTextOptions& o);
+
 // Writes frames in the same format (raw coordinates and times are written as
 // given; `times` absolute).  Title: `title_lines` lines.
 void write_fe_text(const std::string& path, int title_lines, int fnx, int fny, int fnz,

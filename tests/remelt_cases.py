@@ -230,3 +230,32 @@ def read_micro_file(path):
                 grain_cell=np.array([int(g[1]) for g in grs], np.int32),
                 grain_centre=np.array([[float(v) for v in g[2:5]] for g in grs]),
                 grain_orient=np.array([int(g[5]) for g in grs], np.int32))
+
+
+def parse_frames_text(path, title_lines=4, subtitle_lines=5, unit=1e-3, time_unit=1e-3,
+                      offset=(0.5, -0.25, 0.0)):
+    """Reader of the text frame format: per frame `subtitle_lines` lines whose
+    fourth holds "step step time time x0 x1 y0 y1 z0 z1", then one line
+    "x y z T" per node (x fastest)."""
+    with open(path) as fh:
+        rows = fh.read().split("\n")
+    r = title_lines
+    xyz, times, theta, shape = None, [], [], None
+    while r + subtitle_lines <= len(rows):
+        head = rows[r + 3].split()
+        if len(head) < 10:
+            break
+        x0, x1, y0, y1, z0, z1 = (int(v) for v in head[4:10])
+        shape = (x1 - x0 + 1, y1 - y0 + 1, z1 - z0 + 1)
+        n = shape[0] * shape[1] * shape[2]
+        body = rows[r + subtitle_lines:r + subtitle_lines + n]
+        if len(body) < n or not body[-1].strip():
+            break
+        a = np.array([[float(v) for v in b.split()[:4]] for b in body])
+        if xyz is None:
+            xyz = (a[:, :3] + np.asarray(offset)) * unit
+        times.append(float(head[2]))
+        theta.append(a[:, 3])
+        r += subtitle_lines + n
+    t = np.asarray(times)
+    return dict(shape=shape, xyz=xyz, times=(t - t[0]) * time_unit, theta=np.stack(theta))

@@ -81,3 +81,52 @@ def test_microstructure_file_must_fit_the_grid(amca, small):
 def test_refused_options(amca, small, edit, msg):
     with pytest.raises(AmcaError, match=msg):
         amca.Simulation(edit(deck(small)))
+
+
+def _read_cache(path):
+    """<frames>.amcacache: 8-byte magic, key (2 int64, 2 int32, 5 doubles),
+    4 int32 (fnx, fny, fnz, frames), then xyz, times, theta as doubles."""
+    raw = open(path, "rb").read()
+    assert raw[:8] == b"AMCAFR01"
+    o = 8 + 2 * 8 + 2 * 4 + 5 * 8
+    fnx, fny, fnz, nf = np.frombuffer(raw[o:o + 16], np.int32)
+    n = int(fnx) * int(fny) * int(fnz)
+    d = np.frombuffer(raw[o + 16:], np.float64)
+    assert d.size == 3 * n + nf + n * nf
+    return (int(fnx), int(fny), int(fnz)), d[:3 * n].reshape(-1, 3), d[3 * n:3 * n + nf], \
+        d[3 * n + nf:].reshape(nf, n)
+
+
+def test_frame_cache_holds_the_text_and_follows_it(amca, small, tmp_path):
+    """The first load writes <frames>.amcacache with exactly the parsed values;
+    a later load with the same file reads the cache (a corrupted text of the
+    same size and time still loads); a changed file replaces it."""
+    import os
+    import shutil
+    _, case = small
+    src = tmp_path / "frames.txt"
+    rc.write_frames_text(src, case["fe"])
+    text = rc.cafe_deck(case, mesh=str(src))
+    amca.Simulation(text).prepare()
+    cache = tmp_path / "frames.txt.amcacache"
+    assert cache.exists()
+    shape, xyz, times, theta = _read_cache(cache)
+    fe = rc.parse_frames_text(src)
+    assert shape == tuple(case["fe"]["shape"])
+    np.testing.assert_array_equal(xyz, fe["xyz"])
+    np.testing.assert_array_equal(times, fe["times"])
+    np.testing.assert_array_equal(theta, fe["theta"])
+    # a cache hit: the text no longer parses, but size and time are unchanged
+    st = os.stat(src)
+    body = src.read_text()
+    i = body.rindex("\n", 0, len(body) - 2) + 1
+    src.write_text(body[:i] + "x" + body[i + 1:])
+    os.utime(src, ns=(st.st_atime_ns, st.st_mtime_ns))
+    assert amca.Simulation(text).prepare().info()["n_frames"] == 5
+    # a changed file (one frame fewer) is parsed again and replaces the cache
+    shutil.copy(tmp_path / "frames.txt.amcacache", tmp_path / "old.amcacache")
+    rows = body.split("\n")
+    nodes = int(np.prod(case["fe"]["shape"]))
+    src.write_text("\n".join(rows[:4 + 4 * (5 + nodes)]) + "\n")
+    assert amca.Simulation(text).prepare().info()["n_frames"] == 4
+    assert _read_cache(cache)[2].size == 4
