@@ -55,6 +55,12 @@ __device__ __forceinline__ int brick_word(const DevParams& P, int i, int j, int 
 __device__ __forceinline__ unsigned long long brick_bit(int i, int j, int k) {
   return 1ull << (((k & 3) << 4) | ((j & 3) << 2) | (i & 3));
 }
+// Claim words in the same 4x4x4 bricks, 64 int32 (two 128-byte lines) per brick,
+// so that a 3x3x3 neighbourhood touches few lines.
+__device__ __forceinline__ long long claim_index(const DevParams& P, int i, int j, int k) {
+  return 64ll * (((long long)(k >> 2) * P.nbY + (j >> 2)) * P.nbX + (i >> 2)) +
+         (((k & 3) << 4) | ((j & 3) << 2) | (i & 3));
+}
 __device__ __forceinline__ void brick_clear(const DevParams& P, unsigned long long* bm, int word,
                                             int i, int j, int k) {
   atomicAnd(&bm[brick_word(P, i, j, k) + word], ~brick_bit(i, j, k));

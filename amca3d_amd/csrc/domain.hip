@@ -237,7 +237,7 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   }
   ALLOC(B.gid, (size_t)NG);
   ALLOC(B.oid, (size_t)NG);
-  ALLOC(B.claim, (size_t)NG);
+  ALLOC(B.claim, 64 * (size_t)P.nbX * P.nbY * P.nbZ);  // brick order (claim_index)
   ALLOC(B.bmap, 2 * (size_t)P.nbX * P.nbY * P.nbZ);
   ALLOC(B.genv, N);
   ALLOC(B.gco, N);

@@ -67,7 +67,8 @@ struct StepLogRow {
 struct DevBuffers {
   int* gid;               // cellGrainID_ [NG]
   int* oid;               // cellOrientationID_ [NG]
-  int* claim;             // smallest claiming grain id per cell, INT_MAX when free [NG]
+  int* claim;             // smallest claiming grain id per cell, INT_MAX when free, in
+                          // 4x4x4 bricks (claim_index) [64 * nbX*nbY*nbZ]
   unsigned long long* bmap;  // brick words: liquid, claimed [2 * nbX*nbY*nbZ]
   double4* genv;          // grain envelope: xc, yc, zc, L [N]
   int2* gco;              // grain -> (cell, orientation) [N]

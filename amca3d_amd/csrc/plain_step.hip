@@ -62,8 +62,11 @@ __global__ void k_init_grid(DevParams P, DevBuffers B) {
     int v = is_ghost(P, c) ? 0 : -1;
     B.gid[c] = v;
     B.oid[c] = v;
-    B.claim[c] = INT_MAX;
   }
+  // claim words, brick order, padding included: no claim
+  const long long nclaim = 64ll * P.nbX * P.nbY * P.nbZ;
+  for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < nclaim; q += stride)
+    B.claim[q] = INT_MAX;
   // brick map: every interior cell liquid, nothing claimed
   const long long nb = (long long)P.nbX * P.nbY * P.nbZ;
   for (long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x; b < nb; b += stride) {
@@ -273,8 +276,7 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
     const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
     if (inside_envelope(r, e.w, gx, gy, gz)) {
       im |= 1u << s;
-      const int nb = c + P.off[s];
-      atomicMin(&B.claim[nb], g);
+      atomicMin(&B.claim[claim_index(P, ci + di, cj + dj, ck + dk)], g);
       brick_set(P, B.bmap, 1, ci + di, cj + dj, ck + dk);
     }
   }
@@ -316,7 +318,9 @@ __device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevB
   for (int s = 0; s < 26; s++) {
     cl[s] = INT_MAX;
     if ((need >> s) & 1u) {
-      const int* w = &B.claim[c + P.off[s]];
+      int di, dj, dk;
+      slot_delta(s, di, dj, dk);
+      const int* w = &B.claim[claim_index(P, ci + di, cj + dj, ck + dk)];
       cl[s] = kCoherent ? ld_coherent(w) : *w;
     }
   }
