@@ -220,6 +220,36 @@ def test_vtr_series_of_a_deck_run(amca, oracle, tmp_path):
     assert (root / "Grains.pvd").read_text() == vs.pvd_text(times, "Grains")
 
 
+def test_driver_runs_a_deck(amca, oracle, tmp_path):
+    """The amca3d driver (`amca3d -i deck`, the reference's cafe command line)
+    as a child process: it runs the deck to the end, writes the VTR frames the
+    output block names and finalGrains.vtk, and the files hold the oracle's
+    state."""
+    import os
+    import subprocess
+    import sys
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                                    "oracle"))
+    import final_grains as fg
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))),
+                       "amca3d_amd", "_build", "amca3d")
+    text, prm = case(**SMALL[2])
+    r = oracle.run_plain(oracle.plain_params(**prm))
+    freq = r["n_steps"] // 3 + 1
+    text = text.replace("output_frequency: 100", f"output_frequency: {freq}")
+    (tmp_path / "deck.yaml").write_text(text)
+    p = subprocess.run([exe, "-i", "deck.yaml"], cwd=tmp_path, capture_output=True, text=True,
+                       timeout=120)
+    assert p.returncode == 0, p.stderr
+    assert f"steps {r['n_steps']} " in p.stdout
+    frames = sorted((tmp_path / "Results" / "grainsTimeStates").iterdir())
+    assert len(frames) == r["n_steps"] // freq + 1
+    vtk, res = fg.final_grains_text(prm["nx"], prm["ny"], prm["nz"], 0.0, 0.0, 0.0, prm["h"],
+                                    r["ori"], None, r["euler"])
+    assert (tmp_path / "finalGrains.vtk").read_text() == vtk
+    assert (tmp_path / "Results" / "finalResult.txt").read_text() == res
+
+
 def test_nucleation_example_parity(amca, oracle):
     """The reference's nucleation example deck (60^3, 270 bulk draws) to full
     solidification."""
