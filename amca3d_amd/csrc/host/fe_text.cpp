@@ -8,6 +8,8 @@
 #include <sstream>
 #include <stdexcept>
 
+#include <unistd.h>
+
 namespace amca_host {
 
 FeFrames load_fe_text(const std::string& path, const FeTextOptions& o) {
@@ -140,7 +142,8 @@ FeFrames load_fe_frames_cached(const std::string& path, const FeTextOptions& o) 
     }
   }
   FeFrames f = load_fe_text(path, o);
-  const std::string tmp = cache + ".tmp";
+  // a name of this process's own, so that concurrent loads cannot interleave
+  const std::string tmp = cache + "." + std::to_string((long long)getpid()) + ".tmp";
   {
     std::ofstream out(tmp, std::ios::binary);
     const int32_t shape[4] = {f.fnx, f.fny, f.fnz, f.frames()};
