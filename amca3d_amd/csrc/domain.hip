@@ -76,10 +76,10 @@ struct amca_domain {
   }
 
   ~amca_domain() {
-    for (void* p : allocs) hipFree(p);
+    for (void* p : allocs) (void)hipFree(p);
     for (auto& e : ev_pool)
       if (e) (void)hipEventDestroy(e);
-    if (stream) hipStreamDestroy(stream);
+    if (stream) (void)hipStreamDestroy(stream);
   }
 };
 
@@ -172,6 +172,18 @@ amca_status check_err(amca_domain* d, const amca::DevCounters& c) {
   return AMCA_OK;
 }
 
+}  // namespace
+
+namespace {
+template <class F>
+struct ScopeExit {
+  F f;
+  ~ScopeExit() { f(); }
+};
+template <class F>
+ScopeExit<F> on_scope_exit(F f) {
+  return {f};
+}
 }  // namespace
 
 extern "C" {
@@ -280,7 +292,7 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
 
 amca_status amca_domain_destroy(amca_domain* d) {
   if (!d) return AMCA_OK;
-  hipStreamSynchronize(d->stream);
+  (void)hipStreamSynchronize(d->stream);  // the handle goes either way
   delete d;
   return AMCA_OK;
 }
@@ -299,7 +311,7 @@ amca_status amca_domain_set_sites(amca_domain* d, int32_t n, const int32_t* cell
     for (int b = 0; b < 2; b++) {
       for (auto& p : d->allocs) {
         if (p == d->B.site_cell[b] || p == d->B.site_dt[b]) {
-          hipFree(p);
+          (void)hipFree(p);
           p = nullptr;
         }
       }
@@ -392,7 +404,7 @@ amca_status amca_domain_enable_log(amca_domain* d, int32_t capacity) {
   if (d->B.log) {
     for (auto& p : d->allocs)
       if (p == d->B.log) {
-        hipFree(p);
+        (void)hipFree(p);
         p = nullptr;
       }
     d->allocs.erase(std::remove(d->allocs.begin(), d->allocs.end(), nullptr), d->allocs.end());
@@ -533,6 +545,11 @@ amca_status amca_envelope_selftest(int32_t n, const double* env4, const double* 
     return fail(AMCA_ERR_NO_DEVICE, "no HIP device visible");
   double *de = nullptr, *dr = nullptr, *dc = nullptr, *dout = nullptr;
   int* din = nullptr;
+  // device buffers freed on every path, the early error returns included
+  const auto release = on_scope_exit([&] {
+    for (void* q : {(void*)de, (void*)dr, (void*)dc, (void*)dout, (void*)din})
+      if (q) (void)hipFree(q);
+  });
   HIP_TRY(hipMalloc(&de, sizeof(double) * 4 * n));
   HIP_TRY(hipMalloc(&dr, sizeof(double) * 9 * n));
   HIP_TRY(hipMalloc(&dc, sizeof(double) * 3 * n));
@@ -545,11 +562,6 @@ amca_status amca_envelope_selftest(int32_t n, const double* env4, const double* 
   HIP_TRY(hipDeviceSynchronize());
   HIP_TRY(hipMemcpy(inside, din, sizeof(int) * n, hipMemcpyDeviceToHost));
   HIP_TRY(hipMemcpy(child4, dout, sizeof(double) * 4 * n, hipMemcpyDeviceToHost));
-  hipFree(de);
-  hipFree(dr);
-  hipFree(dc);
-  hipFree(dout);
-  hipFree(din);
   return AMCA_OK;
 }
 
