@@ -13,11 +13,13 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <stdexcept>
 #include <string>
 #include <unordered_map>
 #include <vector>
 
 #include "../../include/amca3d.h"
+#include "host/fe_text.h"
 #include "host/generators.h"
 #include "host/sites.h"
 #include "host/voxel_map.h"
@@ -886,6 +888,21 @@ amca_status amca_rosenthal_frames(int64_t n_nodes, const double* node_xyz, int32
     return rm_fail(AMCA_ERR_INVALID, "bad arguments");
   amca_host::RosenthalParams p{P, eta, v, k, alpha, T0, x_start, y_c, z_top, r_min};
   amca_host::rosenthal_frames(n_nodes, node_xyz, n_frames, t0, dt, p, out);
+  return AMCA_OK;
+}
+
+amca_status amca_write_fe_frames(const char* path, int32_t title_lines, int32_t fnx, int32_t fny,
+                                 int32_t fnz, const double* node_xyz, int32_t n_frames,
+                                 const double* times, const double* theta) {
+  if (!path || title_lines < 0 || fnx < 2 || fny < 2 || fnz < 2 || n_frames < 1 || !node_xyz ||
+      !times || !theta)
+    return rm_fail(AMCA_ERR_INVALID, "bad arguments");
+  if ((long long)fnx * fny * fnz > INT_MAX) return rm_fail(AMCA_ERR_INVALID, "mesh too large");
+  try {
+    amca_host::write_fe_text(path, title_lines, fnx, fny, fnz, node_xyz, n_frames, times, theta);
+  } catch (const std::exception& e) {
+    return rm_fail(AMCA_ERR_IO, std::string(path) + ": " + e.what());
+  }
   return AMCA_OK;
 }
 

@@ -93,6 +93,8 @@ SIGNATURES = {
     "amca_voronoi_substrate": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, C.c_double, C.c_double,
                                          C.c_double, C.c_double, C.c_int32, C.c_uint64, I32P,
                                          F64P]),
+    "amca_write_fe_frames": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
+                                       F64P, C.c_int32, F64P, F64P]),
     "amca_remelt_last_error": (C.c_char_p, []),
 }
 
@@ -158,6 +160,21 @@ def rosenthal_frames(xyz, times, *, P, eta, v, k, alpha, T0, x_start, y_c, z_top
                                        P, eta, v, k, alpha, T0, x_start, y_c, z_top, r_min,
                                        _p(out, C.c_double)), "rosenthal_frames")
     return out
+
+
+def write_fe_frames(path, shape, xyz, times, theta, title_lines=4):
+    """Writes frames [n_frames, nodes] on a (fnx, fny, fnz) node mesh as the
+    text a finite_element realm reads (coordinates and absolute times as given)."""
+    fnx, fny, fnz = (int(v) for v in shape)
+    xyz = np.ascontiguousarray(xyz, np.float64)
+    times = np.ascontiguousarray(times, np.float64)
+    theta = np.ascontiguousarray(theta, np.float64)
+    nodes = fnx * fny * fnz
+    if xyz.shape != (nodes, 3) or theta.shape != (len(times), nodes):
+        raise ValueError("write_fe_frames: xyz must be [nodes, 3] and theta [frames, nodes]")
+    _check(lib().amca_write_fe_frames(str(path).encode(), int(title_lines), fnx, fny, fnz,
+                                      _p(xyz, C.c_double), len(times), _p(times, C.c_double),
+                                      _p(theta, C.c_double)), "write_fe_frames")
 
 
 def voronoi_substrate(grid, n_seeds, seed):

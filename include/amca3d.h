@@ -477,6 +477,21 @@ amca_status amca_rosenthal_frames(int64_t n_nodes, const double* node_xyz, int32
 amca_status amca_voronoi_substrate(int32_t nx, int32_t ny, int32_t nz, double h, double x0,
                                    double y0, double z0, int32_t n_seeds, uint64_t seed,
                                    int32_t* label, double* euler3);
+/* Writes nodal frames as the text file a finite_element realm reads
+ * (LoadDataFromFile.cpp:21-191 with lines_for_subtitle 5; format of the
+ * README "PBF_x" example): `title_lines` title lines, then per frame a
+ * five-line subtitle block whose fourth line is
+ * "f+1 f+1 t t 0 fnx-1 0 fny-1 0 fnz-1", and one "x y z T" line per node
+ * (x fastest); frames after the first open with an empty line.  Doubles carry
+ * 17 significant digits, so the reader recovers them exactly.  node_xyz is
+ * [nodes][3] as written (the deck's length unit and offset apply on reading),
+ * times[n_frames] absolute, theta [n_frames][nodes].  No counterpart in the
+ * reference, whose frames come from an external FE code; with
+ * amca_rosenthal_frames it produces deck-ready input.  AMCA_ERR_IO if the
+ * file cannot be written. */
+amca_status amca_write_fe_frames(const char* path, int32_t title_lines, int32_t fnx, int32_t fny,
+                                 int32_t fnz, const double* node_xyz, int32_t n_frames,
+                                 const double* times, const double* theta);
 const char* amca_remelt_last_error(void);
 
 /* ---- realm layer: YAML input deck -> device domain ------------------------

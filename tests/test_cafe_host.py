@@ -130,3 +130,53 @@ def test_frame_cache_holds_the_text_and_follows_it(amca, small, tmp_path):
     src.write_text("\n".join(rows[:4 + 4 * (5 + nodes)]) + "\n")
     assert amca.Simulation(text).prepare().info()["n_frames"] == 4
     assert _read_cache(cache)[2].size == 4
+
+
+def test_generated_frames_written_as_deck_input(amca, small, tmp_path):
+    """amca_write_fe_frames: Rosenthal frames from the library's generator,
+    written by the library, parse back (independent reader) to exactly the
+    values given, parse to the same values as the same frames written by the
+    test writer, and load through a deck (the frame cache then holds what the
+    independent reader computes with the deck's unit and offset)."""
+    from amca3d_amd import remelt
+    _, case = small
+    fe = case["fe"]
+    g = case["grid"]
+    theta = remelt.rosenthal_frames(fe["xyz"], fe["times"], P=30.0, eta=0.35, v=1.0, k=20.0,
+                                    alpha=5e-6, T0=300.0, x_start=0.15 * g["nx"] * g["h"],
+                                    y_c=0.5 * g["ny"] * g["h"], z_top=g["nz"] * g["h"],
+                                    r_min=g["h"])
+    unit, offset, time_unit, t0 = 1e-3, np.array([0.5, -0.25, 0.0]), 1e-3, 5.0
+    raw = np.asarray(fe["xyz"]) / unit - offset
+    t_raw = t0 + np.asarray(fe["times"]) / time_unit
+    src = tmp_path / "frames.txt"
+    remelt.write_fe_frames(src, fe["shape"], raw, t_raw, theta)
+    as_given = rc.parse_frames_text(src, unit=1.0, time_unit=1.0, offset=(0.0, 0.0, 0.0))
+    assert as_given["shape"] == tuple(fe["shape"])
+    np.testing.assert_array_equal(as_given["xyz"], raw)
+    np.testing.assert_array_equal(as_given["times"], t_raw - t_raw[0])
+    np.testing.assert_array_equal(as_given["theta"], theta)
+    ours = rc.parse_frames_text(src)
+    rc.write_frames_text(tmp_path / "py.txt", dict(fe, theta=theta))
+    theirs = rc.parse_frames_text(tmp_path / "py.txt")
+    for key in ("xyz", "times", "theta"):
+        np.testing.assert_array_equal(ours[key], theirs[key])
+    info = amca.Simulation(rc.cafe_deck(case, mesh=str(src))).prepare().info()
+    assert info["n_frames"] == len(t_raw)
+    shape, xyz, times, th = _read_cache(tmp_path / "frames.txt.amcacache")
+    assert shape == tuple(fe["shape"])
+    np.testing.assert_array_equal(xyz, ours["xyz"])
+    np.testing.assert_array_equal(times, ours["times"])
+    np.testing.assert_array_equal(th, ours["theta"])
+
+
+def test_frame_writer_refuses_bad_arguments(amca, tmp_path):
+    from amca3d_amd import remelt
+    xyz = np.zeros((8, 3))
+    with pytest.raises(ValueError):
+        remelt.write_fe_frames(tmp_path / "f.txt", (2, 2, 2), xyz, [0.0], np.zeros((2, 8)))
+    with pytest.raises(AmcaError):
+        remelt.write_fe_frames(tmp_path / "f.txt", (1, 2, 2), xyz[:4], [0.0], np.zeros((1, 4)))
+    with pytest.raises(AmcaError, match="fail to open"):
+        remelt.write_fe_frames(tmp_path / "no_dir" / "f.txt", (2, 2, 2), xyz, [0.0],
+                               np.zeros((1, 8)))
