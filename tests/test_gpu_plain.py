@@ -29,11 +29,11 @@ def case(nx, ny, nz, h, *, surface=(0.0, 2.0, 0.5), faces="all", bulk=(0.0, 3.0,
     return text, prm
 
 
-def run_gpu(amca, text, log_cap=1 << 16):
+def run_gpu(amca, text, log_cap=1 << 16, max_steps=-1):
     s = amca.Simulation(text).initialize()
     d = s.domain
     d.enable_log(log_cap)
-    c = s.run(-1)
+    c = s.run(max_steps)
     out = d.download()
     out["grains"] = d.download_grains()
     out["log"] = d.read_log()
@@ -287,6 +287,32 @@ def test_uniform_cooling_96_parity(amca, oracle):
     g = run_gpu(amca, text)
     r = compare(oracle, prm, g)
     assert r["done"] == 1
+
+
+def test_ragged_grid_parity(amca, oracle):
+    """131 x 97 x 67 cells (ghost-inclusive extents 133, 99, 69: no axis fills
+    whole 4-cell bricks), surface sites on all six faces plus the bulk, seed 7,
+    to full solidification: bit-exact against the oracle."""
+    text, prm = case(131, 97, 67, 2e-6, surface=(1e9, 1.0, 0.5), faces="all",
+                     bulk=(5e13, 3.0, 1.0), Tdot=-200.0, time_step=1e-4, termination_time=1.0,
+                     seed=7)
+    g = run_gpu(amca, text)
+    r = compare(oracle, prm, g)
+    assert r["done"] == 1
+
+
+def test_capture_cascade_beyond_wave_one(amca, oracle):
+    """144^3 with the 512^3 workload's physics: in step 205 a grain captured
+    in that step captures again (wave 2, the path of waves >= 2).  212 steps,
+    bit-exact against the oracle, and both step logs record the wave."""
+    text, prm = case(144, 144, 144, 2e-6, surface=(1e9, 1.0, 0.5), faces="z_min",
+                     bulk=(5e13, 3.0, 1.0), Tdot=-200.0, time_step=1e-4, termination_time=1.0)
+    prm["max_steps"] = 212
+    g = run_gpu(amca, text, max_steps=212)
+    r = compare(oracle, prm, g)
+    assert r["n_steps"] == 212
+    assert r["steps"]["max_wave"][205] == 2
+    assert g["log"]["max_wave"][205] == 2
 
 
 def test_given_sites_identity_nucleus(amca, oracle):
