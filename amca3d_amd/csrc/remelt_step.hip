@@ -54,8 +54,12 @@ __device__ __forceinline__ bool rm_ghost_ijk(const RmParams& P, int i, int j, in
   return i == 0 || j == 0 || k == 0 || i == P.nxG - 1 || j == P.nyG - 1 || k == P.nzG - 1;
 }
 
-// exponents are 1, 2 and 3: pow(x, 1) = x, pow(x, 2) = x*x (glibc's pow is
-// exact for both); the cube term only matters when a3 != 0
+// Exponents are 1, 2 and 3 (ProblemPhysics.cpp:142-158 calls pow(dT, b)).
+// glibc's pow(x, 1) is x exactly.  Its pow(x, 2) is not correctly rounded: it
+// equals x*x except for about 0.08 % of arguments, where it is the neighbouring
+// double.  This path computes x*x, so at such an undercooling a grain's growth
+// increment, and with it the half-diagonal, can differ from the reference's by
+// one ulp.  The cube term only matters when a3 != 0.
 __device__ __forceinline__ double rm_velocity(const RmParams& P, double Tc) {
   const double dTu = P.Tm - Tc;
   double v = 0.0;

@@ -107,7 +107,7 @@ def bits(a):
     return a.view(np.uint64) if a.dtype == np.float64 else a
 
 
-def compare(case, orc, dev):
+def compare(case, orc, dev, pow_ulps=0):
     c = dev["counters"]
     assert c["step"] == orc["n_steps"]
     assert c["n_grains"] == orc["n_grains"]
@@ -127,7 +127,16 @@ def compare(case, orc, dev):
     n = orc["n_grains"]
     np.testing.assert_array_equal(dev["grain_cell"], orc["grain_cell"][:n])
     np.testing.assert_array_equal(dev["grain_orient"], orc["grain_orient"][:n])
-    np.testing.assert_array_equal(bits(dev["grain_env"]), bits(orc["grain_env"][:n]))
+    if pow_ulps:
+        # centres exact; half-diagonals exact but for at most `pow_ulps` grains
+        # one ulp apart (glibc's pow(dT, 2) in the reference's remelting
+        # velocity is not dT*dT for every dT; the device computes dT*dT)
+        a, b = bits(dev["grain_env"]), bits(orc["grain_env"][:n])
+        np.testing.assert_array_equal(a[:, :3], b[:, :3])
+        d = np.abs(a[:, 3].astype(np.int64) - b[:, 3].astype(np.int64))
+        assert d.max() <= 1 and (d != 0).sum() <= pow_ulps, (d.max(), (d != 0).sum())
+    else:
+        np.testing.assert_array_equal(bits(dev["grain_env"]), bits(orc["grain_env"][:n]))
     lg, so = dev["log"], orc["steps"]
     assert len(lg) == len(so)
     for k in ("n_nucleated", "n_captured", "n_fresh", "n_flagged"):
@@ -232,6 +241,27 @@ def test_active_box_equals_whole_grid(amca, chunks):
     nb = boxed["log"]["n_box"]
     assert nb[0] == ng and (whole["log"]["n_box"] == ng).all()
     assert np.median(nb[1:]) < 0.25 * ng, np.median(nb[1:])
+
+
+def test_wider_hotter_track(amca):
+    """240 x 64 x 40 cells under 60 frames of a 60 W source, surface and bulk
+    nucleation, seed 9 (about 630 steps, 1900 nuclei, 47 000 captures): the
+    boxed run equals the oracle bit for bit (fields, temperatures, flags, grain
+    ids and records, step log) except for one grain's half-diagonal, one ulp
+    apart (the reference's pow, below), and the box stays a small part of the
+    grid."""
+    nuc = dict(surface=(2e10, 2.0, 0.5), bulk=(2e15, 5.0, 1.0))
+    case = rc.track_case(nx=240, ny=64, nz=40, n_frames=60, P=60.0, nucleation=nuc, seed=9)
+    orc = ol.run_remelt(case)
+    assert orc["steps"]["n_nucleated"].sum() > 1000
+    assert orc["steps"]["n_captured"].sum() > 10000
+    dev = run_device(case, orc)
+    # one grain's half-diagonal is one ulp off: at step 393 its cell's
+    # undercooling is an argument where glibc's pow(dT, 2) != dT*dT
+    # (test_remelting_velocity_pow_is_not_the_product pins that argument)
+    compare(case, orc, dev, pow_ulps=1)
+    nb = dev["log"]["n_box"]
+    assert np.median(nb[1:]) < 0.25 * (242 * 66 * 42), np.median(nb[1:])
 
 
 def test_track_with_fresh_captures(amca):

@@ -259,3 +259,22 @@ def test_track_bookkeeping_invariants():
     # every solid interior cell's orientation is its grain record's
     solid = (r["ori"] >= 0) & (r["gid"] >= 0)
     np.testing.assert_array_equal(r["grain_orient"][r["gid"][solid]], r["ori"][solid])
+
+
+def test_remelting_velocity_pow_is_not_the_product():
+    """The reference's remelting velocity (ProblemPhysics.cpp:142-158) takes
+    pow(dT, b); the oracle calls glibc's pow, the device computes dT*dT.  glibc's
+    pow(x, 2) is not correctly rounded: it is the neighbouring double of x*x for
+    a small share of arguments, among them the undercooling (1600 K minus this
+    cell temperature) that puts one grain of the 240 x 64 x 40 GPU track one
+    ulp apart."""
+    import math
+    Tc = float.fromhex("0x1.86299cd5e950cp+10")
+    x = 1600.0 - Tc
+    assert math.pow(x, 1.0) == x
+    assert (x * x).hex() == "0x1.831a0b997f4fdp+10"
+    assert math.pow(x, 2.0).hex() == "0x1.831a0b997f4fep+10"
+    rng = np.random.default_rng(7)
+    xs = rng.uniform(0.0, 300.0, 200_000)
+    share = np.mean([math.pow(v, 2.0) != v * v for v in xs])
+    assert 1e-4 < share < 1e-2, share
