@@ -165,6 +165,19 @@ amca_status enqueue_step(amca_domain* d) {
   return AMCA_OK;
 }
 
+// Every live site can nucleate once and takes the next orientation row, so a
+// run may start only when the pushed rows cover the orientations in use plus
+// one per live site.  The kernels never read a row that was not pushed.
+amca_status check_rows(amca_domain* d, const amca::DevCounters& c) {
+  if ((long long)c.n_orient + c.n_sites > d->orient_pushed)
+    return fail(AMCA_ERR_CAPACITY,
+                "orientation rows pushed (" + std::to_string(d->orient_pushed) +
+                    ") do not cover the orientations in use plus one per live site (" +
+                    std::to_string((long long)c.n_orient + c.n_sites) +
+                    "): push more rows with amca_domain_push_orientations");
+  return AMCA_OK;
+}
+
 amca_status check_err(amca_domain* d, const amca::DevCounters& c) {
   if (c.err)
     return fail(AMCA_ERR_CAPACITY,
@@ -355,8 +368,10 @@ amca_status amca_domain_push_orientations(amca_domain* d, int32_t n, const doubl
 
 amca_status amca_domain_step(amca_domain* d, amca_counters* out) {
   if (!d) return fail(AMCA_ERR_INVALID, "null domain");
-  AMCA_TRY(enqueue_step(d));
   amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  AMCA_TRY(check_rows(d, c));
+  AMCA_TRY(enqueue_step(d));
   AMCA_TRY(read_counters(d, &c));
   fill_counters(d, c, out);
   return check_err(d, c);
@@ -366,6 +381,7 @@ amca_status amca_domain_run(amca_domain* d, int64_t max_steps, amca_counters* ou
   if (!d) return fail(AMCA_ERR_INVALID, "null domain");
   amca::DevCounters c;
   AMCA_TRY(read_counters(d, &c));
+  AMCA_TRY(check_rows(d, c));
   const long long start = c.step;
   long long batch = 8;
   while (true) {
@@ -385,6 +401,9 @@ amca_status amca_domain_run(amca_domain* d, int64_t max_steps, amca_counters* ou
 
 amca_status amca_domain_run_fixed(amca_domain* d, int64_t n_steps) {
   if (!d || n_steps < 0) return fail(AMCA_ERR_INVALID, "bad arguments");
+  amca::DevCounters c;
+  AMCA_TRY(read_counters(d, &c));
+  AMCA_TRY(check_rows(d, c));
   for (int64_t q = 0; q < n_steps; q++) AMCA_TRY(enqueue_step(d));
   HIP_TRY(hipStreamSynchronize(d->stream));
   return AMCA_OK;
@@ -773,6 +792,13 @@ amca_status amca_batch_create(amca_domain* const* doms, int32_t n, amca_batch** 
     if (doms[i]->timing) {
       delete b;
       return fail(AMCA_ERR_INVALID, "disable per-kernel timing before batching a domain");
+    }
+    amca::DevCounters c;
+    amca_status rows = read_counters(doms[i], &c);
+    if (rows == AMCA_OK) rows = check_rows(doms[i], c);
+    if (rows != AMCA_OK) {
+      delete b;
+      return rows;
     }
     b->doms.push_back(doms[i]);
   }

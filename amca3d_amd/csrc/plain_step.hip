@@ -100,7 +100,8 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
                                               int nblk) {
   DevCounters* ctr = B.ctr;
   const double t = ctr->t;
-  const int skip = (t < P.final_time && !ctr->done) ? 0 : 1;
+  // an exhausted orientation table (err) stops the run like termination does
+  const int skip = (t < P.final_time && !ctr->done && !ctr->err) ? 0 : 1;
   if (blk == 0 && threadIdx.x == 0) ctr->skip = skip;
   if (skip) return;
 
@@ -180,7 +181,14 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
   if (threadIdx.x == 0 && bid == nblk - 1) {
     const int nn = (int)(run >> 32);
     const int nk = (int)(run & 0xffffffffull);
-    if (no0 + nn > P.orient_cap) ctr->err = 1;
+    if (no0 + nn > P.orient_cap) {
+      // nuclei beyond the table were not written: stop here, so that no
+      // later kernel of this step or a later one reads their list entries
+      ctr->err = 1;
+      ctr->skip = 1;
+      ctr->tickets[0] = 0u;
+      return;
+    }
     const int nA = nA0 + nn;
     ctr->n_grains = ng0 + nn;
     ctr->n_orient = no0 + nn;

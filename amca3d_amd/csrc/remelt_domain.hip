@@ -23,6 +23,7 @@
 #include "host/generators.h"
 #include "host/sites.h"
 #include "host/voxel_map.h"
+#include "libm_pow.h"
 #include "plain_step.h"
 #include "remelt_step.h"
 #include "surrogate.h"
@@ -313,9 +314,10 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
   *out = nullptr;
   if (desc->nx < 1 || desc->ny < 1 || desc->nz < 1 || !(desc->h > 0))
     return rm_fail(AMCA_ERR_INVALID, "bad grid");
-  if (desc->b1 != 1.0 || desc->b2 != 2.0 || desc->b3 != 3.0)
-    return rm_fail(AMCA_ERR_INVALID,
-                   "kinetics exponents b1, b2, b3 must be 1, 2, 3 on the device path");
+  for (double b : {desc->b1, desc->b2, desc->b3})
+    if (!amca_libm::pow_exponent_supported(b))
+      return rm_fail(AMCA_ERR_INVALID,
+                     "kinetics exponents b1, b2, b3 must be 0 or of magnitude in [2^-65, 2^63)");
   const long long nxG = desc->nx + 2LL, nyG = desc->ny + 2LL, nzG = desc->nz + 2LL;
   const long long NG = nxG * nyG * nzG;
   if (NG > (long long)INT_MAX) return rm_fail(AMCA_ERR_INVALID, "grid exceeds int32 cell indices");
@@ -337,6 +339,7 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
   P.h = desc->h; P.x0 = desc->x0; P.y0 = desc->y0; P.z0 = desc->z0;
   P.Tm = desc->Tm; P.Ts = desc->Ts; P.Tmin_nuc = desc->Tmin_nuc;
   P.a1 = desc->a1; P.a2 = desc->a2; P.a3 = desc->a3;
+  P.b1 = desc->b1; P.b2 = desc->b2; P.b3 = desc->b3;
   P.trunc = desc->trunc;
   P.tsf = desc->time_step_factor;
   P.sqrt2f = (double)sqrtf(2.0f);
@@ -974,6 +977,30 @@ amca_status amca_remelt_read_log(amca_remelt* d, int32_t cap, amca_remelt_step_r
       rows[i].n_box = tmp[i].n_box;
     }
   }
+  return AMCA_OK;
+}
+
+amca_status amca_pow_selftest(int64_t n, const double* x, double y, double* out) {
+  if (n < 0 || (n > 0 && (!x || !out))) return rm_fail(AMCA_ERR_INVALID, "bad arguments");
+  if (!amca_libm::pow_exponent_supported(y))
+    return rm_fail(AMCA_ERR_INVALID, "exponent outside [2^-65, 2^63) in magnitude");
+  for (int64_t i = 0; i < n; i++)
+    if (!(x[i] > 0) || !(x[i] < HUGE_VAL))
+      return rm_fail(AMCA_ERR_INVALID, "bases must be finite and positive");
+  if (n == 0) return AMCA_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1)
+    return rm_fail(AMCA_ERR_NO_DEVICE, "no HIP device visible");
+  double *dx = nullptr, *dout = nullptr;
+  hipError_t e = hipMalloc(&dx, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMalloc(&dout, sizeof(double) * n);
+  if (e == hipSuccess) e = hipMemcpy(dx, x, sizeof(double) * n, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = amca::launch_pow_selftest(n, dx, y, dout, 0);
+  if (e == hipSuccess) e = hipDeviceSynchronize();
+  if (e == hipSuccess) e = hipMemcpy(out, dout, sizeof(double) * n, hipMemcpyDeviceToHost);
+  if (dx) (void)hipFree(dx);
+  if (dout) (void)hipFree(dout);
+  if (e != hipSuccess) return rm_fail(AMCA_ERR_HIP, std::string("pow self-test: ") + hipGetErrorString(e));
   return AMCA_OK;
 }
 

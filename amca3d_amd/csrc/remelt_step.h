@@ -12,7 +12,8 @@ struct RmParams {
   int NG;
   long long N;
   double h, x0, y0, z0;
-  double Tm, Ts, Tmin_nuc, a1, a2, a3;   // kinetics with exponents 1, 2, 3
+  double Tm, Ts, Tmin_nuc, a1, a2, a3;   // kinetics: v = a1 dT^b1 + a2 dT^b2 + a3 dT^b3
+  double b1, b2, b3;
   double trunc, tsf, sqrt2f, cutoff, final_time, void_temp;
   int off[26];
   int fnx, fny, fnz, nodes, ncx, ncy;    // structured FE mesh
@@ -141,6 +142,9 @@ hipError_t rm_launch_sur_error(const RmParams& P, const RmBuffers& B, hipStream_
 // T and cooling of the last step for every cell outside its active box, and
 // the coming step's nodal field into B.theta
 hipError_t rm_launch_complete(const RmParams& P, const RmBuffers& B, hipStream_t s);
+// out[i] = pow(x[i], y) as glibc computes it (x[i] > 0, y supported), n >= 1
+hipError_t launch_pow_selftest(long long n, const double* x, double y, double* out,
+                               hipStream_t s);
 // P.sur_hot from the cell table (B.sur_cell), written to *out (6 ints, device)
 hipError_t rm_launch_sur_hot(const RmParams& P, const RmBuffers& B, int* out, hipStream_t s);
 

@@ -35,6 +35,7 @@
 
 #include "device_util.h"
 #include "envelope.h"
+#include "libm_pow.h"
 #include "remelt_step.h"
 
 namespace amca {
@@ -54,17 +55,17 @@ __device__ __forceinline__ bool rm_ghost_ijk(const RmParams& P, int i, int j, in
   return i == 0 || j == 0 || k == 0 || i == P.nxG - 1 || j == P.nyG - 1 || k == P.nzG - 1;
 }
 
-// Exponents are 1, 2 and 3 (ProblemPhysics.cpp:142-158 calls pow(dT, b)).
-// glibc's pow(x, 1) is x exactly.  Its pow(x, 2) is not correctly rounded: it
-// equals x*x except for about 0.08 % of arguments, where it is the neighbouring
-// double.  This path computes x*x, so at such an undercooling a grain's growth
-// increment, and with it the half-diagonal, can differ from the reference's by
-// one ulp.  The cube term only matters when a3 != 0.
+// ProblemPhysics_RappazGandin1993::compute_growth_velocity_with_temperature_input
+// (ProblemPhysics.cpp:142-158): v = a1 pow(dT, b1) + a2 pow(dT, b2) + a3 pow(dT, b3)
+// for dT = Tm - T > 0, summed left to right, clamped at 0.  pow is glibc's
+// (libm_pow.h), so the velocity, and with it every growth increment and the
+// step's dt, is the reference's to the bit for any exponents.
 __device__ __forceinline__ double rm_velocity(const RmParams& P, double Tc) {
   const double dTu = P.Tm - Tc;
   double v = 0.0;
   if (dTu > 0) {
-    v = P.a1 * dTu + P.a2 * (dTu * dTu) + P.a3 * (dTu * dTu * dTu);
+    v = P.a1 * amca_libm::glibc_pow(dTu, P.b1) + P.a2 * amca_libm::glibc_pow(dTu, P.b2) +
+        P.a3 * amca_libm::glibc_pow(dTu, P.b3);
     if (v < 0.0) v = 0.0;
   }
   return v;
@@ -1229,6 +1230,13 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
   block_box_merge(cap_box, &ctr->acc[(st + 1) & 1][2]);
 }
 
+// pow(x[i], y) on the device, for checks of the velocity's arithmetic
+__global__ void k_pow_selftest(long long n, const double* x, double y, double* out) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    out[i] = amca_libm::glibc_pow(x[i], y);
+}
+
 // ---------------------------------------------------------------------------
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s) {
@@ -1300,6 +1308,15 @@ hipError_t rm_launch_sur_hot(const RmParams& P, const RmBuffers& B, int* out, hi
   const double thr = P.Tm - P.sur_band - P.hot_margin;
   hipLaunchKernelGGL(k_rm_sur_hot, dim3((unsigned)cb), dim3(kRmBlock), 0, s, P, B, thr,
                      reinterpret_cast<RmBox*>(out));
+  return hipGetLastError();
+}
+
+hipError_t launch_pow_selftest(long long n, const double* x, double y, double* out,
+                               hipStream_t s) {
+  long long blocks = (n + 255) / 256;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_pow_selftest, dim3((unsigned)blocks), dim3(256), 0, s, n, x, y, out);
   return hipGetLastError();
 }
 

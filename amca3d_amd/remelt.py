@@ -95,6 +95,7 @@ SIGNATURES = {
                                          F64P]),
     "amca_write_fe_frames": (C.c_int, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, C.c_int32,
                                        F64P, C.c_int32, F64P, F64P]),
+    "amca_pow_selftest": (C.c_int, [C.c_int64, F64P, C.c_double, F64P]),
     "amca_remelt_last_error": (C.c_char_p, []),
 }
 
@@ -130,6 +131,15 @@ def run_many(domains, max_steps=-1):
     outs = (RemeltCounters * n)()
     _check(lib().amca_remelt_run_many(hs, n, int(max_steps), outs), "run_many")
     return [o.as_dict() for o in outs]
+
+
+def pow_selftest(x, y):
+    """pow(x, y) element-wise by the device code of the remelting velocity."""
+    x = np.ascontiguousarray(x, np.float64)
+    out = np.empty_like(x)
+    _check(lib().amca_pow_selftest(x.size, _p(x, C.c_double), float(y), _p(out, C.c_double)),
+           "pow_selftest")
+    return out
 
 
 def map_voxels(grid, fe, bin=(5, 5, 5), tol=1e-16, max_iter=20, device=False):

@@ -308,7 +308,7 @@ typedef struct {
   int32_t nx, ny, nz;
   double h, x0, y0, z0;
   double Tm, Ts, Tmin_nuc;        /* melting, solidus, minimum_temperature_for_nucleation */
-  double a1, a2, a3, b1, b2, b3;  /* ProblemPhysics.cpp:142-158; b must be 1, 2, 3 */
+  double a1, a2, a3, b1, b2, b3;  /* ProblemPhysics.cpp:142-158; each b 0 or |b| in [2^-65, 2^63) */
   double trunc;                   /* envelope truncation coefficient */
   double time_step_factor;        /* time_step_factor_ca */
   double cutoff_factor;           /* cell_grain_length_cutoff_factor */
@@ -492,6 +492,13 @@ amca_status amca_voronoi_substrate(int32_t nx, int32_t ny, int32_t nz, double h,
 amca_status amca_write_fe_frames(const char* path, int32_t title_lines, int32_t fnx, int32_t fny,
                                  int32_t fnz, const double* node_xyz, int32_t n_frames,
                                  const double* times, const double* theta);
+
+/* out[i] = pow(x[i], y) evaluated by the device code of the remelting growth
+ * velocity (ProblemPhysics.cpp:152 calls glibc's pow(dT, b)), which restates
+ * glibc 2.35's pow: for finite x[i] > 0 and y == 0 or |y| in [2^-65, 2^63)
+ * the values equal glibc's bit for bit.  A check of the device arithmetic;
+ * AMCA_ERR_INVALID for other arguments. */
+amca_status amca_pow_selftest(int64_t n, const double* x, double y, double* out);
 const char* amca_remelt_last_error(void);
 
 /* ---- realm layer: YAML input deck -> device domain ------------------------

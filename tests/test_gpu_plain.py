@@ -338,6 +338,31 @@ def test_given_sites_identity_nucleus(amca, oracle):
     d.close()
 
 
+def test_rows_must_cover_every_site(amca, oracle):
+    """Each live site can take one orientation row, so run, step and run_fixed
+    refuse (before launching anything) while the pushed rows do not cover the
+    orientations in use plus the live sites; with the missing row pushed the
+    run proceeds."""
+    from amca3d_amd._lib import AmcaError
+    n = 12
+    p = oracle.plain_params(nx=n, ny=n, nz=n, h=1e-6, T0=-5.0, Tdot=-200.0, max_dt=1e-4,
+                            final_time=1.0)
+    cells = [oracle.cell_index(p, 4, 4, 4), oracle.cell_index(p, 8, 8, 8)]
+    eul = np.zeros((2, 3))
+    d = amca.OwnedDomain(nx=n, ny=n, nz=n, h=1e-6, orient_capacity=2, T0=-5.0, Tdot=-200.0,
+                         max_dt=1e-4, final_time=1.0)
+    d.set_sites(cells, [0.5, 0.5])
+    d.push_orientations(eul[:1], oracle.rotations(eul[:1]))
+    for call in (lambda: d.run(-1), d.step, lambda: d.run_fixed(3)):
+        with pytest.raises(AmcaError, match="do not cover"):
+            call()
+    assert d.counters()["step"] == 0
+    d.push_orientations(eul[1:], oracle.rotations(eul[1:]))
+    cnt = d.run(-1)
+    assert cnt["done"] == 1 and cnt["n_orient"] == 2
+    d.close()
+
+
 def test_stepwise_equals_run_and_repeatable(amca, oracle):
     text, prm = case(**SMALL[1], seed=5)
     a = run_gpu(amca, text)

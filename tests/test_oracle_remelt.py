@@ -263,11 +263,11 @@ def test_track_bookkeeping_invariants():
 
 def test_remelting_velocity_pow_is_not_the_product():
     """The reference's remelting velocity (ProblemPhysics.cpp:142-158) takes
-    pow(dT, b); the oracle calls glibc's pow, the device computes dT*dT.  glibc's
-    pow(x, 2) is not correctly rounded: it is the neighbouring double of x*x for
-    a small share of arguments, among them the undercooling (1600 K minus this
-    cell temperature) that puts one grain of the 240 x 64 x 40 GPU track one
-    ulp apart."""
+    pow(dT, b); the oracle calls glibc's pow.  glibc's pow(x, 2) is not
+    correctly rounded: it is the neighbouring double of x*x for a small share of
+    arguments, among them the undercooling (1600 K minus this cell temperature)
+    that a grain of the 240 x 64 x 40 GPU track meets at step 393, so the
+    velocity cannot be taken as products."""
     import math
     Tc = float.fromhex("0x1.86299cd5e950cp+10")
     x = 1600.0 - Tc
