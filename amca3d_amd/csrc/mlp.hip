@@ -12,8 +12,10 @@
 
 #include <cmath>
 #include <cstring>
+#include <mutex>
 #include <random>
 #include <string>
+#include <unordered_map>
 #include <vector>
 
 #include "../../include/amca3d.h"
@@ -263,8 +265,18 @@ hipError_t launch_fused_timed(amca_mlp* h, int mode, long long n, const float* X
 
 using namespace mlpk;
 
+// Activations of amca_mlp_forward_device_noalloc, which allocates nothing: two
+// ping-pong [kNoAllocRows][128] BF16 buffers (16 MiB), one copy per GPU the
+// library runs on, held until the process ends.  Host code never passes the
+// name itself to a kernel (on the host it is only a stand-in for the symbol);
+// each device's address comes from hipGetSymbolAddress (noalloc_scratch).
+constexpr int kNoAllocRows = AMCA_MLP_NOALLOC_MAX_ROWS;
+static __device__ __align__(256) __bf16 scratch[2 * (size_t)kNoAllocRows * kHid];
+
 struct amca_mlp {
   hipStream_t stream = nullptr;
+  int device = -1;                 // the device current at create
+  __bf16* noalloc_act = nullptr;   // that device's copy of `scratch` (two [rows][128] buffers)
   Layout lay;
   float* params = nullptr;  // FP32 master copy
   float* grads = nullptr;
@@ -304,6 +316,11 @@ namespace {
 
 thread_local std::string g_mlp_error;
 
+// per-device address of `scratch`, resolved once per device with that device
+// current (amca_mlp_create)
+std::mutex g_scratch_mu;
+std::unordered_map<int, __bf16*> g_scratch_addr;
+
 amca_status mlp_fail(amca_status s, const std::string& m) {
   g_mlp_error = m;
   return s;
@@ -321,6 +338,21 @@ amca_status mlp_fail(amca_status s, const std::string& m) {
     amca_status s_ = (expr);      \
     if (s_ != AMCA_OK) return s_; \
   } while (0)
+
+amca_status noalloc_scratch(int device, __bf16** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  auto it = g_scratch_addr.find(device);
+  if (it == g_scratch_addr.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return mlp_fail(AMCA_ERR_HIP,
+                      std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+    it = g_scratch_addr.emplace(device, static_cast<__bf16*>(p)).first;
+  }
+  *out = it->second;
+  return AMCA_OK;
+}
 
 template <class T>
 amca_status mlp_alloc(amca_mlp* h, T** p, size_t n) {
@@ -378,23 +410,31 @@ hipError_t gemm(hipStream_t s, const TA* A, int lda, const TB* B, int ldb, int M
 }
 
 // forward of n rows already in h->X; keeps Z (for a backward pass) when keep_z
-amca_status forward(amca_mlp* h, int n, bool keep_z = true) {
+// forward of n rows X[n][5] -> T[n] through the activation buffers act[4]
+// ([n][128] each; act[l] is written by layer l and read by layer l + 1), keeping
+// the pre-activations in h->Z when keep_z
+amca_status forward_rows(amca_mlp* h, int n, const float* X, __bf16* const act[4], float* T,
+                         bool keep_z) {
   const Layout& L = h->lay;
   hipStream_t s = h->stream;
   for (int l = 0; l < 4; l++) {
-    EpiArgs e{nullptr, kHid, h->params + L.b[l], keep_z ? h->Z[l] : nullptr, h->Aact[l]};
+    EpiArgs e{nullptr, kHid, h->params + L.b[l], keep_z ? h->Z[l] : nullptr, act[l]};
     if (l == 0)
-      MLP_HIP((gemm<float, float, false, false, kEpiBiasSilu>(s, h->X, kIn, h->params + L.w[0], kHid,
+      MLP_HIP((gemm<float, float, false, false, kEpiBiasSilu>(s, X, kIn, h->params + L.w[0], kHid,
                                                                n, kHid, kIn, e)));
     else
-      MLP_HIP((gemm<__bf16, float, false, false, kEpiBiasSilu>(s, h->Aact[l - 1], kHid,
+      MLP_HIP((gemm<__bf16, float, false, false, kEpiBiasSilu>(s, act[l - 1], kHid,
                                                                 h->params + L.w[l], kHid, n, kHid,
                                                                 kHid, e)));
   }
-  EpiArgs e{h->T, 1, h->params + L.b[4], nullptr, nullptr};
-  MLP_HIP((gemm<__bf16, float, false, false, kEpiStore>(s, h->Aact[3], kHid, h->params + L.w[4], 1, n,
+  EpiArgs e{T, 1, h->params + L.b[4], nullptr, nullptr};
+  MLP_HIP((gemm<__bf16, float, false, false, kEpiStore>(s, act[3], kHid, h->params + L.w[4], 1, n,
                                                          1, kHid, e)));
   return AMCA_OK;
+}
+
+amca_status forward(amca_mlp* h, int n, bool keep_z = true) {
+  return forward_rows(h, n, h->X, h->Aact, h->T, keep_z);
 }
 
 // gradients of the mean squared error over n rows (X, Y on the device) into h->grads
@@ -462,6 +502,11 @@ amca_status amca_mlp_create(uint64_t seed, amca_mlp** out) {
       st = mlp_fail(AMCA_ERR_HIP, "stream");
       break;
     }
+    if (hipGetDevice(&h->device) != hipSuccess) {
+      st = mlp_fail(AMCA_ERR_HIP, "hipGetDevice");
+      break;
+    }
+    if ((st = noalloc_scratch(h->device, &h->noalloc_act)) != AMCA_OK) break;
     const int P = h->lay.total;
     if ((st = mlp_alloc(h, &h->params, P)) != AMCA_OK) break;
     if ((st = mlp_alloc(h, &h->grads, P)) != AMCA_OK) break;
@@ -616,6 +661,29 @@ amca_status amca_mlp_forward_device(amca_mlp* h, int64_t n, const float* dX, flo
     MLP_TRY(forward(h, m, false));
     MLP_HIP(hipMemcpyAsync(dT + r, h->T, sizeof(float) * m, hipMemcpyDeviceToDevice, h->stream));
   }
+  MLP_HIP(hipStreamSynchronize(h->stream));
+  return AMCA_OK;
+}
+
+// amca_mlp_forward_device's kernels with the activations in the device's copy
+// of `scratch` and the caller's rows read and written in place: no hipMalloc,
+// no hipFree, no staging copies.  Every buffer element a layer reads was written
+// by the layer before it in the same call, so nothing needs clearing.
+amca_status amca_mlp_forward_device_noalloc(amca_mlp* h, int64_t n, const float* dX, float* dT) {
+  if (!h || n < 1 || !dX || !dT) return mlp_fail(AMCA_ERR_INVALID, "bad arguments");
+  if (n > kNoAllocRows)
+    return mlp_fail(AMCA_ERR_INVALID, "n = " + std::to_string(n) +
+                                          " rows exceeds the no-allocation maximum of " +
+                                          std::to_string(kNoAllocRows) + " rows per call");
+  int dev = -1;
+  MLP_HIP(hipGetDevice(&dev));
+  if (dev != h->device || !h->noalloc_act)
+    return mlp_fail(AMCA_ERR_INVALID, "make the network's device (" + std::to_string(h->device) +
+                                          ") current before the call");
+  __bf16* a0 = h->noalloc_act;
+  __bf16* a1 = h->noalloc_act + (size_t)kNoAllocRows * kHid;
+  __bf16* const act[4] = {a0, a1, a0, a1};
+  MLP_TRY(forward_rows(h, (int)n, dX, act, dT, false));
   MLP_HIP(hipStreamSynchronize(h->stream));
   return AMCA_OK;
 }

@@ -36,6 +36,8 @@ SIGNATURES = {
     "amca_mlp_train_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_int32,
                                         C.c_int32, C.c_float, C.c_uint64, F32P]),
     "amca_mlp_forward_device": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p]),
+    "amca_mlp_forward_device_noalloc": (C.c_int, [C.c_void_p, C.c_int64, C.c_void_p,
+                                                  C.c_void_p]),
     "amca_mlp_forward_fused": (C.c_int, [C.c_void_p, C.c_int32, F32P, F32P]),
     "amca_mlp_enable_timing": (C.c_int, [C.c_void_p, C.c_int32]),
     "amca_mlp_fused_times": (C.c_int, [C.c_void_p, F64P, C.POINTER(C.c_int64),

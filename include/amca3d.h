@@ -225,6 +225,15 @@ amca_status amca_mlp_train_device(amca_mlp* m, int64_t n, const float* dX, const
 amca_status amca_mlp_forward_fused(amca_mlp* h, int32_t n, const float* x5, float* t);
 /* Device rows dX[n][5] -> dT[n]. */
 amca_status amca_mlp_forward_device(amca_mlp* m, int64_t n, const float* dX, float* dT);
+/* amca_mlp_forward_device without any allocation: no counterpart in the
+ * reference; for callers that cannot allocate (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The same kernels and
+ * results; the activations go through a file-scope device array of 16 MiB per
+ * GPU (held until the process ends, shared by every call on that GPU, so
+ * calls that overlap on one GPU race).  n <= AMCA_MLP_NOALLOC_MAX_ROWS per call
+ * (AMCA_ERR_INVALID beyond it); the network's device must be current. */
+#define AMCA_MLP_NOALLOC_MAX_ROWS 32768
+amca_status amca_mlp_forward_device_noalloc(amca_mlp* m, int64_t n, const float* dX, float* dT);
 /* Device time of the fused kernel's launches (amca_mlp_forward_fused and the
  * lattice evaluations of amca_remelt_use_surrogate), from HIP events on the
  * network's stream: off by default; enabling resets the totals (milliseconds,
