@@ -41,6 +41,7 @@ class StepRecord(C.Structure):
         ("n_nucleated", C.c_int32), ("n_active_before", C.c_int32),
         ("n_captured", C.c_int32), ("n_active_after", C.c_int32),
         ("n_liquid", C.c_int64), ("n_wave1", C.c_int32), ("max_wave", C.c_int32),
+        ("n_sites", C.c_int32),
     ]
 
 

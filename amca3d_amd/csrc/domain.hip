@@ -459,6 +459,7 @@ amca_status amca_domain_read_log(amca_domain* d, int32_t cap, amca_step_record* 
       rows[i].n_liquid = tmp[i].n_liquid;
       rows[i].n_wave1 = tmp[i].n_wave1;
       rows[i].max_wave = tmp[i].max_wave;
+      rows[i].n_sites = tmp[i].n_sites;
     }
   }
   return AMCA_OK;

@@ -38,6 +38,7 @@ struct DevCounters {
   int nW[2];              // capture-wave list lengths
   int n_sites;
   int step_nuc, step_cap, step_nA0;
+  int step_sites;         // live sites at the running step's start
   int max_wave;
   int step_wave;          // deepest capturing wave of the running step
   int wave_gbase;         // first new grain id of the running wave
@@ -51,6 +52,7 @@ struct StepLogRow {
   int n_nucleated, n_active_before, n_captured, n_active_after;
   long long n_liquid;
   int n_wave1, max_wave;
+  int n_sites;
 };
 
 // Device buffers of one domain (all owned by the handle).

@@ -196,6 +196,7 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
     ctr->n_sites = nk;
     ctr->site_sel = sel ^ 1;
     ctr->step_nuc = nn;
+    ctr->step_sites = n;
     ctr->step_nA0 = nA;
     ctr->step_cap = 0;
     ctr->step_wave = 0;
@@ -651,6 +652,7 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
       row.n_liquid = liquid;
       row.n_wave1 = n_wave1;
       row.max_wave = ctr->step_wave;
+      row.n_sites = ctr->step_sites;
       B.log[st] = row;
     }
     ctr->parity ^= 1;

@@ -1,5 +1,7 @@
 """Summarise rocprofv3 output into the short per-kernel tables kept under profiles/.
 
+Whole process:
+
     python -m amca3d_amd.prof_summary --stats KT/bench_kernel_stats.csv \
         --fetch PMC_F/bench_counter_collection.csv --write PMC_W/bench_counter_collection.csv \
         --copy-fetch CF/copy_counter_collection.csv --copy-write CW/copy_counter_collection.csv \
@@ -8,6 +10,17 @@
 FETCH_SIZE / WRITE_SIZE are KiB per dispatch as the counters report them; they
 are scaled by the factor that maps the copy probe's counters onto its known
 traffic (k_copy_probe reads and writes `--copy-bytes` bytes per dispatch).
+
+A window of plain CA steps (bench.py's timed window is the process's last
+steps):
+
+    python -m amca3d_amd.prof_summary --trace KT/bench_kernel_trace.csv --window-steps 20 \
+        --fetch ... --write ... --l2 PMC_L2/bench_counter_collection.csv ...
+
+keeps the last 10 x window-steps dispatches of the step kernels (k_nucleate,
+then k_claim, k_resolve, k_scan, k_emit for wave 0 and for wave 1, then
+k_tail: ten per step, checked), from the trace and from every counter file,
+and reports each kernel over the window and per wave.
 """
 import argparse
 import csv
@@ -42,9 +55,114 @@ def counter_means(path, counter):
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
 
+STEP_KERNELS = ["k_nucleate", "k_claim", "k_resolve", "k_scan", "k_emit", "k_claim", "k_resolve",
+                "k_scan", "k_emit", "k_tail"]
+STEP_LABELS = ["k_nucleate", "k_claim.w0", "k_resolve.w0", "k_scan.w0", "k_emit.w0",
+               "k_claim.w1", "k_resolve.w1", "k_scan.w1", "k_emit.w1", "k_tail"]
+
+
+def window_rows(rows, steps):
+    """The last `steps` plain steps among (dispatch id, kernel, value) rows:
+    {label: [values]} with labels per position in the step."""
+    rows = sorted((r for r in rows if r[1] in STEP_KERNELS), key=lambda r: r[0])
+    n = len(STEP_KERNELS) * steps
+    if len(rows) < n:
+        raise SystemExit(f"only {len(rows)} step-kernel dispatches, need {n}")
+    rows = rows[-n:]
+    out = defaultdict(list)
+    for i, (_, name, v) in enumerate(rows):
+        q = i % len(STEP_KERNELS)
+        if name != STEP_KERNELS[q]:
+            raise SystemExit(f"dispatch {i} of the window is {name}, expected {STEP_KERNELS[q]}")
+        out[STEP_LABELS[q]].append(v)
+    return out
+
+
+def trace_rows(path):
+    with open(path) as f:
+        return [(int(r["Dispatch_Id"]), short_name(r["Kernel_Name"]),
+                 (int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) * 1e-3)
+                for r in csv.DictReader(f)]
+
+
+def counter_rows(path, counter):
+    with open(path) as f:
+        return [(int(r["Dispatch_Id"]), short_name(r["Kernel_Name"]), float(r["Counter_Value"]))
+                for r in csv.DictReader(f) if r["Counter_Name"] == counter]
+
+
+def window_main(a):
+    """Per-kernel and per-wave table of the window (see the module docstring)."""
+    res = {"workload": a.workload, "command": a.command, "window_steps": a.window_steps,
+           "kernels": {}}
+    fscale = wscale = None
+    if a.copy_fetch and a.copy_write and a.copy_bytes > 0:
+        cf = counter_means(a.copy_fetch, "FETCH_SIZE")["k_copy_probe"][0]
+        cw = counter_means(a.copy_write, "WRITE_SIZE")["k_copy_probe"][0]
+        kib = a.copy_bytes / 1024.0
+        fscale, wscale = kib / cf, kib / cw
+        res["copy_calibration"] = {"bytes": a.copy_bytes, "fetch_kib": cf, "write_kib": cw,
+                                   "fetch_scale": fscale, "write_scale": wscale}
+    dur = window_rows(trace_rows(a.trace), a.window_steps)
+    fetch = window_rows(counter_rows(a.fetch, "FETCH_SIZE"), a.window_steps) if a.fetch else {}
+    write = window_rows(counter_rows(a.write, "WRITE_SIZE"), a.window_steps) if a.write else {}
+    hit = window_rows(counter_rows(a.l2, "TCC_HIT"), a.window_steps) if a.l2 else {}
+    miss = window_rows(counter_rows(a.l2, "TCC_MISS"), a.window_steps) if a.l2 else {}
+
+    def row(labels):
+        d = [v for l in labels for v in dur[l]]
+        r = {"calls": len(d), "total_ms": sum(d) * 1e-3, "avg_us": sum(d) / len(d)}
+        if fscale and all(l in fetch and l in write for l in labels):
+            fk = [v for l in labels for v in fetch[l]]
+            wk = [v for l in labels for v in write[l]]
+            r["fetch_kib"] = sum(fk) / len(fk)
+            r["write_kib"] = sum(wk) / len(wk)
+            r["bytes_per_launch"] = 1024.0 * (r["fetch_kib"] * fscale + r["write_kib"] * wscale)
+        if all(l in hit and l in miss for l in labels):
+            h = sum(v for l in labels for v in hit[l])
+            m = sum(v for l in labels for v in miss[l])
+            r["l2_hit_rate"] = h / (h + m) if h + m > 0 else None
+        return r
+
+    for k in dict.fromkeys(STEP_KERNELS):
+        res["kernels"][k] = row([l for l in STEP_LABELS if l.split(".")[0] == k])
+    res["by_wave"] = {l: row([l]) for l in STEP_LABELS}
+    step_us = sum(sum(v) for v in dur.values()) / a.window_steps
+    res["kernel_us_per_step"] = step_us
+    if a.json:
+        with open(a.json, "w") as f:
+            json.dump(res, f, indent=1)
+    lines = [f"# {a.workload}", "", "Command:", "", "```", a.command, "```", "",
+             f"Window: the last {a.window_steps} steps of the process (bench.py's timed window), "
+             f"{len(STEP_KERNELS)} dispatches per step; kernel time {step_us:.1f} us per step.", ""]
+    if fscale:
+        lines += [f"Copy calibration: k_copy_probe moves {a.copy_bytes:.0f} B each way per "
+                  f"dispatch; FETCH_SIZE reads {res['copy_calibration']['fetch_kib']:.1f} KiB, "
+                  f"WRITE_SIZE {res['copy_calibration']['write_kib']:.1f} KiB -> scale "
+                  f"{fscale:.4f} (fetch), {wscale:.4f} (write).", ""]
+
+    def table(title, rows):
+        out = [title, "", "| kernel | calls | total ms | avg us | HBM bytes/launch (scaled PMC) "
+                          "| L2 hit rate |", "|---|---|---|---|---|---|"]
+        for k, r in rows.items():
+            b, hr = r.get("bytes_per_launch"), r.get("l2_hit_rate")
+            out.append(f"| {k} | {r['calls']} | {r['total_ms']:.3f} | {r['avg_us']:.2f} | "
+                       f"{'' if b is None else f'{b:.4g}'} | {'' if hr is None else f'{hr:.3f}'} |")
+        return out + [""]
+    lines += table("By kernel:", res["kernels"]) + table("By wave:", res["by_wave"])
+    text = "\n".join(lines)
+    if a.md:
+        with open(a.md, "w") as f:
+            f.write(text)
+    print(text)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--stats", required=True)
+    ap.add_argument("--stats")
+    ap.add_argument("--trace")
+    ap.add_argument("--window-steps", type=int, default=0)
+    ap.add_argument("--l2")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
     ap.add_argument("--copy-fetch")
@@ -55,6 +173,8 @@ def main():
     ap.add_argument("--json")
     ap.add_argument("--md")
     a = ap.parse_args()
+    if a.window_steps > 0:
+        return window_main(a)
 
     st = kernel_stats(a.stats)
     res = {"workload": a.workload, "command": a.command, "kernels": {}}

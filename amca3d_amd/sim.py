@@ -47,11 +47,12 @@ class Domain:
         rows = (StepRecord * max(m, 1))()
         check(load().amca_domain_read_log(self.h, m, rows, C.byref(n)), "read_log")
         return np.array([(r.time_before, r.dt, r.n_nucleated, r.n_active_before, r.n_captured,
-                          r.n_active_after, r.n_liquid, r.n_wave1, r.max_wave) for r in rows[:m]],
+                          r.n_active_after, r.n_liquid, r.n_wave1, r.max_wave, r.n_sites)
+                         for r in rows[:m]],
                         dtype=[("time_before", "f8"), ("dt", "f8"), ("n_nucleated", "i4"),
                                ("n_active_before", "i4"), ("n_captured", "i4"),
                                ("n_active_after", "i4"), ("n_liquid", "i8"), ("n_wave1", "i4"),
-                               ("max_wave", "i4")])
+                               ("max_wave", "i4"), ("n_sites", "i4")])
 
     def download(self, env=True, site_cap=1 << 22):
         ori = np.empty(self.ng, np.int32)

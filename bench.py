@@ -1,11 +1,14 @@
-"""Benchmark of the plain CA step on one MI355X.
+"""Benchmark of the CA step on one MI355X.
 
-Workload: 512^3 cells of 2e-6 m, uniform cooling -200 K/s from T0 = -0.2 K,
-bulk Gaussian nucleation 5e13 m^-3 N(3, 1), surface 1e9 m^-2 N(1, 0.5) on the
-z = 0 face, RappazGandin kinetics (a1 = -0.544e-4, a2 = 2.03e-4), FP64
-envelopes, maxDt 1e-4 s.  A "step" is one CA step (nucleation, dt, growth,
-capture cascade, time advance) over the whole grid; value = cells x executed
-steps / wall time of the timed region, with the grid resident in HBM.
+Default workload: 512^3 cells of 2e-6 m, uniform cooling -200 K/s from
+T0 = -0.2 K, bulk Gaussian nucleation 5e13 m^-3 N(3, 1), surface 1e9 m^-2
+N(1, 0.5) on the z = 0 face, RappazGandin kinetics (a1 = -0.544e-4,
+a2 = 2.03e-4), FP64 envelopes, maxDt 1e-4 s.  A "step" is one CA step
+(nucleation, dt, growth, capture cascade, time advance) over the whole grid.
+The timed window is --steps steps centred on the run's largest active front,
+after --warmup untimed steps; value = cells x executed steps / wall time of
+the window, with the grid resident in HBM.  The full run's figure is reported
+beside it.  Other workloads: --workload pbf | batch | surrogate.
 
 Prints one JSON line (rank 0).  --gpus above 1 is refused: this build runs on
 one GPU.
@@ -23,46 +26,94 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "CA cell-updates/sec + % HBM peak; batched+surrogate vs plain GPU at fixed grain-ID error"
 
 
-def algorithmic_bytes(log):
-    """Algorithmic HBM bytes per kernel over a step log (unit figures of the
-    plain step: 48 B per active grain, 44 B per capture, 12 B per live site).
+def algorithmic_bytes(log, chunk=4096):
+    """Algorithmic HBM bytes per kernel over a step log, from the unit figures
+    of the plain step (48 B per active grain, 44 B per capture, 12 B per live
+    site):
 
+    k_nucleate 12 B per live site (cell index 4 + critical undercooling 8)
     k_claim    wave 0: 48 B per active grain (centre + L read 32, orientation id
                4, grain id 4, grown L written 8); wave 1: 40 B per wave-1 grain
                (no growth write)
     k_resolve  4 B per survivor-list entry written
+    k_scan     16 B per list chunk of `chunk` entries (its aggregate read and
+               its prefix written), for wave 0 and wave 1
     k_emit     44 B per capture: child envelope 32, orientation 4, grain id 4,
                wave-list entry 4 (captures of waves >= 2, made in k_tail, are a
                handful per step and are counted here too)
-    k_nucleate 12 B per live site (cell index 4 + critical undercooling 8)
+    k_tail     120 B per step (the step's log row 56 and counter block 64)
     """
     A = log["n_active_before"].astype(np.float64)
     W1 = log["n_wave1"].astype(np.float64)
     C = log["n_captured"].astype(np.float64)
     Aout = log["n_active_after"].astype(np.float64)
+    S = log["n_sites"].astype(np.float64)
+    chunks = np.maximum(1.0, np.ceil(A / chunk)) + np.maximum(1.0, np.ceil(W1 / chunk))
     return {
+        "k_nucleate": 12.0 * S,
         "k_claim": 48.0 * A + 40.0 * W1,
         "k_resolve": 4.0 * Aout,
+        "k_scan": 16.0 * chunks,
         "k_emit": 44.0 * C,
+        "k_tail": 120.0 * np.ones_like(A),
     }
 
 
-def cpu_baseline(seconds_target=20.0):
-    """The oracle (one core) on the same physics at 160^3, to full solidification
-    (about 10 s of CPU work)."""
+def step_formula_bytes(log, cells):
+    """Whole-step algorithmic bytes of the plain step, 4 N + 48 A + 44 C + 12 S
+    (N cells, A grains active at the step's start, C captures, S live sites):
+    the grain-ID field scan that finds the interface, the active grains'
+    records, the captures' records and the sites."""
+    return (4.0 * cells + 48.0 * log["n_active_before"].astype(np.float64) +
+            44.0 * log["n_captured"].astype(np.float64) +
+            12.0 * log["n_sites"].astype(np.float64))
+
+
+def _oracle_replica(job):
+    """One oracle run of the 512^3 workload's physics at edge n, seed `seed`
+    (a process of its own: cell-updates, wall seconds, steps)."""
+    n, seed = job
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    n = 160
     p = oracle_lib.plain_params(nx=n, ny=n, nz=n, h=2e-6, ns=1e9, dTs_mean=1.0, dTs_sd=0.5,
                                 surface_mode=1, nv=5e13, dTv_mean=3.0, dTv_sd=1.0, Tdot=-200.0,
-                                max_dt=1e-4, final_time=1.0)
+                                max_dt=1e-4, final_time=1.0, seed=seed)
     t0 = time.perf_counter()
     r = oracle_lib.run_plain(p, want_grains=False)
+    return n ** 3 * r["n_steps"], time.perf_counter() - t0, r["n_steps"]
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(n=160, cores=None):
+    """The CPU path as a throughput reference: one oracle replica per host core
+    (seeds 0..cores-1, the 512^3 workload's physics at n^3, each to full
+    solidification, about 15-25 s), run in child processes started before this
+    process touches the GPU.  value = all replicas' cell-updates / the wall
+    time of the slowest; the one-core figure is replica 0's rate."""
+    import multiprocessing as mp
+    cores = cores or max(1, min(16, os.cpu_count() or 1))
+    ctx = mp.get_context("spawn")
+    t0 = time.perf_counter()
+    with ctx.Pool(cores) as pool:
+        runs = pool.map(_oracle_replica, [(n, seed) for seed in range(cores)])
     wall = time.perf_counter() - t0
-    return {"value": n ** 3 * r["n_steps"] / wall, "unit": "cell-updates/s", "cores": 1,
-            "kind": "port",
-            "sample": f"CPU oracle (sequential restatement, g++ -O2) on the same physics at "
-                      f"{n}^3, {r['n_steps']} steps to full solidification, {wall:.1f} s"}
+    work = sum(r[0] for r in runs)
+    return {"value": work / wall, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "one_core_value": runs[0][0] / runs[0][1],
+            "sample": f"CPU oracle (sequential restatement, g++ -O2), {cores} replicas in "
+                      f"parallel, one per core, seeds 0..{cores - 1}: the same physics at "
+                      f"{n}^3 to full solidification ({min(r[2] for r in runs)}-"
+                      f"{max(r[2] for r in runs)} steps), {wall:.1f} s wall"}
 
 
 def copy_probe(mib=2048, reps=20):
@@ -227,7 +278,7 @@ def pbf_main(args):
         result["kernel_ms"] = {k: round(v, 3) for k, v in total.items()}
     if not args.no_cpu_baseline:
         result["cpu_baseline"] = pbf_cpu_baseline()
-    print(json.dumps(result))
+    return result
 
 
 def plain_step_bytes(log):
@@ -297,7 +348,7 @@ def batch_main(args):
                      "achieved": algo / wall / 1e9, "frac": algo / wall / 1e9 / HBM_PEAK_GBS,
                      "traffic": None},
     }
-    print(json.dumps(result))
+    return result
 
 
 MLP_FLOPS_PER_ROW = 2 * (5 * 128 + 3 * 128 * 128 + 128)  # 99,840
@@ -429,16 +480,20 @@ def surrogate_main(args):
                      "traffic": None},
     }
     m.close()
-    print(json.dumps(result))
+    return result
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=450)
-    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: 100 for plain512, 450 otherwise)")
+    ap.add_argument("--warmup", type=int, default=None,
+                    help="untimed steps before them (default: 10 for plain512, 20 otherwise)")
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-sub-results", action="store_true",
+                    help="plain512: skip the batched and surrogate sub-results")
     ap.add_argument("--no-timing", action="store_true", help="skip per-kernel HIP events")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run the stream-copy probe (for counter calibration)")
@@ -454,54 +509,137 @@ def main():
     ap.add_argument("--whole-grid", action="store_true",
                     help="pbf: every step over the whole grid (no active box)")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 100 if args.workload == "plain512" else 450
+    if args.warmup is None:
+        args.warmup = 10 if args.workload == "plain512" else 20
     if args.gpus != 1:
         print(f"bench.py: --gpus {args.gpus} is not supported; this build runs the CA step on "
               "one GPU only (a single device-resident grid)", file=sys.stderr)
         sys.exit(2)
 
-    import amca3d_amd
-    from amca3d_amd import decks
+    # the CPU baseline runs first, in child processes, before anything here
+    # touches the GPU
+    cpu = None
+    if args.workload == "plain512" and not args.copy_probe and not args.no_cpu_baseline:
+        cpu = cpu_baseline()
 
     if args.copy_probe:
         print(json.dumps({"copy_probe": copy_probe()}))
         return
     if args.workload == "pbf":
-        pbf_main(args)
+        print(json.dumps(pbf_main(args)))
         return
     if args.workload == "batch":
-        batch_main(args)
+        print(json.dumps(batch_main(args)))
         return
     if args.workload == "surrogate":
-        surrogate_main(args)
+        print(json.dumps(surrogate_main(args)))
         return
+    # the sub-results run before the headline, so that the headline's window
+    # stays the process's last plain steps (the profile summary selects them so)
+    sub = None if args.no_sub_results else sub_results(args)
+    result = plain_main(args, cpu)
+    if sub is not None:
+        result["sub_results"] = sub
+    print(json.dumps(result))
 
+
+def sub_results(args):
+    """The metric's second half, measured in the same run: the batched +
+    surrogate-driven coupled form against the plain GPU form at fixed grain-ID
+    error (--workload surrogate, 450 steps after 20), and the batched plain
+    form against the same domains one after another (--workload batch)."""
+    sub = argparse.Namespace(**vars(args))
+    sub.steps, sub.warmup, sub.no_cpu_baseline = 450, 20, True
+    t0 = time.perf_counter()
+    s = surrogate_main(sub)
+    c = s["config"]
+    sur = {"workload": c["workload"], "value": s["value"], "unit": s["unit"],
+           "steps": s["steps"], "warmup": s["warmup"], "ms_per_step": s["ms_per_step"],
+           "plain_gpu_whole_grid_value": c["plain_gpu_whole_grid_value"],
+           "plain_gpu_active_box_value": c["plain_gpu_active_box_value"],
+           "fe_batched_value": c["fe_batched_value"],
+           "over_plain_gpu_whole_grid": c["over_plain_gpu_whole_grid"],
+           "over_plain_gpu_active_box": c["over_plain_gpu_active_box"],
+           "surrogate_over_fe_batched": c["surrogate_over_fe_batched"],
+           "grain_id_mismatch_fraction_max": c["grain_id_mismatch_fraction_max"],
+           "temperature_rms_served_K_max": c["temperature_rms_served_K_max"],
+           "train_seconds": c["train_seconds"], "roofline": s["roofline"],
+           "seconds": time.perf_counter() - t0}
+    t0 = time.perf_counter()
+    b = batch_main(sub)
+    c = b["config"]
+    bat = {"workload": c["workload"], "value": b["value"], "unit": b["unit"],
+           "sequential_single_domain_value": c["sequential_single_domain_value"],
+           "batched_over_sequential": c["batched_over_sequential"],
+           "bit_equal_to_single_runs": c["bit_equal_to_single_runs"],
+           "schedule": c["schedule"], "roofline": b["roofline"],
+           "seconds": time.perf_counter() - t0}
+    return {"batched_surrogate_pbf": sur, "batched_plain": bat}
+
+
+def plain_main(args, cpu):
+    """The headline: exactly --steps steps of the 512^3 run, timed where the
+    active front is largest.
+
+    1. The whole run once, to full solidification: its wall time gives the
+       full-run figure and its step log the step with the most active grains.
+    2. A second domain from the same deck is fast-forwarded, untimed, so that
+       after --warmup more untimed steps the timed window of --steps steps is
+       centred on that peak; then the window is timed (per-kernel HIP events
+       on the domain's stream).  Its kernels are the process's last
+       10 x --steps step-kernel dispatches, which is how the profile summary
+       (amca3d_amd/prof_summary.py --window-steps) picks them out.  The window's
+       grain-ID field equals the first run's at the same step count by
+       construction of the path (the tests check stepwise == run)."""
+    import amca3d_amd
+    from amca3d_amd import decks
     n = args.size
+    cells = n ** 3
     deck = decks.uniform_cooling_512_deck(n=n)
-    if args.warmup > 0:
-        with amca3d_amd.Simulation(deck) as w:
-            w.initialize()
-            w.domain.run_fixed(args.warmup)
+
+    full = amca3d_amd.Simulation(deck).initialize()
+    fd = full.domain
+    fd.enable_log(1 << 15)
+    fd.counters()
+    t0 = time.perf_counter()
+    cf = fd.run(-1)
+    full_wall = time.perf_counter() - t0
+    full_log = fd.read_log()
+    full.close()
+    total = int(cf["step"])
+    peak = int(np.argmax(full_log["n_active_before"]))
+    K, W = args.steps, args.warmup
+    first = min(max(peak - K // 2, W), max(total - K, W))
+    ff = first - W
+
     sim = amca3d_amd.Simulation(deck).initialize()
     dom = sim.domain
-    dom.enable_log(args.steps + 8)
+    dom.enable_log(max(total, first + K) + 8)
+    if ff > 0:
+        dom.run_fixed(ff)
+    if W > 0:
+        dom.run_fixed(W)
     if not args.no_timing:
         dom.enable_timing(True)
-    dom.counters()  # drains the stream
+    c0 = dom.counters()  # drains the stream
     t0 = time.perf_counter()
-    dom.run_fixed(args.steps)
+    dom.run_fixed(K)
     wall = time.perf_counter() - t0
     c = dom.counters()
-    executed = int(c["step"])
-    cells = n ** 3
+    s0, s1 = int(c0["step"]), int(c["step"])
+    executed = s1 - s0
     value = cells * executed / wall
-    log = dom.read_log()
+    log = dom.read_log()[s0:s1]
+    step_bytes = float(step_formula_bytes(log, cells).sum())
     result = {
         "metric": METRIC,
         "value": value,
         "unit": "cell-updates/s",
         "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
+        "steps": K,
+        "warmup": W,
         "ms_per_step": 1e3 * wall / max(executed, 1),
         "higher_is_better": True,
         "scaling": "weak",
@@ -510,36 +648,56 @@ def main():
         "data": "synthetic (host-drawn nucleation sites and orientations, seed 0)",
         "config": {"workload": f"plain CA, uniform cooling, {n}^3 cells of 2e-6 m, -200 K/s, "
                                "bulk 5e13 m^-3 N(3,1), surface 1e9 m^-2 N(1,0.5) on z=0",
-                   "cells": cells, "executed_steps": executed, "sim_time_s": c["time"],
-                   "liquid_left": int(c["n_liquid"]), "grains": int(c["n_grains"]),
-                   "max_cascade_wave": int(c["max_wave"])},
+                   "cells": cells,
+                   "window": {"first_step": s0, "last_step": s1 - 1, "executed_steps": executed,
+                              "placement": "centred on the step with the most active grains "
+                                           f"({peak}) of the {total}-step run",
+                              "active_grains_mean": float(log["n_active_before"].mean()),
+                              "captures_per_step_mean": float(log["n_captured"].mean()),
+                              "liquid_left_at_end": int(c["n_liquid"]),
+                              "max_cascade_wave": int(log["max_wave"].max()) if executed else 0},
+                   "full_run": {"steps": total, "wall_s": full_wall,
+                                "value": cells * total / full_wall,
+                                "ms_per_step": 1e3 * full_wall / max(total, 1),
+                                "grains": int(cf["n_grains"]),
+                                "max_cascade_wave": int(cf["max_wave"])},
+                   "step_formula": {"bytes_per_step": step_bytes / max(executed, 1),
+                                    "achieved_gbs": step_bytes / wall / 1e9,
+                                    "frac": step_bytes / wall / 1e9 / HBM_PEAK_GBS,
+                                    "formula": "4 N + 48 A + 44 C + 12 S per step"}},
     }
     if not args.no_timing:
         kt = dom.kernel_times()
-        total = {k: v[0] for k, v in kt.items()}
-        dom_k = max(total, key=total.get)
+        tot = {k: v[0] for k, v in kt.items()}
+        dom_k = max(tot, key=tot.get)
         ab = algorithmic_bytes(log)
-        roof = {"bound": "hbm", "peak": HBM_PEAK_GBS, "unit": "GB/s", "kernel": dom_k,
-                "kernel_ms_total": total[dom_k], "launches": kt[dom_k][1],
-                "avg_launch_us": 1e3 * total[dom_k] / max(kt[dom_k][1], 1),
-                "kernel_share_of_wall": total[dom_k] / (1e3 * wall)}
-        if dom_k in ab:
-            bytes_total = float(ab[dom_k].sum())
-            achieved = bytes_total / (total[dom_k] * 1e-3) / 1e9
-            roof.update(achieved=achieved, frac=achieved / HBM_PEAK_GBS,
-                        bytes_per_launch=bytes_total / max(kt[dom_k][1], 1))
-        else:
-            roof.update(achieved=None, frac=None)
-        roof["traffic"] = measured_traffic(dom_k)
+        per = {}
+        for k, (ms, launches) in kt.items():
+            if launches == 0:
+                continue
+            b = float(ab[k].sum()) if k in ab else None
+            per[k] = {"ms": round(ms, 4), "launches": launches,
+                      "avg_launch_us": round(1e3 * ms / launches, 3),
+                      "algorithmic_bytes_per_launch": None if b is None else b / launches,
+                      "achieved_gbs": None if b is None else b / (ms * 1e-3) / 1e9}
+        b = float(ab[dom_k].sum())
+        achieved = b / (tot[dom_k] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": achieved / HBM_PEAK_GBS, "traffic": measured_traffic(dom_k),
+                "kernel": dom_k, "kernel_ms_total": tot[dom_k], "launches": kt[dom_k][1],
+                "avg_launch_us": 1e3 * tot[dom_k] / max(kt[dom_k][1], 1),
+                "bytes_per_launch": b / max(kt[dom_k][1], 1),
+                "kernel_share_of_wall": tot[dom_k] / (1e3 * wall),
+                "kernels_share_of_wall": sum(tot.values()) / (1e3 * wall)}
         cp = copy_probe()
         if cp:
             roof["measured_copy_gbs"] = cp["gbps"]
         result["roofline"] = roof
-        result["kernel_ms"] = {k: round(v, 3) for k, v in total.items()}
-    if not args.no_cpu_baseline:
-        result["cpu_baseline"] = cpu_baseline()
+        result["kernels"] = per
+    if cpu is not None:
+        result["cpu_baseline"] = cpu
     sim.close()
-    print(json.dumps(result))
+    return result
 
 
 if __name__ == "__main__":

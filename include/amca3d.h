@@ -75,6 +75,7 @@ typedef struct {
   int64_t n_liquid;
   int32_t n_wave1;    /* grains captured by the grains active at the step's start */
   int32_t max_wave;   /* deepest capture wave that captured a cell (0: none) */
+  int32_t n_sites;    /* live nucleation sites at the step's start */
 } amca_step_record;
 
 /* ---- domain lifetime ---------------------------------------------------- */

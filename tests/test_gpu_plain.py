@@ -76,6 +76,12 @@ def compare(oracle, prm_kw, g, r=None):
     for f in ("time_before", "dt", "n_nucleated", "n_active_before", "n_captured",
               "n_active_after", "n_liquid", "n_wave1", "max_wave"):
         assert np.array_equal(lg[f][:n], st[f][:n].astype(lg[f].dtype)), f
+    # live sites at each step's start: all of them first, then each step drops
+    # at least its nuclei, and the last step leaves the final live list
+    ns, nn = lg["n_sites"][:n].astype(np.int64), lg["n_nucleated"][:n]
+    assert ns[0] == len(sc)
+    assert (ns[:-1] - ns[1:] >= nn[:-1]).all()
+    assert ns[-1] - nn[-1] >= len(g["sites_cell"])
     return r
 
 
