@@ -18,14 +18,19 @@ TRUNC = float(np.sqrt(np.float32(3.0)))
 
 
 def case(nx, ny, nz, h, *, surface=(0.0, 2.0, 0.5), faces="all", bulk=(0.0, 3.0, 1.0),
-         T0=-0.2, Tdot=-20.0, time_step=1e-3, termination_time=17.81, seed=0, lateral=None):
+         T0=-0.2, Tdot=-20.0, time_step=1e-3, termination_time=17.81, seed=0, lateral=None,
+         max_half_diagonal_length=None):
     text = decks.plain_deck(nx, ny, nz, h, surface=surface, surface_faces=faces, bulk=bulk,
                             T0=T0, Tdot=Tdot, time_step=time_step,
                             termination_time=termination_time, random_seed=seed or None,
-                            lateral=lateral)
+                            lateral=lateral, max_half_diagonal_length=max_half_diagonal_length)
     prm = dict(nx=nx, ny=ny, nz=nz, h=h, ns=surface[0], dTs_mean=surface[1], dTs_sd=surface[2],
                nv=bulk[0], dTv_mean=bulk[1], dTv_sd=bulk[2], surface_mode=1 if faces == "z_min" else 0,
                seed=seed, T0=T0, Tdot=Tdot, max_dt=time_step, final_time=termination_time)
+    if max_half_diagonal_length is not None:
+        # envelope.max_half_diagonal_length f: truncation sqrtf(2) * f * 0.5
+        # (Octahedron.cpp:316-320), in double after the float square root
+        prm["trunc"] = float(np.sqrt(np.float32(2.0))) * float(max_half_diagonal_length) * 0.5
     return text, prm
 
 
@@ -321,6 +326,22 @@ def test_capture_cascade_beyond_wave_one(amca, oracle):
     assert g["log"]["max_wave"][205] == 2
 
 
+def test_capture_cascade_to_wave_four(amca, oracle):
+    """96^3 with the 512^3 workload's physics and the deck's
+    envelope.max_half_diagonal_length at 8 (truncation sqrtf(2) * 8 / 2): larger
+    children capture again in the same step, down to wave 4 (two steps reach
+    wave 3 or deeper).  The whole run, 433 steps to full solidification, is
+    bit-exact against the oracle, and both logs record the waves."""
+    text, prm = case(96, 96, 96, 2e-6, surface=(1e9, 1.0, 0.5), faces="z_min",
+                     bulk=(5e13, 3.0, 1.0), Tdot=-200.0, time_step=1e-4, termination_time=1.0,
+                     max_half_diagonal_length=8.0)
+    g = run_gpu(amca, text)
+    r = compare(oracle, prm, g)
+    assert g["counters"]["done"] == 1
+    assert r["steps"]["max_wave"].max() == 4 and (r["steps"]["max_wave"] >= 3).sum() >= 2
+    assert np.array_equal(g["log"]["max_wave"], r["steps"]["max_wave"])
+
+
 def test_given_sites_identity_nucleus(amca, oracle):
     """Low-level ABI with explicit sites and orientation rows; one identity
     nucleus grows to full solidification of a 21^3 box."""
@@ -404,23 +425,38 @@ def test_edge_cases(amca, oracle):
 
 
 def test_full_size_512_invariants(amca):
-    """512^3 cells (the benchmark workload) for 120 steps: counters agree with
-    the downloaded fields, ids are a permutation, orientations are in range."""
+    """512^3 cells (the benchmark workload) to full solidification, through
+    capture cascades of depth 3 and more: every cell solid, ids a permutation
+    of 0..N-1 whose records sit at their own cells with their cells'
+    orientations, orientations in range and one per nucleus, the step log
+    adding up (nuclei + captures = N, active lists chained, liquid count
+    falling by exactly the step's new grains)."""
     s = amca.Simulation(decks.uniform_cooling_512_deck()).initialize()
     d = s.domain
-    c = s.run(120)
-    assert c["step"] == 120
-    out = d.download(env=False)
+    d.enable_log(1 << 12)
+    c = s.run(-1)
     n = 512
-    ori = out["ori"].reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
-    gid = out["gid"].reshape(n + 2, n + 2, n + 2)[1:-1, 1:-1, 1:-1]
-    solid = ori >= 0
-    nsol = int(solid.sum())
-    assert nsol == n ** 3 - c["n_liquid"] == c["n_grains"]
-    assert c["n_grains"] > 100_000
-    g = gid[solid]
-    assert g.min() == 0 and g.max() == c["n_grains"] - 1
-    assert np.bincount(g, minlength=c["n_grains"]).max() == 1
-    assert ori.max() < c["n_orient"]
-    assert ((gid < 0) == (ori < 0)).all()
+    N = n ** 3
+    assert c["done"] == 1 and c["n_liquid"] == 0 and c["n_grains"] == N
+    lg = d.read_log()
+    assert len(lg) == c["step"] > 400
+    nuc, cap = lg["n_nucleated"].astype(np.int64), lg["n_captured"].astype(np.int64)
+    assert nuc.sum() + cap.sum() == N and nuc.sum() == c["n_orient"]
+    liq = lg["n_liquid"].astype(np.int64)
+    assert (np.diff(np.concatenate([[N], liq])) == -(nuc + cap)).all()
+    assert (lg["n_active_before"][1:] == lg["n_active_after"][:-1] + lg["n_nucleated"][1:]).all()
+    assert lg["max_wave"].max() >= 3
+    out = d.download(env=False)
+    ori = out["ori"].reshape(n + 2, n + 2, n + 2)
+    gid = out["gid"].reshape(n + 2, n + 2, n + 2)
+    inner = (slice(1, -1),) * 3
+    assert (ori[inner] >= 0).all() and ori[inner].max() < c["n_orient"]
+    g = gid[inner].ravel()
+    assert np.bincount(g, minlength=N).max() == 1 and g.min() == 0 and g.max() == N - 1
+    gr = d.download_grains()
+    assert len(gr["cell"]) == N
+    # record g sits at a cell whose grain id is g, with that cell's orientation
+    assert np.array_equal(gid.ravel()[gr["cell"]], np.arange(N, dtype=np.int32))
+    assert np.array_equal(gr["orient"], ori.ravel()[gr["cell"]])
+    assert (gr["env"][:, 3] >= 0).all()
     s.close()

@@ -113,9 +113,7 @@ def test_training_and_surrogate_driven_track(amca, small_track):
     ts, tf = untrained.eval(desc, scan, pts)
     rms0 = np.sqrt(np.mean((ts - np.minimum(tf, desc.T_clip)) ** 2))
     untrained.close()
-    m, _, losses = pbf.train_surrogate(w, n_samples=4_000_000,
-                                       schedule=((8192, 4000, 1e-3), (8192, 2000, 2e-4),
-                                                 (16384, 1000, 5e-5)))
+    m, _, losses = pbf.train_surrogate(w)
     ts, tf = m.eval(desc, scan, pts)
     rms1 = np.sqrt(np.mean((ts - np.minimum(tf, desc.T_clip)) ** 2))
     assert losses[-1] < 1e-3
@@ -127,7 +125,7 @@ def test_training_and_surrogate_driven_track(amca, small_track):
         d = pbf.domain(w)
         d.enable_log(512)
         if mode == "sur":
-            d.use_surrogate(m, desc, scan, band=50.0, resid=5.0)
+            d.use_surrogate(m, desc, scan, band=50.0, resid=2.0)
         d.run(200)
         out[mode] = (d.download()["ori"], d.read_log(),
                      d.surrogate_error() if mode == "sur" else None)
@@ -137,10 +135,12 @@ def test_training_and_surrogate_driven_track(amca, small_track):
     cells = g["nx"] * g["ny"] * g["nz"]
     mismatch = (ori_fe != ori_s).sum() / cells
     assert log_fe["n_captured"].sum() > 1000
-    # this box is mostly melt-pool surroundings; far-field cells are where the
-    # surrogate serves most on large tracks
+    # this box is mostly melt-pool surroundings, where the served cells' error
+    # is largest (the far field, where the surrogate serves most on a full
+    # track, is small here): 1.5 K; the nine-scan test on the full track holds
+    # the 1 K bar (test_gpu_configs.py::test_nine_scans_surrogate_against_fe)
     assert err["n_served"] > 0.2 * err["n_all"], err
-    assert err["rms_served"] < 3.0, err
+    assert err["rms_served"] <= 1.5, err
     assert mismatch < 5e-3, mismatch
     assert log_s["n_fallback"].min() > 0
     m.close()
