@@ -35,10 +35,12 @@ amca_status fail(amca_status st, const std::string& msg) {
       return fail(AMCA_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));     \
   } while (0)
 
-enum KernelId { K_NUCLEATE = 0, K_CLAIM, K_RESOLVE, K_SCAN, K_EMIT, K_TAIL, K_COUNT };
+enum KernelId {
+  K_NUCLEATE = 0, K_CLAIM, K_RESOLVE, K_SCAN, K_EMIT, K_TAIL, K_IDSCAN, K_COMPACT, K_COUNT
+};
 // named as the kernels are, so event times line up with a kernel trace
-const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim", "k_resolve",
-                                     "k_scan",     "k_emit",  "k_tail"};
+const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim", "k_resolve", "k_scan",
+                                     "k_emit",     "k_tail",  "k_idscan",  "k_compact"};
 
 }  // namespace
 
@@ -53,6 +55,8 @@ struct amca_domain {
   int nuc_blocks = 1;
   int list_blocks = 1;
   int emit_blocks = 1;
+  int ids_blocks = 1;
+  int cmp_blocks = 1;
   bool timing = false;
   // event pairs recorded around timed launches, resolved when times are read
   std::vector<hipEvent_t> ev_pool;
@@ -154,10 +158,17 @@ amca_status enqueue_step(amca_domain* d) {
   const amca::DevBuffers& B = d->B;
   hipStream_t s = d->stream;
   AMCA_TRY(timed(d, K_NUCLEATE, [&] { return amca::launch_nucleate(P, B, d->nuc_blocks, s); }));
+  AMCA_TRY(timed(d, K_COMPACT, [&] { return amca::launch_compact(P, B, d->cmp_blocks, s); }));
   for (int wave = 0; wave < 2; wave++) {
     AMCA_TRY(timed(d, K_CLAIM, [&] { return amca::launch_claim(P, B, wave, d->list_blocks, s); }));
     AMCA_TRY(timed(d, K_RESOLVE,
                    [&] { return amca::launch_resolve(P, B, wave, d->emit_blocks, s); }));
+    if (wave == 0) {
+      AMCA_TRY(timed(d, K_IDSCAN,
+                     [&] { return amca::launch_idscan(P, B, d->ids_blocks, d->emit_blocks, s); }));
+      AMCA_TRY(timed(d, K_EMIT, [&] { return amca::launch_emit(P, B, wave, d->emit_blocks, s); }));
+      continue;
+    }
     AMCA_TRY(timed(d, K_SCAN, [&] { return amca::launch_scan(P, B, wave, d->emit_blocks, s); }));
     AMCA_TRY(timed(d, K_EMIT, [&] { return amca::launch_emit(P, B, wave, d->emit_blocks, s); }));
   }
@@ -233,6 +244,11 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   P.nbY = (int)((nyG + 3) / 4);
   P.nbZ = (int)((nzG + 3) / 4);
   P.N = (long long)desc->nx * desc->ny * desc->nz;
+  const long long nB = (long long)P.nbX * P.nbY * P.nbZ;
+  if (64 * nB > (long long)INT_MAX) {
+    delete d;
+    return fail(AMCA_ERR_INVALID, "grid exceeds 2^31-1 brick slots (int32 slot index)");
+  }
   P.h = desc->h; P.x0 = desc->x0; P.y0 = desc->y0; P.z0 = desc->z0;
   P.T0 = desc->T0; P.Tm = desc->Tm; P.Tdot = desc->Tdot;
   P.a1 = desc->a1; P.a2 = desc->a2; P.a3 = desc->a3;
@@ -262,12 +278,18 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   }
   ALLOC(B.gid, (size_t)NG);
   ALLOC(B.oid, (size_t)NG);
-  ALLOC(B.claim, 64 * (size_t)P.nbX * P.nbY * P.nbZ);  // brick order (claim_index)
-  ALLOC(B.bmap, 2 * (size_t)P.nbX * P.nbY * P.nbZ);
+  ALLOC(B.claim, 64 * (size_t)nB);  // brick order (claim_index)
+  ALLOC(B.bmap, 2 * (size_t)nB);
+  ALLOC(B.amap, (size_t)nB);
+  ALLOC(B.rec, 64 * (size_t)nB);
+  ALLOC(B.skey, 64 * (size_t)nB);
+  ALLOC(B.sori, 64 * (size_t)nB);
   ALLOC(B.genv, N);
   ALLOC(B.gco, N);
-  ALLOC(B.listA[0], N);
-  ALLOC(B.listA[1], N);
+  ALLOC(B.alist, N);
+  ALLOC(B.plist, N);
+  ALLOC(B.cnt, N + 64);  // the id scan reads whole 16-byte groups past the last id
+  ALLOC(B.coff, N);
   ALLOC(B.listW[0], N);
   ALLOC(B.listW[1], N);
   ALLOC(B.masks, N);
@@ -285,6 +307,7 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   B.log = nullptr;
   d->site_cap = 1;
   HIP_TRY(hipMemsetAsync(B.lb_flag, 0, sizeof(unsigned) * amca::kLbSites * amca::kMaxBlocks, d->stream));
+  HIP_TRY(hipMemsetAsync(B.cnt, 0, N + 64, d->stream));
   amca::DevCounters c{};
   c.t = 0.0;
   HIP_TRY(hipMemcpyAsync(B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream));
@@ -298,6 +321,10 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   d->list_blocks = (int)std::max(8LL, std::min<long long>((want + 7) / 8 * 8, 16LL * cus));
   d->emit_blocks = (int)std::max(8LL, std::min<long long>((want + 7) / 8 * 8, std::min(16LL * cus, (long long)amca::kMaxBlocks)));
   d->nuc_blocks = 1;
+  // id scan and compaction: two blocks per CU at most (a short look-back
+  // chain), at least 32 K ids / 512 bricks per block
+  d->ids_blocks = (int)std::max(1LL, std::min<long long>((P.N + 32767) / 32768, 2LL * cus));
+  d->cmp_blocks = (int)std::max(1LL, std::min<long long>((nB + 511) / 512, 2LL * cus));
   HIP_TRY(hipStreamSynchronize(d->stream));
   *out = d;
   return AMCA_OK;
@@ -475,7 +502,9 @@ amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain
   if (orient) HIP_TRY(hipMemcpy(orient, d->B.oid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
   if (grain) HIP_TRY(hipMemcpy(grain, d->B.gid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
   if (env) {
-    // grain records are stored by grain id; scatter them to their cells
+    // grain records by id (materialized from the slots), scattered to their cells
+    HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
     std::memset(env, 0, sizeof(double) * 4 * NG);
     const size_t ng = (size_t)std::max(c.n_grains, 0);
     if (ng > 0) {
@@ -504,7 +533,11 @@ amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env
   *n_grains = c.n_grains;
   const size_t m = (size_t)std::max(0, std::min(cap, c.n_grains));
   if (m == 0) return AMCA_OK;
-  if (env4) HIP_TRY(hipMemcpy(env4, d->B.genv, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
+  if (env4) {
+    HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+    HIP_TRY(hipMemcpy(env4, d->B.genv, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
+  }
   if (cell || orient) {
     std::vector<int32_t> co(2 * m);
     HIP_TRY(hipMemcpy(co.data(), d->B.gco, sizeof(int32_t) * 2 * m, hipMemcpyDeviceToHost));
@@ -519,6 +552,10 @@ amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env
 amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
                                       void** stream) {
   if (!d) return fail(AMCA_ERR_INVALID, "null domain");
+  if (env) {
+    HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+  }
   if (orient) *orient = d->B.oid;
   if (grain) *grain = d->B.gid;
   if (env) *env = d->B.genv;
@@ -603,7 +640,7 @@ struct amca_batch {
   hipStream_t stream = nullptr;
   amca::DevParams* dP = nullptr;
   amca::DevBuffers* dB = nullptr;
-  int4* maps = nullptr;  // nuc | list | emit | scan, back to back
+  int4* maps = nullptr;  // nuc | list | emit | scan | ids | compaction, back to back
   int4* status = nullptr;
   amca::BatchLaunch L{};
   amca_batch_schedule sched{};
@@ -680,21 +717,26 @@ void build_map(const std::vector<int>& order, const std::vector<int>& cnt, int i
 
 amca_status batch_apply_schedule(amca_batch* b) {
   const int n = (int)b->doms.size();
-  std::vector<int> nl, ne, nn(n), one(n, 1);
+  std::vector<int> nl, ne, nn(n), one(n, 1), ni(n), nc(n);
   batch_geometry(b, nl, ne);
   for (int i = 0; i < n; i++) nn[i] = b->doms[i]->nuc_blocks;
+  for (int i = 0; i < n; i++) ni[i] = b->doms[i]->ids_blocks;
+  for (int i = 0; i < n; i++) nc[i] = b->doms[i]->cmp_blocks;
   std::vector<int> order(n);
   for (int i = 0; i < n; i++) order[i] = i;
   // larger domains first: their blocks start earliest
   std::stable_sort(order.begin(), order.end(),
                    [&](int a, int c) { return b->doms[a]->P.N > b->doms[c]->P.N; });
-  std::vector<int4> mn, ml, me, ms;
+  std::vector<int4> mn, ml, me, ms, mi, mc;
   build_map(order, nn, b->sched.interleave, mn);
+  build_map(order, ni, b->sched.interleave, mi);
+  for (auto& m : mi) m.w = ne[m.x];  // the id scan sums the domain's k_resolve blocks
+  build_map(order, nc, b->sched.interleave, mc);
   build_map(order, nl, b->sched.interleave, ml);
   build_map(order, ne, b->sched.interleave, me);
   build_map(order, one, 0, ms);
   for (auto& m : ms) m.z = ne[m.x];  // the scan of domain d spans its emit blocks
-  const size_t total = mn.size() + ml.size() + me.size() + ms.size();
+  const size_t total = mn.size() + ml.size() + me.size() + ms.size() + mi.size() + mc.size();
   if (b->maps) HIP_TRY(hipFree(b->maps));
   b->maps = nullptr;
   HIP_TRY(hipMalloc(&b->maps, sizeof(int4) * total));
@@ -703,15 +745,21 @@ amca_status batch_apply_schedule(amca_batch* b) {
   all.insert(all.end(), ml.begin(), ml.end());
   all.insert(all.end(), me.begin(), me.end());
   all.insert(all.end(), ms.begin(), ms.end());
+  all.insert(all.end(), mi.begin(), mi.end());
+  all.insert(all.end(), mc.begin(), mc.end());
   HIP_TRY(hipMemcpy(b->maps, all.data(), sizeof(int4) * total, hipMemcpyHostToDevice));
   b->L.map_nuc = b->maps;
   b->L.map_list = b->maps + mn.size();
   b->L.map_emit = b->L.map_list + ml.size();
   b->L.map_scan = b->L.map_emit + me.size();
+  b->L.map_ids = b->L.map_scan + ms.size();
+  b->L.map_cmp = b->L.map_ids + mi.size();
   b->L.n_nuc = (int)mn.size();
   b->L.n_list = (int)ml.size();
   b->L.n_emit = (int)me.size();
   b->L.n_one = (int)ms.size();
+  b->L.n_ids = (int)mi.size();
+  b->L.n_cmp = (int)mc.size();
   b->drop_graph();
   return AMCA_OK;
 }

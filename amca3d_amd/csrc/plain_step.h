@@ -57,27 +57,39 @@ struct StepLogRow {
 
 // Device buffers of one domain (all owned by the handle).
 //
-// Cell-indexed (ghosted grid, NG cells): the two grain-ID fields and the claim
-// word of the capture waves.  The liquid and claimed states are 1 bit per cell
-// in 4x4x4-cell bricks: brick b holds two 64-bit words, [2b] liquid and [2b+1]
-// claimed, bit ((k&3)*16 + (j&3)*4 + (i&3)); bricks run x fastest.  A cell's 26
-// neighbours then lie in at most 8 bricks and usually in two or three cache
-// lines, against nine rows of a cell-linear map.  Grain-indexed (grain id =
-// creation order, capacity N): envelope, cell and orientation.  Lists hold
-// grain ids, so list order is id order and grain records are read in
-// ascending address order.
+// Cell-indexed (ghosted grid, NG cells, x fastest as the reference's arrays):
+// the two grain-ID fields.  Brick-indexed: the grid is cut into 4x4x4-cell
+// bricks (x fastest); cell (i, j, k) is lane ((k&3)*16 + (j&3)*4 + (i&3)) of
+// brick b, and its "slot" is 64 b + lane (claim_index).  Per brick: two 64-bit
+// words of the 1-bit maps, [2b] liquid and [2b+1] claimed, and the active word
+// (a grain whose cell has the bit set is on the reference's active list).  Per
+// slot: the grain occupying the cell (envelope, id, orientation), the claim
+// word of the capture waves and the wave-0 neighbour masks.  A wave-0 pass
+// walks bricks, so a cell's 26 neighbours' bits come from the 27 words around
+// its brick, read once per 64 cells, and the records of a brick's active
+// cells are neighbours in memory.  Grain-indexed (grain id = creation order,
+// capacity N): (cell, orientation), and the envelope copy that downloads
+// read (materialized from the slots on request).  Wave lists of waves >= 1
+// hold grain ids in id order.
 struct DevBuffers {
   int* gid;               // cellGrainID_ [NG]
   int* oid;               // cellOrientationID_ [NG]
-  int* claim;             // smallest claiming grain id per cell, INT_MAX when free, in
-                          // 4x4x4 bricks (claim_index) [64 * nbX*nbY*nbZ]
-  unsigned long long* bmap;  // brick words: liquid, claimed [2 * nbX*nbY*nbZ]
-  double4* genv;          // grain envelope: xc, yc, zc, L [N]
+  int* claim;             // smallest claiming grain id per slot, INT_MAX when free [64 nB]
+  unsigned long long* bmap;  // brick words: liquid, claimed [2 nB]
+  unsigned long long* amap;  // brick words: active grains [nB]
+  double4* rec;           // envelope xc, yc, zc, L of the grain in each slot [64 nB]
+  int* skey;              // grain id of each slot [64 nB]
+  int* sori;              // orientation of each slot [64 nB]
+  double4* genv;          // envelope by grain id, materialized for downloads [N]
   int2* gco;              // grain -> (cell, orientation) [N]
-  int* listA[2];          // active lists (grain ids), ping-pong by step parity [N]
+  int* alist;             // wave-0 list: active slots in brick order [N]
+  int4* plist;            // wave-0 capturing grains: (id, slot, captured slots), one segment
+                          // per k_resolve block from its first list entry [N]
+  unsigned char* cnt;     // captures per grain id, nonzero only inside a wave-0 step [N]
+  int* coff;              // first child (wave-relative) per capturing grain id [N]
   int* listW[2];          // capture-wave lists (grain ids) [N]
   unsigned long long* masks;  // per wave entry: liquid bits | inside bits << 32 [N]
-  unsigned* res;          // per wave entry: captured bits | keep << 31 [N]
+  unsigned* res;          // per wave entry (waves >= 1): captured bits | keep << 31 [N]
   int* site_cell[2];      // live nucleation sites, ascending cell, ping-pong
   double* site_dt[2];
   double* l2g;            // orientation table, 9 doubles per row
@@ -91,7 +103,7 @@ struct DevBuffers {
 constexpr int kBlock = 256;       // threads per block of the list kernels
 constexpr int kItems = 4;         // consecutive list entries per thread in scan passes
 constexpr int kMaxBlocks = 4096;  // blocks of a look-back kernel
-constexpr int kLbSites = 4;       // look-back launch sites per step
+constexpr int kLbSites = 5;       // look-back launch sites per step (the last: block statistics)
 
 // Kernel launchers (plain_step.hip).  Each returns hipGetLastError().
 hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t s);
@@ -104,6 +116,15 @@ hipError_t launch_resolve(const DevParams& P, const DevBuffers& B, int wave, int
                           hipStream_t s);
 hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                        hipStream_t s);
+// wave 0: child offsets of the capturing grains, an exclusive scan over their
+// ids (`blocks` <= kMaxBlocks look-back blocks; `nres` = blocks of the wave-0
+// k_resolve, whose statistics words it sums)
+// wave 0's list: the active slots in brick order (look-back over the bricks)
+hipError_t launch_compact(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
+hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int blocks, int nres,
+                         hipStream_t s);
+// genv[g] = the envelope in grain g's slot, for g < n_grains
+hipError_t launch_materialize(const DevParams& P, const DevBuffers& B, hipStream_t s);
 hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                        hipStream_t s);
 hipError_t launch_tail(const DevParams& P, const DevBuffers& B, hipStream_t s);
@@ -117,7 +138,9 @@ struct BatchLaunch {
   const int4* map_list;  // claim geometry
   const int4* map_emit;  // resolve / emit geometry
   const int4* map_scan;  // one block per domain (scan and tail), z = emit block count
-  int n_nuc, n_list, n_emit, n_one;
+  const int4* map_ids;   // id-scan geometry
+  const int4* map_cmp;   // compaction geometry
+  int n_nuc, n_list, n_emit, n_one, n_ids, n_cmp;
 };
 hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s);
 // out[d] = (finished, error, step & 0x7fffffff, step >> 31) for each of n domains

@@ -80,6 +80,7 @@ __global__ void k_init_grid(DevParams P, DevBuffers B) {
     }
     B.bmap[2 * b] = bits;
     B.bmap[2 * b + 1] = 0ull;
+    B.amap[b] = 0ull;
   }
 }
 
@@ -115,12 +116,10 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
   const int sel = ctr->site_sel;
   const int n = ctr->n_sites;
   const int ng0 = ctr->n_grains, no0 = ctr->n_orient, nA0 = ctr->nA_in;
-  const int par = ctr->parity;
   const int* __restrict__ sc = B.site_cell[sel];
   const double* __restrict__ sd = B.site_dt[sel];
   int* __restrict__ nc = B.site_cell[sel ^ 1];
   double* __restrict__ nd = B.site_dt[sel ^ 1];
-  int* __restrict__ Ain = B.listA[par];
   const double T = P.T0 + P.Tdot * t;
   const double under = P.Tm - T;
 
@@ -162,13 +161,16 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
         int ci, cj, ck;
         cell_ijk(P, c, ci, cj, ck);
         const int id = ng0 + r;
-        B.genv[id] = make_double4(centre_coord(P.x0, P.h, ci), centre_coord(P.y0, P.h, cj),
-                                  centre_coord(P.z0, P.h, ck), 0.0);
+        const long long q = claim_index(P, ci, cj, ck);
+        B.rec[q] = make_double4(centre_coord(P.x0, P.h, ci), centre_coord(P.y0, P.h, cj),
+                                centre_coord(P.z0, P.h, ck), 0.0);
+        B.skey[q] = id;
+        B.sori[q] = o;
         B.gco[id] = make_int2(c, o);
         B.gid[c] = id;
         B.oid[c] = o;
         brick_clear(P, B.bmap, 0, ci, cj, ck);
-        Ain[nA0 + r] = id;
+        atomicOr(&B.amap[q >> 6], 1ull << (q & 63));
       }
     }
     if (keep) {
@@ -218,38 +220,23 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
 // ---------------------------------------------------------------------------
 // capture waves
 
+// Input list of waves >= 1: the grains created by the previous wave, in id
+// order.
 __device__ __forceinline__ void wave_input(const DevBuffers& B, int wave, const int*& list,
                                            int& n) {
   const DevCounters* ctr = B.ctr;
-  if (wave == 0) {
-    list = B.listA[ctr->parity];
-    n = ctr->nA_in;
-  } else {
-    list = B.listW[(wave - 1) & 1];
-    n = ctr->nW[(wave - 1) & 1];
-  }
+  list = B.listW[(wave - 1) & 1];
+  n = ctr->nW[(wave - 1) & 1];
 }
 
-// Envelope tests of one wave entry (grain id g).  Wave 0 first grows the grain
-// (Grain::grow, Grain.h:27, over the whole list as in :521-538).
-// Entries of wave w >= 1 are exactly the cells captured in wave w-1, whose
-// claimed bits are cleared here (not while wave w-1 still reads the map).
-template <bool kCoherent>
-__device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
-                                                          int g, bool grow, double inc) {
-  const int2 co = B.gco[g];
-  const int c = co.x;
-  const int o = co.y;
-  int ci, cj, ck;
-  cell_ijk(P, c, ci, cj, ck);
-  if (!grow) brick_clear(P, B.bmap, 1, ci, cj, ck);
-  double4 e = B.genv[g];
-  if (grow) {
-    e.w = e.w + inc;
-    B.genv[g].w = e.w;
-  }
-  const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
-  if (lm == 0 || e.w == 0) return lm;  // no liquid neighbour, or an envelope that captures nothing
+// Envelope tests of grain g (orientation o, envelope e) at cell (ci, cj, ck)
+// against its liquid neighbours lm: each neighbour inside the envelope gets an
+// atomicMin of g into its claim word and its claimed bit.  Returns the inside
+// bits (none when L == 0: such an envelope captures nothing).
+__device__ __forceinline__ unsigned claim_tests(const DevParams& P, const DevBuffers& B, int g,
+                                                int o, int ci, int cj, int ck, unsigned lm,
+                                                const double4& e) {
+  if (lm == 0 || e.w == 0) return 0u;
   // The L1 norm in the grain frame is at least the Euclidean distance, so a
   // neighbour farther than L (with a margin far above rounding) cannot be
   // inside; the orientation row is read only when some neighbour can be.
@@ -268,7 +255,7 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
       if (gx * gx + gy * gy + gz * gz <= reach) cand |= 1u << s;
     }
   }
-  if (cand == 0) return lm;
+  if (cand == 0) return 0u;
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
 #pragma unroll
@@ -289,35 +276,146 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
       brick_set(P, B.bmap, 1, ci + di, cj + dj, ck + dk);
     }
   }
+  return im;
+}
+
+// Brick range [lo, hi) of block blk of nblk.
+__device__ __forceinline__ void brick_range(const DevParams& P, int blk, int nblk, long long& lo,
+                                            long long& hi) {
+  const long long nB = (long long)P.nbX * P.nbY * P.nbZ;
+  const long long per = (nB + nblk - 1) / nblk;
+  lo = (long long)blk * per;
+  hi = lo + per < nB ? lo + per : nB;
+  if (lo > nB) lo = nB;
+}
+
+// The active slots in brick order: a look-back scan over the bricks' active
+// words.  Each thread counts its brick's active cells; in the second pass each
+// wavefront writes the slots of its 64 bricks one brick at a time (a ballot
+// and a coalesced store per brick).  The list's length is the active count.
+__device__ __forceinline__ void compact_body(const DevParams& P, const DevBuffers& B, int nblk) {
+  DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ int s_bid;
+  __shared__ unsigned long long s_sm[kBlock / 64];
+  __shared__ unsigned long long s_excl;
+  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[3], 1u);
+  __syncthreads();
+  const int bid = s_bid;
+  const long long nB = (long long)P.nbX * P.nbY * P.nbZ;
+  const long long per = ((nB + nblk - 1) / nblk + kBlock - 1) / kBlock * kBlock;
+  const long long lo = (long long)bid * per < nB ? (long long)bid * per : nB;
+  const long long hi = lo + per < nB ? lo + per : nB;
+  unsigned long long mine = 0;
+  for (long long b = lo + threadIdx.x; b < hi; b += kBlock) mine += __popcll(B.amap[b]);
+  unsigned long long agg;
+  block_scan_excl(mine, &agg, s_sm);
+  if (threadIdx.x < 64) {
+    const unsigned long long e =
+        lookback(B.lb_flag + 3 * kMaxBlocks, B.lb_agg + 3 * kMaxBlocks, B.lb_pref + 3 * kMaxBlocks,
+                 bid, (unsigned)(ctr->step * kLbSites + 3), agg);
+    if (threadIdx.x == 0) s_excl = e;
+  }
+  __syncthreads();
+  unsigned long long run = s_excl;
+  const int lane = threadIdx.x & 63;
+  for (long long base = lo; base < hi; base += kBlock) {
+    const long long b = base + threadIdx.x;
+    const unsigned long long w = b < hi ? B.amap[b] : 0ull;
+    unsigned long long tot;
+    const unsigned long long off = run + block_scan_excl(__popcll(w), &tot, s_sm);
+    // the wavefront's 64 bricks, one after another
+    const long long wb = base + (threadIdx.x & ~63);
+    for (int t = 0; t < 64; t++) {
+      const unsigned long long bw = __shfl(w, t, 64);
+      if (!bw) continue;
+      const unsigned long long bo = __shfl(off, t, 64);
+      if ((bw >> lane) & 1ull)
+        B.alist[bo + __popcll(bw & ((1ull << lane) - 1ull))] = (int)(64 * (wb + t) + lane);
+    }
+    run += tot;
+  }
+  if (threadIdx.x == 0 && bid == nblk - 1) ctr->tickets[3] = 0u;
+}
+
+// Cell of a slot.
+__device__ __forceinline__ void slot_ijk(const DevParams& P, long long q, int& i, int& j, int& k) {
+  int bx, by, bz;
+  brick_xyz(P, q >> 6, bx, by, bz);
+  const int l = (int)(q & 63);
+  i = 4 * bx + (l & 3);
+  j = 4 * by + ((l >> 2) & 3);
+  k = 4 * bz + (l >> 4);
+}
+
+// Wave 0 over the active slots in brick order (nA_in of them): each active
+// grain grows (Grain::grow, Grain.h:27, over the whole list as in :521-538),
+// then tests its liquid neighbours.  Neighbouring entries sit in the same
+// bricks, so their map words and records share cache lines.
+__device__ __forceinline__ void claim0_body(const DevParams& P, const DevBuffers& B, int blk,
+                                            int nblk) {
+  const DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  const double inc = ctr->dt * ctr->v;
+  const int n = ctr->nA_in;
+  int lo, hi;
+  chunk_of(n, blk, nblk, kBlock, lo, hi);
+  for (int t = lo + threadIdx.x; t < hi; t += kBlock) {
+    const long long q = B.alist[t];
+    int i, j, k;
+    slot_ijk(P, q, i, j, k);
+    double4 e = B.rec[q];
+    e.w = e.w + inc;
+    B.rec[q].w = e.w;
+    const unsigned lm = brick_bits26<0, false>(P, B.bmap, i, j, k);
+    const unsigned im = claim_tests(P, B, B.skey[q], B.sori[q], i, j, k, lm, e);
+    B.masks[t] = ((unsigned long long)im << 32) | lm;
+  }
+}
+
+// Envelope tests of one entry of a wave >= 1 (grain g): these are the cells
+// captured by the previous wave, whose claimed bits are cleared here (not
+// while that wave still reads the map).
+template <bool kCoherent>
+__device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
+                                                          int g) {
+  const int2 co = B.gco[g];
+  int ci, cj, ck;
+  cell_ijk(P, co.x, ci, cj, ck);
+  brick_clear(P, B.bmap, 1, ci, cj, ck);
+  const double4 e = B.rec[claim_index(P, ci, cj, ck)];
+  const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
+  const unsigned im = claim_tests(P, B, g, co.y, ci, cj, ck, lm, e);
   return ((unsigned long long)im << 32) | lm;
 }
 
 __device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers& B, int wave,
                                            int blk, int nblk) {
+  if (wave == 0) {
+    claim0_body(P, B, blk, nblk);
+    return;
+  }
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   const int* list;
   int n;
   wave_input(B, wave, list, n);
-  const double inc = ctr->dt * ctr->v;
   int lo, hi;
   chunk_of(n, blk, nblk, kBlock, lo, hi);
-  for (int i = lo + threadIdx.x; i < hi; i += kBlock)
-    B.masks[i] = claim_entry<false>(P, B, list[i], wave == 0, inc);
+  for (int i = lo + threadIdx.x; i < hi; i += kBlock) B.masks[i] = claim_entry<false>(P, B, list[i]);
 }
 
-// Captured slots and the keep bit of one entry.  A liquid slot nobody claimed
+// Captured slots and the keep bit of grain g at (ci, cj, ck) from its masks m
+// and the claimed bits cmask of its neighbours.  A liquid slot nobody claimed
 // in this wave needs no claim word (its claim is "none", larger than any id);
 // only claimed slots read theirs, all loads issued before any is used.
 template <bool kCoherent>
-__device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevBuffers& B, int g,
-                                                  unsigned long long m) {
+__device__ __forceinline__ unsigned resolve_bits(const DevParams& P, const DevBuffers& B, int g,
+                                                 unsigned long long m, unsigned cmask, int ci,
+                                                 int cj, int ck) {
   const unsigned lm = (unsigned)m, im = (unsigned)(m >> 32);
   if (lm == 0) return 0u;
-  const int c = B.gco[g].x;
-  int ci, cj, ck;
-  cell_ijk(P, c, ci, cj, ck);
-  const unsigned cm = brick_bits26<1, kCoherent>(P, B.bmap, ci, cj, ck) & lm;
+  const unsigned cm = cmask & lm;
   // an unclaimed liquid neighbour outside the envelope keeps the grain active
   unsigned keep = (lm & ~im & ~cm) ? 1u : 0u;
   const unsigned need = keep ? (cm & im) : cm;
@@ -347,18 +445,209 @@ __device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevB
   return cap | (keep << 31);
 }
 
-// Octahedron::capture_cell via CellularAutomataManager_3D::capture_cell
-// (:676-707): each captured cell gets a new grain (ids id0, id0+1, ... in slot
-// order) with the parent orientation.
-__device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffers& B, int g,
-                                              unsigned cap, int id0, int* __restrict__ kids,
-                                              int pos0) {
-  const int2 co = B.gco[g];
-  const int c = co.x;
-  const int o = co.y;
-  const double4 e = B.genv[g];
+// One entry of a wave >= 1; a grain that does not stay active leaves the
+// active map.
+template <bool kCoherent>
+__device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevBuffers& B, int g,
+                                                  unsigned long long m) {
+  const int c = B.gco[g].x;
   int ci, cj, ck;
   cell_ijk(P, c, ci, cj, ck);
+  unsigned r = 0u;
+  if ((unsigned)m != 0u)
+    r = resolve_bits<kCoherent>(P, B, g, m, brick_bits26<1, kCoherent>(P, B.bmap, ci, cj, ck),
+                                ci, cj, ck);
+  if (!(r >> 31)) active_clear(P, B.amap, ci, cj, ck);
+  return r;
+}
+
+// Wave 0 over the same list: a grain that does not stay active leaves the
+// active map; a grain that captured writes (id, slot, captured slots) to its
+// block's segment of the parent list (which starts at the block's first list
+// entry) and its capture count to the id-indexed count array.  Each block
+// leaves (parents << 32 | survivors) in its statistics word; no counter is
+// shared between blocks.
+__device__ __forceinline__ void resolve0_body(const DevParams& P, const DevBuffers& B, int blk,
+                                              int nblk) {
+  const DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ int s_np, s_surv;
+  if (threadIdx.x == 0) {
+    s_np = 0;
+    s_surv = 0;
+  }
+  __syncthreads();
+  const int n = ctr->nA_in;
+  int lo, hi;
+  chunk_of(n, blk, nblk, kBlock, lo, hi);
+  int4* __restrict__ seg = B.plist + lo;
+  const int lane = threadIdx.x & 63;
+  for (int base = lo; base < hi; base += kBlock) {
+    const int t = base + threadIdx.x;
+    unsigned r = 0u;
+    int g = 0;
+    long long q = 0;
+    if (t < hi) {
+      q = B.alist[t];
+      int i, j, k;
+      slot_ijk(P, q, i, j, k);
+      g = B.skey[q];
+      const unsigned long long m = B.masks[t];
+      if ((unsigned)m != 0u)
+        r = resolve_bits<false>(P, B, g, m, brick_bits26<1, false>(P, B.bmap, i, j, k), i, j, k);
+      if (!(r >> 31)) atomicAnd(&B.amap[q >> 6], ~(1ull << (q & 63)));
+    }
+    const unsigned long long keepm = __ballot(t < hi && (r >> 31));
+    const unsigned cap = r & 0x3ffffffu;
+    const unsigned long long capm = __ballot(cap != 0u);
+    int pbase = 0;
+    if (lane == 0) {
+      if (keepm) atomicAdd(&s_surv, (int)__popcll(keepm));
+      if (capm) pbase = atomicAdd(&s_np, (int)__popcll(capm));
+    }
+    if (capm) {
+      pbase = __shfl(pbase, 0, 64);
+      if (cap) {
+        seg[pbase + (int)__popcll(capm & ((1ull << lane) - 1ull))] = make_int4(g, (int)q, (int)cap, 0);
+        B.cnt[g] = (unsigned char)__builtin_popcount(cap);
+      }
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0)
+    B.lb_agg[4 * kMaxBlocks + blk] = ((unsigned long long)s_np << 32) | (unsigned)s_surv;
+}
+
+__device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffers& B, int wave,
+                                             int blk, int nblk);
+
+// Wave 0's children are numbered in (parent id, slot) order, as the
+// reference's sweep appends them: the first child of a capturing grain is the
+// sum of the capture counts of the capturing grains with smaller ids.  The
+// counts sit in an id-indexed byte array; this scan runs over the ids of all
+// grains, each thread over a contiguous run of 4-byte words and the blocks
+// chained by the decoupled look-back, writes each capturing grain's first
+// child and zeroes the counts it has read.  Its last block also adds the
+// wave's survivors (the statistics words of the `nres` k_resolve blocks) to
+// the active count.
+__device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers& B, int nblk,
+                                            int nres) {
+  DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  __shared__ int s_bid;
+  __shared__ unsigned long long s_sm[kBlock / 64];
+  __shared__ unsigned long long s_excl;
+  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[2], 1u);
+  __syncthreads();
+  const int bid = s_bid;
+  const int ng = ctr->n_grains;
+  // 16 ids (one uint4 of counts) per thread and tile; tiles of 256 x 16 ids
+  const long long n4 = ((long long)ng + 15) >> 4;
+  const long long per = ((n4 + nblk - 1) / nblk + kBlock - 1) / kBlock * kBlock;
+  const long long lo = (long long)bid * per < n4 ? (long long)bid * per : n4;
+  const long long hi = lo + per < n4 ? lo + per : n4;
+  uint4* __restrict__ c4 = reinterpret_cast<uint4*>(B.cnt);
+  auto bytesum = [](unsigned v) {
+    return (v & 0xffu) + ((v >> 8) & 0xffu) + ((v >> 16) & 0xffu) + (v >> 24);
+  };
+  unsigned long long mine = 0;
+  for (long long u = lo + threadIdx.x; u < hi; u += kBlock) {
+    const uint4 v = c4[u];
+    if (v.x | v.y | v.z | v.w) mine += bytesum(v.x) + bytesum(v.y) + bytesum(v.z) + bytesum(v.w);
+  }
+  unsigned long long agg;
+  block_scan_excl(mine, &agg, s_sm);
+  if (threadIdx.x < 64) {
+    const unsigned long long e =
+        lookback(B.lb_flag + 2 * kMaxBlocks, B.lb_agg + 2 * kMaxBlocks, B.lb_pref + 2 * kMaxBlocks,
+                 bid, (unsigned)(ctr->step * kLbSites + 2), agg);
+    if (threadIdx.x == 0) s_excl = e;
+  }
+  __syncthreads();
+  unsigned long long run = s_excl;
+  if (agg) {
+    for (long long base = lo; base < hi; base += kBlock) {
+      const long long u = base + threadIdx.x;
+      uint4 v = make_uint4(0u, 0u, 0u, 0u);
+      if (u < hi) v = c4[u];
+      const bool any = (v.x | v.y | v.z | v.w) != 0u;
+      const unsigned sum = any ? bytesum(v.x) + bytesum(v.y) + bytesum(v.z) + bytesum(v.w) : 0u;
+      unsigned long long tot;
+      unsigned long long off = run + block_scan_excl(sum, &tot, s_sm);
+      if (any) {
+        const unsigned w[4] = {v.x, v.y, v.z, v.w};
+        const int id = (int)(16 * u);
+#pragma unroll
+        for (int t = 0; t < 16; t++) {
+          const unsigned c = (w[t >> 2] >> (8 * (t & 3))) & 0xffu;
+          if (c) {
+            B.coff[id + t] = (int)off;
+            off += c;
+          }
+        }
+        c4[u] = make_uint4(0u, 0u, 0u, 0u);
+      }
+      run += tot;
+    }
+  }
+  if (bid == nblk - 1) {
+    // survivors of wave 0, summed over the k_resolve blocks' statistics
+    unsigned long long sv = 0;
+    for (int q = threadIdx.x; q < nres; q += kBlock)
+      sv += (unsigned)B.lb_agg[4 * kMaxBlocks + q];
+    unsigned long long svt;
+    block_scan_excl(sv, &svt, s_sm);
+    if (threadIdx.x == 0) {
+      // the last block's running prefix is the wave's child count
+      const unsigned long long nc = s_excl + agg;
+      const int gbase = ctr->n_grains;
+      ctr->wave_gbase = gbase;
+      ctr->n_grains = gbase + (int)nc;
+      ctr->nW[0] = (int)nc;
+      ctr->step_cap += (int)nc;
+      ctr->nA_out += (int)svt;
+      if (nc > 0 && ctr->max_wave < 1) ctr->max_wave = 1;
+      if (nc > 0 && ctr->step_wave < 1) ctr->step_wave = 1;
+      ctr->tickets[2] = 0u;
+    }
+  }
+}
+
+// A captured cell's new grain: record in the cell's slot, grain-ID fields,
+// (cell, orientation) by id, liquid bit cleared, active bit set
+// (CellularAutomataManager_3D::capture_cell, :676-707; Grain.cpp:13-18).
+__device__ __forceinline__ void write_child(const DevParams& P, const DevBuffers& B, int id,
+                                            int ni, int nj, int nk, int o, const double4& env) {
+  const long long q = claim_index(P, ni, nj, nk);
+  const int nb = (nk * P.nyG + nj) * P.nxG + ni;
+  B.rec[q] = env;
+  B.skey[q] = id;
+  B.sori[q] = o;
+  B.gco[id] = make_int2(nb, o);
+  B.gid[nb] = id;
+  B.oid[nb] = o;
+  brick_clear(P, B.bmap, 0, ni, nj, nk);
+  atomicOr(&B.amap[q >> 6], 1ull << (q & 63));
+}
+
+// Octahedron::capture_cell (Octahedron.cpp:133-275): the child envelope of
+// parent envelope e (orientation row r) for its neighbour slot s.
+__device__ __forceinline__ double4 child_of(const DevParams& P, const double* r, const double4& e,
+                                            int ci, int cj, int ck, int di, int dj, int dk) {
+  const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
+  const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
+  const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
+  double out[4];
+  child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
+  return make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
+}
+
+// The children of a parent at (ci, cj, ck) with orientation o and envelope e:
+// each captured cell (slots ascending) gets a new grain, ids id0, id0+1, ...,
+// with the parent orientation, listed at kids[pos0 ...].
+__device__ __forceinline__ void emit_from(const DevParams& P, const DevBuffers& B, int o,
+                                          const double4& e, int ci, int cj, int ck, unsigned cap,
+                                          int id0, int* __restrict__ kids, int pos0) {
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
 #pragma unroll
@@ -369,56 +658,68 @@ __device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffe
     cap &= cap - 1;
     int di, dj, dk;
     slot_delta(s, di, dj, dk);
-    const int nb = c + P.off[s];
-    const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
-    const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
-    const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
-    double out[4];
-    child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
-    const int id = id0 + j;
-    B.genv[id] = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
-    B.gco[id] = make_int2(nb, o);
-    B.gid[nb] = id;
-    B.oid[nb] = o;
-    brick_clear(P, B.bmap, 0, ci + di, cj + dj, ck + dk);
-    kids[pos0 + j] = id;
+    write_child(P, B, id0 + j, ci + di, cj + dj, ck + dk, o, child_of(P, r, e, ci, cj, ck, di, dj, dk));
+    kids[pos0 + j] = id0 + j;
     j++;
   }
 }
 
-// One child of emit_children: the capture of parent g's neighbour slot s as
-// grain `id`, written to the wave list at `pos`.
+// Parent g's children (grain id known, cell from its record).
+__device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffers& B, int g,
+                                              unsigned cap, int id0, int* __restrict__ kids,
+                                              int pos0) {
+  const int2 co = B.gco[g];
+  int ci, cj, ck;
+  cell_ijk(P, co.x, ci, cj, ck);
+  emit_from(P, B, co.y, B.rec[claim_index(P, ci, cj, ck)], ci, cj, ck, cap, id0, kids, pos0);
+}
+
+// One child: the capture of parent g's neighbour slot s as grain `id`, listed
+// at kids[pos].
 __device__ __forceinline__ void emit_child(const DevParams& P, const DevBuffers& B, int g, int s,
                                            int id, int* __restrict__ kids, int pos) {
   const int2 co = B.gco[g];
-  const int c = co.x;
   const int o = co.y;
-  const double4 e = B.genv[g];
   int ci, cj, ck;
-  cell_ijk(P, c, ci, cj, ck);
+  cell_ijk(P, co.x, ci, cj, ck);
+  const double4 e = B.rec[claim_index(P, ci, cj, ck)];
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
 #pragma unroll
   for (int q = 0; q < 9; q++) r[q] = rp[q];
   int di, dj, dk;
   slot_delta(s, di, dj, dk);
-  const int nb = c + P.off[s];
-  const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
-  const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
-  const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
-  double out[4];
-  child_envelope(r, e.w, P.h, P.trunc, P.sqrt2f, gx, gy, gz, out);
-  B.genv[id] = make_double4(out[0] + e.x, out[1] + e.y, out[2] + e.z, out[3]);
-  B.gco[id] = make_int2(nb, o);
-  B.gid[nb] = id;
-  B.oid[nb] = o;
-  brick_clear(P, B.bmap, 0, ci + di, cj + dj, ck + dk);
+  write_child(P, B, id, ci + di, cj + dj, ck + dk, o, child_of(P, r, e, ci, cj, ck, di, dj, dk));
   kids[pos] = id;
 }
 
-// The list of wave entries is cut into `nb` contiguous chunks (nb <= grid):
-// k_resolve decides every entry and writes per-chunk totals, k_scan turns the
-// totals into absolute output positions, k_emit writes survivors and children.
+// Wave-0 children: one thread per capturing grain of this block's parent-list
+// segment (the same list chunk as its k_resolve block), its children in slot
+// order with ids wave_gbase + first child + 0, 1, ..., placed at those
+// positions of the wave-1 list, which is therefore in id order.
+__device__ __forceinline__ void emit0_body(const DevParams& P, const DevBuffers& B, int blk,
+                                           int nblk) {
+  const DevCounters* ctr = B.ctr;
+  if (ctr->skip) return;
+  int lo, hi;
+  chunk_of(ctr->nA_in, blk, nblk, kBlock, lo, hi);
+  const int np = (int)(B.lb_agg[4 * kMaxBlocks + blk] >> 32);
+  const int4* __restrict__ seg = B.plist + lo;
+  const int gbase = ctr->wave_gbase;
+  int* __restrict__ kids = B.listW[0];
+  for (int t = threadIdx.x; t < np; t += kBlock) {
+    const int4 pe = seg[t];
+    int i, j, k;
+    slot_ijk(P, pe.y, i, j, k);
+    const int p0 = B.coff[pe.x];
+    emit_from(P, B, B.sori[pe.y], B.rec[pe.y], i, j, k, (unsigned)pe.z, gbase + p0, kids, p0);
+  }
+}
+
+// The list of a wave >= 1 is cut into `nb` contiguous chunks (nb <= grid):
+// k_resolve decides every entry and writes per-chunk totals (children,
+// survivors), k_scan turns the totals into absolute output positions, k_emit
+// writes the children.
 __device__ __forceinline__ int wave_chunks(int n, int grid) {
   constexpr int kTile = kBlock * kItems;
   return max(1, min(grid, (n + kTile - 1) / kTile));
@@ -426,6 +727,10 @@ __device__ __forceinline__ int wave_chunks(int n, int grid) {
 
 __device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffers& B, int wave,
                                              int blk, int nblk) {
+  if (wave == 0) {
+    resolve0_body(P, B, blk, nblk);
+    return;
+  }
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   __shared__ unsigned long long s_sm[kBlock / 64];
@@ -450,6 +755,8 @@ __device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffer
 
 constexpr int kScanBlock = 1024;
 
+// Waves >= 1: chunk totals -> absolute positions; children ids start at
+// n_grains, survivors are counted on top of nA_out.
 __device__ __forceinline__ void scan_body(const DevParams& P, const DevBuffers& B, int wave,
                                           int grid) {
   DevCounters* ctr = B.ctr;
@@ -459,8 +766,6 @@ __device__ __forceinline__ void scan_body(const DevParams& P, const DevBuffers& 
   int n;
   wave_input(B, wave, list, n);
   const int nb = wave_chunks(n, grid);
-  // absolute bases: children ids and wave-list positions start at n_grains / 0,
-  // survivors at nA_out; stored as (child offset << 32 | survivor position)
   const int gbase = ctr->n_grains;
   const int sbase = ctr->nA_out;
   constexpr int kPer = kMaxBlocks / kScanBlock;
@@ -495,6 +800,10 @@ __device__ __forceinline__ void scan_body(const DevParams& P, const DevBuffers& 
 
 __device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& B, int wave,
                                           int blk, int nblk) {
+  if (wave == 0) {
+    emit0_body(P, B, blk, nblk);
+    return;
+  }
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   constexpr int kEmitStage = 1024;
@@ -507,7 +816,6 @@ __device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& 
   const int bid = blk;
   if (bid >= nb) return;
   int* __restrict__ kids = B.listW[wave & 1];
-  int* __restrict__ surv = B.listA[ctr->parity ^ 1];
   const int gbase = ctr->wave_gbase;
   constexpr int kTile = kBlock * kItems;
   int lo, hi;
@@ -536,7 +844,6 @@ __device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& 
 #pragma unroll
     for (int q = 0; q < kItems; q++) {
       const unsigned cap = r[q] & 0x3ffffffu;
-      if (r[q] >> 31) surv[(int)(off & 0xffffffffull)] = g[q];
       cpos[q] = (int)(off >> 32) - cbase;
       off += ((unsigned long long)__builtin_popcount(cap) << 32) | (r[q] >> 31);
     }
@@ -591,9 +898,8 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
     if (n == 0) break;
     const int* list = B.listW[in];
     int* kids = B.listW[wave & 1];
-    int* surv = B.listA[ctr->parity ^ 1];
     for (int i = threadIdx.x; i < n; i += kTailBlock)
-      B.masks[i] = claim_entry<true>(P, B, list[i], false, 0.0);
+      B.masks[i] = claim_entry<true>(P, B, list[i]);
     __threadfence();
     __syncthreads();
     unsigned long long run = 0;
@@ -610,7 +916,6 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
       unsigned long long off =
           run + block_scan_excl(((unsigned long long)__builtin_popcount(cap) << 32) | (r >> 31),
                                 &tot, s_sm);
-      if (r >> 31) surv[s_sbase + (int)(off & 0xffffffffull)] = g;
       if (cap) {
         const int pos = (int)(off >> 32);
         emit_children(P, B, g, cap, s_gbase + pos, kids, pos);
@@ -705,6 +1010,19 @@ __global__ void __launch_bounds__(kBlock) k_emit(DevParams P, DevBuffers B, int 
   emit_body(P, B, wave, xcd_block(), gridDim.x);
 }
 __global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) { tail_body(P, B); }
+__global__ void __launch_bounds__(kBlock) k_compact(DevParams P, DevBuffers B) {
+  compact_body(P, B, gridDim.x);
+}
+__global__ void __launch_bounds__(kBlock) k_idscan(DevParams P, DevBuffers B, int nres) {
+  idscan_body(P, B, gridDim.x, nres);
+}
+// genv[g] = the envelope in the slot of grain g's cell (downloads read genv)
+__global__ void k_materialize(DevParams P, DevBuffers B) {
+  const int n = B.ctr->n_grains;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += stride)
+    B.genv[g] = B.rec[cell_slot(P, B.gco[g].x)];
+}
 
 // ---------------------------------------------------------------------------
 // batched kernels: block b serves domain map[b].x as its block map[b].y of
@@ -740,6 +1058,16 @@ __global__ void __launch_bounds__(kTailBlock) kb_tail(const DevParams* Ps, const
   const int4 m = map[blockIdx.x];
   tail_body(Ps[m.x], Bs[m.x]);
 }
+__global__ void __launch_bounds__(kBlock) kb_compact(const DevParams* Ps, const DevBuffers* Bs,
+                                                     const int4* map) {
+  const int4 m = map[blockIdx.x];
+  compact_body(Ps[m.x], Bs[m.x], m.z);
+}
+__global__ void __launch_bounds__(kBlock) kb_idscan(const DevParams* Ps, const DevBuffers* Bs,
+                                                    const int4* map) {
+  const int4 m = map[blockIdx.x];
+  idscan_body(Ps[m.x], Bs[m.x], m.z, m.w);
+}
 
 // per-domain status after a batch of steps: (finished, error, steps low, steps high)
 __global__ void kb_status(const DevParams* Ps, const DevBuffers* Bs, int n, int4* out) {
@@ -758,10 +1086,15 @@ hipError_t launch_batch_status(const BatchLaunch& L, int n, int4* out, hipStream
 
 hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
   hipLaunchKernelGGL(kb_nucleate, dim3(L.n_nuc), dim3(kBlock), 0, s, L.P, L.B, L.map_nuc);
+  hipLaunchKernelGGL(kb_compact, dim3(L.n_cmp), dim3(kBlock), 0, s, L.P, L.B, L.map_cmp);
   for (int wave = 0; wave < 2; wave++) {
     hipLaunchKernelGGL(kb_claim, dim3(L.n_list), dim3(kBlock), 0, s, L.P, L.B, L.map_list, wave);
     hipLaunchKernelGGL(kb_resolve, dim3(L.n_emit), dim3(kBlock), 0, s, L.P, L.B, L.map_emit, wave);
-    hipLaunchKernelGGL(kb_scan, dim3(L.n_one), dim3(kScanBlock), 0, s, L.P, L.B, L.map_scan, wave);
+    if (wave == 0)
+      hipLaunchKernelGGL(kb_idscan, dim3(L.n_ids), dim3(kBlock), 0, s, L.P, L.B, L.map_ids);
+    else
+      hipLaunchKernelGGL(kb_scan, dim3(L.n_one), dim3(kScanBlock), 0, s, L.P, L.B, L.map_scan,
+                         wave);
     hipLaunchKernelGGL(kb_emit, dim3(L.n_emit), dim3(kBlock), 0, s, L.P, L.B, L.map_emit, wave);
   }
   hipLaunchKernelGGL(kb_tail, dim3(L.n_one), dim3(kTailBlock), 0, s, L.P, L.B, L.map_scan);
@@ -801,6 +1134,25 @@ hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int bl
 hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
+  return hipGetLastError();
+}
+
+hipError_t launch_compact(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kBlock), 0, s, P, B);
+  return hipGetLastError();
+}
+
+hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int blocks, int nres,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_idscan, dim3(blocks), dim3(kBlock), 0, s, P, B, nres);
+  return hipGetLastError();
+}
+
+hipError_t launch_materialize(const DevParams& P, const DevBuffers& B, hipStream_t s) {
+  long long blocks = (P.N + 255) / 256;
+  if (blocks > 16384) blocks = 16384;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_materialize, dim3((unsigned)blocks), dim3(256), 0, s, P, B);
   return hipGetLastError();
 }
 

@@ -26,7 +26,7 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "CA cell-updates/sec + % HBM peak; batched+surrogate vs plain GPU at fixed grain-ID error"
 
 
-def algorithmic_bytes(log, chunk=4096):
+def algorithmic_bytes(log, cells, chunk=4096):
     """Algorithmic HBM bytes per kernel over a step log, from the unit figures
     of the plain step (48 B per active grain, 44 B per capture, 12 B per live
     site):
@@ -36,8 +36,12 @@ def algorithmic_bytes(log, chunk=4096):
                4, grain id 4, grown L written 8); wave 1: 40 B per wave-1 grain
                (no growth write)
     k_resolve  4 B per survivor-list entry written
+    k_compact  16 B per 4x4x4 brick (its active word, read twice) and 4 B per
+               active grain (its list entry written)
+    k_idscan   1 B per grain id below the step's first new id (the capture
+               counts it scans; an upper bound of the scanned range)
     k_scan     16 B per list chunk of `chunk` entries (its aggregate read and
-               its prefix written), for wave 0 and wave 1
+               its prefix written), wave 1
     k_emit     44 B per capture: child envelope 32, orientation 4, grain id 4,
                wave-list entry 4 (captures of waves >= 2, made in k_tail, are a
                handful per step and are counted here too)
@@ -48,9 +52,14 @@ def algorithmic_bytes(log, chunk=4096):
     C = log["n_captured"].astype(np.float64)
     Aout = log["n_active_after"].astype(np.float64)
     S = log["n_sites"].astype(np.float64)
-    chunks = np.maximum(1.0, np.ceil(A / chunk)) + np.maximum(1.0, np.ceil(W1 / chunk))
+    chunks = np.maximum(1.0, np.ceil(W1 / chunk))
+    # grains after nucleation: one per solid cell, N minus the liquid cells
+    # before the step's captures
+    grains = cells - (log["n_liquid"].astype(np.float64) + C)
     return {
         "k_nucleate": 12.0 * S,
+        "k_idscan": 1.0 * grains,
+        "k_compact": 16.0 * cells / 64.0 + 4.0 * A,
         "k_claim": 48.0 * A + 40.0 * W1,
         "k_resolve": 4.0 * Aout,
         "k_scan": 16.0 * chunks,
@@ -589,7 +598,7 @@ def plain_main(args, cpu):
        after --warmup more untimed steps the timed window of --steps steps is
        centred on that peak; then the window is timed (per-kernel HIP events
        on the domain's stream).  Its kernels are the process's last
-       10 x --steps step-kernel dispatches, which is how the profile summary
+       11 x --steps step-kernel dispatches, which is how the profile summary
        (amca3d_amd/prof_summary.py --window-steps) picks them out.  The window's
        grain-ID field equals the first run's at the same step count by
        construction of the path (the tests check stepwise == run)."""
@@ -670,7 +679,7 @@ def plain_main(args, cpu):
         kt = dom.kernel_times()
         tot = {k: v[0] for k, v in kt.items()}
         dom_k = max(tot, key=tot.get)
-        ab = algorithmic_bytes(log)
+        ab = algorithmic_bytes(log, cells)
         per = {}
         for k, (ms, launches) in kt.items():
             if launches == 0:

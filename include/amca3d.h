@@ -147,8 +147,9 @@ amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env
                                         int32_t* orient, int32_t* n_grains);
 
 /* Device pointers of the grain-ID fields, of the grain envelopes (double4 per
- * grain id) and the stream (for callers that keep the data on the GPU).
- * Valid until amca_domain_destroy. */
+ * grain id, as of this call: the step keeps envelopes by cell and this call
+ * copies them into grain-id order) and the stream (for callers that keep the
+ * data on the GPU).  Valid until amca_domain_destroy. */
 amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
                                       void** stream);
 
