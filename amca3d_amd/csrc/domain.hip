@@ -36,11 +36,11 @@ amca_status fail(amca_status st, const std::string& msg) {
   } while (0)
 
 enum KernelId {
-  K_NUCLEATE = 0, K_CLAIM, K_RESOLVE, K_SCAN, K_EMIT, K_TAIL, K_IDSCAN, K_COMPACT, K_COUNT
+  K_NUCLEATE = 0, K_CLAIM, K_RESOLVE, K_SCAN, K_EMIT, K_TAIL, K_IDSCAN, K_COMPACT, K_CNT, K_COUNT
 };
 // named as the kernels are, so event times line up with a kernel trace
-const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim", "k_resolve", "k_scan",
-                                     "k_emit",     "k_tail",  "k_idscan",  "k_compact"};
+const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim", "k_resolve", "k_scan", "k_emit",
+                                     "k_tail",     "k_idscan", "k_compact", "k_count"};
 
 }  // namespace
 
@@ -158,14 +158,14 @@ amca_status enqueue_step(amca_domain* d) {
   const amca::DevBuffers& B = d->B;
   hipStream_t s = d->stream;
   AMCA_TRY(timed(d, K_NUCLEATE, [&] { return amca::launch_nucleate(P, B, d->nuc_blocks, s); }));
+  AMCA_TRY(timed(d, K_CNT, [&] { return amca::launch_count(P, B, d->cmp_blocks, s); }));
   AMCA_TRY(timed(d, K_COMPACT, [&] { return amca::launch_compact(P, B, d->cmp_blocks, s); }));
   for (int wave = 0; wave < 2; wave++) {
     AMCA_TRY(timed(d, K_CLAIM, [&] { return amca::launch_claim(P, B, wave, d->list_blocks, s); }));
     AMCA_TRY(timed(d, K_RESOLVE,
                    [&] { return amca::launch_resolve(P, B, wave, d->emit_blocks, s); }));
     if (wave == 0) {
-      AMCA_TRY(timed(d, K_IDSCAN,
-                     [&] { return amca::launch_idscan(P, B, d->ids_blocks, d->emit_blocks, s); }));
+      AMCA_TRY(timed(d, K_IDSCAN, [&] { return amca::launch_idscan(P, B, d->emit_blocks, s); }));
       AMCA_TRY(timed(d, K_EMIT, [&] { return amca::launch_emit(P, B, wave, d->emit_blocks, s); }));
       continue;
     }
@@ -290,6 +290,8 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   ALLOC(B.plist, N);
   ALLOC(B.cnt, N + 64);  // the id scan reads whole 16-byte groups past the last id
   ALLOC(B.coff, N);
+  ALLOC(B.idtot, amca::kMaxIdsBlocks);
+  ALLOC(B.cmptot, amca::kMaxBlocks);
   ALLOC(B.listW[0], N);
   ALLOC(B.listW[1], N);
   ALLOC(B.masks, N);
@@ -321,10 +323,13 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   d->list_blocks = (int)std::max(8LL, std::min<long long>((want + 7) / 8 * 8, 16LL * cus));
   d->emit_blocks = (int)std::max(8LL, std::min<long long>((want + 7) / 8 * 8, std::min(16LL * cus, (long long)amca::kMaxBlocks)));
   d->nuc_blocks = 1;
-  // id scan and compaction: two blocks per CU at most (a short look-back
-  // chain), at least 32 K ids / 512 bricks per block
-  d->ids_blocks = (int)std::max(1LL, std::min<long long>((P.N + 32767) / 32768, 2LL * cus));
-  d->cmp_blocks = (int)std::max(1LL, std::min<long long>((nB + 511) / 512, 2LL * cus));
+  // id scan and compaction: eight blocks per CU at most (each block's tiles
+  // run one after another), at least 32 K ids / 512 bricks per block
+  d->ids_blocks = (int)std::max(1LL, std::min<long long>((P.N + 32767) / 32768,
+                                                         std::min(8LL * cus, (long long)amca::kMaxIdsBlocks)));
+  P.ids_blocks = d->ids_blocks;
+  d->cmp_blocks = (int)std::max(1LL, std::min<long long>((nB + 511) / 512,
+                                                         std::min(8LL * cus, (long long)amca::kMaxBlocks)));
   HIP_TRY(hipStreamSynchronize(d->stream));
   *out = d;
   return AMCA_OK;

@@ -19,6 +19,7 @@ struct DevParams {
   int off[26];            // 26-neighbour offsets in the reference's k,j,i order
   int orient_cap;
   int nbX, nbY, nbZ;      // 4x4x4-cell bricks per axis of the liquid / claimed map
+  int ids_blocks;         // blocks of the id scan (k_idscan)
 };
 
 // Mutable scalars that live on the device, so that a whole step (or many
@@ -87,6 +88,8 @@ struct DevBuffers {
                           // per k_resolve block from its first list entry [N]
   unsigned char* cnt;     // captures per grain id, nonzero only inside a wave-0 step [N]
   int* coff;              // first child (wave-relative) per capturing grain id [N]
+  unsigned* idtot;        // wave-0 captures per id-scan block [kMaxIdsBlocks]
+  unsigned* cmptot;       // active cells per compaction block [kMaxBlocks]
   int* listW[2];          // capture-wave lists (grain ids) [N]
   unsigned long long* masks;  // per wave entry: liquid bits | inside bits << 32 [N]
   unsigned* res;          // per wave entry (waves >= 1): captured bits | keep << 31 [N]
@@ -104,6 +107,7 @@ constexpr int kBlock = 256;       // threads per block of the list kernels
 constexpr int kItems = 4;         // consecutive list entries per thread in scan passes
 constexpr int kMaxBlocks = 4096;  // blocks of a look-back kernel
 constexpr int kLbSites = 5;       // look-back launch sites per step (the last: block statistics)
+constexpr int kMaxIdsBlocks = 2048;  // blocks of the id scan (a k_resolve block keeps one count each)
 
 // Kernel launchers (plain_step.hip).  Each returns hipGetLastError().
 hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t s);
@@ -116,13 +120,14 @@ hipError_t launch_resolve(const DevParams& P, const DevBuffers& B, int wave, int
                           hipStream_t s);
 hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                        hipStream_t s);
-// wave 0: child offsets of the capturing grains, an exclusive scan over their
-// ids (`blocks` <= kMaxBlocks look-back blocks; `nres` = blocks of the wave-0
-// k_resolve, whose statistics words it sums)
-// wave 0's list: the active slots in brick order (look-back over the bricks)
+// wave 0's list: the active slots in brick order (k_count: active cells per
+// block of bricks, then k_compact over the same blocks)
+hipError_t launch_count(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
 hipError_t launch_compact(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
-hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int blocks, int nres,
-                         hipStream_t s);
+// wave 0: child offsets of the capturing grains, an exclusive scan over their
+// ids (P.ids_blocks blocks; `nres` = blocks of the wave-0 k_resolve, whose
+// statistics words it sums)
+hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int nres, hipStream_t s);
 // genv[g] = the envelope in grain g's slot, for g < n_grains
 hipError_t launch_materialize(const DevParams& P, const DevBuffers& B, hipStream_t s);
 hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
@@ -139,7 +144,7 @@ struct BatchLaunch {
   const int4* map_emit;  // resolve / emit geometry
   const int4* map_scan;  // one block per domain (scan and tail), z = emit block count
   const int4* map_ids;   // id-scan geometry
-  const int4* map_cmp;   // compaction geometry
+  const int4* map_cmp;   // compaction geometry (k_count and k_compact)
   int n_nuc, n_list, n_emit, n_one, n_ids, n_cmp;
 };
 hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s);

@@ -289,53 +289,55 @@ __device__ __forceinline__ void brick_range(const DevParams& P, int blk, int nbl
   if (lo > nB) lo = nB;
 }
 
-// The active slots in brick order: a look-back scan over the bricks' active
-// words.  Each thread counts its brick's active cells; in the second pass each
-// wavefront writes the slots of its 64 bricks one brick at a time (a ballot
-// and a coalesced store per brick).  The list's length is the active count.
-__device__ __forceinline__ void compact_body(const DevParams& P, const DevBuffers& B, int nblk) {
-  DevCounters* ctr = B.ctr;
-  if (ctr->skip) return;
-  __shared__ int s_bid;
-  __shared__ unsigned long long s_sm[kBlock / 64];
-  __shared__ unsigned long long s_excl;
-  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[3], 1u);
-  __syncthreads();
-  const int bid = s_bid;
+// The active slots in brick order.  k_count leaves the active-cell count of
+// each block of bricks; k_compact block b starts from the sum of the counts of
+// the blocks before it, and each thread lists its own brick's active slots
+// (bricks in 256-brick tiles, one block scan per tile).  The list's length is
+// the active count.
+__device__ __forceinline__ void cmp_range(const DevParams& P, int blk, int nblk, long long& lo,
+                                          long long& hi) {
   const long long nB = (long long)P.nbX * P.nbY * P.nbZ;
   const long long per = ((nB + nblk - 1) / nblk + kBlock - 1) / kBlock * kBlock;
-  const long long lo = (long long)bid * per < nB ? (long long)bid * per : nB;
-  const long long hi = lo + per < nB ? lo + per : nB;
+  lo = (long long)blk * per < nB ? (long long)blk * per : nB;
+  hi = lo + per < nB ? lo + per : nB;
+}
+
+__device__ __forceinline__ void count_body(const DevParams& P, const DevBuffers& B, int blk,
+                                           int nblk) {
+  if (B.ctr->skip) return;
+  __shared__ unsigned long long s_sm[kBlock / 64];
+  long long lo, hi;
+  cmp_range(P, blk, nblk, lo, hi);
   unsigned long long mine = 0;
   for (long long b = lo + threadIdx.x; b < hi; b += kBlock) mine += __popcll(B.amap[b]);
-  unsigned long long agg;
-  block_scan_excl(mine, &agg, s_sm);
-  if (threadIdx.x < 64) {
-    const unsigned long long e =
-        lookback(B.lb_flag + 3 * kMaxBlocks, B.lb_agg + 3 * kMaxBlocks, B.lb_pref + 3 * kMaxBlocks,
-                 bid, (unsigned)(ctr->step * kLbSites + 3), agg);
-    if (threadIdx.x == 0) s_excl = e;
-  }
-  __syncthreads();
-  unsigned long long run = s_excl;
-  const int lane = threadIdx.x & 63;
+  unsigned long long tot;
+  block_scan_excl(mine, &tot, s_sm);
+  if (threadIdx.x == 0) B.cmptot[blk] = (unsigned)tot;
+}
+
+__device__ __forceinline__ void compact_body(const DevParams& P, const DevBuffers& B, int blk,
+                                             int nblk) {
+  if (B.ctr->skip) return;
+  __shared__ unsigned long long s_sm[kBlock / 64];
+  long long lo, hi;
+  cmp_range(P, blk, nblk, lo, hi);
+  unsigned long long pre = 0;
+  for (int q = threadIdx.x; q < blk; q += kBlock) pre += B.cmptot[q];
+  unsigned long long run;
+  block_scan_excl(pre, &run, s_sm);
+  if (B.cmptot[blk] == 0u) return;
   for (long long base = lo; base < hi; base += kBlock) {
     const long long b = base + threadIdx.x;
-    const unsigned long long w = b < hi ? B.amap[b] : 0ull;
+    unsigned long long w = b < hi ? B.amap[b] : 0ull;
     unsigned long long tot;
-    const unsigned long long off = run + block_scan_excl(__popcll(w), &tot, s_sm);
-    // the wavefront's 64 bricks, one after another
-    const long long wb = base + (threadIdx.x & ~63);
-    for (int t = 0; t < 64; t++) {
-      const unsigned long long bw = __shfl(w, t, 64);
-      if (!bw) continue;
-      const unsigned long long bo = __shfl(off, t, 64);
-      if ((bw >> lane) & 1ull)
-        B.alist[bo + __popcll(bw & ((1ull << lane) - 1ull))] = (int)(64 * (wb + t) + lane);
+    unsigned long long off = run + block_scan_excl(__popcll(w), &tot, s_sm);
+    while (w) {
+      const int l = __builtin_ctzll(w);
+      w &= w - 1;
+      B.alist[off++] = (int)(64 * b + l);
     }
     run += tot;
   }
-  if (threadIdx.x == 0 && bid == nblk - 1) ctr->tickets[3] = 0u;
 }
 
 // Cell of a slot.
@@ -348,6 +350,13 @@ __device__ __forceinline__ void slot_ijk(const DevParams& P, long long q, int& i
   k = 4 * bz + (l >> 4);
 }
 
+// The id scan's block geometry over `ng` grain ids: block b covers the ids
+// [16 b per, 16 (b + 1) per), per a whole number of 256-thread tiles.
+__device__ __forceinline__ long long ids_per(const DevParams& P, int ng) {
+  const long long n4 = ((long long)ng + 15) >> 4;
+  return ((n4 + P.ids_blocks - 1) / P.ids_blocks + kBlock - 1) / kBlock * kBlock;
+}
+
 // Wave 0 over the active slots in brick order (nA_in of them): each active
 // grain grows (Grain::grow, Grain.h:27, over the whole list as in :521-538),
 // then tests its liquid neighbours.  Neighbouring entries sit in the same
@@ -356,6 +365,8 @@ __device__ __forceinline__ void claim0_body(const DevParams& P, const DevBuffers
                                             int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
+  // the id scan's per-block capture totals start from zero (k_resolve adds)
+  for (int q = blk * kBlock + threadIdx.x; q < P.ids_blocks; q += nblk * kBlock) B.idtot[q] = 0u;
   const double inc = ctr->dt * ctr->v;
   const int n = ctr->nA_in;
   int lo, hi;
@@ -472,15 +483,18 @@ __device__ __forceinline__ void resolve0_body(const DevParams& P, const DevBuffe
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   __shared__ int s_np, s_surv;
+  __shared__ unsigned s_hist[kMaxIdsBlocks];  // captures per id-scan block
   if (threadIdx.x == 0) {
     s_np = 0;
     s_surv = 0;
   }
+  for (int q = threadIdx.x; q < P.ids_blocks; q += kBlock) s_hist[q] = 0u;
   __syncthreads();
   const int n = ctr->nA_in;
   int lo, hi;
   chunk_of(n, blk, nblk, kBlock, lo, hi);
   int4* __restrict__ seg = B.plist + lo;
+  const long long iper = ids_per(P, ctr->n_grains);
   const int lane = threadIdx.x & 63;
   for (int base = lo; base < hi; base += kBlock) {
     const int t = base + threadIdx.x;
@@ -510,12 +524,15 @@ __device__ __forceinline__ void resolve0_body(const DevParams& P, const DevBuffe
       if (cap) {
         seg[pbase + (int)__popcll(capm & ((1ull << lane) - 1ull))] = make_int4(g, (int)q, (int)cap, 0);
         B.cnt[g] = (unsigned char)__builtin_popcount(cap);
+        atomicAdd(&s_hist[(g >> 4) / iper], (unsigned)__builtin_popcount(cap));
       }
     }
   }
   __syncthreads();
   if (threadIdx.x == 0)
     B.lb_agg[4 * kMaxBlocks + blk] = ((unsigned long long)s_np << 32) | (unsigned)s_surv;
+  for (int q = threadIdx.x; q < P.ids_blocks; q += kBlock)
+    if (s_hist[q]) atomicAdd(&B.idtot[q], s_hist[q]);
 }
 
 __device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffers& B, int wave,
@@ -524,48 +541,34 @@ __device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffer
 // Wave 0's children are numbered in (parent id, slot) order, as the
 // reference's sweep appends them: the first child of a capturing grain is the
 // sum of the capture counts of the capturing grains with smaller ids.  The
-// counts sit in an id-indexed byte array; this scan runs over the ids of all
-// grains, each thread over a contiguous run of 4-byte words and the blocks
-// chained by the decoupled look-back, writes each capturing grain's first
-// child and zeroes the counts it has read.  Its last block also adds the
-// wave's survivors (the statistics words of the `nres` k_resolve blocks) to
-// the active count.
-__device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers& B, int nblk,
+// counts sit in an id-indexed byte array, and k_resolve also added them up per
+// block of this scan (idtot).  Block b starts from the sum of the totals of
+// the blocks before it, scans its ids in tiles of 4096 (16 per thread, one
+// uint4 of counts), writes each capturing grain's first child and zeroes the
+// counts it has read.  The last block also adds the wave's survivors (the
+// statistics words of the `nres` k_resolve blocks) to the active count.
+__device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers& B, int blk,
                                             int nres) {
   DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
-  __shared__ int s_bid;
   __shared__ unsigned long long s_sm[kBlock / 64];
-  __shared__ unsigned long long s_excl;
-  if (threadIdx.x == 0) s_bid = (int)atomicAdd(&ctr->tickets[2], 1u);
-  __syncthreads();
-  const int bid = s_bid;
+  const int nblk = P.ids_blocks;
   const int ng = ctr->n_grains;
-  // 16 ids (one uint4 of counts) per thread and tile; tiles of 256 x 16 ids
   const long long n4 = ((long long)ng + 15) >> 4;
-  const long long per = ((n4 + nblk - 1) / nblk + kBlock - 1) / kBlock * kBlock;
-  const long long lo = (long long)bid * per < n4 ? (long long)bid * per : n4;
+  const long long per = ids_per(P, ng);
+  const long long lo = (long long)blk * per < n4 ? (long long)blk * per : n4;
   const long long hi = lo + per < n4 ? lo + per : n4;
+  unsigned long long pre = 0;
+  for (int q = threadIdx.x; q < blk; q += kBlock) pre += B.idtot[q];
+  unsigned long long start;
+  block_scan_excl(pre, &start, s_sm);
+  const unsigned mine = B.idtot[blk];
   uint4* __restrict__ c4 = reinterpret_cast<uint4*>(B.cnt);
   auto bytesum = [](unsigned v) {
     return (v & 0xffu) + ((v >> 8) & 0xffu) + ((v >> 16) & 0xffu) + (v >> 24);
   };
-  unsigned long long mine = 0;
-  for (long long u = lo + threadIdx.x; u < hi; u += kBlock) {
-    const uint4 v = c4[u];
-    if (v.x | v.y | v.z | v.w) mine += bytesum(v.x) + bytesum(v.y) + bytesum(v.z) + bytesum(v.w);
-  }
-  unsigned long long agg;
-  block_scan_excl(mine, &agg, s_sm);
-  if (threadIdx.x < 64) {
-    const unsigned long long e =
-        lookback(B.lb_flag + 2 * kMaxBlocks, B.lb_agg + 2 * kMaxBlocks, B.lb_pref + 2 * kMaxBlocks,
-                 bid, (unsigned)(ctr->step * kLbSites + 2), agg);
-    if (threadIdx.x == 0) s_excl = e;
-  }
-  __syncthreads();
-  unsigned long long run = s_excl;
-  if (agg) {
+  unsigned long long run = start;
+  if (mine) {
     for (long long base = lo; base < hi; base += kBlock) {
       const long long u = base + threadIdx.x;
       uint4 v = make_uint4(0u, 0u, 0u, 0u);
@@ -590,7 +593,7 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
       run += tot;
     }
   }
-  if (bid == nblk - 1) {
+  if (blk == nblk - 1) {
     // survivors of wave 0, summed over the k_resolve blocks' statistics
     unsigned long long sv = 0;
     for (int q = threadIdx.x; q < nres; q += kBlock)
@@ -598,8 +601,7 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
     unsigned long long svt;
     block_scan_excl(sv, &svt, s_sm);
     if (threadIdx.x == 0) {
-      // the last block's running prefix is the wave's child count
-      const unsigned long long nc = s_excl + agg;
+      const unsigned long long nc = start + mine;
       const int gbase = ctr->n_grains;
       ctr->wave_gbase = gbase;
       ctr->n_grains = gbase + (int)nc;
@@ -608,7 +610,6 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
       ctr->nA_out += (int)svt;
       if (nc > 0 && ctr->max_wave < 1) ctr->max_wave = 1;
       if (nc > 0 && ctr->step_wave < 1) ctr->step_wave = 1;
-      ctr->tickets[2] = 0u;
     }
   }
 }
@@ -1010,11 +1011,14 @@ __global__ void __launch_bounds__(kBlock) k_emit(DevParams P, DevBuffers B, int 
   emit_body(P, B, wave, xcd_block(), gridDim.x);
 }
 __global__ void __launch_bounds__(kTailBlock) k_tail(DevParams P, DevBuffers B) { tail_body(P, B); }
+__global__ void __launch_bounds__(kBlock) k_count(DevParams P, DevBuffers B) {
+  count_body(P, B, blockIdx.x, gridDim.x);
+}
 __global__ void __launch_bounds__(kBlock) k_compact(DevParams P, DevBuffers B) {
-  compact_body(P, B, gridDim.x);
+  compact_body(P, B, blockIdx.x, gridDim.x);
 }
 __global__ void __launch_bounds__(kBlock) k_idscan(DevParams P, DevBuffers B, int nres) {
-  idscan_body(P, B, gridDim.x, nres);
+  idscan_body(P, B, blockIdx.x, nres);
 }
 // genv[g] = the envelope in the slot of grain g's cell (downloads read genv)
 __global__ void k_materialize(DevParams P, DevBuffers B) {
@@ -1061,12 +1065,17 @@ __global__ void __launch_bounds__(kTailBlock) kb_tail(const DevParams* Ps, const
 __global__ void __launch_bounds__(kBlock) kb_compact(const DevParams* Ps, const DevBuffers* Bs,
                                                      const int4* map) {
   const int4 m = map[blockIdx.x];
-  compact_body(Ps[m.x], Bs[m.x], m.z);
+  compact_body(Ps[m.x], Bs[m.x], m.y, m.z);
+}
+__global__ void __launch_bounds__(kBlock) kb_count(const DevParams* Ps, const DevBuffers* Bs,
+                                                   const int4* map) {
+  const int4 m = map[blockIdx.x];
+  count_body(Ps[m.x], Bs[m.x], m.y, m.z);
 }
 __global__ void __launch_bounds__(kBlock) kb_idscan(const DevParams* Ps, const DevBuffers* Bs,
                                                     const int4* map) {
   const int4 m = map[blockIdx.x];
-  idscan_body(Ps[m.x], Bs[m.x], m.z, m.w);
+  idscan_body(Ps[m.x], Bs[m.x], m.y, m.w);
 }
 
 // per-domain status after a batch of steps: (finished, error, steps low, steps high)
@@ -1086,6 +1095,7 @@ hipError_t launch_batch_status(const BatchLaunch& L, int n, int4* out, hipStream
 
 hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
   hipLaunchKernelGGL(kb_nucleate, dim3(L.n_nuc), dim3(kBlock), 0, s, L.P, L.B, L.map_nuc);
+  hipLaunchKernelGGL(kb_count, dim3(L.n_cmp), dim3(kBlock), 0, s, L.P, L.B, L.map_cmp);
   hipLaunchKernelGGL(kb_compact, dim3(L.n_cmp), dim3(kBlock), 0, s, L.P, L.B, L.map_cmp);
   for (int wave = 0; wave < 2; wave++) {
     hipLaunchKernelGGL(kb_claim, dim3(L.n_list), dim3(kBlock), 0, s, L.P, L.B, L.map_list, wave);
@@ -1137,14 +1147,18 @@ hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int bl
   return hipGetLastError();
 }
 
+hipError_t launch_count(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s) {
+  hipLaunchKernelGGL(k_count, dim3(blocks), dim3(kBlock), 0, s, P, B);
+  return hipGetLastError();
+}
+
 hipError_t launch_compact(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s) {
   hipLaunchKernelGGL(k_compact, dim3(blocks), dim3(kBlock), 0, s, P, B);
   return hipGetLastError();
 }
 
-hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int blocks, int nres,
-                         hipStream_t s) {
-  hipLaunchKernelGGL(k_idscan, dim3(blocks), dim3(kBlock), 0, s, P, B, nres);
+hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int nres, hipStream_t s) {
+  hipLaunchKernelGGL(k_idscan, dim3(P.ids_blocks), dim3(kBlock), 0, s, P, B, nres);
   return hipGetLastError();
 }
 

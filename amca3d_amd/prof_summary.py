@@ -17,11 +17,11 @@ steps):
     python -m amca3d_amd.prof_summary --trace KT/bench_kernel_trace.csv --window-steps 20 \
         --fetch ... --write ... --l2 PMC_L2/bench_counter_collection.csv ...
 
-keeps the last 11 x window-steps dispatches of the step kernels (k_nucleate;
-k_compact, k_claim, k_resolve, k_idscan, k_emit for wave 0; k_claim, k_resolve,
-k_scan, k_emit for wave 1; k_tail: eleven per step, checked), from the trace
-and from every counter file, and reports each kernel over the window and per
-wave.
+keeps the last 12 x window-steps dispatches of the step kernels (k_nucleate;
+k_count, k_compact, k_claim, k_resolve, k_idscan, k_emit for wave 0; k_claim,
+k_resolve, k_scan, k_emit for wave 1; k_tail: twelve per step, checked), from
+the trace and from every counter file, and reports each kernel over the window
+and per wave.
 """
 import argparse
 import csv
@@ -56,10 +56,10 @@ def counter_means(path, counter):
     return {k: (sum(v) / len(v), len(v)) for k, v in acc.items()}
 
 
-STEP_KERNELS = ["k_nucleate", "k_compact", "k_claim", "k_resolve", "k_idscan", "k_emit",
-                "k_claim", "k_resolve", "k_scan", "k_emit", "k_tail"]
-STEP_LABELS = ["k_nucleate", "k_compact", "k_claim.w0", "k_resolve.w0", "k_idscan", "k_emit.w0",
-               "k_claim.w1", "k_resolve.w1", "k_scan.w1", "k_emit.w1", "k_tail"]
+STEP_KERNELS = ["k_nucleate", "k_count", "k_compact", "k_claim", "k_resolve", "k_idscan",
+                "k_emit", "k_claim", "k_resolve", "k_scan", "k_emit", "k_tail"]
+STEP_LABELS = ["k_nucleate", "k_count", "k_compact", "k_claim.w0", "k_resolve.w0", "k_idscan",
+               "k_emit.w0", "k_claim.w1", "k_resolve.w1", "k_scan.w1", "k_emit.w1", "k_tail"]
 
 
 def window_rows(rows, steps):

@@ -36,8 +36,9 @@ def algorithmic_bytes(log, cells, chunk=4096):
                4, grain id 4, grown L written 8); wave 1: 40 B per wave-1 grain
                (no growth write)
     k_resolve  4 B per survivor-list entry written
-    k_compact  16 B per 4x4x4 brick (its active word, read twice) and 4 B per
-               active grain (its list entry written)
+    k_count    8 B per 4x4x4 brick (its active word)
+    k_compact  8 B per 4x4x4 brick (its active word) and 4 B per active grain
+               (its list entry written)
     k_idscan   1 B per grain id below the step's first new id (the capture
                counts it scans; an upper bound of the scanned range)
     k_scan     16 B per list chunk of `chunk` entries (its aggregate read and
@@ -59,7 +60,8 @@ def algorithmic_bytes(log, cells, chunk=4096):
     return {
         "k_nucleate": 12.0 * S,
         "k_idscan": 1.0 * grains,
-        "k_compact": 16.0 * cells / 64.0 + 4.0 * A,
+        "k_count": 8.0 * cells / 64.0 * np.ones_like(A),
+        "k_compact": 8.0 * cells / 64.0 + 4.0 * A,
         "k_claim": 48.0 * A + 40.0 * W1,
         "k_resolve": 4.0 * Aout,
         "k_scan": 16.0 * chunks,
@@ -598,7 +600,7 @@ def plain_main(args, cpu):
        after --warmup more untimed steps the timed window of --steps steps is
        centred on that peak; then the window is timed (per-kernel HIP events
        on the domain's stream).  Its kernels are the process's last
-       11 x --steps step-kernel dispatches, which is how the profile summary
+       12 x --steps step-kernel dispatches, which is how the profile summary
        (amca3d_amd/prof_summary.py --window-steps) picks them out.  The window's
        grain-ID field equals the first run's at the same step count by
        construction of the path (the tests check stepwise == run)."""
