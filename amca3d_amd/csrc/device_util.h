@@ -125,12 +125,19 @@ __device__ __forceinline__ long long cell_slot(const DevParams& P, int c) {
   cell_ijk(P, c, i, j, k);
   return claim_index(P, i, j, k);
 }
+__device__ __forceinline__ unsigned fast_div(unsigned n, const FastDiv& f) {
+  const unsigned t = __umulhi(f.m, n);
+  return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+// Brick coordinates of brick b (b < 2^32).
 __device__ __forceinline__ void brick_xyz(const DevParams& P, long long b, int& bx, int& by,
                                           int& bz) {
-  bx = (int)(b % P.nbX);
-  const long long r = b / P.nbX;
-  by = (int)(r % P.nbY);
-  bz = (int)(r / P.nbY);
+  const unsigned ub = (unsigned)b;
+  const unsigned r = fast_div(ub, P.fdX);
+  bx = (int)(ub - r * (unsigned)P.nbX);
+  const unsigned z = fast_div(r, P.fdY);
+  by = (int)(r - z * (unsigned)P.nbY);
+  bz = (int)z;
 }
 __device__ __forceinline__ void active_set(const DevParams& P, unsigned long long* am, int i,
                                            int j, int k) {

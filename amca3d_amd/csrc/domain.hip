@@ -36,11 +36,11 @@ amca_status fail(amca_status st, const std::string& msg) {
   } while (0)
 
 enum KernelId {
-  K_NUCLEATE = 0, K_CLAIM, K_RESOLVE, K_SCAN, K_EMIT, K_TAIL, K_IDSCAN, K_COMPACT, K_CNT, K_COUNT
+  K_NUCLEATE = 0, K_CLAIM, K_RESOLVE, K_EMIT, K_TAIL, K_IDSCAN, K_COMPACT, K_CNT, K_COUNT
 };
 // named as the kernels are, so event times line up with a kernel trace
-const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim", "k_resolve", "k_scan", "k_emit",
-                                     "k_tail",     "k_idscan", "k_compact", "k_count"};
+const char* kKernelNames[K_COUNT] = {"k_nucleate", "k_claim",   "k_resolve", "k_emit",
+                                     "k_tail",     "k_idscan",  "k_compact", "k_count"};
 
 }  // namespace
 
@@ -164,12 +164,8 @@ amca_status enqueue_step(amca_domain* d) {
     AMCA_TRY(timed(d, K_CLAIM, [&] { return amca::launch_claim(P, B, wave, d->list_blocks, s); }));
     AMCA_TRY(timed(d, K_RESOLVE,
                    [&] { return amca::launch_resolve(P, B, wave, d->emit_blocks, s); }));
-    if (wave == 0) {
-      AMCA_TRY(timed(d, K_IDSCAN, [&] { return amca::launch_idscan(P, B, d->emit_blocks, s); }));
-      AMCA_TRY(timed(d, K_EMIT, [&] { return amca::launch_emit(P, B, wave, d->emit_blocks, s); }));
-      continue;
-    }
-    AMCA_TRY(timed(d, K_SCAN, [&] { return amca::launch_scan(P, B, wave, d->emit_blocks, s); }));
+    AMCA_TRY(timed(d, K_IDSCAN,
+                   [&] { return amca::launch_idscan(P, B, wave, d->emit_blocks, s); }));
     AMCA_TRY(timed(d, K_EMIT, [&] { return amca::launch_emit(P, B, wave, d->emit_blocks, s); }));
   }
   AMCA_TRY(timed(d, K_TAIL, [&] { return amca::launch_tail(P, B, s); }));
@@ -243,6 +239,8 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   P.nbX = (int)((nxG + 3) / 4);
   P.nbY = (int)((nyG + 3) / 4);
   P.nbZ = (int)((nzG + 3) / 4);
+  P.fdX = amca::make_fastdiv((unsigned)P.nbX);
+  P.fdY = amca::make_fastdiv((unsigned)P.nbY);
   P.N = (long long)desc->nx * desc->ny * desc->nz;
   const long long nB = (long long)P.nbX * P.nbY * P.nbZ;
   if (64 * nB > (long long)INT_MAX) {
@@ -282,8 +280,8 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   ALLOC(B.bmap, 2 * (size_t)nB);
   ALLOC(B.amap, (size_t)nB);
   ALLOC(B.rec, 64 * (size_t)nB);
+  ALLOC(B.sL, 64 * (size_t)nB);
   ALLOC(B.skey, 64 * (size_t)nB);
-  ALLOC(B.sori, 64 * (size_t)nB);
   ALLOC(B.genv, N);
   ALLOC(B.gco, N);
   ALLOC(B.alist, N);
@@ -504,12 +502,15 @@ amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain
   const size_t NG = (size_t)d->P.NG;
   amca::DevCounters c;
   AMCA_TRY(read_counters(d, &c));
+  // the grain-ID fields and the records by id, materialized from the slots
+  if (orient || grain || env) {
+    HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
+    HIP_TRY(hipStreamSynchronize(d->stream));
+  }
   if (orient) HIP_TRY(hipMemcpy(orient, d->B.oid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
   if (grain) HIP_TRY(hipMemcpy(grain, d->B.gid, sizeof(int32_t) * NG, hipMemcpyDeviceToHost));
   if (env) {
-    // grain records by id (materialized from the slots), scattered to their cells
-    HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
+    // grain records by id, scattered to their cells
     std::memset(env, 0, sizeof(double) * 4 * NG);
     const size_t ng = (size_t)std::max(c.n_grains, 0);
     if (ng > 0) {
@@ -538,9 +539,9 @@ amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env
   *n_grains = c.n_grains;
   const size_t m = (size_t)std::max(0, std::min(cap, c.n_grains));
   if (m == 0) return AMCA_OK;
+  HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
+  HIP_TRY(hipStreamSynchronize(d->stream));
   if (env4) {
-    HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
-    HIP_TRY(hipStreamSynchronize(d->stream));
     HIP_TRY(hipMemcpy(env4, d->B.genv, sizeof(double) * 4 * m, hipMemcpyDeviceToHost));
   }
   if (cell || orient) {
@@ -557,7 +558,7 @@ amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env
 amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
                                       void** stream) {
   if (!d) return fail(AMCA_ERR_INVALID, "null domain");
-  if (env) {
+  if (orient || grain || env) {
     HIP_TRY(amca::launch_materialize(d->P, d->B, d->stream));
     HIP_TRY(hipStreamSynchronize(d->stream));
   }
@@ -740,7 +741,6 @@ amca_status batch_apply_schedule(amca_batch* b) {
   build_map(order, nl, b->sched.interleave, ml);
   build_map(order, ne, b->sched.interleave, me);
   build_map(order, one, 0, ms);
-  for (auto& m : ms) m.z = ne[m.x];  // the scan of domain d spans its emit blocks
   const size_t total = mn.size() + ml.size() + me.size() + ms.size() + mi.size() + mc.size();
   if (b->maps) HIP_TRY(hipFree(b->maps));
   b->maps = nullptr;
@@ -756,8 +756,8 @@ amca_status batch_apply_schedule(amca_batch* b) {
   b->L.map_nuc = b->maps;
   b->L.map_list = b->maps + mn.size();
   b->L.map_emit = b->L.map_list + ml.size();
-  b->L.map_scan = b->L.map_emit + me.size();
-  b->L.map_ids = b->L.map_scan + ms.size();
+  b->L.map_one = b->L.map_emit + me.size();
+  b->L.map_ids = b->L.map_one + ms.size();
   b->L.map_cmp = b->L.map_ids + mi.size();
   b->L.n_nuc = (int)mn.size();
   b->L.n_list = (int)ml.size();

@@ -5,6 +5,14 @@
 
 namespace amca {
 
+// Division by a fixed 32-bit divisor d as a multiply and shifts (the
+// round-up method of Granlund and Montgomery, exact for every 32-bit n):
+// t = mulhi(m, n), n / d = (t + ((n - t) >> s1)) >> s2.
+struct FastDiv {
+  unsigned m;
+  int s1, s2;
+};
+
 // Constant description of a domain, passed by value to every kernel.
 struct DevParams {
   int nx, ny, nz;
@@ -19,6 +27,7 @@ struct DevParams {
   int off[26];            // 26-neighbour offsets in the reference's k,j,i order
   int orient_cap;
   int nbX, nbY, nbZ;      // 4x4x4-cell bricks per axis of the liquid / claimed map
+  FastDiv fdX, fdY;       // division by nbX and by nbY
   int ids_blocks;         // blocks of the id scan (k_idscan)
 };
 
@@ -43,6 +52,7 @@ struct DevCounters {
   int max_wave;
   int step_wave;          // deepest capturing wave of the running step
   int wave_gbase;         // first new grain id of the running wave
+  int scan_lo[2], scan_n[2];  // ids of the entries of list waves 0 and 1
   int err;                // 1: orientation table exhausted
   int log_cap;
   unsigned tickets[8];    // block tickets of the look-back kernels
@@ -56,33 +66,38 @@ struct StepLogRow {
   int n_sites;
 };
 
+// The grain in a cell's slot: its centre and (id, orientation).  The
+// half-diagonal L sits in an array of its own, since every step rewrites it
+// for each active grain while the rest stays as written at capture.
+struct alignas(32) SlotRec {
+  double x, y, z;
+  int id, o;
+};
+
 // Device buffers of one domain (all owned by the handle).
 //
-// Cell-indexed (ghosted grid, NG cells, x fastest as the reference's arrays):
-// the two grain-ID fields.  Brick-indexed: the grid is cut into 4x4x4-cell
-// bricks (x fastest); cell (i, j, k) is lane ((k&3)*16 + (j&3)*4 + (i&3)) of
-// brick b, and its "slot" is 64 b + lane (claim_index).  Per brick: two 64-bit
-// words of the 1-bit maps, [2b] liquid and [2b+1] claimed, and the active word
-// (a grain whose cell has the bit set is on the reference's active list).  Per
-// slot: the grain occupying the cell (envelope, id, orientation), the claim
-// word of the capture waves and the wave-0 neighbour masks.  A wave-0 pass
-// walks bricks, so a cell's 26 neighbours' bits come from the 27 words around
-// its brick, read once per 64 cells, and the records of a brick's active
-// cells are neighbours in memory.  Grain-indexed (grain id = creation order,
-// capacity N): (cell, orientation), and the envelope copy that downloads
-// read (materialized from the slots on request).  Wave lists of waves >= 1
-// hold grain ids in id order.
+// The grid is cut into 4x4x4-cell bricks (x fastest); cell (i, j, k) is lane
+// ((k&3)*16 + (j&3)*4 + (i&3)) of brick b, and its "slot" is 64 b + lane
+// (claim_index).  Per brick: two 64-bit words of the 1-bit maps, [2b] liquid
+// and [2b+1] claimed, and the active word (a grain whose cell has the bit set
+// is on the reference's active list).  Per slot: the grain occupying the cell
+// (record, L, id) and the claim word of the capture waves.  Steps read and
+// write only these: wave 0 walks the active slots in brick order, so
+// neighbouring entries share map words and record lines, and the lists of
+// waves >= 1 hold slots in grain-id order.  The reference's layouts
+// (cellGrainID_ / cellOrientationID_ on the ghosted grid, x fastest; records
+// by grain id) are materialized from the slots when a download asks for them.
 struct DevBuffers {
-  int* gid;               // cellGrainID_ [NG]
-  int* oid;               // cellOrientationID_ [NG]
+  int* gid;               // cellGrainID_ [NG], materialized for downloads
+  int* oid;               // cellOrientationID_ [NG], materialized for downloads
   int* claim;             // smallest claiming grain id per slot, INT_MAX when free [64 nB]
   unsigned long long* bmap;  // brick words: liquid, claimed [2 nB]
   unsigned long long* amap;  // brick words: active grains [nB]
-  double4* rec;           // envelope xc, yc, zc, L of the grain in each slot [64 nB]
-  int* skey;              // grain id of each slot [64 nB]
-  int* sori;              // orientation of each slot [64 nB]
+  SlotRec* rec;           // centre, id and orientation of the grain in each slot [64 nB]
+  double* sL;             // half-diagonal L of the grain in each slot [64 nB]
+  int* skey;              // grain id of each slot (the id of rec, packed alone) [64 nB]
   double4* genv;          // envelope by grain id, materialized for downloads [N]
-  int2* gco;              // grain -> (cell, orientation) [N]
+  int2* gco;              // grain -> (cell, orientation), materialized for downloads [N]
   int* alist;             // wave-0 list: active slots in brick order [N]
   int4* plist;            // wave-0 capturing grains: (id, slot, captured slots), one segment
                           // per k_resolve block from its first list entry [N]
@@ -90,7 +105,7 @@ struct DevBuffers {
   int* coff;              // first child (wave-relative) per capturing grain id [N]
   unsigned* idtot;        // wave-0 captures per id-scan block [kMaxIdsBlocks]
   unsigned* cmptot;       // active cells per compaction block [kMaxBlocks]
-  int* listW[2];          // capture-wave lists (grain ids) [N]
+  int* listW[2];          // capture-wave lists (slots, in grain-id order) [N]
   unsigned long long* masks;  // per wave entry: liquid bits | inside bits << 32 [N]
   unsigned* res;          // per wave entry (waves >= 1): captured bits | keep << 31 [N]
   int* site_cell[2];      // live nucleation sites, ascending cell, ping-pong
@@ -109,26 +124,35 @@ constexpr int kMaxBlocks = 4096;  // blocks of a look-back kernel
 constexpr int kLbSites = 5;       // look-back launch sites per step (the last: block statistics)
 constexpr int kMaxIdsBlocks = 2048;  // blocks of the id scan (a k_resolve block keeps one count each)
 
+inline FastDiv make_fastdiv(unsigned d) {
+  int l = 0;
+  while ((1ull << l) < d) l++;  // ceil(log2 d)
+  FastDiv f;
+  f.m = (unsigned)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+  f.s1 = l < 1 ? l : 1;
+  f.s2 = l > 1 ? l - 1 : 0;
+  return f;
+}
+
 // Kernel launchers (plain_step.hip).  Each returns hipGetLastError().
 hipError_t launch_init_grid(const DevParams& P, const DevBuffers& B, hipStream_t s);
 hipError_t launch_nucleate(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
 hipError_t launch_claim(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                         hipStream_t s);
-// k_resolve, k_scan and k_emit of one wave, launched in that order with the
-// same `blocks` (<= kMaxBlocks chunks)
+// Waves 0 and 1: k_claim, k_resolve, k_idscan and k_emit in that order;
+// k_resolve and k_emit with the same `blocks` (<= kMaxBlocks list chunks)
 hipError_t launch_resolve(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                           hipStream_t s);
-hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int blocks,
-                       hipStream_t s);
 // wave 0's list: the active slots in brick order (k_count: active cells per
 // block of bricks, then k_compact over the same blocks)
 hipError_t launch_count(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
 hipError_t launch_compact(const DevParams& P, const DevBuffers& B, int blocks, hipStream_t s);
-// wave 0: child offsets of the capturing grains, an exclusive scan over their
-// ids (P.ids_blocks blocks; `nres` = blocks of the wave-0 k_resolve, whose
-// statistics words it sums)
-hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int nres, hipStream_t s);
-// genv[g] = the envelope in grain g's slot, for g < n_grains
+// child offsets of a wave's capturing grains, an exclusive scan over their
+// ids (P.ids_blocks blocks; `nres` = blocks of the wave's k_resolve, whose
+// statistics words it reads)
+hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int wave, int nres,
+                         hipStream_t s);
+// gid / oid on the ghosted grid, genv / gco by grain id, from the slots
 hipError_t launch_materialize(const DevParams& P, const DevBuffers& B, hipStream_t s);
 hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                        hipStream_t s);
@@ -142,7 +166,7 @@ struct BatchLaunch {
   const int4* map_nuc;   // nucleation geometry per domain
   const int4* map_list;  // claim geometry
   const int4* map_emit;  // resolve / emit geometry
-  const int4* map_scan;  // one block per domain (scan and tail), z = emit block count
+  const int4* map_one;   // one block per domain (tail)
   const int4* map_ids;   // id-scan geometry
   const int4* map_cmp;   // compaction geometry (k_count and k_compact)
   int n_nuc, n_list, n_emit, n_one, n_ids, n_cmp;

@@ -12,14 +12,16 @@
 // neighbour takes it; new ids follow (parent id, neighbour slot) order; a grain
 // stays active iff one of its liquid neighbours is outside its envelope and not
 // taken by a smaller id of the same wave.  The kernels compute exactly that:
+//   k_count/k_compact: wave 0's list, the active slots in brick order;
 //   k_claim         : per wave entry, liquid test of the 26 neighbours on the
 //                     1-bit liquid map, envelope tests, and an atomicMin of the
 //                     grain id into each enclosed neighbour's claim word;
-//   k_resolve       : per entry, captured / keep decisions from the claims and
-//                     per-chunk totals; k_scan turns the totals into output
-//                     positions; k_emit writes the child envelopes and the
-//                     survivor list;
+//   k_resolve       : per entry, captured / keep decisions from the claims;
+//                     capturing grains go to a parent list with their counts;
+//   k_idscan        : first child of each capturing grain, a scan over ids;
+//   k_emit          : the child records and the next wave's list;
 //   k_tail          : waves >= 2 (rare) in one work-group, then the time advance.
+// Waves 0 and 1 each run k_claim, k_resolve, k_idscan and k_emit.
 // Claims need no reset: every claimed cell is captured in the same wave and is
 // never liquid again.
 #include <hip/hip_runtime.h>
@@ -94,6 +96,22 @@ __global__ void k_init_grid(DevParams P, DevBuffers B) {
 // next grain id; a non-liquid site is dropped.  Ranks come from the look-back
 // scan, so ids follow cell order exactly as in the reference loop.
 
+// Index of slot q's record.  A 128-byte line holds four 32-byte records; with
+// this order (x-bit 1 and y-bit 0 of the lane exchanged) they are a 2 x 2
+// patch of cells in x and y rather than a row of four in x, which holds more
+// cells of a growth front per line on average.
+__device__ __forceinline__ long long rec_slot(long long q) {
+  return (q & ~6ll) | ((q & 2) << 1) | ((q & 4) >> 1);
+}
+
+// Whether ghosted cell c is liquid: its bit in the brick map (ghost cells
+// never have one).
+__device__ __forceinline__ bool cell_liquid(const DevParams& P, const DevBuffers& B, int c) {
+  int i, j, k;
+  cell_ijk(P, c, i, j, k);
+  return (B.bmap[brick_word(P, i, j, k)] & brick_bit(i, j, k)) != 0ull;
+}
+
 // Kernel bodies take the domain by reference and their block number / block
 // count explicitly, so that one launch can also serve many domains (the
 // batched kernels `kb_*` map each block to a (domain, block, count) entry).
@@ -127,9 +145,8 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
   chunk_of(n, bid, nblk, kBlock, lo, hi);
   unsigned long long mine = 0;
   for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
-    int c = sc[i];
-    bool liq = B.gid[c] == -1;
-    bool nuc = liq && (under > sd[i]);
+    const bool liq = cell_liquid(P, B, sc[i]);
+    const bool nuc = liq && (under > sd[i]);
     mine += ((unsigned long long)nuc << 32) | (unsigned long long)(liq && !nuc);
   }
   unsigned long long agg;
@@ -147,7 +164,7 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
     bool nuc = false, keep = false;
     if (i < hi) {
       c = sc[i];
-      bool liq = B.gid[c] == -1;
+      const bool liq = cell_liquid(P, B, c);
       nuc = liq && (under > sd[i]);
       keep = liq && !nuc;
     }
@@ -162,13 +179,15 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
         cell_ijk(P, c, ci, cj, ck);
         const int id = ng0 + r;
         const long long q = claim_index(P, ci, cj, ck);
-        B.rec[q] = make_double4(centre_coord(P.x0, P.h, ci), centre_coord(P.y0, P.h, cj),
-                                centre_coord(P.z0, P.h, ck), 0.0);
+        SlotRec r;
+        r.x = centre_coord(P.x0, P.h, ci);
+        r.y = centre_coord(P.y0, P.h, cj);
+        r.z = centre_coord(P.z0, P.h, ck);
+        r.id = id;
+        r.o = o;
+        B.rec[rec_slot(q)] = r;
+        B.sL[q] = 0.0;
         B.skey[q] = id;
-        B.sori[q] = o;
-        B.gco[id] = make_int2(c, o);
-        B.gid[c] = id;
-        B.oid[c] = o;
         brick_clear(P, B.bmap, 0, ci, cj, ck);
         atomicOr(&B.amap[q >> 6], 1ull << (q & 63));
       }
@@ -202,6 +221,8 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
     ctr->step_nA0 = nA;
     ctr->step_cap = 0;
     ctr->step_wave = 0;
+    ctr->scan_lo[0] = 0;
+    ctr->scan_n[0] = ng0 + nn;
     ctr->nA_out = 0;
     ctr->nW[0] = 0;
     ctr->nW[1] = 0;
@@ -220,73 +241,96 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
 // ---------------------------------------------------------------------------
 // capture waves
 
-// Input list of waves >= 1: the grains created by the previous wave, in id
-// order.
-__device__ __forceinline__ void wave_input(const DevBuffers& B, int wave, const int*& list,
-                                           int& n) {
+// Waves 0 and 1 run as "list waves" over many work-groups: wave 0 over the
+// active slots in brick order (k_count / k_compact), wave 1 over the slots of
+// wave 0's children in the order wave 0's k_emit wrote them (parent blocks in
+// brick order).  The processing order of a wave does not change its outcome
+// (claims are an atomicMin, each entry's decision depends only on the claim
+// words); its children are numbered by an exclusive scan over the capturing
+// grains' ids (k_idscan).
+__device__ __forceinline__ void list_wave(const DevBuffers& B, int wave, const int*& list,
+                                          int& n) {
   const DevCounters* ctr = B.ctr;
-  list = B.listW[(wave - 1) & 1];
-  n = ctr->nW[(wave - 1) & 1];
+  if (wave == 0) {
+    list = B.alist;
+    n = ctr->nA_in;
+  } else {
+    list = B.listW[0];
+    n = ctr->nW[0];
+  }
 }
 
 // Envelope tests of grain g (orientation o, envelope e) at cell (ci, cj, ck)
 // against its liquid neighbours lm: each neighbour inside the envelope gets an
 // atomicMin of g into its claim word and its claimed bit.  Returns the inside
 // bits (none when L == 0: such an envelope captures nothing).
+//
+// inside_envelope's arithmetic, reorganised without changing a value: along
+// each axis the neighbour centres sit at three offsets from the envelope
+// centre, and to_local multiplies each offset by the same three rotation
+// entries, so the 27 products are formed once; a neighbour's local
+// coordinates are then sums of three of them, added in to_local's order
+// (its leading 0.0 + p is p up to the sign of a zero, which fabs drops).
 __device__ __forceinline__ unsigned claim_tests(const DevParams& P, const DevBuffers& B, int g,
                                                 int o, int ci, int cj, int ck, unsigned lm,
                                                 const double4& e) {
   if (lm == 0 || e.w == 0) return 0u;
-  // The L1 norm in the grain frame is at least the Euclidean distance, so a
-  // neighbour farther than L (with a margin far above rounding) cannot be
-  // inside; the orientation row is read only when some neighbour can be.
-  const double reach = e.w * e.w * (1.0 + 1e-9);
-  unsigned cand = 0;
-  {
-    unsigned rest = lm;
-    while (rest) {
-      const int s = __builtin_ctz(rest);
-      rest &= rest - 1;
-      int di, dj, dk;
-      slot_delta(s, di, dj, dk);
-      const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
-      const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
-      const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
-      if (gx * gx + gy * gy + gz * gz <= reach) cand |= 1u << s;
-    }
-  }
-  if (cand == 0) return 0u;
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
 #pragma unroll
   for (int q = 0; q < 9; q++) r[q] = rp[q];
+  double px[3][3], py[3][3], pz[3][3];  // [offset + 1][local axis]
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const double gx = centre_coord(P.x0, P.h, ci + d - 1) - e.x;
+    const double gy = centre_coord(P.y0, P.h, cj + d - 1) - e.y;
+    const double gz = centre_coord(P.z0, P.h, ck + d - 1) - e.z;
+#pragma unroll
+    for (int c = 0; c < 3; c++) {
+      px[d][c] = gx * r[c];
+      py[d][c] = gy * r[3 + c];
+      pz[d][c] = gz * r[6 + c];
+    }
+  }
+  // x + y parts once per (di, dj) column, then the three z offsets
   unsigned im = 0;
-  unsigned rest = cand;
+#pragma unroll
+  for (int dj = -1; dj <= 1; dj++) {
+#pragma unroll
+    for (int di = -1; di <= 1; di++) {
+      unsigned col = 0;
+#pragma unroll
+      for (int dk = -1; dk <= 1; dk++) {
+        const int idx = (dk + 1) * 9 + (dj + 1) * 3 + (di + 1);
+        if (idx != 13) col |= 1u << (idx < 13 ? idx : idx - 1);
+      }
+      if (!(lm & col)) continue;
+      const double sx = px[di + 1][0] + py[dj + 1][0];
+      const double sy = px[di + 1][1] + py[dj + 1][1];
+      const double sz = px[di + 1][2] + py[dj + 1][2];
+#pragma unroll
+      for (int dk = -1; dk <= 1; dk++) {
+        const int idx = (dk + 1) * 9 + (dj + 1) * 3 + (di + 1);
+        if (idx == 13) continue;
+        const int s = idx < 13 ? idx : idx - 1;
+        if (!((lm >> s) & 1u)) continue;
+        const double lx = sx + pz[dk + 1][0];
+        const double ly = sy + pz[dk + 1][1];
+        const double lz = sz + pz[dk + 1][2];
+        if ((fabs(lx) + fabs(ly)) + fabs(lz) <= e.w) im |= 1u << s;
+      }
+    }
+  }
+  unsigned rest = im;
   while (rest) {
     const int s = __builtin_ctz(rest);
     rest &= rest - 1;
     int di, dj, dk;
     slot_delta(s, di, dj, dk);
-    const double gx = centre_coord(P.x0, P.h, ci + di) - e.x;
-    const double gy = centre_coord(P.y0, P.h, cj + dj) - e.y;
-    const double gz = centre_coord(P.z0, P.h, ck + dk) - e.z;
-    if (inside_envelope(r, e.w, gx, gy, gz)) {
-      im |= 1u << s;
-      atomicMin(&B.claim[claim_index(P, ci + di, cj + dj, ck + dk)], g);
-      brick_set(P, B.bmap, 1, ci + di, cj + dj, ck + dk);
-    }
+    atomicMin(&B.claim[claim_index(P, ci + di, cj + dj, ck + dk)], g);
+    brick_set(P, B.bmap, 1, ci + di, cj + dj, ck + dk);
   }
   return im;
-}
-
-// Brick range [lo, hi) of block blk of nblk.
-__device__ __forceinline__ void brick_range(const DevParams& P, int blk, int nblk, long long& lo,
-                                            long long& hi) {
-  const long long nB = (long long)P.nbX * P.nbY * P.nbZ;
-  const long long per = (nB + nblk - 1) / nblk;
-  lo = (long long)blk * per;
-  hi = lo + per < nB ? lo + per : nB;
-  if (lo > nB) lo = nB;
 }
 
 // The active slots in brick order.  k_count leaves the active-cell count of
@@ -350,80 +394,84 @@ __device__ __forceinline__ void slot_ijk(const DevParams& P, long long q, int& i
   k = 4 * bz + (l >> 4);
 }
 
-// The id scan's block geometry over `ng` grain ids: block b covers the ids
-// [16 b per, 16 (b + 1) per), per a whole number of 256-thread tiles.
-__device__ __forceinline__ long long ids_per(const DevParams& P, int ng) {
-  const long long n4 = ((long long)ng + 15) >> 4;
-  return ((n4 + P.ids_blocks - 1) / P.ids_blocks + kBlock - 1) / kBlock * kBlock;
+// The id scan of a list wave covers the 16-id groups [g0, g0 + n4) that hold
+// the ids of the wave's entries (scan_lo, scan_n: all grains for wave 0, wave
+// 0's children for wave 1); block b takes the groups g0 + [b per, (b + 1) per),
+// per a whole number of 256-thread tiles.  The counters it reads are written
+// by an earlier kernel, so every block sees the same range.
+struct IdRange {
+  long long g0, n4, per;
+};
+__device__ __forceinline__ IdRange id_range(const DevParams& P, const DevCounters* ctr, int wave) {
+  const long long lo = ctr->scan_lo[wave], hi = lo + ctr->scan_n[wave];
+  IdRange r;
+  r.g0 = lo >> 4;
+  r.n4 = hi > lo ? ((hi + 15) >> 4) - r.g0 : 0;
+  r.per = ((r.n4 + P.ids_blocks - 1) / P.ids_blocks + kBlock - 1) / kBlock * kBlock;
+  if (r.per == 0) r.per = kBlock;
+  return r;
 }
 
-// Wave 0 over the active slots in brick order (nA_in of them): each active
-// grain grows (Grain::grow, Grain.h:27, over the whole list as in :521-538),
-// then tests its liquid neighbours.  Neighbouring entries sit in the same
-// bricks, so their map words and records share cache lines.
-__device__ __forceinline__ void claim0_body(const DevParams& P, const DevBuffers& B, int blk,
-                                            int nblk) {
+// Envelope tests of one entry of a wave >= 1 (the grain in slot q): these are
+// the cells captured by the previous wave, whose claimed bits are cleared here
+// (not while that wave still reads the map).
+template <bool kCoherent>
+__device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
+                                                          int q) {
+  int ci, cj, ck;
+  slot_ijk(P, q, ci, cj, ck);
+  brick_clear(P, B.bmap, 1, ci, cj, ck);
+  const SlotRec r = B.rec[rec_slot(q)];
+  const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
+  const unsigned im =
+      claim_tests(P, B, r.id, r.o, ci, cj, ck, lm, make_double4(r.x, r.y, r.z, B.sL[q]));
+  return ((unsigned long long)im << 32) | lm;
+}
+
+// A list wave's envelope tests, one entry per thread.  Wave 0 first grows
+// each active grain (Grain::grow, Grain.h:27, over the whole list as in
+// :521-538); wave 1's grains were created in this step and do not grow.
+// Neighbouring entries sit in the same bricks, so their map words and
+// records share cache lines.
+__device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers& B, int wave,
+                                           int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   // the id scan's per-block capture totals start from zero (k_resolve adds)
   for (int q = blk * kBlock + threadIdx.x; q < P.ids_blocks; q += nblk * kBlock) B.idtot[q] = 0u;
-  const double inc = ctr->dt * ctr->v;
-  const int n = ctr->nA_in;
-  int lo, hi;
-  chunk_of(n, blk, nblk, kBlock, lo, hi);
-  for (int t = lo + threadIdx.x; t < hi; t += kBlock) {
-    const long long q = B.alist[t];
-    int i, j, k;
-    slot_ijk(P, q, i, j, k);
-    double4 e = B.rec[q];
-    e.w = e.w + inc;
-    B.rec[q].w = e.w;
-    const unsigned lm = brick_bits26<0, false>(P, B.bmap, i, j, k);
-    const unsigned im = claim_tests(P, B, B.skey[q], B.sori[q], i, j, k, lm, e);
-    B.masks[t] = ((unsigned long long)im << 32) | lm;
-  }
-}
-
-// Envelope tests of one entry of a wave >= 1 (grain g): these are the cells
-// captured by the previous wave, whose claimed bits are cleared here (not
-// while that wave still reads the map).
-template <bool kCoherent>
-__device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
-                                                          int g) {
-  const int2 co = B.gco[g];
-  int ci, cj, ck;
-  cell_ijk(P, co.x, ci, cj, ck);
-  brick_clear(P, B.bmap, 1, ci, cj, ck);
-  const double4 e = B.rec[claim_index(P, ci, cj, ck)];
-  const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
-  const unsigned im = claim_tests(P, B, g, co.y, ci, cj, ck, lm, e);
-  return ((unsigned long long)im << 32) | lm;
-}
-
-__device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers& B, int wave,
-                                           int blk, int nblk) {
-  if (wave == 0) {
-    claim0_body(P, B, blk, nblk);
-    return;
-  }
-  const DevCounters* ctr = B.ctr;
-  if (ctr->skip) return;
   const int* list;
   int n;
-  wave_input(B, wave, list, n);
+  list_wave(B, wave, list, n);
   int lo, hi;
   chunk_of(n, blk, nblk, kBlock, lo, hi);
-  for (int i = lo + threadIdx.x; i < hi; i += kBlock) B.masks[i] = claim_entry<false>(P, B, list[i]);
+  if (wave == 0) {
+    const double inc = ctr->dt * ctr->v;
+    for (int t = lo + threadIdx.x; t < hi; t += kBlock) {
+      const long long q = list[t];
+      int i, j, k;
+      slot_ijk(P, q, i, j, k);
+      const SlotRec r = B.rec[rec_slot(q)];
+      const double L = B.sL[q] + inc;
+      B.sL[q] = L;
+      const unsigned lm = brick_bits26<0, false>(P, B.bmap, i, j, k);
+      const unsigned im = claim_tests(P, B, r.id, r.o, i, j, k, lm, make_double4(r.x, r.y, r.z, L));
+      B.masks[t] = ((unsigned long long)im << 32) | lm;
+    }
+    return;
+  }
+  for (int t = lo + threadIdx.x; t < hi; t += kBlock) B.masks[t] = claim_entry<false>(P, B, list[t]);
 }
 
-// Captured slots and the keep bit of grain g at (ci, cj, ck) from its masks m
-// and the claimed bits cmask of its neighbours.  A liquid slot nobody claimed
-// in this wave needs no claim word (its claim is "none", larger than any id);
-// only claimed slots read theirs, all loads issued before any is used.
+// Captured slots and the keep bit of the grain in slot q at (ci, cj, ck) from
+// its masks m and the claimed bits cmask of its neighbours.  A liquid slot
+// nobody claimed in this wave needs no claim word (its claim is "none", larger
+// than any id); only claimed slots read theirs, all loads issued before any is
+// used.  The grain's id is read (into g) only when a claim word is compared
+// with it, which every capture needs.
 template <bool kCoherent>
-__device__ __forceinline__ unsigned resolve_bits(const DevParams& P, const DevBuffers& B, int g,
-                                                 unsigned long long m, unsigned cmask, int ci,
-                                                 int cj, int ck) {
+__device__ __forceinline__ unsigned resolve_bits(const DevParams& P, const DevBuffers& B,
+                                                 long long q, unsigned long long m, unsigned cmask,
+                                                 int ci, int cj, int ck, int& g) {
   const unsigned lm = (unsigned)m, im = (unsigned)(m >> 32);
   if (lm == 0) return 0u;
   const unsigned cm = cmask & lm;
@@ -431,6 +479,7 @@ __device__ __forceinline__ unsigned resolve_bits(const DevParams& P, const DevBu
   unsigned keep = (lm & ~im & ~cm) ? 1u : 0u;
   const unsigned need = keep ? (cm & im) : cm;
   if (need == 0) return keep << 31;
+  g = B.skey[q];
   int cl[26];
 #pragma unroll
   for (int s = 0; s < 26; s++) {
@@ -456,45 +505,62 @@ __device__ __forceinline__ unsigned resolve_bits(const DevParams& P, const DevBu
   return cap | (keep << 31);
 }
 
-// One entry of a wave >= 1; a grain that does not stay active leaves the
-// active map.
+// One entry of a wave >= 1 (slot q); a grain that does not stay active leaves
+// the active map.
 template <bool kCoherent>
-__device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevBuffers& B, int g,
+__device__ __forceinline__ unsigned resolve_entry(const DevParams& P, const DevBuffers& B, int q,
                                                   unsigned long long m) {
-  const int c = B.gco[g].x;
-  int ci, cj, ck;
-  cell_ijk(P, c, ci, cj, ck);
   unsigned r = 0u;
-  if ((unsigned)m != 0u)
-    r = resolve_bits<kCoherent>(P, B, g, m, brick_bits26<1, kCoherent>(P, B.bmap, ci, cj, ck),
-                                ci, cj, ck);
-  if (!(r >> 31)) active_clear(P, B.amap, ci, cj, ck);
+  if ((unsigned)m != 0u) {
+    int ci, cj, ck;
+    slot_ijk(P, q, ci, cj, ck);
+    int g;
+    r = resolve_bits<kCoherent>(P, B, q, m, brick_bits26<1, kCoherent>(P, B.bmap, ci, cj, ck), ci,
+                                cj, ck, g);
+  }
+  if (!(r >> 31)) atomicAnd(&B.amap[q >> 6], ~(1ull << (q & 63)));
   return r;
 }
 
-// Wave 0 over the same list: a grain that does not stay active leaves the
-// active map; a grain that captured writes (id, slot, captured slots) to its
-// block's segment of the parent list (which starts at the block's first list
-// entry) and its capture count to the id-indexed count array.  Each block
-// leaves (parents << 32 | survivors) in its statistics word; no counter is
+// Inclusive prefix sum over the 64 lanes of a wave (every lane must call it).
+__device__ __forceinline__ unsigned wave_incl(unsigned v) {
+  const int lane = threadIdx.x & 63;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const unsigned u = __shfl_up(v, d, 64);
+    if (lane >= d) v += u;
+  }
+  return v;
+}
+
+// A list wave's decisions, over the same list chunks as its k_emit.  A grain
+// that does not stay active leaves the active map; a grain that captured
+// writes (id, slot, captured slots, first child in the block) to its block's
+// segment of the parent list, which starts at the block's first list entry,
+// and its capture count to the id-indexed count array.  Each block leaves
+// (parents << 32 | survivors) and its child count in its statistics words
+// and adds its captures per id-scan block into idtot; no other counter is
 // shared between blocks.
-__device__ __forceinline__ void resolve0_body(const DevParams& P, const DevBuffers& B, int blk,
-                                              int nblk) {
+__device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffers& B, int wave,
+                                             int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
-  __shared__ int s_np, s_surv;
+  __shared__ unsigned long long s_pc;         // parents << 32 | children
+  __shared__ int s_surv;
   __shared__ unsigned s_hist[kMaxIdsBlocks];  // captures per id-scan block
   if (threadIdx.x == 0) {
-    s_np = 0;
+    s_pc = 0ull;
     s_surv = 0;
   }
   for (int q = threadIdx.x; q < P.ids_blocks; q += kBlock) s_hist[q] = 0u;
   __syncthreads();
-  const int n = ctr->nA_in;
+  const int* list;
+  int n;
+  list_wave(B, wave, list, n);
   int lo, hi;
   chunk_of(n, blk, nblk, kBlock, lo, hi);
   int4* __restrict__ seg = B.plist + lo;
-  const long long iper = ids_per(P, ctr->n_grains);
+  const IdRange ir = id_range(P, ctr, wave);
   const int lane = threadIdx.x & 63;
   for (int base = lo; base < hi; base += kBlock) {
     const int t = base + threadIdx.x;
@@ -502,68 +568,70 @@ __device__ __forceinline__ void resolve0_body(const DevParams& P, const DevBuffe
     int g = 0;
     long long q = 0;
     if (t < hi) {
-      q = B.alist[t];
-      int i, j, k;
-      slot_ijk(P, q, i, j, k);
-      g = B.skey[q];
+      q = list[t];
       const unsigned long long m = B.masks[t];
-      if ((unsigned)m != 0u)
-        r = resolve_bits<false>(P, B, g, m, brick_bits26<1, false>(P, B.bmap, i, j, k), i, j, k);
+      if ((unsigned)m != 0u) {
+        int i, j, k;
+        slot_ijk(P, q, i, j, k);
+        r = resolve_bits<false>(P, B, q, m, brick_bits26<1, false>(P, B.bmap, i, j, k), i, j, k, g);
+      }
       if (!(r >> 31)) atomicAnd(&B.amap[q >> 6], ~(1ull << (q & 63)));
     }
     const unsigned long long keepm = __ballot(t < hi && (r >> 31));
+    if (lane == 0 && keepm) atomicAdd(&s_surv, (int)__popcll(keepm));
     const unsigned cap = r & 0x3ffffffu;
     const unsigned long long capm = __ballot(cap != 0u);
-    int pbase = 0;
-    if (lane == 0) {
-      if (keepm) atomicAdd(&s_surv, (int)__popcll(keepm));
-      if (capm) pbase = atomicAdd(&s_np, (int)__popcll(capm));
-    }
     if (capm) {
-      pbase = __shfl(pbase, 0, 64);
+      const unsigned pc = (unsigned)__builtin_popcount(cap);
+      const unsigned inc = wave_incl(pc);
+      const unsigned wtot = __shfl(inc, 63, 64);  // the wave's children
+      unsigned long long wb = 0ull;
+      if (lane == 0) wb = atomicAdd(&s_pc, ((unsigned long long)__popcll(capm) << 32) | wtot);
+      wb = __shfl(wb, 0, 64);
       if (cap) {
-        seg[pbase + (int)__popcll(capm & ((1ull << lane) - 1ull))] = make_int4(g, (int)q, (int)cap, 0);
-        B.cnt[g] = (unsigned char)__builtin_popcount(cap);
-        atomicAdd(&s_hist[(g >> 4) / iper], (unsigned)__builtin_popcount(cap));
+        const int pp = (int)(wb >> 32) + (int)__popcll(capm & ((1ull << lane) - 1ull));
+        const int cp = (int)(unsigned)wb + (int)(inc - pc);
+        seg[pp] = make_int4(g, (int)q, (int)cap, cp);
+        B.cnt[g] = (unsigned char)pc;
+        atomicAdd(&s_hist[((g >> 4) - ir.g0) / ir.per], pc);
       }
     }
   }
   __syncthreads();
-  if (threadIdx.x == 0)
-    B.lb_agg[4 * kMaxBlocks + blk] = ((unsigned long long)s_np << 32) | (unsigned)s_surv;
+  if (threadIdx.x == 0) {
+    B.lb_agg[4 * kMaxBlocks + blk] = (s_pc & 0xffffffff00000000ull) | (unsigned)s_surv;
+    B.lb_pref[4 * kMaxBlocks + blk] = s_pc & 0xffffffffull;
+  }
   for (int q = threadIdx.x; q < P.ids_blocks; q += kBlock)
     if (s_hist[q]) atomicAdd(&B.idtot[q], s_hist[q]);
 }
 
-__device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffers& B, int wave,
-                                             int blk, int nblk);
-
-// Wave 0's children are numbered in (parent id, slot) order, as the
+// A list wave's children are numbered in (parent id, slot) order, as the
 // reference's sweep appends them: the first child of a capturing grain is the
 // sum of the capture counts of the capturing grains with smaller ids.  The
 // counts sit in an id-indexed byte array, and k_resolve also added them up per
 // block of this scan (idtot).  Block b starts from the sum of the totals of
-// the blocks before it, scans its ids in tiles of 4096 (16 per thread, one
-// uint4 of counts), writes each capturing grain's first child and zeroes the
-// counts it has read.  The last block also adds the wave's survivors (the
-// statistics words of the `nres` k_resolve blocks) to the active count.
-__device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers& B, int blk,
-                                            int nres) {
+// the blocks before it, scans its id groups in tiles of 4096 ids (16 per
+// thread, one uint4 of counts), writes each capturing grain's first child
+// (wave-relative) and zeroes the counts it has read.  The last block also
+// turns the `nres` k_resolve blocks' child counts into each block's first
+// child, adds the wave's survivors to the active count, and publishes the
+// wave's children (the next id range and list length).
+__device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers& B, int wave,
+                                            int blk, int nres) {
   DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
   __shared__ unsigned long long s_sm[kBlock / 64];
   const int nblk = P.ids_blocks;
-  const int ng = ctr->n_grains;
-  const long long n4 = ((long long)ng + 15) >> 4;
-  const long long per = ids_per(P, ng);
-  const long long lo = (long long)blk * per < n4 ? (long long)blk * per : n4;
-  const long long hi = lo + per < n4 ? lo + per : n4;
+  const IdRange ir = id_range(P, ctr, wave);
+  const long long lo = (long long)blk * ir.per < ir.n4 ? (long long)blk * ir.per : ir.n4;
+  const long long hi = lo + ir.per < ir.n4 ? lo + ir.per : ir.n4;
   unsigned long long pre = 0;
   for (int q = threadIdx.x; q < blk; q += kBlock) pre += B.idtot[q];
   unsigned long long start;
   block_scan_excl(pre, &start, s_sm);
   const unsigned mine = B.idtot[blk];
-  uint4* __restrict__ c4 = reinterpret_cast<uint4*>(B.cnt);
+  uint4* __restrict__ c4 = reinterpret_cast<uint4*>(B.cnt) + ir.g0;
   auto bytesum = [](unsigned v) {
     return (v & 0xffu) + ((v >> 8) & 0xffu) + ((v >> 16) & 0xffu) + (v >> 24);
   };
@@ -579,7 +647,7 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
       unsigned long long off = run + block_scan_excl(sum, &tot, s_sm);
       if (any) {
         const unsigned w[4] = {v.x, v.y, v.z, v.w};
-        const int id = (int)(16 * u);
+        const int id = (int)(16 * (ir.g0 + u));
 #pragma unroll
         for (int t = 0; t < 16; t++) {
           const unsigned c = (w[t >> 2] >> (8 * (t & 3))) & 0xffu;
@@ -594,10 +662,20 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
     }
   }
   if (blk == nblk - 1) {
-    // survivors of wave 0, summed over the k_resolve blocks' statistics
-    unsigned long long sv = 0;
-    for (int q = threadIdx.x; q < nres; q += kBlock)
-      sv += (unsigned)B.lb_agg[4 * kMaxBlocks + q];
+    // survivors of the wave and each k_resolve block's first child
+    unsigned long long sv = 0, crun = 0;
+    for (int base = 0; base < nres; base += kBlock) {
+      const int q = base + threadIdx.x;
+      unsigned long long c = 0;
+      if (q < nres) {
+        sv += (unsigned)B.lb_agg[4 * kMaxBlocks + q];
+        c = B.lb_pref[4 * kMaxBlocks + q];
+      }
+      unsigned long long ctot;
+      const unsigned long long off = crun + block_scan_excl(c, &ctot, s_sm);
+      if (q < nres) B.lb_pref[4 * kMaxBlocks + q] = off;
+      crun += ctot;
+    }
     unsigned long long svt;
     block_scan_excl(sv, &svt, s_sm);
     if (threadIdx.x == 0) {
@@ -605,30 +683,37 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
       const int gbase = ctr->n_grains;
       ctr->wave_gbase = gbase;
       ctr->n_grains = gbase + (int)nc;
-      ctr->nW[0] = (int)nc;
+      ctr->nW[wave] = (int)nc;
       ctr->step_cap += (int)nc;
       ctr->nA_out += (int)svt;
-      if (nc > 0 && ctr->max_wave < 1) ctr->max_wave = 1;
-      if (nc > 0 && ctr->step_wave < 1) ctr->step_wave = 1;
+      if (wave == 0) {
+        ctr->scan_lo[1] = gbase;
+        ctr->scan_n[1] = (int)nc;
+      }
+      if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
+      if (nc > 0 && ctr->step_wave < wave + 1) ctr->step_wave = wave + 1;
     }
   }
 }
 
-// A captured cell's new grain: record in the cell's slot, grain-ID fields,
-// (cell, orientation) by id, liquid bit cleared, active bit set
-// (CellularAutomataManager_3D::capture_cell, :676-707; Grain.cpp:13-18).
-__device__ __forceinline__ void write_child(const DevParams& P, const DevBuffers& B, int id,
-                                            int ni, int nj, int nk, int o, const double4& env) {
+// A captured cell's new grain: record, L and id in the cell's slot, liquid bit
+// cleared, active bit set (CellularAutomataManager_3D::capture_cell,
+// :676-707; Grain.cpp:13-18).  Returns the slot.
+__device__ __forceinline__ int write_child(const DevParams& P, const DevBuffers& B, int id, int ni,
+                                           int nj, int nk, int o, const double4& env) {
   const long long q = claim_index(P, ni, nj, nk);
-  const int nb = (nk * P.nyG + nj) * P.nxG + ni;
-  B.rec[q] = env;
+  SlotRec r;
+  r.x = env.x;
+  r.y = env.y;
+  r.z = env.z;
+  r.id = id;
+  r.o = o;
+  B.rec[rec_slot(q)] = r;
+  B.sL[q] = env.w;
   B.skey[q] = id;
-  B.sori[q] = o;
-  B.gco[id] = make_int2(nb, o);
-  B.gid[nb] = id;
-  B.oid[nb] = o;
   brick_clear(P, B.bmap, 0, ni, nj, nk);
   atomicOr(&B.amap[q >> 6], 1ull << (q & 63));
+  return (int)q;
 }
 
 // Octahedron::capture_cell (Octahedron.cpp:133-275): the child envelope of
@@ -645,7 +730,7 @@ __device__ __forceinline__ double4 child_of(const DevParams& P, const double* r,
 
 // The children of a parent at (ci, cj, ck) with orientation o and envelope e:
 // each captured cell (slots ascending) gets a new grain, ids id0, id0+1, ...,
-// with the parent orientation, listed at kids[pos0 ...].
+// with the parent orientation; their slots are listed at kids[pos0 ...].
 __device__ __forceinline__ void emit_from(const DevParams& P, const DevBuffers& B, int o,
                                           const double4& e, int ci, int cj, int ck, unsigned cap,
                                           int id0, int* __restrict__ kids, int pos0) {
@@ -659,220 +744,54 @@ __device__ __forceinline__ void emit_from(const DevParams& P, const DevBuffers& 
     cap &= cap - 1;
     int di, dj, dk;
     slot_delta(s, di, dj, dk);
-    write_child(P, B, id0 + j, ci + di, cj + dj, ck + dk, o, child_of(P, r, e, ci, cj, ck, di, dj, dk));
-    kids[pos0 + j] = id0 + j;
+    kids[pos0 + j] =
+        write_child(P, B, id0 + j, ci + di, cj + dj, ck + dk, o, child_of(P, r, e, ci, cj, ck, di, dj, dk));
     j++;
   }
 }
 
-// Parent g's children (grain id known, cell from its record).
-__device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffers& B, int g,
+// The envelope of the grain in slot q.
+__device__ __forceinline__ double4 slot_env(const DevBuffers& B, const SlotRec& r, long long q) {
+  return make_double4(r.x, r.y, r.z, B.sL[q]);
+}
+
+// The children of the parent in slot qp.
+__device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffers& B, int qp,
                                               unsigned cap, int id0, int* __restrict__ kids,
                                               int pos0) {
-  const int2 co = B.gco[g];
   int ci, cj, ck;
-  cell_ijk(P, co.x, ci, cj, ck);
-  emit_from(P, B, co.y, B.rec[claim_index(P, ci, cj, ck)], ci, cj, ck, cap, id0, kids, pos0);
+  slot_ijk(P, qp, ci, cj, ck);
+  const SlotRec pr = B.rec[rec_slot(qp)];
+  emit_from(P, B, pr.o, slot_env(B, pr, qp), ci, cj, ck, cap, id0, kids, pos0);
 }
 
-// One child: the capture of parent g's neighbour slot s as grain `id`, listed
-// at kids[pos].
-__device__ __forceinline__ void emit_child(const DevParams& P, const DevBuffers& B, int g, int s,
-                                           int id, int* __restrict__ kids, int pos) {
-  const int2 co = B.gco[g];
-  const int o = co.y;
-  int ci, cj, ck;
-  cell_ijk(P, co.x, ci, cj, ck);
-  const double4 e = B.rec[claim_index(P, ci, cj, ck)];
-  double r[9];
-  const double* rp = B.l2g + 9 * (size_t)o;
-#pragma unroll
-  for (int q = 0; q < 9; q++) r[q] = rp[q];
-  int di, dj, dk;
-  slot_delta(s, di, dj, dk);
-  write_child(P, B, id, ci + di, cj + dj, ck + dk, o, child_of(P, r, e, ci, cj, ck, di, dj, dk));
-  kids[pos] = id;
-}
-
-// Wave-0 children: one thread per capturing grain of this block's parent-list
-// segment (the same list chunk as its k_resolve block), its children in slot
-// order with ids wave_gbase + first child + 0, 1, ..., placed at those
-// positions of the wave-1 list, which is therefore in id order.
-__device__ __forceinline__ void emit0_body(const DevParams& P, const DevBuffers& B, int blk,
-                                           int nblk) {
+// A list wave's children: one thread per capturing grain of this block's
+// parent-list segment (the same list chunk as its k_resolve block), its
+// children in slot order with ids wave_gbase + first child + 0, 1, ....  Wave
+// 0 lists their slots for wave 1 by parent block and position in the block
+// (parents in brick order); wave 1 lists them for the tail in id order.
+__device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& B, int wave,
+                                          int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
+  const int* list;
+  int n;
+  list_wave(B, wave, list, n);
   int lo, hi;
-  chunk_of(ctr->nA_in, blk, nblk, kBlock, lo, hi);
+  chunk_of(n, blk, nblk, kBlock, lo, hi);
   const int np = (int)(B.lb_agg[4 * kMaxBlocks + blk] >> 32);
+  const int cb = (int)B.lb_pref[4 * kMaxBlocks + blk];
   const int4* __restrict__ seg = B.plist + lo;
   const int gbase = ctr->wave_gbase;
-  int* __restrict__ kids = B.listW[0];
+  int* __restrict__ kids = B.listW[wave];
   for (int t = threadIdx.x; t < np; t += kBlock) {
     const int4 pe = seg[t];
     int i, j, k;
     slot_ijk(P, pe.y, i, j, k);
     const int p0 = B.coff[pe.x];
-    emit_from(P, B, B.sori[pe.y], B.rec[pe.y], i, j, k, (unsigned)pe.z, gbase + p0, kids, p0);
-  }
-}
-
-// The list of a wave >= 1 is cut into `nb` contiguous chunks (nb <= grid):
-// k_resolve decides every entry and writes per-chunk totals (children,
-// survivors), k_scan turns the totals into absolute output positions, k_emit
-// writes the children.
-__device__ __forceinline__ int wave_chunks(int n, int grid) {
-  constexpr int kTile = kBlock * kItems;
-  return max(1, min(grid, (n + kTile - 1) / kTile));
-}
-
-__device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffers& B, int wave,
-                                             int blk, int nblk) {
-  if (wave == 0) {
-    resolve0_body(P, B, blk, nblk);
-    return;
-  }
-  const DevCounters* ctr = B.ctr;
-  if (ctr->skip) return;
-  __shared__ unsigned long long s_sm[kBlock / 64];
-  const int* list;
-  int n;
-  wave_input(B, wave, list, n);
-  const int nb = wave_chunks(n, nblk);
-  const int bid = blk;
-  if (bid >= nb) return;
-  int lo, hi;
-  chunk_of(n, bid, nb, kBlock * kItems, lo, hi);
-  unsigned long long mine = 0;
-  for (int i = lo + threadIdx.x; i < hi; i += kBlock) {
-    const unsigned r = resolve_entry<false>(P, B, list[i], B.masks[i]);
-    B.res[i] = r;
-    mine += ((unsigned long long)__builtin_popcount(r & 0x3ffffffu) << 32) | (r >> 31);
-  }
-  unsigned long long tot;
-  block_scan_excl(mine, &tot, s_sm);
-  if (threadIdx.x == 0) B.lb_agg[kMaxBlocks + bid] = tot;  // chunk totals
-}
-
-constexpr int kScanBlock = 1024;
-
-// Waves >= 1: chunk totals -> absolute positions; children ids start at
-// n_grains, survivors are counted on top of nA_out.
-__device__ __forceinline__ void scan_body(const DevParams& P, const DevBuffers& B, int wave,
-                                          int grid) {
-  DevCounters* ctr = B.ctr;
-  if (ctr->skip) return;
-  __shared__ unsigned long long s_sm[kScanBlock / 64];
-  const int* list;
-  int n;
-  wave_input(B, wave, list, n);
-  const int nb = wave_chunks(n, grid);
-  const int gbase = ctr->n_grains;
-  const int sbase = ctr->nA_out;
-  constexpr int kPer = kMaxBlocks / kScanBlock;
-  unsigned long long v[kPer];
-  unsigned long long mine = 0;
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const int b = threadIdx.x * kPer + q;
-    v[q] = b < nb ? B.lb_agg[kMaxBlocks + b] : 0ull;
-    mine += v[q];
-  }
-  unsigned long long tot;
-  unsigned long long off = block_scan_excl(mine, &tot, s_sm) + (unsigned long long)sbase;
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const int b = threadIdx.x * kPer + q;
-    if (b < nb) B.lb_pref[kMaxBlocks + b] = off;
-    off += v[q];
-  }
-  if (threadIdx.x == 0) {
-    const int nc = (int)(tot >> 32);
-    const int nk = (int)(tot & 0xffffffffull);
-    ctr->wave_gbase = gbase;
-    ctr->n_grains = gbase + nc;
-    ctr->nA_out = sbase + nk;
-    ctr->nW[wave & 1] = nc;
-    ctr->step_cap += nc;
-    if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
-    if (nc > 0 && ctr->step_wave < wave + 1) ctr->step_wave = wave + 1;
-  }
-}
-
-__device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& B, int wave,
-                                          int blk, int nblk) {
-  if (wave == 0) {
-    emit0_body(P, B, blk, nblk);
-    return;
-  }
-  const DevCounters* ctr = B.ctr;
-  if (ctr->skip) return;
-  constexpr int kEmitStage = 1024;
-  __shared__ unsigned long long s_sm[kBlock / 64];
-  __shared__ int2 s_stage[kEmitStage];
-  const int* list;
-  int n;
-  wave_input(B, wave, list, n);
-  const int nb = wave_chunks(n, nblk);
-  const int bid = blk;
-  if (bid >= nb) return;
-  int* __restrict__ kids = B.listW[wave & 1];
-  const int gbase = ctr->wave_gbase;
-  constexpr int kTile = kBlock * kItems;
-  int lo, hi;
-  chunk_of(n, bid, nb, kTile, lo, hi);
-  unsigned long long run = B.lb_pref[kMaxBlocks + bid];
-  for (int base = lo; base < hi; base += kTile) {
-    unsigned r[kItems];
-    int g[kItems];
-    unsigned long long cnt = 0;
-#pragma unroll
-    for (int q = 0; q < kItems; q++) {
-      const int i = base + threadIdx.x * kItems + q;
-      r[q] = 0;
-      g[q] = 0;
-      if (i < hi) {
-        r[q] = B.res[i];
-        g[q] = list[i];
-      }
-      cnt += ((unsigned long long)__builtin_popcount(r[q] & 0x3ffffffu) << 32) | (r[q] >> 31);
-    }
-    unsigned long long tot;
-    unsigned long long off = run + block_scan_excl(cnt, &tot, s_sm);
-    const int cbase = (int)(run >> 32);  // first child position of this tile
-    const int ntile = (int)(tot >> 32);  // children of this tile
-    int cpos[kItems];                    // first child of each entry, tile-relative
-#pragma unroll
-    for (int q = 0; q < kItems; q++) {
-      const unsigned cap = r[q] & 0x3ffffffu;
-      cpos[q] = (int)(off >> 32) - cbase;
-      off += ((unsigned long long)__builtin_popcount(cap) << 32) | (r[q] >> 31);
-    }
-    // The tile's children are built one per thread: (parent, slot) pairs are
-    // staged in LDS in child order, kEmitStage at a time, so that the few
-    // capturing entries of a wave do not serialise its lanes.
-    for (int w0 = 0; w0 < ntile; w0 += kEmitStage) {
-#pragma unroll
-      for (int q = 0; q < kItems; q++) {
-        unsigned cap = r[q] & 0x3ffffffu;
-        int j = cpos[q];
-        while (cap) {
-          const int sl = __builtin_ctz(cap);
-          cap &= cap - 1;
-          if (j >= w0 && j < w0 + kEmitStage) s_stage[j - w0] = make_int2(g[q], sl);
-          j++;
-        }
-      }
-      __syncthreads();
-      const int m = min(kEmitStage, ntile - w0);
-      for (int t = threadIdx.x; t < m; t += kBlock) {
-        const int2 it = s_stage[t];
-        const int pos = cbase + w0 + t;
-        emit_child(P, B, it.x, it.y, gbase + pos, kids, pos);
-      }
-      __syncthreads();
-    }
-    run += tot;
+    const SlotRec pr = B.rec[rec_slot(pe.y)];
+    emit_from(P, B, pr.o, slot_env(B, pr, pe.y), i, j, k, (unsigned)pe.z, gbase + p0, kids,
+              wave == 0 ? cb + pe.w : p0);
   }
 }
 
@@ -907,10 +826,10 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
     for (int base = 0; base < n; base += kTailBlock) {
       const int i = base + threadIdx.x;
       unsigned r = 0;
-      int g = 0;
+      int q = 0;
       if (i < n) {
-        g = list[i];
-        r = resolve_entry<true>(P, B, g, B.masks[i]);
+        q = list[i];
+        r = resolve_entry<true>(P, B, q, B.masks[i]);
       }
       const unsigned cap = r & 0x3ffffffu;
       unsigned long long tot;
@@ -919,7 +838,7 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
                                 &tot, s_sm);
       if (cap) {
         const int pos = (int)(off >> 32);
-        emit_children(P, B, g, cap, s_gbase + pos, kids, pos);
+        emit_children(P, B, q, cap, s_gbase + pos, kids, pos);
       }
       run += tot;
     }
@@ -1004,9 +923,6 @@ __global__ void __launch_bounds__(kBlock) k_claim(DevParams P, DevBuffers B, int
 __global__ void __launch_bounds__(kBlock) k_resolve(DevParams P, DevBuffers B, int wave) {
   resolve_body(P, B, wave, xcd_block(), gridDim.x);
 }
-__global__ void __launch_bounds__(kScanBlock) k_scan(DevParams P, DevBuffers B, int wave, int grid) {
-  scan_body(P, B, wave, grid);
-}
 __global__ void __launch_bounds__(kBlock) k_emit(DevParams P, DevBuffers B, int wave) {
   emit_body(P, B, wave, xcd_block(), gridDim.x);
 }
@@ -1017,15 +933,32 @@ __global__ void __launch_bounds__(kBlock) k_count(DevParams P, DevBuffers B) {
 __global__ void __launch_bounds__(kBlock) k_compact(DevParams P, DevBuffers B) {
   compact_body(P, B, blockIdx.x, gridDim.x);
 }
-__global__ void __launch_bounds__(kBlock) k_idscan(DevParams P, DevBuffers B, int nres) {
-  idscan_body(P, B, blockIdx.x, nres);
+__global__ void __launch_bounds__(kBlock) k_idscan(DevParams P, DevBuffers B, int wave, int nres) {
+  idscan_body(P, B, wave, blockIdx.x, nres);
 }
-// genv[g] = the envelope in the slot of grain g's cell (downloads read genv)
+// The reference's layouts from the slots, for downloads: per interior cell,
+// cellGrainID_ / cellOrientationID_ (-1 when liquid), and per solid cell's
+// grain its envelope and (cell, orientation) by id.  Ghost cells keep the 0
+// that k_init_grid wrote.  In this path a cell never melts again, so every
+// solid cell holds the grain created there.
 __global__ void k_materialize(DevParams P, DevBuffers B) {
-  const int n = B.ctr->n_grains;
   const long long stride = (long long)gridDim.x * blockDim.x;
-  for (long long g = (long long)blockIdx.x * blockDim.x + threadIdx.x; g < n; g += stride)
-    B.genv[g] = B.rec[cell_slot(P, B.gco[g].x)];
+  for (long long c = (long long)blockIdx.x * blockDim.x + threadIdx.x; c < P.NG; c += stride) {
+    int i, j, k;
+    cell_ijk(P, (int)c, i, j, k);
+    if (i == 0 || j == 0 || k == 0 || i > P.nx || j > P.ny || k > P.nz) continue;
+    if (B.bmap[brick_word(P, i, j, k)] & brick_bit(i, j, k)) {
+      B.gid[c] = -1;
+      B.oid[c] = -1;
+      continue;
+    }
+    const long long q = claim_index(P, i, j, k);
+    const SlotRec r = B.rec[rec_slot(q)];
+    B.gid[c] = r.id;
+    B.oid[c] = r.o;
+    B.genv[r.id] = make_double4(r.x, r.y, r.z, B.sL[q]);
+    B.gco[r.id] = make_int2((int)c, r.o);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1046,11 +979,6 @@ __global__ void __launch_bounds__(kBlock) kb_resolve(const DevParams* Ps, const 
                                                      const int4* map, int wave) {
   const int4 m = map[blockIdx.x];
   resolve_body(Ps[m.x], Bs[m.x], wave, m.y, m.z);
-}
-__global__ void __launch_bounds__(kScanBlock) kb_scan(const DevParams* Ps, const DevBuffers* Bs,
-                                                      const int4* map, int wave) {
-  const int4 m = map[blockIdx.x];
-  scan_body(Ps[m.x], Bs[m.x], wave, m.z);
 }
 __global__ void __launch_bounds__(kBlock) kb_emit(const DevParams* Ps, const DevBuffers* Bs,
                                                   const int4* map, int wave) {
@@ -1073,9 +1001,9 @@ __global__ void __launch_bounds__(kBlock) kb_count(const DevParams* Ps, const De
   count_body(Ps[m.x], Bs[m.x], m.y, m.z);
 }
 __global__ void __launch_bounds__(kBlock) kb_idscan(const DevParams* Ps, const DevBuffers* Bs,
-                                                    const int4* map) {
+                                                    const int4* map, int wave) {
   const int4 m = map[blockIdx.x];
-  idscan_body(Ps[m.x], Bs[m.x], m.y, m.w);
+  idscan_body(Ps[m.x], Bs[m.x], wave, m.y, m.w);
 }
 
 // per-domain status after a batch of steps: (finished, error, steps low, steps high)
@@ -1100,14 +1028,10 @@ hipError_t launch_batch_step(const BatchLaunch& L, hipStream_t s) {
   for (int wave = 0; wave < 2; wave++) {
     hipLaunchKernelGGL(kb_claim, dim3(L.n_list), dim3(kBlock), 0, s, L.P, L.B, L.map_list, wave);
     hipLaunchKernelGGL(kb_resolve, dim3(L.n_emit), dim3(kBlock), 0, s, L.P, L.B, L.map_emit, wave);
-    if (wave == 0)
-      hipLaunchKernelGGL(kb_idscan, dim3(L.n_ids), dim3(kBlock), 0, s, L.P, L.B, L.map_ids);
-    else
-      hipLaunchKernelGGL(kb_scan, dim3(L.n_one), dim3(kScanBlock), 0, s, L.P, L.B, L.map_scan,
-                         wave);
+    hipLaunchKernelGGL(kb_idscan, dim3(L.n_ids), dim3(kBlock), 0, s, L.P, L.B, L.map_ids, wave);
     hipLaunchKernelGGL(kb_emit, dim3(L.n_emit), dim3(kBlock), 0, s, L.P, L.B, L.map_emit, wave);
   }
-  hipLaunchKernelGGL(kb_tail, dim3(L.n_one), dim3(kTailBlock), 0, s, L.P, L.B, L.map_scan);
+  hipLaunchKernelGGL(kb_tail, dim3(L.n_one), dim3(kTailBlock), 0, s, L.P, L.B, L.map_one);
   return hipGetLastError();
 }
 
@@ -1135,12 +1059,6 @@ hipError_t launch_resolve(const DevParams& P, const DevBuffers& B, int wave, int
   return hipGetLastError();
 }
 
-hipError_t launch_scan(const DevParams& P, const DevBuffers& B, int wave, int blocks,
-                       hipStream_t s) {
-  hipLaunchKernelGGL(k_scan, dim3(1), dim3(kScanBlock), 0, s, P, B, wave, blocks);
-  return hipGetLastError();
-}
-
 hipError_t launch_emit(const DevParams& P, const DevBuffers& B, int wave, int blocks,
                        hipStream_t s) {
   hipLaunchKernelGGL(k_emit, dim3(blocks), dim3(kBlock), 0, s, P, B, wave);
@@ -1157,13 +1075,14 @@ hipError_t launch_compact(const DevParams& P, const DevBuffers& B, int blocks, h
   return hipGetLastError();
 }
 
-hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int nres, hipStream_t s) {
-  hipLaunchKernelGGL(k_idscan, dim3(P.ids_blocks), dim3(kBlock), 0, s, P, B, nres);
+hipError_t launch_idscan(const DevParams& P, const DevBuffers& B, int wave, int nres,
+                         hipStream_t s) {
+  hipLaunchKernelGGL(k_idscan, dim3(P.ids_blocks), dim3(kBlock), 0, s, P, B, wave, nres);
   return hipGetLastError();
 }
 
 hipError_t launch_materialize(const DevParams& P, const DevBuffers& B, hipStream_t s) {
-  long long blocks = (P.N + 255) / 256;
+  long long blocks = ((long long)P.NG + 255) / 256;
   if (blocks > 16384) blocks = 16384;
   if (blocks < 1) blocks = 1;
   hipLaunchKernelGGL(k_materialize, dim3((unsigned)blocks), dim3(256), 0, s, P, B);

@@ -10,6 +10,13 @@ Whole process:
 FETCH_SIZE / WRITE_SIZE are KiB per dispatch as the counters report them; they
 are scaled by the factor that maps the copy probe's counters onto its known
 traffic (k_copy_probe reads and writes `--copy-bytes` bytes per dispatch).
+FETCH_SIZE prices every read request that is not 32 bytes at 64 bytes, while
+gfx950 also issues 128-byte reads (streaming reads are all of that size), so
+one scale cannot fit kernels whose request sizes differ.  `--rdreq` (a pass
+with TCC_EA0_RDREQ_32B_sum, TCC_EA0_RDREQ_64B_sum, TCC_EA0_RDREQ_128B_sum and
+TCC_EA0_RDREQ_sum) gives the read bytes from the request counts themselves,
+32 a + 64 b + 128 c; when it is given, reads come from it and FETCH_SIZE is
+kept as a cross-check.
 
 A window of plain CA steps (bench.py's timed window is the process's last
 steps):
@@ -18,8 +25,8 @@ steps):
         --fetch ... --write ... --l2 PMC_L2/bench_counter_collection.csv ...
 
 keeps the last 12 x window-steps dispatches of the step kernels (k_nucleate;
-k_count, k_compact, k_claim, k_resolve, k_idscan, k_emit for wave 0; k_claim,
-k_resolve, k_scan, k_emit for wave 1; k_tail: twelve per step, checked), from
+k_count, k_compact; k_claim, k_resolve, k_idscan, k_emit for wave 0 and again
+for wave 1; k_tail: twelve per step, checked), from
 the trace and from every counter file, and reports each kernel over the window
 and per wave.
 """
@@ -57,9 +64,9 @@ def counter_means(path, counter):
 
 
 STEP_KERNELS = ["k_nucleate", "k_count", "k_compact", "k_claim", "k_resolve", "k_idscan",
-                "k_emit", "k_claim", "k_resolve", "k_scan", "k_emit", "k_tail"]
-STEP_LABELS = ["k_nucleate", "k_count", "k_compact", "k_claim.w0", "k_resolve.w0", "k_idscan",
-               "k_emit.w0", "k_claim.w1", "k_resolve.w1", "k_scan.w1", "k_emit.w1", "k_tail"]
+                "k_emit", "k_claim", "k_resolve", "k_idscan", "k_emit", "k_tail"]
+STEP_LABELS = ["k_nucleate", "k_count", "k_compact", "k_claim.w0", "k_resolve.w0", "k_idscan.w0",
+               "k_emit.w0", "k_claim.w1", "k_resolve.w1", "k_idscan.w1", "k_emit.w1", "k_tail"]
 
 
 def window_rows(rows, steps):
@@ -107,6 +114,16 @@ def window_main(a):
     dur = window_rows(trace_rows(a.trace), a.window_steps)
     fetch = window_rows(counter_rows(a.fetch, "FETCH_SIZE"), a.window_steps) if a.fetch else {}
     write = window_rows(counter_rows(a.write, "WRITE_SIZE"), a.window_steps) if a.write else {}
+    rd = {}
+    if a.rdreq:
+        parts = {sz: window_rows(counter_rows(a.rdreq, f"TCC_EA0_RDREQ_{sz}B_sum"), a.window_steps)
+                 for sz in (32, 64, 128)}
+        rd = {l: [sum(sz * parts[sz][l][i] for sz in parts) for i in range(len(parts[32][l]))]
+              for l in parts[32]}
+        rdn = window_rows(counter_rows(a.rdreq, "TCC_EA0_RDREQ_sum"), a.window_steps)
+        rd_split = {l: sum(parts[sz][l][i] for sz in parts for i in range(len(parts[32][l])))
+                    for l in parts[32]}
+        res["rdreq_unsized"] = {l: sum(rdn[l]) - rd_split[l] for l in rdn}
     hit = window_rows(counter_rows(a.l2, "TCC_HIT"), a.window_steps) if a.l2 else {}
     miss = window_rows(counter_rows(a.l2, "TCC_MISS"), a.window_steps) if a.l2 else {}
 
@@ -119,6 +136,12 @@ def window_main(a):
             r["fetch_kib"] = sum(fk) / len(fk)
             r["write_kib"] = sum(wk) / len(wk)
             r["bytes_per_launch"] = 1024.0 * (r["fetch_kib"] * fscale + r["write_kib"] * wscale)
+        if rd and all(l in rd and l in write for l in labels):
+            rb = [v for l in labels for v in rd[l]]
+            wk = [v for l in labels for v in write[l]]
+            r["read_bytes"] = sum(rb) / len(rb)
+            r["write_bytes"] = 1024.0 * sum(wk) / len(wk)
+            r["bytes_per_launch"] = r["read_bytes"] + r["write_bytes"]
         if all(l in hit and l in miss for l in labels):
             h = sum(v for l in labels for v in hit[l])
             m = sum(v for l in labels for v in miss[l])
@@ -141,14 +164,30 @@ def window_main(a):
                   f"dispatch; FETCH_SIZE reads {res['copy_calibration']['fetch_kib']:.1f} KiB, "
                   f"WRITE_SIZE {res['copy_calibration']['write_kib']:.1f} KiB -> scale "
                   f"{fscale:.4f} (fetch), {wscale:.4f} (write).", ""]
+    if rd:
+        lines += ["Reads from the request counts (32 a + 64 b + 128 c B, TCC_EA0_RDREQ_*B_sum); "
+                  "writes from WRITE_SIZE; FETCH_SIZE x scale beside them as a cross-check.", ""]
 
     def table(title, rows):
-        out = [title, "", "| kernel | calls | total ms | avg us | HBM bytes/launch (scaled PMC) "
-                          "| L2 hit rate |", "|---|---|---|---|---|---|"]
+        if rd:
+            out = [title, "", "| kernel | calls | total ms | avg us | HBM read B/launch "
+                              "| HBM write B/launch | FETCH_SIZE x scale | L2 hit rate |",
+                   "|---|---|---|---|---|---|---|---|"]
+        else:
+            out = [title, "", "| kernel | calls | total ms | avg us | HBM bytes/launch (scaled PMC) "
+                              "| L2 hit rate |", "|---|---|---|---|---|---|"]
         for k, r in rows.items():
             b, hr = r.get("bytes_per_launch"), r.get("l2_hit_rate")
-            out.append(f"| {k} | {r['calls']} | {r['total_ms']:.3f} | {r['avg_us']:.2f} | "
-                       f"{'' if b is None else f'{b:.4g}'} | {'' if hr is None else f'{hr:.3f}'} |")
+            hs = '' if hr is None else f'{hr:.3f}'
+            if rd:
+                fx = r.get("fetch_kib")
+                fxs = '' if fx is None or not fscale else f"{1024.0 * fx * fscale:.4g}"
+                out.append(f"| {k} | {r['calls']} | {r['total_ms']:.3f} | {r['avg_us']:.2f} | "
+                           f"{r.get('read_bytes', 0):.4g} | {r.get('write_bytes', 0):.4g} | "
+                           f"{fxs} | {hs} |")
+            else:
+                out.append(f"| {k} | {r['calls']} | {r['total_ms']:.3f} | {r['avg_us']:.2f} | "
+                           f"{'' if b is None else f'{b:.4g}'} | {hs} |")
         return out + [""]
     lines += table("By kernel:", res["kernels"]) + table("By wave:", res["by_wave"])
     text = "\n".join(lines)
@@ -166,6 +205,7 @@ def main():
     ap.add_argument("--l2")
     ap.add_argument("--fetch")
     ap.add_argument("--write")
+    ap.add_argument("--rdreq")
     ap.add_argument("--copy-fetch")
     ap.add_argument("--copy-write")
     ap.add_argument("--copy-bytes", type=float, default=0.0)

@@ -26,45 +26,41 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "CA cell-updates/sec + % HBM peak; batched+surrogate vs plain GPU at fixed grain-ID error"
 
 
-def algorithmic_bytes(log, cells, chunk=4096):
+def algorithmic_bytes(log, cells):
     """Algorithmic HBM bytes per kernel over a step log, from the unit figures
     of the plain step (48 B per active grain, 44 B per capture, 12 B per live
-    site):
+    site).  k_claim, k_resolve, k_idscan and k_emit run once for wave 0 and
+    once for wave 1; the figures are per step, both launches together.
 
     k_nucleate 12 B per live site (cell index 4 + critical undercooling 8)
-    k_claim    wave 0: 48 B per active grain (centre + L read 32, orientation id
-               4, grain id 4, grown L written 8); wave 1: 40 B per wave-1 grain
-               (no growth write)
-    k_resolve  4 B per survivor-list entry written
     k_count    8 B per 4x4x4 brick (its active word)
     k_compact  8 B per 4x4x4 brick (its active word) and 4 B per active grain
                (its list entry written)
-    k_idscan   1 B per grain id below the step's first new id (the capture
-               counts it scans; an upper bound of the scanned range)
-    k_scan     16 B per list chunk of `chunk` entries (its aggregate read and
-               its prefix written), wave 1
-    k_emit     44 B per capture: child envelope 32, orientation 4, grain id 4,
-               wave-list entry 4 (captures of waves >= 2, made in k_tail, are a
-               handful per step and are counted here too)
+    k_claim    wave 0: 48 B per active grain (centre 24, L 8, grain id 4 and
+               orientation 4 read, grown L written 8); wave 1: 40 B per
+               wave-1 grain (no growth write)
+    k_resolve  1 bit per entry of waves 0 and 1 (its active bit)
+    k_idscan   1 B per grain id of the scanned ranges: every grain before
+               wave 0's children, then wave 0's children (the capture counts)
+    k_emit     44 B per capture: child centre 24, L 8, grain id 4, orientation
+               4, next-wave list entry 4 (captures of waves >= 2, made in
+               k_tail, are a handful per step and are counted here too)
     k_tail     120 B per step (the step's log row 56 and counter block 64)
     """
     A = log["n_active_before"].astype(np.float64)
     W1 = log["n_wave1"].astype(np.float64)
     C = log["n_captured"].astype(np.float64)
-    Aout = log["n_active_after"].astype(np.float64)
     S = log["n_sites"].astype(np.float64)
-    chunks = np.maximum(1.0, np.ceil(W1 / chunk))
     # grains after nucleation: one per solid cell, N minus the liquid cells
     # before the step's captures
     grains = cells - (log["n_liquid"].astype(np.float64) + C)
     return {
         "k_nucleate": 12.0 * S,
-        "k_idscan": 1.0 * grains,
         "k_count": 8.0 * cells / 64.0 * np.ones_like(A),
         "k_compact": 8.0 * cells / 64.0 + 4.0 * A,
         "k_claim": 48.0 * A + 40.0 * W1,
-        "k_resolve": 4.0 * Aout,
-        "k_scan": 16.0 * chunks,
+        "k_resolve": (A + W1) / 8.0,
+        "k_idscan": grains + W1,
         "k_emit": 44.0 * C,
         "k_tail": 120.0 * np.ones_like(A),
     }

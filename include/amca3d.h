@@ -146,10 +146,12 @@ amca_status amca_domain_download(amca_domain* d, int32_t* orient, int32_t* grain
 amca_status amca_domain_download_grains(amca_domain* d, int32_t cap, double* env4, int32_t* cell,
                                         int32_t* orient, int32_t* n_grains);
 
-/* Device pointers of the grain-ID fields, of the grain envelopes (double4 per
- * grain id, as of this call: the step keeps envelopes by cell and this call
- * copies them into grain-id order) and the stream (for callers that keep the
- * data on the GPU).  Valid until amca_domain_destroy. */
+/* Device pointers of the grain-ID fields (int32 per ghosted cell) and of the
+ * grain envelopes (double4 per grain id), and the stream, for callers that
+ * keep the data on the GPU.  The contents are as of this call: the step keeps
+ * grains by cell in its own layout and this call writes these arrays from it
+ * (later steps do not update them; call again after stepping).  Valid until
+ * amca_domain_destroy. */
 amca_status amca_domain_device_fields(amca_domain* d, void** orient, void** grain, void** env,
                                       void** stream);
 
