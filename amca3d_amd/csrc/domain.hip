@@ -65,6 +65,18 @@ struct amca_domain {
   double kernel_ms[K_COUNT] = {};
   int64_t kernel_launches[K_COUNT] = {};
   std::vector<void*> allocs;
+  // kGraphSteps steps captured once and replayed by run / run_fixed when
+  // kernel timing is off; dropped whenever a launch argument changes
+  static constexpr int kGraphSteps = 16;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+
+  void drop_graph() {
+    if (gexec) (void)hipGraphExecDestroy(gexec);
+    if (graph) (void)hipGraphDestroy(graph);
+    gexec = nullptr;
+    graph = nullptr;
+  }
 
   template <class T>
   amca_status alloc(T** p, size_t count) {
@@ -80,6 +92,8 @@ struct amca_domain {
   }
 
   ~amca_domain() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graph();
     for (void* p : allocs) (void)hipFree(p);
     for (auto& e : ev_pool)
       if (e) (void)hipEventDestroy(e);
@@ -169,6 +183,36 @@ amca_status enqueue_step(amca_domain* d) {
     AMCA_TRY(timed(d, K_EMIT, [&] { return amca::launch_emit(P, B, wave, d->emit_blocks, s); }));
   }
   AMCA_TRY(timed(d, K_TAIL, [&] { return amca::launch_tail(P, B, s); }));
+  return AMCA_OK;
+}
+
+amca_status capture_steps(amca_domain* d) {
+  d->drop_graph();
+  HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
+  amca_status st = AMCA_OK;
+  for (int q = 0; q < amca_domain::kGraphSteps && st == AMCA_OK; q++) st = enqueue_step(d);
+  hipGraph_t g = nullptr;
+  const hipError_t e = hipStreamEndCapture(d->stream, &g);
+  if (st != AMCA_OK || e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return st != AMCA_OK ? st : fail(AMCA_ERR_HIP, std::string("graph capture: ") + hipGetErrorString(e));
+  }
+  d->graph = g;
+  HIP_TRY(hipGraphInstantiate(&d->gexec, g, nullptr, nullptr, 0));
+  return AMCA_OK;
+}
+
+// n steps: whole graphs of kGraphSteps steps, then single launches (every
+// launch is single, with its events, when kernel timing is on).  Steps past
+// termination do nothing, so a graph may run beyond the last live step.
+amca_status enqueue_steps(amca_domain* d, long long n) {
+  if (!d->timing) {
+    for (; n >= amca_domain::kGraphSteps; n -= amca_domain::kGraphSteps) {
+      if (!d->gexec) AMCA_TRY(capture_steps(d));
+      HIP_TRY(hipGraphLaunch(d->gexec, d->stream));
+    }
+  }
+  for (; n > 0; n--) AMCA_TRY(enqueue_step(d));
   return AMCA_OK;
 }
 
@@ -349,6 +393,7 @@ amca_status amca_domain_set_sites(amca_domain* d, int32_t n, const int32_t* cell
       return fail(AMCA_ERR_INVALID, "sites must be strictly ascending by cell");
   }
   HIP_TRY(hipStreamSynchronize(d->stream));
+  d->drop_graph();  // site buffers and the nucleation grid may change
   int cap = std::max(n, 1);
   if (cap > d->site_cap) {
     for (int b = 0; b < 2; b++) {
@@ -421,7 +466,7 @@ amca_status amca_domain_run(amca_domain* d, int64_t max_steps, amca_counters* ou
     if (max_steps >= 0 && taken >= max_steps) break;
     long long k = batch;
     if (max_steps >= 0) k = std::min<long long>(k, max_steps - taken);
-    for (long long q = 0; q < k; q++) AMCA_TRY(enqueue_step(d));
+    AMCA_TRY(enqueue_steps(d, k));
     AMCA_TRY(read_counters(d, &c));
     batch = std::min<long long>(batch * 2, 64);
   }
@@ -434,7 +479,7 @@ amca_status amca_domain_run_fixed(amca_domain* d, int64_t n_steps) {
   amca::DevCounters c;
   AMCA_TRY(read_counters(d, &c));
   AMCA_TRY(check_rows(d, c));
-  for (int64_t q = 0; q < n_steps; q++) AMCA_TRY(enqueue_step(d));
+  AMCA_TRY(enqueue_steps(d, n_steps));
   HIP_TRY(hipStreamSynchronize(d->stream));
   return AMCA_OK;
 }
@@ -450,6 +495,7 @@ amca_status amca_domain_counters(amca_domain* d, amca_counters* out) {
 amca_status amca_domain_enable_log(amca_domain* d, int32_t capacity) {
   if (!d || capacity < 0) return fail(AMCA_ERR_INVALID, "bad arguments");
   HIP_TRY(hipStreamSynchronize(d->stream));
+  d->drop_graph();  // the log pointer is a launch argument
   if (d->B.log) {
     for (auto& p : d->allocs)
       if (p == d->B.log) {

@@ -346,6 +346,28 @@ __device__ __forceinline__ void cmp_range(const DevParams& P, int blk, int nblk,
   hi = lo + per < nB ? lo + per : nB;
 }
 
+// Position of the r-th set bit (from 0) of w, r < popcount(w).
+__device__ __forceinline__ int nth_bit(unsigned long long w, int r) {
+  int pos = 0;
+  unsigned c = (unsigned)__popc((unsigned)w);
+  if ((unsigned)r >= c) {
+    r -= (int)c;
+    w >>= 32;
+    pos = 32;
+  }
+  unsigned x = (unsigned)w;
+#pragma unroll
+  for (int half = 16; half >= 1; half >>= 1) {
+    c = (unsigned)__popc(x & ((1u << half) - 1u));
+    if ((unsigned)r >= c) {
+      r -= (int)c;
+      x >>= half;
+      pos += half;
+    }
+  }
+  return pos;
+}
+
 __device__ __forceinline__ void count_body(const DevParams& P, const DevBuffers& B, int blk,
                                            int nblk) {
   if (B.ctr->skip) return;
@@ -370,16 +392,27 @@ __device__ __forceinline__ void compact_body(const DevParams& P, const DevBuffer
   unsigned long long run;
   block_scan_excl(pre, &run, s_sm);
   if (B.cmptot[blk] == 0u) return;
+  // Per tile of 256 bricks the slots are written one per thread at
+  // consecutive positions: position p of the tile belongs to the last brick
+  // whose tile offset is <= p, as its (p - offset)-th active lane.
+  __shared__ unsigned s_off[kBlock];
+  __shared__ unsigned long long s_w[kBlock];
   for (long long base = lo; base < hi; base += kBlock) {
     const long long b = base + threadIdx.x;
-    unsigned long long w = b < hi ? B.amap[b] : 0ull;
+    const unsigned long long w = b < hi ? B.amap[b] : 0ull;
     unsigned long long tot;
-    unsigned long long off = run + block_scan_excl(__popcll(w), &tot, s_sm);
-    while (w) {
-      const int l = __builtin_ctzll(w);
-      w &= w - 1;
-      B.alist[off++] = (int)(64 * b + l);
+    const unsigned long long off = block_scan_excl(__popcll(w), &tot, s_sm);
+    s_off[threadIdx.x] = (unsigned)off;
+    s_w[threadIdx.x] = w;
+    __syncthreads();
+    for (unsigned p = threadIdx.x; p < (unsigned)tot; p += kBlock) {
+      int t = 0;
+#pragma unroll
+      for (int step = kBlock / 2; step >= 1; step >>= 1)
+        if (s_off[t + step] <= p) t += step;
+      B.alist[run + p] = (int)(64 * (base + t) + nth_bit(s_w[t], (int)(p - s_off[t])));
     }
+    __syncthreads();
     run += tot;
   }
 }
@@ -480,6 +513,15 @@ __device__ __forceinline__ unsigned resolve_bits(const DevParams& P, const DevBu
   const unsigned need = keep ? (cm & im) : cm;
   if (need == 0) return keep << 31;
   g = B.skey[q];
+  // claim_index of neighbour (di, dj, dk) = cx[di + 1] + cy[dj + 1] + cz[dk + 1]
+  unsigned cx[3], cy[3], cz[3];
+#pragma unroll
+  for (int d = 0; d < 3; d++) {
+    const unsigned x = (unsigned)(ci + d - 1), y = (unsigned)(cj + d - 1), z = (unsigned)(ck + d - 1);
+    cx[d] = 64u * (x >> 2) + (x & 3u);
+    cy[d] = 64u * (unsigned)P.nbX * (y >> 2) + ((y & 3u) << 2);
+    cz[d] = 64u * (unsigned)P.nbX * (unsigned)P.nbY * (z >> 2) + ((z & 3u) << 4);
+  }
   int cl[26];
 #pragma unroll
   for (int s = 0; s < 26; s++) {
@@ -487,7 +529,7 @@ __device__ __forceinline__ unsigned resolve_bits(const DevParams& P, const DevBu
     if ((need >> s) & 1u) {
       int di, dj, dk;
       slot_delta(s, di, dj, dk);
-      const int* w = &B.claim[claim_index(P, ci + di, cj + dj, ck + dk)];
+      const int* w = B.claim + (cx[di + 1] + cy[dj + 1] + cz[dk + 1]);
       cl[s] = kCoherent ? ld_coherent(w) : *w;
     }
   }

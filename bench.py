@@ -594,12 +594,16 @@ def plain_main(args, cpu):
        full-run figure and its step log the step with the most active grains.
     2. A second domain from the same deck is fast-forwarded, untimed, so that
        after --warmup more untimed steps the timed window of --steps steps is
-       centred on that peak; then the window is timed (per-kernel HIP events
-       on the domain's stream).  Its kernels are the process's last
-       12 x --steps step-kernel dispatches, which is how the profile summary
-       (amca3d_amd/prof_summary.py --window-steps) picks them out.  The window's
+       centred on that peak; then the window is timed as the library runs it
+       (whole steps replayed from a captured HIP graph).  The window's
        grain-ID field equals the first run's at the same step count by
-       construction of the path (the tests check stepwise == run)."""
+       construction of the path (the tests check stepwise == run).
+    3. Unless --no-timing, a third domain runs the same window with every
+       launch bracketed by HIP events on its stream, for the per-kernel
+       durations.  Its kernels are the process's last 12 x --steps
+       step-kernel dispatches, which is how the profile summary
+       (amca3d_amd/prof_summary.py --window-steps) picks them out; with
+       --no-timing the last ones are those of step 2's window."""
     import amca3d_amd
     from amca3d_amd import decks
     n = args.size
@@ -621,15 +625,19 @@ def plain_main(args, cpu):
     first = min(max(peak - K // 2, W), max(total - K, W))
     ff = first - W
 
-    sim = amca3d_amd.Simulation(deck).initialize()
-    dom = sim.domain
-    dom.enable_log(max(total, first + K) + 8)
-    if ff > 0:
-        dom.run_fixed(ff)
-    if W > 0:
-        dom.run_fixed(W)
-    if not args.no_timing:
-        dom.enable_timing(True)
+    def at_window():
+        sim = amca3d_amd.Simulation(deck).initialize()
+        dom = sim.domain
+        dom.enable_log(max(total, first + K) + 8)
+        if ff > 0:
+            dom.run_fixed(ff)
+        if W > 0:
+            dom.run_fixed(W)
+        return sim, dom
+
+    # the headline: the window as the library runs it (steps replayed from a
+    # captured HIP graph, no per-kernel events)
+    sim, dom = at_window()
     c0 = dom.counters()  # drains the stream
     t0 = time.perf_counter()
     dom.run_fixed(K)
@@ -639,6 +647,7 @@ def plain_main(args, cpu):
     executed = s1 - s0
     value = cells * executed / wall
     log = dom.read_log()[s0:s1]
+    sim.close()
     step_bytes = float(step_formula_bytes(log, cells).sum())
     result = {
         "metric": METRIC,
@@ -674,7 +683,17 @@ def plain_main(args, cpu):
                                     "formula": "4 N + 48 A + 44 C + 12 S per step"}},
     }
     if not args.no_timing:
+        # per-kernel durations: the same window again on a second domain
+        # fast-forwarded to the same step, each launch bracketed by HIP events
+        sim, dom = at_window()
+        dom.enable_timing(True)
+        ct0 = dom.counters()
+        dom.run_fixed(K)
+        ct1 = dom.counters()
+        if (int(ct0["step"]), int(ct1["step"])) != (s0, s1):
+            raise SystemExit("the timed replica of the window ran other steps")
         kt = dom.kernel_times()
+        sim.close()
         tot = {k: v[0] for k, v in kt.items()}
         dom_k = max(tot, key=tot.get)
         ab = algorithmic_bytes(log, cells)
