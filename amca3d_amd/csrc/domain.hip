@@ -694,6 +694,8 @@ struct amca_batch {
   amca::DevBuffers* dB = nullptr;
   int4* maps = nullptr;  // nuc | list | emit | scan | ids | compaction, back to back
   int4* status = nullptr;
+  int4* hstatus = nullptr;  // pinned: two slots of n statuses, one per replay in flight
+  hipEvent_t hready[2] = {nullptr, nullptr};
   amca::BatchLaunch L{};
   amca_batch_schedule sched{};
   hipGraph_t graph = nullptr;
@@ -716,6 +718,9 @@ struct amca_batch {
     if (dB) (void)hipFree(dB);
     if (maps) (void)hipFree(maps);
     if (status) (void)hipFree(status);
+    if (hstatus) (void)hipHostFree(hstatus);
+    for (auto& e : hready)
+      if (e) (void)hipEventDestroy(e);
     if (stream) (void)hipStreamDestroy(stream);
   }
 };
@@ -835,29 +840,51 @@ amca_status batch_capture(amca_batch* b, int steps) {
 }
 
 // Replays the graph until every domain has finished or `max_graphs` replays
-// ran; returns the number of finished domains and the elapsed time.
+// ran; returns the number of finished domains and the elapsed time.  Each
+// replay ends with a status kernel whose output is copied to a pinned slot;
+// replay r + 1 is queued before the host reads replay r's slot, so the GPU
+// does not wait for the host's check.  A replay queued after every domain
+// finished runs steps that all skip.
 amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, double* ms,
                          long long* replays) {
   if (!b->exec || b->graph_steps != b->sched.steps_per_graph)
     AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
   const int n = (int)b->doms.size();
-  std::vector<int4> st(n);
   hipEvent_t e0, e1;
   HIP_TRY(hipEventCreate(&e0));
   HIP_TRY(hipEventCreate(&e1));
+  auto cleanup = [&] {
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+  };
+  auto queue = [&](long long r) -> hipError_t {
+    hipError_t e = hipGraphLaunch(b->exec, b->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(b->hstatus + (r & 1) * n, b->status, sizeof(int4) * n,
+                         hipMemcpyDeviceToHost, b->stream);
+    if (e == hipSuccess) e = hipEventRecord(b->hready[r & 1], b->stream);
+    return e;
+  };
   HIP_TRY(hipEventRecord(e0, b->stream));
-  long long r = 0;
+  long long r = 0, queued = 0;
   int fin = 0;
-  while (max_graphs < 0 || r < max_graphs) {
-    HIP_TRY(hipGraphLaunch(b->exec, b->stream));
+  if (max_graphs != 0) {
+    HIP_TRY(queue(0));
+    queued = 1;
+  }
+  while (r < queued) {
+    if (max_graphs < 0 || queued < max_graphs) {
+      HIP_TRY(queue(queued));
+      queued++;
+    }
+    HIP_TRY(hipEventSynchronize(b->hready[r & 1]));
+    const int4* st = b->hstatus + (r & 1) * n;
     r++;
-    HIP_TRY(hipMemcpyAsync(st.data(), b->status, sizeof(int4) * n, hipMemcpyDeviceToHost, b->stream));
-    HIP_TRY(hipStreamSynchronize(b->stream));
     fin = 0;
     for (int i = 0; i < n; i++) {
       if (st[i].y) {
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(b->stream);
+        cleanup();
         return fail(AMCA_ERR_CAPACITY, "orientation table exhausted in domain " + std::to_string(i));
       }
       fin += st[i].x;
@@ -868,8 +895,8 @@ amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, dou
   HIP_TRY(hipEventSynchronize(e1));
   float t = 0.f;
   HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  cleanup();
+  r = queued;
   if (finished) *finished = fin;
   if (ms) *ms = t;
   if (replays) *replays = r;
@@ -923,7 +950,10 @@ amca_status amca_batch_create(amca_domain* const* doms, int32_t n, amca_batch** 
     if (st != AMCA_OK) break;
     if (hipMalloc(&b->dP, sizeof(amca::DevParams) * n) != hipSuccess ||
         hipMalloc(&b->dB, sizeof(amca::DevBuffers) * n) != hipSuccess ||
-        hipMalloc(&b->status, sizeof(int4) * n) != hipSuccess) {
+        hipMalloc(&b->status, sizeof(int4) * n) != hipSuccess ||
+        hipHostMalloc(&b->hstatus, sizeof(int4) * 2 * n) != hipSuccess ||
+        hipEventCreateWithFlags(&b->hready[0], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&b->hready[1], hipEventDisableTiming) != hipSuccess) {
       st = fail(AMCA_ERR_HIP, "batch allocation failed");
       break;
     }
@@ -984,44 +1014,74 @@ amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finishe
 
 // The autotuner runs the real batch: each candidate schedule takes
 // `trial_graphs` graph replays in turn, for `rounds` rounds, and the schedule
-// with the least total time wins.  Schedules do not change results, so the
+// with the least mean time per replay wins.  Stage 1 tries one block budget
+// for all list kernels (4, 16, 64 blocks per CU) in both block orders; stage 2
+// sizes the claim grid and the resolve / emit grid apart, a factor 4 up and
+// down from the stage-1 choice.  Schedules do not change results, so the
 // trial steps are ordinary steps of the run.
 amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rounds,
                                 amca_batch_schedule* chosen) {
   if (!b || trial_graphs < 1 || rounds < 1) return fail(AMCA_ERR_INVALID, "bad arguments");
+  const int emit_cap = (int)amca::kMaxBlocks * 8;
+  auto make = [&](int list_budget, int emit_budget, int inter) {
+    amca_batch_schedule s = b->sched;
+    s.list_blocks = list_budget * b->cus;
+    s.emit_blocks = std::min(emit_budget * b->cus, emit_cap);
+    s.min_blocks = 1;
+    s.interleave = inter;
+    return s;
+  };
+  bool all_done = false;
+  // mean ms per replay of each candidate over `rounds` passes; the best index
+  auto pick = [&](const std::vector<amca_batch_schedule>& cand, size_t* best) -> amca_status {
+    std::vector<double> tot(cand.size(), 0.0);
+    std::vector<int> runs(cand.size(), 0);
+    for (int r = 0; r < rounds && !all_done; r++)
+      for (size_t c = 0; c < cand.size() && !all_done; c++) {
+        b->sched.list_blocks = cand[c].list_blocks;
+        b->sched.emit_blocks = cand[c].emit_blocks;
+        b->sched.min_blocks = cand[c].min_blocks;
+        b->sched.interleave = cand[c].interleave;
+        AMCA_TRY(batch_apply_schedule(b));
+        int fin = 0;
+        double ms = 0.0;
+        long long rep = 0;
+        AMCA_TRY(batch_replay(b, trial_graphs, &fin, &ms, &rep));
+        tot[c] += ms / std::max(1LL, rep);
+        runs[c]++;
+        if (fin == (int)b->doms.size()) all_done = true;
+      }
+    size_t k = 0;
+    for (size_t c = 1; c < cand.size(); c++)
+      if (runs[c] > 0 && (runs[k] == 0 || tot[c] / runs[c] < tot[k] / runs[k])) k = c;
+    *best = k;
+    return AMCA_OK;
+  };
+  // stage 1: one budget for every list kernel, both block orders
   std::vector<amca_batch_schedule> cand;
   for (int budget : {4, 16, 64})
-    for (int inter : {0, 1}) {
-      amca_batch_schedule s = b->sched;
-      s.list_blocks = budget * b->cus;
-      s.emit_blocks = std::min(budget * b->cus, (int)amca::kMaxBlocks * 8);
-      s.min_blocks = 1;
-      s.interleave = inter;
-      cand.push_back(s);
-    }
-  std::vector<double> tot(cand.size(), 0.0);
-  bool all_done = false;
-  for (int r = 0; r < rounds && !all_done; r++)
-    for (size_t c = 0; c < cand.size() && !all_done; c++) {
-      b->sched.list_blocks = cand[c].list_blocks;
-      b->sched.emit_blocks = cand[c].emit_blocks;
-      b->sched.min_blocks = cand[c].min_blocks;
-      b->sched.interleave = cand[c].interleave;
-      AMCA_TRY(batch_apply_schedule(b));
-      int fin = 0;
-      double ms = 0.0;
-      long long rep = 0;
-      AMCA_TRY(batch_replay(b, trial_graphs, &fin, &ms, &rep));
-      tot[c] += ms / std::max(1LL, rep);
-      if (fin == (int)b->doms.size()) all_done = true;
-    }
+    for (int inter : {0, 1}) cand.push_back(make(budget, budget, inter));
   size_t best = 0;
-  for (size_t c = 1; c < cand.size(); c++)
-    if (tot[c] < tot[best]) best = c;
-  b->sched.list_blocks = cand[best].list_blocks;
-  b->sched.emit_blocks = cand[best].emit_blocks;
-  b->sched.min_blocks = cand[best].min_blocks;
-  b->sched.interleave = cand[best].interleave;
+  AMCA_TRY(pick(cand, &best));
+  amca_batch_schedule win = cand[best];
+  // stage 2: the claim kernel's grid and the resolve / emit grid apart, a
+  // factor 4 either way from the stage-1 choice
+  if (!all_done) {
+    const int lb = win.list_blocks / b->cus, eb = std::max(1, win.emit_blocks / b->cus);
+    std::vector<amca_batch_schedule> c2 = {win};
+    for (int f : {4, -4}) {
+      const int l2 = f > 0 ? lb * f : std::max(1, lb / -f);
+      const int e2 = f > 0 ? eb * f : std::max(1, eb / -f);
+      if (l2 != lb) c2.push_back(make(l2, eb, win.interleave));
+      if (e2 != eb) c2.push_back(make(lb, e2, win.interleave));
+    }
+    AMCA_TRY(pick(c2, &best));
+    win = c2[best];
+  }
+  b->sched.list_blocks = win.list_blocks;
+  b->sched.emit_blocks = win.emit_blocks;
+  b->sched.min_blocks = win.min_blocks;
+  b->sched.interleave = win.interleave;
   AMCA_TRY(batch_apply_schedule(b));
   if (chosen) *chosen = b->sched;
   return AMCA_OK;

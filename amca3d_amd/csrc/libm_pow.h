@@ -169,4 +169,58 @@ AMCA_LIBM_HD inline double glibc_pow(double x, double y) {
   return pow_exp(ehi, elo);
 }
 
+// Whether r is the correctly rounded value of an exact r + res (|res| <= 1/2
+// ulp) that lies more than 1/16 ulp from a rounding midpoint: r normal, not a
+// power of two (the spacing below it halves) and not in the top binade.
+AMCA_LIBM_HD inline bool clear_of_midpoint(double r, double res) {
+  const uint32_t e = top12(r);
+  if (e < 53 || e >= 0x7fe) return false;
+  if ((as_u64(r) & 0xfffffffffffffull) == 0) return false;
+  const double ulp = as_f64((uint64_t)(e - 52) << 52);
+  return __builtin_fabs(res) < 0.4375 * ulp;
+}
+
+// glibc's pow(x, y) for finite x > 0, evaluated from exact products for the
+// exponents 1, 2 and 3 where that is glibc's result (pow_shortcut says whether
+// it is).  glibc's pow is within
+// 0.54 ulp of the exact x^y (its source bounds the error of exp by 0.509 ulp
+// and the log's contribution by 1024 ln2 1.3 2^-68 2^53 < 0.03 ulp), so when
+// the exact x^y lies more than 0.04 ulp from a rounding midpoint glibc returns
+// the correctly rounded x^y.  x^1 is x.  x^2 = hi + lo exactly with
+// lo = fma(x, x, -hi).  x^3 = x hi + x lo, where x hi = p + pe exactly and the
+// rounded remainder t = pe + x lo is within 2^-100 of x^3 of the exact one, so
+// r = p + t rounds like x^3 and res = (p - r) + t is its residual.  Results
+// within 1/16 ulp of a midpoint, and everything else, take glibc_pow.
+AMCA_LIBM_HD inline bool pow_shortcut(double x, double y, double* out) {
+  if (top12(x) - 1u >= 0x7feu) return false;  // x not normal
+  if (y == 1.0) {
+    *out = x;
+    return true;
+  }
+  if (y == 2.0) {
+    const double hi = x * x;
+    const double lo = __builtin_fma(x, x, -hi);
+    *out = hi;
+    return clear_of_midpoint(hi, lo);
+  }
+  if (y == 3.0) {
+    const double h2 = x * x;
+    const double l2 = __builtin_fma(x, x, -h2);
+    const double p = x * h2;
+    const double pe = __builtin_fma(x, h2, -p);
+    const double t = pe + x * l2;
+    const double r = p + t;
+    const double res = (p - r) + t;
+    *out = r;
+    return clear_of_midpoint(r, res);
+  }
+  return false;
+}
+
+AMCA_LIBM_HD inline double glibc_pow_fast(double x, double y) {
+  double r;
+  if (pow_shortcut(x, y, &r)) return r;
+  return glibc_pow(x, y);
+}
+
 }  // namespace amca_libm

@@ -58,14 +58,15 @@ __device__ __forceinline__ bool rm_ghost_ijk(const RmParams& P, int i, int j, in
 // ProblemPhysics_RappazGandin1993::compute_growth_velocity_with_temperature_input
 // (ProblemPhysics.cpp:142-158): v = a1 pow(dT, b1) + a2 pow(dT, b2) + a3 pow(dT, b3)
 // for dT = Tm - T > 0, summed left to right, clamped at 0.  pow is glibc's
-// (libm_pow.h), so the velocity, and with it every growth increment and the
-// step's dt, is the reference's to the bit for any exponents.
+// (libm_pow.h, with exact-product shortcuts for the exponents 1, 2, 3), so the
+// velocity, and with it every growth increment and the step's dt, is the
+// reference's to the bit for any exponents.
 __device__ __forceinline__ double rm_velocity(const RmParams& P, double Tc) {
   const double dTu = P.Tm - Tc;
   double v = 0.0;
   if (dTu > 0) {
-    v = P.a1 * amca_libm::glibc_pow(dTu, P.b1) + P.a2 * amca_libm::glibc_pow(dTu, P.b2) +
-        P.a3 * amca_libm::glibc_pow(dTu, P.b3);
+    v = P.a1 * amca_libm::glibc_pow_fast(dTu, P.b1) +
+        P.a2 * amca_libm::glibc_pow_fast(dTu, P.b2) + P.a3 * amca_libm::glibc_pow_fast(dTu, P.b3);
     if (v < 0.0) v = 0.0;
   }
   return v;
@@ -833,7 +834,7 @@ __device__ __forceinline__ void rm_nucleate_body(const RmParams& P, const RmBuff
 }
 
 // dt = min(dT_FE, min(factor*h/v(minT), dT_FE)) from the minimum flagged
-// temperature (every thread of k_rm_claim computes it the same way)
+// temperature (every block of k_rm_claim computes it the same way)
 __device__ __forceinline__ double rm_step_dt(const RmParams& P, const RmCounters* ctr) {
   double minT = P.Tm;
   unsigned long long key = ~0ull;
@@ -940,14 +941,21 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) 
   if (ctr->skip) return;
   const RmRows w = rm_rows(P, ctr);
   const int total = w.ex * w.ey * w.ez;
-  const double dt = rm_step_dt(P, ctr);
+  // dt and the velocity at Ts + 0.01 K are the same for every thread: one
+  // thread per block evaluates them (six pow calls) for the rest
+  __shared__ double s_dt, s_vts;
+  if (threadIdx.x == 0) {
+    s_dt = rm_step_dt(P, ctr);
+    s_vts = rm_velocity(P, P.Ts + 0.01);
+  }
+  __syncthreads();
+  const double dt = s_dt, vts = s_vts;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // read by later kernels only
     B.ctr->dt = dt;
     B.ctr->step_cap = 0;
     B.ctr->step_fresh = 0;
   }
-  const double vts = rm_velocity(P, P.Ts + 0.01);
   const int stride = gridDim.x * blockDim.x;
   for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < total; q += stride) {
     int i, j, k;
@@ -1234,7 +1242,7 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_emit(RmParams P, RmBuffers B) {
 __global__ void k_pow_selftest(long long n, const double* x, double y, double* out) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    out[i] = amca_libm::glibc_pow(x[i], y);
+    out[i] = amca_libm::glibc_pow_fast(x[i], y);
 }
 
 // ---------------------------------------------------------------------------

@@ -191,9 +191,11 @@ amca_status amca_batch_get_schedule(amca_batch* b, amca_batch_schedule* s);
  * returns the finished-domain count, device milliseconds and replays. */
 amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finished, double* ms,
                            int64_t* replays);
-/* Tries the candidate schedules on the running batch (trial_graphs replays
- * each, `rounds` rounds) and keeps the fastest; the trial steps are ordinary
- * steps of the run. */
+/* Tries candidate schedules on the running batch (trial_graphs replays each,
+ * `rounds` rounds) and keeps the fastest: first one block budget for every
+ * list kernel (4, 16 or 64 blocks per CU) in both block orders, then the claim
+ * grid and the resolve / emit grid sized apart around that choice.  The trial
+ * steps are ordinary steps of the run. */
 amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rounds,
                                 amca_batch_schedule* chosen);
 
@@ -508,9 +510,10 @@ amca_status amca_write_fe_frames(const char* path, int32_t title_lines, int32_t 
 
 /* out[i] = pow(x[i], y) evaluated by the device code of the remelting growth
  * velocity (ProblemPhysics.cpp:152 calls glibc's pow(dT, b)), which restates
- * glibc 2.35's pow: for finite x[i] > 0 and y == 0 or |y| in [2^-65, 2^63)
- * the values equal glibc's bit for bit.  A check of the device arithmetic;
- * AMCA_ERR_INVALID for other arguments. */
+ * glibc 2.35's pow and answers the exponents 1, 2 and 3 from exact products
+ * where that gives glibc's result: for finite x[i] > 0 and y == 0 or |y| in
+ * [2^-65, 2^63) the values equal glibc's bit for bit.  A check of the device
+ * arithmetic; AMCA_ERR_INVALID for other arguments. */
 amca_status amca_pow_selftest(int64_t n, const double* x, double y, double* out);
 const char* amca_remelt_last_error(void);
 

@@ -17,6 +17,8 @@ def build(dirpath):
     L.pow_mismatches.restype = C.c_long
     L.pow_mismatches.argtypes = [C.c_long, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int,
                                  C.c_double, C.POINTER(C.c_long)]
+    L.pow_fast_mismatches.restype = C.c_long
+    L.pow_fast_mismatches.argtypes = L.pow_mismatches.argtypes
     L.pow_args.restype = None
     L.pow_args.argtypes = [C.c_long, C.c_double, C.c_int, C.c_int, C.c_uint64, C.c_int,
                            C.c_double, C.POINTER(C.c_double)]
@@ -36,6 +38,12 @@ def build(dirpath):
         naive = C.c_long(0)
         bad = L.pow_mismatches(n, y, lo_e, hi_e, seed, hard, band, C.byref(naive))
         return bad, naive.value
+
+    def fast_mismatches(n, y, lo_e, hi_e, seed=1, hard=0, band=0.0):
+        sc = C.c_long(0)
+        bad = L.pow_fast_mismatches(n, y, lo_e, hi_e, seed, hard, band, C.byref(sc))
+        return bad, sc.value
+    mismatches.fast = fast_mismatches
 
     def args(n, y, lo_e, hi_e, seed=1, hard=0, band=0.0):
         out = np.empty(n, np.float64)

@@ -68,6 +68,25 @@ long pow_mismatches(long n, double y, int lo_e, int hi_e, uint64_t seed, int har
   return bad;
 }
 
+// The same draws against glibc_pow_fast (the form the device velocity calls);
+// *shortcut counts the arguments it answered from exact products.
+long pow_fast_mismatches(long n, double y, int lo_e, int hi_e, uint64_t seed, int hard,
+                         double band, long* shortcut) {
+  g_state = 0x9e3779b97f4a7c15ull ^ (seed * 0xbf58476d1ce4e5b9ull) ^ 1;
+  long bad = 0, kept = 0, sc = 0;
+  while (kept < n) {
+    const double x = draw(lo_e, hi_e);
+    if (!(x > 0)) continue;
+    if (hard && midpoint_distance(x, (int)y) > band) continue;
+    kept++;
+    double r;
+    if (amca_libm::pow_shortcut(x, y, &r)) sc++;
+    if (as_u64(pow(x, y)) != as_u64(amca_libm::glibc_pow_fast(x, y))) bad++;
+  }
+  if (shortcut) *shortcut = sc;
+  return bad;
+}
+
 // The arguments pow_mismatches(n, ...) draws, written to out[n].
 void pow_args(long n, double y, int lo_e, int hi_e, uint64_t seed, int hard, double band,
               double* out) {

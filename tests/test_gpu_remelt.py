@@ -273,15 +273,18 @@ def test_pbf_deck_kinetics(amca):
 
 
 def test_device_pow_equals_glibc(amca, tmp_path):
-    """The device velocity's pow (amca_pow_selftest) equals glibc's pow bit for
-    bit: random undercoolings for the exponents 1, 2, 3, arguments whose exact
-    square or cube lies within 1e-3 ulp of a rounding midpoint, and the
-    subnormal and rescaled branches."""
+    """The device velocity's pow (amca_pow_selftest: glibc's pow with the
+    exact-product shortcut for the exponents 1, 2, 3) equals glibc's pow bit
+    for bit: random undercoolings for the exponents 1, 2, 3, arguments whose
+    exact square or cube lies within 1e-3 ulp of a rounding midpoint and within
+    0.09 ulp (both sides of the shortcut's threshold), and the subnormal and
+    rescaled branches."""
     import pow_check
     from amca3d_amd.remelt import pow_selftest
     _, args, glibc_pow = pow_check.build(tmp_path)
     sets = [(y, args(200_000, y, -45, 12, seed=3 + int(y))) for y in (1.0, 2.0, 3.0)]
     sets += [(y, args(50_000, y, -30, 11, seed=7 + int(y), hard=1, band=1e-3)) for y in (2.0, 3.0)]
+    sets += [(y, args(100_000, y, -30, 11, seed=9 + int(y), hard=1, band=0.09)) for y in (2.0, 3.0)]
     sets += [(3.0, args(50_000, 3.0, -1074, -1023, seed=5)), (3.0, args(50_000, 3.0, -400, -100, seed=6)),
              (2.5, args(50_000, 2.5, -1074, 1023, seed=8))]
     for y, x in sets:

@@ -61,3 +61,28 @@ def test_next_to_rounding_midpoints(check, y):
 def test_every_branch(check, y, lo_e, hi_e):
     bad, _ = check(300_000, y, lo_e, hi_e, seed=lo_e & 0xffff)
     assert bad == 0
+
+
+@pytest.mark.parametrize("y", [1.0, 2.0, 3.0])
+def test_exact_product_shortcut(check, y):
+    """glibc_pow_fast, the form the device velocity calls, answers most
+    undercoolings from exact products and equals glibc everywhere."""
+    bad, sc = check.fast(2_000_000, y, -45, 12, seed=20 + int(y))
+    assert bad == 0
+    assert sc > 1_700_000, sc
+
+
+@pytest.mark.parametrize("y", [2.0, 3.0])
+def test_exact_product_shortcut_at_its_threshold(check, y):
+    """Arguments whose exact square or cube lies within 0.09 ulp of a rounding
+    midpoint, on both sides of the shortcut's 1/16-ulp threshold."""
+    bad, sc = check.fast(300_000, y, -30, 11, seed=30 + int(y), hard=1, band=0.09)
+    assert bad == 0
+    assert 50_000 < sc < 250_000, sc
+
+
+@pytest.mark.parametrize("y,lo_e,hi_e", [(1.0, -1074, 1023), (2.0, -1074, 1023),
+                                         (3.0, -1074, 1023), (2.5, -60, 60)])
+def test_exact_product_shortcut_whole_range(check, y, lo_e, hi_e):
+    bad, _ = check.fast(300_000, y, lo_e, hi_e, seed=40 + int(4 * y))
+    assert bad == 0
