@@ -120,11 +120,6 @@ __device__ __forceinline__ unsigned brick_bits26(const DevParams& P, const unsig
 }
 
 // ---- per-cell slots (brick-major, the claim words' order) -----------------
-__device__ __forceinline__ long long cell_slot(const DevParams& P, int c) {
-  int i, j, k;
-  cell_ijk(P, c, i, j, k);
-  return claim_index(P, i, j, k);
-}
 __device__ __forceinline__ unsigned fast_div(unsigned n, const FastDiv& f) {
   const unsigned t = __umulhi(f.m, n);
   return (t + ((n - t) >> f.s1)) >> f.s2;
@@ -138,79 +133,6 @@ __device__ __forceinline__ void brick_xyz(const DevParams& P, long long b, int& 
   const unsigned z = fast_div(r, P.fdY);
   by = (int)(r - z * (unsigned)P.nbY);
   bz = (int)z;
-}
-__device__ __forceinline__ void active_set(const DevParams& P, unsigned long long* am, int i,
-                                           int j, int k) {
-  const long long q = claim_index(P, i, j, k);
-  atomicOr(&am[q >> 6], 1ull << (q & 63));
-}
-__device__ __forceinline__ void active_clear(const DevParams& P, unsigned long long* am, int i,
-                                             int j, int k) {
-  const long long q = claim_index(P, i, j, k);
-  atomicAnd(&am[q >> 6], ~(1ull << (q & 63)));
-}
-
-// Word `kWord` of the brick at offset (lane % 3 - 1, lane / 3 % 3 - 1,
-// lane / 9 - 1) from brick (bx, by, bz), for lanes 0..26 (0 outside the grid
-// and for the other lanes): the 27 words of a brick's 3x3x3 neighbourhood,
-// one per lane.
-template <int kWord>
-__device__ __forceinline__ unsigned long long tile_word(const DevParams& P,
-                                                        const unsigned long long* bm, int bx,
-                                                        int by, int bz, int lane) {
-  unsigned long long w = 0;
-  if (lane < 27) {
-    const int x = bx + lane % 3 - 1, y = by + (lane / 3) % 3 - 1, z = bz + lane / 9 - 1;
-    if (x >= 0 && y >= 0 && z >= 0 && x < P.nbX && y < P.nbY && z < P.nbZ)
-      w = bm[2 * (((long long)z * P.nbY + y) * P.nbX + x) + kWord];
-  }
-  return w;
-}
-
-// The 26 neighbour bits of cell (i, j, k) of brick (bx, by, bz) from the 27
-// words tile_word left one per lane, in slot order (k, j, i, centre skipped):
-// the same extraction as brick_bits26, with the 8 corner words fetched from
-// the lanes that hold them.  Every lane of the wave must execute it.
-__device__ __forceinline__ unsigned tile_bits26(unsigned long long held, int bx, int by, int bz,
-                                                int i, int j, int k) {
-  const int bx0 = (i - 1) >> 2, bx1 = (i + 1) >> 2;
-  const int by0 = (j - 1) >> 2, by1 = (j + 1) >> 2;
-  const int bz0 = (k - 1) >> 2, bz1 = (k + 1) >> 2;
-  unsigned long long w[8];
-#pragma unroll
-  for (int q = 0; q < 8; q++) {
-    const int rx = ((q & 1) ? bx1 : bx0) - bx + 1;
-    const int ry = ((q & 2) ? by1 : by0) - by + 1;
-    const int rz = ((q & 4) ? bz1 : bz0) - bz + 1;
-    w[q] = __shfl(held, (rz * 3 + ry) * 3 + rx, 64);
-  }
-  const int xs = (i - 1) & 3;
-  unsigned m = 0;
-  int s = 0;
-#pragma unroll
-  for (int dk = -1; dk <= 1; dk++) {
-    const int z = k + dk;
-    const bool hz = (z >> 2) != bz0;
-#pragma unroll
-    for (int dj = -1; dj <= 1; dj++) {
-      const int y = j + dj;
-      const bool hy = (y >> 2) != by0;
-      const unsigned long long lo = hz ? (hy ? w[6] : w[4]) : (hy ? w[2] : w[0]);
-      const unsigned long long hi = hz ? (hy ? w[7] : w[5]) : (hy ? w[3] : w[1]);
-      const int sh = ((z & 3) << 4) | ((y & 3) << 2);
-      const unsigned row8 = (unsigned)((lo >> sh) & 0xfull) | ((unsigned)((hi >> sh) & 0xfull) << 4);
-      const unsigned b = (row8 >> xs) & 7u;
-      if (dk == 0 && dj == 0) {
-        m |= (b & 1u) << s;
-        m |= ((b >> 2) & 1u) << (s + 1);
-        s += 2;
-      } else {
-        m |= b << s;
-        s += 3;
-      }
-    }
-  }
-  return m;
 }
 
 // wave-wide sum
@@ -321,6 +243,5 @@ __device__ __forceinline__ int xcd_block() {
   if (nb & 7) return blockIdx.x;
   return (blockIdx.x & 7) * (nb >> 3) + (blockIdx.x >> 3);
 }
-
 
 }  // namespace amca

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <climits>
 #include <cmath>
 #include <cstdio>
@@ -65,9 +66,11 @@ struct amca_domain {
   double kernel_ms[K_COUNT] = {};
   int64_t kernel_launches[K_COUNT] = {};
   std::vector<void*> allocs;
-  // kGraphSteps steps captured once and replayed by run / run_fixed when
-  // kernel timing is off; dropped whenever a launch argument changes
-  static constexpr int kGraphSteps = 16;
+  // graph_steps steps captured once and replayed by run / run_fixed when
+  // kernel timing is off; dropped whenever a launch argument changes.
+  // AMCA_DOMAIN_GRAPH_STEPS sets the count (0: no graphs, every step launched
+  // on its own, for tools that do not follow graph launches).
+  int graph_steps = 16;
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
 
@@ -190,7 +193,7 @@ amca_status capture_steps(amca_domain* d) {
   d->drop_graph();
   HIP_TRY(hipStreamBeginCapture(d->stream, hipStreamCaptureModeThreadLocal));
   amca_status st = AMCA_OK;
-  for (int q = 0; q < amca_domain::kGraphSteps && st == AMCA_OK; q++) st = enqueue_step(d);
+  for (int q = 0; q < d->graph_steps && st == AMCA_OK; q++) st = enqueue_step(d);
   hipGraph_t g = nullptr;
   const hipError_t e = hipStreamEndCapture(d->stream, &g);
   if (st != AMCA_OK || e != hipSuccess) {
@@ -202,12 +205,12 @@ amca_status capture_steps(amca_domain* d) {
   return AMCA_OK;
 }
 
-// n steps: whole graphs of kGraphSteps steps, then single launches (every
+// n steps: whole graphs of graph_steps steps, then single launches (every
 // launch is single, with its events, when kernel timing is on).  Steps past
 // termination do nothing, so a graph may run beyond the last live step.
 amca_status enqueue_steps(amca_domain* d, long long n) {
-  if (!d->timing) {
-    for (; n >= amca_domain::kGraphSteps; n -= amca_domain::kGraphSteps) {
+  if (!d->timing && d->graph_steps > 0) {
+    for (; n >= d->graph_steps; n -= d->graph_steps) {
       if (!d->gexec) AMCA_TRY(capture_steps(d));
       HIP_TRY(hipGraphLaunch(d->gexec, d->stream));
     }
@@ -337,7 +340,6 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   ALLOC(B.listW[0], N);
   ALLOC(B.listW[1], N);
   ALLOC(B.masks, N);
-  ALLOC(B.res, N);
   ALLOC(B.l2g, 9 * (size_t)std::max(orient_capacity, 1));
   ALLOC(B.lb_flag, (size_t)amca::kLbSites * amca::kMaxBlocks);
   ALLOC(B.lb_agg, (size_t)amca::kLbSites * amca::kMaxBlocks);
@@ -372,6 +374,15 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   P.ids_blocks = d->ids_blocks;
   d->cmp_blocks = (int)std::max(1LL, std::min<long long>((nB + 511) / 512,
                                                          std::min(8LL * cus, (long long)amca::kMaxBlocks)));
+  if (const char* gs = std::getenv("AMCA_DOMAIN_GRAPH_STEPS")) {
+    char* end = nullptr;
+    const long v = std::strtol(gs, &end, 10);
+    if (end == gs || *end != '\0' || v < 0 || v > 1024) {
+      delete d;
+      return fail(AMCA_ERR_INVALID, "AMCA_DOMAIN_GRAPH_STEPS must be an integer in [0, 1024]");
+    }
+    d->graph_steps = (int)v;
+  }
   HIP_TRY(hipStreamSynchronize(d->stream));
   *out = d;
   return AMCA_OK;

@@ -107,7 +107,6 @@ struct DevBuffers {
   unsigned* cmptot;       // active cells per compaction block [kMaxBlocks]
   int* listW[2];          // capture-wave lists (slots, in grain-id order) [N]
   unsigned long long* masks;  // per wave entry: liquid bits | inside bits << 32 [N]
-  unsigned* res;          // per wave entry (waves >= 1): captured bits | keep << 31 [N]
   int* site_cell[2];      // live nucleation sites, ascending cell, ping-pong
   double* site_dt[2];
   double* l2g;            // orientation table, 9 doubles per row
@@ -121,7 +120,9 @@ struct DevBuffers {
 constexpr int kBlock = 256;       // threads per block of the list kernels
 constexpr int kItems = 4;         // consecutive list entries per thread in scan passes
 constexpr int kMaxBlocks = 4096;  // blocks of a look-back kernel
-constexpr int kLbSites = 5;       // look-back launch sites per step (the last: block statistics)
+constexpr int kLbSites = 2;       // look-back arrays: [0] k_nucleate's look-back, [1] k_resolve's
+                                  // per-block statistics (lb_agg, lb_pref at kStatBase + block)
+constexpr int kStatBase = kMaxBlocks;
 constexpr int kMaxIdsBlocks = 2048;  // blocks of the id scan (a k_resolve block keeps one count each)
 
 inline FastDiv make_fastdiv(unsigned d) {

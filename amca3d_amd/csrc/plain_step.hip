@@ -641,8 +641,8 @@ __device__ __forceinline__ void resolve_body(const DevParams& P, const DevBuffer
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    B.lb_agg[4 * kMaxBlocks + blk] = (s_pc & 0xffffffff00000000ull) | (unsigned)s_surv;
-    B.lb_pref[4 * kMaxBlocks + blk] = s_pc & 0xffffffffull;
+    B.lb_agg[kStatBase + blk] = (s_pc & 0xffffffff00000000ull) | (unsigned)s_surv;
+    B.lb_pref[kStatBase + blk] = s_pc & 0xffffffffull;
   }
   for (int q = threadIdx.x; q < P.ids_blocks; q += kBlock)
     if (s_hist[q]) atomicAdd(&B.idtot[q], s_hist[q]);
@@ -710,12 +710,12 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
       const int q = base + threadIdx.x;
       unsigned long long c = 0;
       if (q < nres) {
-        sv += (unsigned)B.lb_agg[4 * kMaxBlocks + q];
-        c = B.lb_pref[4 * kMaxBlocks + q];
+        sv += (unsigned)B.lb_agg[kStatBase + q];
+        c = B.lb_pref[kStatBase + q];
       }
       unsigned long long ctot;
       const unsigned long long off = crun + block_scan_excl(c, &ctot, s_sm);
-      if (q < nres) B.lb_pref[4 * kMaxBlocks + q] = off;
+      if (q < nres) B.lb_pref[kStatBase + q] = off;
       crun += ctot;
     }
     unsigned long long svt;
@@ -821,8 +821,8 @@ __device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& 
   list_wave(B, wave, list, n);
   int lo, hi;
   chunk_of(n, blk, nblk, kBlock, lo, hi);
-  const int np = (int)(B.lb_agg[4 * kMaxBlocks + blk] >> 32);
-  const int cb = (int)B.lb_pref[4 * kMaxBlocks + blk];
+  const int np = (int)(B.lb_agg[kStatBase + blk] >> 32);
+  const int cb = (int)B.lb_pref[kStatBase + blk];
   const int4* __restrict__ seg = B.plist + lo;
   const int gbase = ctr->wave_gbase;
   int* __restrict__ kids = B.listW[wave];

@@ -229,6 +229,10 @@ def main():
                                    "fetch_scale": fscale, "write_scale": wscale}
     fm = counter_means(a.fetch, "FETCH_SIZE") if a.fetch else {}
     wm = counter_means(a.write, "WRITE_SIZE") if a.write else {}
+    rdm = {}
+    if a.rdreq:
+        parts = {sz: counter_means(a.rdreq, f"TCC_EA0_RDREQ_{sz}B_sum") for sz in (32, 64, 128)}
+        rdm = {k: sum(sz * parts[sz][k][0] for sz in parts if k in parts[sz]) for k in parts[128]}
     for k, s in sorted(st.items(), key=lambda kv: -kv[1]["total_ms"]):
         row = dict(s)
         if k in fm and k in wm and fscale:
@@ -236,6 +240,10 @@ def main():
             row["write_kib"] = wm[k][0]
             row["pmc_dispatches"] = fm[k][1]
             row["bytes_per_launch"] = 1024.0 * (fm[k][0] * fscale + wm[k][0] * wscale)
+        if k in rdm and k in wm:
+            row["read_bytes"] = rdm[k]
+            row["write_bytes"] = 1024.0 * wm[k][0]
+            row["bytes_per_launch"] = row["read_bytes"] + row["write_bytes"]
         res["kernels"][k] = row
     if a.json:
         with open(a.json, "w") as f:
@@ -246,12 +254,23 @@ def main():
                   f"dispatch; FETCH_SIZE reads {res['copy_calibration']['fetch_kib']:.1f} KiB, "
                   f"WRITE_SIZE {res['copy_calibration']['write_kib']:.1f} KiB -> scale "
                   f"{fscale:.4f} (fetch), {wscale:.4f} (write).", ""]
-    lines += ["| kernel | calls | total ms | avg us | HBM bytes/launch (scaled PMC) |",
-              "|---|---|---|---|---|"]
-    for k, r in res["kernels"].items():
-        b = r.get("bytes_per_launch")
-        lines.append(f"| {k} | {r['calls']} | {r['total_ms']:.3f} | {r['avg_us']:.2f} | "
-                     f"{'' if b is None else f'{b:.4g}'} |")
+    if rdm:
+        lines += ["Reads from the request counts (32 a + 64 b + 128 c B, TCC_EA0_RDREQ_*B_sum); "
+                  "writes from WRITE_SIZE.", "",
+                  "| kernel | calls | total ms | avg us | HBM read B/launch | HBM write B/launch |",
+                  "|---|---|---|---|---|---|"]
+        for k, r in res["kernels"].items():
+            rb, wb = r.get("read_bytes"), r.get("write_bytes")
+            lines.append(f"| {k} | {r['calls']} | {r['total_ms']:.3f} | {r['avg_us']:.2f} | "
+                         f"{'' if rb is None else f'{rb:.4g}'} | "
+                         f"{'' if wb is None else f'{wb:.4g}'} |")
+    else:
+        lines += ["| kernel | calls | total ms | avg us | HBM bytes/launch (scaled PMC) |",
+                  "|---|---|---|---|---|"]
+        for k, r in res["kernels"].items():
+            b = r.get("bytes_per_launch")
+            lines.append(f"| {k} | {r['calls']} | {r['total_ms']:.3f} | {r['avg_us']:.2f} | "
+                         f"{'' if b is None else f'{b:.4g}'} |")
     text = "\n".join(lines) + "\n"
     if a.md:
         with open(a.md, "w") as f:
