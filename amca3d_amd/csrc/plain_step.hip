@@ -807,33 +807,78 @@ __device__ __forceinline__ void emit_children(const DevParams& P, const DevBuffe
   emit_from(P, B, pr.o, slot_env(B, pr, qp), ci, cj, ck, cap, id0, kids, pos0);
 }
 
-// A list wave's children: one thread per capturing grain of this block's
-// parent-list segment (the same list chunk as its k_resolve block), its
-// children in slot order with ids wave_gbase + first child + 0, 1, ....  Wave
-// 0 lists their slots for wave 1 by parent block and position in the block
-// (parents in brick order); wave 1 lists them for the tail in id order.
+// A list wave's children, one per thread.  A capturing grain's children take
+// the ids wave_gbase + first child + 0, 1, ... in slot order.  The block's
+// parent-list segment (the same list chunk as its k_resolve block) holds each
+// parent's first child within the block, increasing along the segment; per
+// tile of 256 parents, every parent stages (parent, neighbour slot) for its
+// children in LDS at their block positions, kStage at a time, and the block
+// then builds one child per thread, so that a parent with many children does
+// not hold up the other lanes of its wave.  Wave 0 lists the children's slots
+// for wave 1 by parent block and position in the block (parents in brick
+// order); wave 1 lists them for the tail in id order.
 __device__ __forceinline__ void emit_body(const DevParams& P, const DevBuffers& B, int wave,
                                           int blk, int nblk) {
   const DevCounters* ctr = B.ctr;
   if (ctr->skip) return;
+  constexpr int kStage = 1024;
+  __shared__ int2 s_stage[kStage];
   const int* list;
   int n;
   list_wave(B, wave, list, n);
   int lo, hi;
   chunk_of(n, blk, nblk, kBlock, lo, hi);
   const int np = (int)(B.lb_agg[kStatBase + blk] >> 32);
+  if (np == 0) return;
   const int cb = (int)B.lb_pref[kStatBase + blk];
+  // children of this block: up to the next block's first child, or the wave's
+  // total for the last block
+  const int cend = (blk + 1 < nblk ? (int)B.lb_pref[kStatBase + blk + 1] : ctr->nW[wave]) - cb;
   const int4* __restrict__ seg = B.plist + lo;
   const int gbase = ctr->wave_gbase;
   int* __restrict__ kids = B.listW[wave];
-  for (int t = threadIdx.x; t < np; t += kBlock) {
-    const int4 pe = seg[t];
-    int i, j, k;
-    slot_ijk(P, pe.y, i, j, k);
-    const int p0 = B.coff[pe.x];
-    const SlotRec pr = B.rec[rec_slot(pe.y)];
-    emit_from(P, B, pr.o, slot_env(B, pr, pe.y), i, j, k, (unsigned)pe.z, gbase + p0, kids,
-              wave == 0 ? cb + pe.w : p0);
+  for (int pbase = 0; pbase < np; pbase += kBlock) {
+    const int pt = pbase + threadIdx.x;
+    int4 pe = make_int4(0, 0, 0, 0);
+    if (pt < np) pe = seg[pt];
+    const int c0 = seg[pbase].w;
+    const int c1 = pbase + kBlock < np ? seg[pbase + kBlock].w : cend;
+    for (int w0 = c0; w0 < c1; w0 += kStage) {
+      if (pt < np) {
+        unsigned cap = (unsigned)pe.z;
+        int c = pe.w;
+        while (cap) {
+          const int sl = __builtin_ctz(cap);
+          cap &= cap - 1;
+          if (c >= w0 && c < w0 + kStage) s_stage[c - w0] = make_int2(pt, sl);
+          c++;
+        }
+      }
+      __syncthreads();
+      const int m = min(kStage, c1 - w0);
+      for (int u = threadIdx.x; u < m; u += kBlock) {
+        const int2 it = s_stage[u];
+        const int4 pp = seg[it.x];
+        const int q = pp.y;
+        int ci, cj, ck;
+        slot_ijk(P, q, ci, cj, ck);
+        const SlotRec pr = B.rec[rec_slot(q)];
+        const double4 e = slot_env(B, pr, q);
+        double r[9];
+        const double* rp = B.l2g + 9 * (size_t)pr.o;
+#pragma unroll
+        for (int x = 0; x < 9; x++) r[x] = rp[x];
+        int di, dj, dk;
+        slot_delta(it.y, di, dj, dk);
+        const int c = w0 + u;         // block position of the child
+        const int rank = c - pp.w;    // its rank among its parent's children
+        const int p0 = B.coff[pp.x];  // the parent's first child in the wave
+        const int pos = wave == 0 ? cb + c : p0 + rank;
+        kids[pos] = write_child(P, B, gbase + p0 + rank, ci + di, cj + dj, ck + dk, pr.o,
+                                child_of(P, r, e, ci, cj, ck, di, dj, dk));
+      }
+      __syncthreads();
+    }
   }
 }
 
