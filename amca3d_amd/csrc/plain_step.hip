@@ -447,17 +447,19 @@ __device__ __forceinline__ IdRange id_range(const DevParams& P, const DevCounter
 
 // Envelope tests of one entry of a wave >= 1 (the grain in slot q): these are
 // the cells captured by the previous wave, whose claimed bits are cleared here
-// (not while that wave still reads the map).
+// (not while that wave still reads the map).  The clear comes after the map
+// reads: issued first, it would hold them until it completed (the compiler
+// cannot tell the words apart).
 template <bool kCoherent>
 __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
                                                           int q) {
   int ci, cj, ck;
   slot_ijk(P, q, ci, cj, ck);
-  brick_clear(P, B.bmap, 1, ci, cj, ck);
   const SlotRec r = B.rec[rec_slot(q)];
   const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
   const unsigned im =
       claim_tests(P, B, r.id, r.o, ci, cj, ck, lm, make_double4(r.x, r.y, r.z, B.sL[q]));
+  brick_clear(P, B.bmap, 1, ci, cj, ck);
   return ((unsigned long long)im << 32) | lm;
 }
 
@@ -668,6 +670,9 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
   const IdRange ir = id_range(P, ctr, wave);
   const long long lo = (long long)blk * ir.per < ir.n4 ? (long long)blk * ir.per : ir.n4;
   const long long hi = lo + ir.per < ir.n4 ? lo + ir.per : ir.n4;
+  // a block past the wave's ids has nothing to scan (the last one still sums
+  // the totals and publishes the wave)
+  if (lo >= hi && blk != nblk - 1) return;
   unsigned long long pre = 0;
   for (int q = threadIdx.x; q < blk; q += kBlock) pre += B.idtot[q];
   unsigned long long start;

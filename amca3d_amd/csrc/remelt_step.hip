@@ -61,12 +61,19 @@ __device__ __forceinline__ bool rm_ghost_ijk(const RmParams& P, int i, int j, in
 // (libm_pow.h, with exact-product shortcuts for the exponents 1, 2, 3), so the
 // velocity, and with it every growth increment and the step's dt, is the
 // reference's to the bit for any exponents.
+// a * pow(x, b) for x > 0.  For 0 <= b < 15 and x < 2^64, pow(x, b) is finite
+// and not negative, so a zero coefficient gives a itself (a zero of a's sign)
+// and the pow need not be evaluated; the reference decks set a1 or a3 to 0.
+__device__ __forceinline__ double rm_term(double a, double x, double b) {
+  if (a == 0.0 && x < 0x1p64 && b >= 0.0 && b < 15.0) return a;
+  return a * amca_libm::glibc_pow_fast(x, b);
+}
+
 __device__ __forceinline__ double rm_velocity(const RmParams& P, double Tc) {
   const double dTu = P.Tm - Tc;
   double v = 0.0;
   if (dTu > 0) {
-    v = P.a1 * amca_libm::glibc_pow_fast(dTu, P.b1) +
-        P.a2 * amca_libm::glibc_pow_fast(dTu, P.b2) + P.a3 * amca_libm::glibc_pow_fast(dTu, P.b3);
+    v = rm_term(P.a1, dTu, P.b1) + rm_term(P.a2, dTu, P.b2) + rm_term(P.a3, dTu, P.b3);
     if (v < 0.0) v = 0.0;
   }
   return v;
