@@ -861,13 +861,15 @@ amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, dou
   if (!b->exec || b->graph_steps != b->sched.steps_per_graph)
     AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
   const int n = (int)b->doms.size();
-  hipEvent_t e0, e1;
-  HIP_TRY(hipEventCreate(&e0));
-  HIP_TRY(hipEventCreate(&e1));
-  auto cleanup = [&] {
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-  };
+  struct Events {  // destroyed on every return
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ~Events() {
+      if (e0) (void)hipEventDestroy(e0);
+      if (e1) (void)hipEventDestroy(e1);
+    }
+  } ev;
+  HIP_TRY(hipEventCreate(&ev.e0));
+  HIP_TRY(hipEventCreate(&ev.e1));
   auto queue = [&](long long r) -> hipError_t {
     hipError_t e = hipGraphLaunch(b->exec, b->stream);
     if (e == hipSuccess)
@@ -876,7 +878,7 @@ amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, dou
     if (e == hipSuccess) e = hipEventRecord(b->hready[r & 1], b->stream);
     return e;
   };
-  HIP_TRY(hipEventRecord(e0, b->stream));
+  HIP_TRY(hipEventRecord(ev.e0, b->stream));
   long long r = 0, queued = 0;
   int fin = 0;
   if (max_graphs != 0) {
@@ -895,18 +897,16 @@ amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, dou
     for (int i = 0; i < n; i++) {
       if (st[i].y) {
         (void)hipStreamSynchronize(b->stream);
-        cleanup();
         return fail(AMCA_ERR_CAPACITY, "orientation table exhausted in domain " + std::to_string(i));
       }
       fin += st[i].x;
     }
     if (fin == n) break;
   }
-  HIP_TRY(hipEventRecord(e1, b->stream));
-  HIP_TRY(hipEventSynchronize(e1));
+  HIP_TRY(hipEventRecord(ev.e1, b->stream));
+  HIP_TRY(hipEventSynchronize(ev.e1));
   float t = 0.f;
-  HIP_TRY(hipEventElapsedTime(&t, e0, e1));
-  cleanup();
+  HIP_TRY(hipEventElapsedTime(&t, ev.e0, ev.e1));
   r = queued;
   if (finished) *finished = fin;
   if (ms) *ms = t;

@@ -407,6 +407,35 @@ def test_stepwise_equals_run_and_repeatable(amca, oracle):
     assert np.array_equal(a["env"], a2["env"]) and np.array_equal(a["gid"], a2["gid"])
 
 
+@pytest.mark.parametrize("graph_steps", ["0", "3"])
+def test_graph_length_changes_nothing(amca, graph_steps, monkeypatch):
+    """Whole steps replayed from HIP graphs of 16 steps (the default), of 3, or
+    launched one by one (AMCA_DOMAIN_GRAPH_STEPS=0): fields, records and the
+    step log are the same bit for bit; run_fixed past termination changes
+    nothing."""
+    text, prm = case(**SMALL[0])
+    a = run_gpu(amca, text)
+    monkeypatch.setenv("AMCA_DOMAIN_GRAPH_STEPS", graph_steps)
+    b = run_gpu(amca, text)
+    for k in ("ori", "gid", "env"):
+        assert np.array_equal(a[k], b[k]), k
+    for f in a["log"].dtype.names:
+        assert np.array_equal(a["log"][f], b["log"][f]), f
+    s = amca.Simulation(text).initialize()
+    s.domain.run_fixed(a["counters"]["step"] + 37)
+    c = s.domain.counters()
+    o = s.domain.download()
+    s.close()
+    assert c["step"] == a["counters"]["step"] and np.array_equal(o["gid"], a["gid"])
+
+
+def test_graph_length_is_checked(amca, monkeypatch):
+    monkeypatch.setenv("AMCA_DOMAIN_GRAPH_STEPS", "sixteen")
+    text, _ = case(6, 5, 4, 1e-3)
+    with pytest.raises(Exception, match="AMCA_DOMAIN_GRAPH_STEPS"):
+        amca.Simulation(text).initialize()
+
+
 def test_edge_cases(amca, oracle):
     # no sites: every step is empty until the termination time
     text, prm = case(6, 5, 4, 1e-3, termination_time=0.0105, time_step=1e-3)
