@@ -714,6 +714,10 @@ struct amca_batch {
   int graph_steps = 0;
   int cus = 256;
   double last_ms_per_step = 0.0;
+  // domains still running when the maps were last built (finished domains get
+  // no blocks) and the finished count then
+  std::vector<char> active;
+  int mapped_finished = 0;
 
   void drop_graph() {
     if (exec) (void)hipGraphExecDestroy(exec);
@@ -746,10 +750,12 @@ namespace {
 void batch_geometry(const amca_batch* b, std::vector<int>& nl, std::vector<int>& ne) {
   const size_t n = b->doms.size();
   double total = 0.0;
-  for (auto* d : b->doms) total += (double)d->P.N;
-  nl.resize(n);
-  ne.resize(n);
+  for (size_t i = 0; i < n; i++)
+    if (b->active[i]) total += (double)b->doms[i]->P.N;
+  nl.assign(n, 0);
+  ne.assign(n, 0);
   for (size_t i = 0; i < n; i++) {
+    if (!b->active[i]) continue;
     const double share = (double)b->doms[i]->P.N / total;
     const long long cap = std::max(1LL, (b->doms[i]->P.N + amca::kBlock - 1) / amca::kBlock);
     long long l = (long long)std::llround(share * b->sched.list_blocks);
@@ -787,9 +793,13 @@ amca_status batch_apply_schedule(amca_batch* b) {
   const int n = (int)b->doms.size();
   std::vector<int> nl, ne, nn(n), one(n, 1), ni(n), nc(n);
   batch_geometry(b, nl, ne);
-  for (int i = 0; i < n; i++) nn[i] = b->doms[i]->nuc_blocks;
-  for (int i = 0; i < n; i++) ni[i] = b->doms[i]->ids_blocks;
-  for (int i = 0; i < n; i++) nc[i] = b->doms[i]->cmp_blocks;
+  for (int i = 0; i < n; i++) {
+    // a finished domain gets no blocks (its kernels would return at once)
+    nn[i] = b->active[i] ? b->doms[i]->nuc_blocks : 0;
+    ni[i] = b->active[i] ? b->doms[i]->ids_blocks : 0;
+    nc[i] = b->active[i] ? b->doms[i]->cmp_blocks : 0;
+    one[i] = b->active[i] ? 1 : 0;
+  }
   std::vector<int> order(n);
   for (int i = 0; i < n; i++) order[i] = i;
   // larger domains first: their blocks start earliest
@@ -855,9 +865,12 @@ amca_status batch_capture(amca_batch* b, int steps) {
 // replay ends with a status kernel whose output is copied to a pinned slot;
 // replay r + 1 is queued before the host reads replay r's slot, so the GPU
 // does not wait for the host's check.  A replay queued after every domain
-// finished runs steps that all skip.
+// finished runs steps that all skip.  With `remap`, once max(4, n/8) more
+// domains have finished the maps are rebuilt over the domains still running,
+// so that the budgets of the list kernels go to them alone (a finished
+// domain's kernels return at once, so leaving it out changes no result).
 amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, double* ms,
-                         long long* replays) {
+                         long long* replays, bool remap) {
   if (!b->exec || b->graph_steps != b->sched.steps_per_graph)
     AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
   const int n = (int)b->doms.size();
@@ -902,6 +915,28 @@ amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, dou
       fin += st[i].x;
     }
     if (fin == n) break;
+    if (remap && fin - b->mapped_finished >= std::max(4, n / 8)) {
+      // rebuild the maps over the running domains: wait for the queued replay,
+      // take the finished set from its status, and capture a new graph
+      HIP_TRY(hipStreamSynchronize(b->stream));
+      const int4* last = b->hstatus + ((queued - 1) & 1) * n;
+      fin = 0;
+      for (int i = 0; i < n; i++) {
+        if (last[i].y)
+          return fail(AMCA_ERR_CAPACITY, "orientation table exhausted in domain " + std::to_string(i));
+        fin += last[i].x;
+        b->active[i] = last[i].x ? 0 : 1;
+      }
+      r = queued;
+      if (fin == n) break;
+      b->mapped_finished = fin;
+      AMCA_TRY(batch_apply_schedule(b));
+      AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
+      if (max_graphs < 0 || queued < max_graphs) {
+        HIP_TRY(queue(queued));
+        queued++;
+      }
+    }
   }
   HIP_TRY(hipEventRecord(ev.e1, b->stream));
   HIP_TRY(hipEventSynchronize(ev.e1));
@@ -980,6 +1015,7 @@ amca_status amca_batch_create(amca_domain* const* doms, int32_t n, amca_batch** 
     b->sched.min_blocks = 1;
     b->sched.interleave = 0;
     b->sched.steps_per_graph = 16;
+    b->active.assign(n, 1);
     st = batch_apply_schedule(b);
   } while (0);
   if (st != AMCA_OK) {
@@ -1016,7 +1052,7 @@ amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finishe
   int fin = 0;
   double t = 0.0;
   long long r = 0;
-  AMCA_TRY(batch_replay(b, max_graphs, &fin, &t, &r));
+  AMCA_TRY(batch_replay(b, max_graphs, &fin, &t, &r, true));
   if (n_finished) *n_finished = fin;
   if (ms) *ms = t;
   if (replays) *replays = r;
@@ -1057,7 +1093,7 @@ amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rou
         int fin = 0;
         double ms = 0.0;
         long long rep = 0;
-        AMCA_TRY(batch_replay(b, trial_graphs, &fin, &ms, &rep));
+        AMCA_TRY(batch_replay(b, trial_graphs, &fin, &ms, &rep, false));
         tot[c] += ms / std::max(1LL, rep);
         runs[c]++;
         if (fin == (int)b->doms.size()) all_done = true;

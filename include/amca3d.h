@@ -188,7 +188,10 @@ amca_status amca_batch_destroy(amca_batch* b);
 amca_status amca_batch_set_schedule(amca_batch* b, const amca_batch_schedule* s);
 amca_status amca_batch_get_schedule(amca_batch* b, amca_batch_schedule* s);
 /* Replays until all domains finished (max_graphs < 0) or max_graphs replays;
- * returns the finished-domain count, device milliseconds and replays. */
+ * returns the finished-domain count, device milliseconds and replays.  Each
+ * time max(4, n/8) more domains have finished, the block budgets are shared
+ * again over the running domains and the graph is captured anew (results do
+ * not change; finished domains get no blocks). */
 amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finished, double* ms,
                            int64_t* replays);
 /* Tries candidate schedules on the running batch (trial_graphs replays each,

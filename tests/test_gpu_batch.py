@@ -93,3 +93,19 @@ def test_batch_member_matches_oracle(amca, oracle, singles):
     r = ol.run_plain(prm, want_grains=False)
     np.testing.assert_array_equal(singles[0]["ori"], r["ori"])
     assert singles[0]["c"]["step"] == r["n_steps"]
+
+
+def test_batch_rebuilds_maps_as_domains_finish(amca, singles):
+    """Ten domains finishing at different steps, one step per graph replay:
+    the block maps are rebuilt over the running domains each time four more
+    have finished, and every domain still ends bit-equal to its own run."""
+    extra = [dict(edge=e, seed=s, Tdot=t, density=r) for e, s, t, r in [
+        (18, 13, -40.0, 5e15), (24, 17, -700.0, 3e15), (14, 19, -90.0, 7e15),
+        (30, 23, -2500.0, 1e15)]]
+    mine = singles + [_single(s) for s in extra]
+    r, out = _batched(SPECS + extra, dict(list_blocks=64, emit_blocks=64, min_blocks=1,
+                                          interleave=1, steps_per_graph=1))
+    assert r["finished"] == len(mine)
+    for s, b in zip(mine, out):
+        assert s["c"]["done"] == 1
+        _same(s, b)
