@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "../../include/amca3d.h"
+#include "env.h"
 #include "plain_step.h"
 
 namespace {
@@ -374,14 +375,9 @@ amca_status amca_domain_create(const amca_domain_desc* desc, int32_t orient_capa
   P.ids_blocks = d->ids_blocks;
   d->cmp_blocks = (int)std::max(1LL, std::min<long long>((nB + 511) / 512,
                                                          std::min(8LL * cus, (long long)amca::kMaxBlocks)));
-  if (const char* gs = std::getenv("AMCA_DOMAIN_GRAPH_STEPS")) {
-    char* end = nullptr;
-    const long v = std::strtol(gs, &end, 10);
-    if (end == gs || *end != '\0' || v < 0 || v > 1024) {
-      delete d;
-      return fail(AMCA_ERR_INVALID, "AMCA_DOMAIN_GRAPH_STEPS must be an integer in [0, 1024]");
-    }
-    d->graph_steps = (int)v;
+  if (amca::env_int("AMCA_DOMAIN_GRAPH_STEPS", 0, 1024, &d->graph_steps) < 0) {
+    delete d;
+    return fail(AMCA_ERR_INVALID, "AMCA_DOMAIN_GRAPH_STEPS must be an integer in [0, 1024]");
   }
   HIP_TRY(hipStreamSynchronize(d->stream));
   *out = d;
@@ -712,6 +708,7 @@ struct amca_batch {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   int graph_steps = 0;
+  bool use_graph = true;  // false: every replay launches its kernels one by one
   int cus = 256;
   double last_ms_per_step = 0.0;
   // domains still running when the maps were last built (finished domains get
@@ -871,7 +868,7 @@ amca_status batch_capture(amca_batch* b, int steps) {
 // domain's kernels return at once, so leaving it out changes no result).
 amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, double* ms,
                          long long* replays, bool remap) {
-  if (!b->exec || b->graph_steps != b->sched.steps_per_graph)
+  if (b->use_graph && (!b->exec || b->graph_steps != b->sched.steps_per_graph))
     AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
   const int n = (int)b->doms.size();
   struct Events {  // destroyed on every return
@@ -884,7 +881,14 @@ amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, dou
   HIP_TRY(hipEventCreate(&ev.e0));
   HIP_TRY(hipEventCreate(&ev.e1));
   auto queue = [&](long long r) -> hipError_t {
-    hipError_t e = hipGraphLaunch(b->exec, b->stream);
+    hipError_t e = hipSuccess;
+    if (b->use_graph) {
+      e = hipGraphLaunch(b->exec, b->stream);
+    } else {
+      for (int q = 0; q < b->sched.steps_per_graph && e == hipSuccess; q++)
+        e = amca::launch_batch_step(b->L, b->stream);
+      if (e == hipSuccess) e = amca::launch_batch_status(b->L, n, b->status, b->stream);
+    }
     if (e == hipSuccess)
       e = hipMemcpyAsync(b->hstatus + (r & 1) * n, b->status, sizeof(int4) * n,
                          hipMemcpyDeviceToHost, b->stream);
@@ -931,7 +935,7 @@ amca_status batch_replay(amca_batch* b, long long max_graphs, int* finished, dou
       if (fin == n) break;
       b->mapped_finished = fin;
       AMCA_TRY(batch_apply_schedule(b));
-      AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
+      if (b->use_graph) AMCA_TRY(batch_capture(b, b->sched.steps_per_graph));
       if (max_graphs < 0 || queued < max_graphs) {
         HIP_TRY(queue(queued));
         queued++;
@@ -1016,6 +1020,12 @@ amca_status amca_batch_create(amca_domain* const* doms, int32_t n, amca_batch** 
     b->sched.interleave = 0;
     b->sched.steps_per_graph = 16;
     b->active.assign(n, 1);
+    int g = 1;
+    if (amca::env_int("AMCA_BATCH_GRAPHS", 0, 1, &g) < 0) {
+      st = fail(AMCA_ERR_INVALID, "AMCA_BATCH_GRAPHS must be 0 or 1");
+      break;
+    }
+    b->use_graph = g != 0;
     st = batch_apply_schedule(b);
   } while (0);
   if (st != AMCA_OK) {

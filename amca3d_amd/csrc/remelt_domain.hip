@@ -23,6 +23,7 @@
 #include "host/generators.h"
 #include "host/sites.h"
 #include "host/voxel_map.h"
+#include "env.h"
 #include "libm_pow.h"
 #include "plain_step.h"
 #include "remelt_step.h"
@@ -326,6 +327,10 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
     return rm_fail(AMCA_ERR_NO_DEVICE, "no HIP device visible");
   auto* d = new amca_remelt();
   d->desc = *desc;
+  if (amca::env_int("AMCA_REMELT_GRAPH_STEPS", 0, 1024, &d->graph_steps) < 0) {
+    delete d;
+    return rm_fail(AMCA_ERR_INVALID, "AMCA_REMELT_GRAPH_STEPS must be an integer in [0, 1024]");
+  }
   if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
     delete d;
     return rm_fail(AMCA_ERR_HIP, "stream creation failed");
@@ -412,9 +417,18 @@ amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capa
     c.box[q].hi[2] = P.nzG - 1;
   }
   rm_restart_box(c);
-  if (hipMemcpyAsync(B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream) != hipSuccess ||
-      amca::rm_launch_init(P, B, d->stream) != hipSuccess ||
-      hipStreamSynchronize(d->stream) != hipSuccess) {
+  // the velocity at Ts + 0.01 K, through the step's own kinetics code
+  P.vts = 0.0;
+  double* vts_dev = nullptr;
+  const bool ok = hipMalloc(&vts_dev, sizeof(double)) == hipSuccess &&
+                  hipMemcpyAsync(B.ctr, &c, sizeof(c), hipMemcpyHostToDevice, d->stream) == hipSuccess &&
+                  amca::rm_launch_init(P, B, d->stream) == hipSuccess &&
+                  amca::rm_launch_vts(P, vts_dev, d->stream) == hipSuccess &&
+                  hipMemcpyAsync(&P.vts, vts_dev, sizeof(double), hipMemcpyDeviceToHost,
+                                 d->stream) == hipSuccess &&
+                  hipStreamSynchronize(d->stream) == hipSuccess;
+  if (vts_dev) (void)hipFree(vts_dev);
+  if (!ok) {
     delete d;
     return rm_fail(AMCA_ERR_HIP, "initialisation failed");
   }

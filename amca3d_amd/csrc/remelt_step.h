@@ -15,6 +15,7 @@ struct RmParams {
   double Tm, Ts, Tmin_nuc, a1, a2, a3;   // kinetics: v = a1 dT^b1 + a2 dT^b2 + a3 dT^b3
   double b1, b2, b3;
   double trunc, tsf, sqrt2f, cutoff, final_time, void_temp;
+  double vts;  // growth velocity at Ts + 0.01 K (cells below Ts grow with it), set by rm_launch_vts
   int off[26];
   int fnx, fny, fnz, nodes, ncx, ncy;    // structured FE mesh
   int n_frames;
@@ -131,6 +132,8 @@ struct RmBuffers {
 };
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s);
+// writes the growth velocity at Ts + 0.01 K to *out (device memory)
+hipError_t rm_launch_vts(const RmParams& P, double* out, hipStream_t s);
 // One step: kernels 0..kRmKernels-1 in order on stream s.
 constexpr int kRmKernels = 10;
 extern const char* const kRmKernelNames[kRmKernels];

@@ -382,6 +382,14 @@ __global__ void k_rm_init(RmParams P, RmBuffers B) {
   }
 }
 
+// The velocity every cell below Ts grows with (the reference evaluates it at
+// Ts + 0.01 K per cell, CellularAutomataManager_3DRemelting.cpp:572, with the
+// kinetics of ProblemPhysics.cpp:142-158); a constant of the domain, evaluated
+// once on the device with the same kinetics code as every other velocity.
+__global__ void k_rm_vts(RmParams P, double* out) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) *out = rm_velocity(P, P.Ts + 0.01);
+}
+
 // Start of a step: the sites whose cell is liquid (the most nuclei the step
 // can make), the hot region of the active box into acc[step & 1][0], and with
 // the surrogate the step's nodal field into B.theta for the fallback test.
@@ -948,15 +956,12 @@ __global__ void __launch_bounds__(kRmBlock) k_rm_claim(RmParams P, RmBuffers B) 
   if (ctr->skip) return;
   const RmRows w = rm_rows(P, ctr);
   const int total = w.ex * w.ey * w.ez;
-  // dt and the velocity at Ts + 0.01 K are the same for every thread: one
-  // thread per block evaluates them (six pow calls) for the rest
-  __shared__ double s_dt, s_vts;
-  if (threadIdx.x == 0) {
-    s_dt = rm_step_dt(P, ctr);
-    s_vts = rm_velocity(P, P.Ts + 0.01);
-  }
+  // dt is the same for every thread: one thread per block evaluates it (three
+  // pow calls) for the rest
+  __shared__ double s_dt;
+  if (threadIdx.x == 0) s_dt = rm_step_dt(P, ctr);
   __syncthreads();
-  const double dt = s_dt, vts = s_vts;
+  const double dt = s_dt, vts = P.vts;
   if (blockIdx.x == 0 && threadIdx.x == 0) {
     // read by later kernels only
     B.ctr->dt = dt;
@@ -1253,6 +1258,11 @@ __global__ void k_pow_selftest(long long n, const double* x, double y, double* o
 }
 
 // ---------------------------------------------------------------------------
+
+hipError_t rm_launch_vts(const RmParams& P, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_rm_vts, dim3(1), dim3(64), 0, s, P, out);
+  return hipGetLastError();
+}
 
 hipError_t rm_launch_init(const RmParams& P, const RmBuffers& B, hipStream_t s) {
   long long blocks = (P.NG + 255) / 256;

@@ -183,6 +183,8 @@ typedef struct {
   int32_t steps_per_graph;  /* steps captured in one graph replay */
 } amca_batch_schedule;
 
+/* AMCA_BATCH_GRAPHS=0 in the environment: replays launch their kernels one by
+ * one instead of from a captured graph (any other value than 0 or 1 fails). */
 amca_status amca_batch_create(amca_domain* const* domains, int32_t n, amca_batch** out);
 amca_status amca_batch_destroy(amca_batch* b);
 amca_status amca_batch_set_schedule(amca_batch* b, const amca_batch_schedule* s);
@@ -355,7 +357,9 @@ typedef struct {
  * rotation) continuing the host's rand() stream; returns 0 on success. */
 typedef int (*amca_orientation_source)(void* user, int32_t n, double* euler3, double* l2g9);
 
-/* CellularAutomataManager_3DRemelting::initialize (:81-135) storage. */
+/* CellularAutomataManager_3DRemelting::initialize (:81-135) storage.
+ * AMCA_REMELT_GRAPH_STEPS in the environment (0..1024) sets the graph length,
+ * as amca_remelt_set_graph_steps does. */
 amca_status amca_remelt_create(const amca_remelt_desc* desc, int32_t orient_capacity,
                                amca_remelt** out);
 amca_status amca_remelt_destroy(amca_remelt* d);

@@ -77,6 +77,27 @@ def test_batch_equals_single_runs(amca, singles, schedule):
         _same(s, b)
 
 
+def test_batch_without_graphs(amca, singles, monkeypatch):
+    """Replays launched kernel by kernel (AMCA_BATCH_GRAPHS=0) give the same
+    runs, maps rebuilt as domains finish included."""
+    monkeypatch.setenv("AMCA_BATCH_GRAPHS", "0")
+    r, out = _batched(SPECS, dict(list_blocks=64, emit_blocks=64, min_blocks=1, interleave=0,
+                                  steps_per_graph=2))
+    assert r["finished"] == len(SPECS)
+    for s, b in zip(singles, out):
+        _same(s, b)
+
+
+def test_batch_graph_setting_is_checked(amca, monkeypatch):
+    from amca3d_amd.batch import Batch, simulations
+    sims = simulations(SPECS[:1], termination_time=1.0)
+    monkeypatch.setenv("AMCA_BATCH_GRAPHS", "2")
+    with pytest.raises(Exception, match="AMCA_BATCH_GRAPHS"):
+        Batch([s.domain for s in sims])
+    for s in sims:
+        s.close()
+
+
 def test_batch_after_autotune(amca, singles):
     r, out = _batched(SPECS, autotune=True)
     assert r["finished"] == len(SPECS)

@@ -209,6 +209,22 @@ def test_run_many_equals_the_oracle(amca):
         np.testing.assert_array_equal(bits(dev["euler"]), bits(orc["euler"]))
 
 
+@pytest.mark.parametrize("graph_steps", ["0", "5"])
+def test_graph_length_changes_nothing(amca, track_nuc, graph_steps, monkeypatch):
+    """Steps launched one by one (AMCA_REMELT_GRAPH_STEPS=0) or replayed from
+    5-step graphs give the oracle's run."""
+    monkeypatch.setenv("AMCA_REMELT_GRAPH_STEPS", graph_steps)
+    case, orc = track_nuc
+    compare(case, orc, run_device(case, orc))
+
+
+def test_graph_length_is_checked(amca, track_nuc, monkeypatch):
+    monkeypatch.setenv("AMCA_REMELT_GRAPH_STEPS", "-1")
+    case, orc = track_nuc
+    with pytest.raises(Exception, match="AMCA_REMELT_GRAPH_STEPS"):
+        run_device(case, orc)
+
+
 def test_track_in_chunks_equals_one_run(amca, track_nuc):
     case, orc = track_nuc
     dev = run_device(case, orc, chunks=[1, 2, 5, 13, 40, 1000])
