@@ -758,7 +758,8 @@ void batch_geometry(const amca_batch* b, std::vector<int>& nl, std::vector<int>&
     long long l = (long long)std::llround(share * b->sched.list_blocks);
     long long e = (long long)std::llround(share * b->sched.emit_blocks);
     l = std::max<long long>(b->sched.min_blocks, std::min(l, cap));
-    e = std::max<long long>(b->sched.min_blocks, std::min(std::min(e, cap), (long long)amca::kMaxBlocks));
+    // resolve / emit blocks keep per-block stats in kMaxBlocks slots
+    e = std::min(std::max<long long>(b->sched.min_blocks, std::min(e, cap)), (long long)amca::kMaxBlocks);
     nl[i] = (int)std::max(1LL, l);
     ne[i] = (int)std::max(1LL, e);
   }

@@ -684,10 +684,14 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
   };
   unsigned long long run = start;
   if (mine) {
+    // each tile's counts are loaded while the tile before is scanned
+    uint4 vn = make_uint4(0u, 0u, 0u, 0u);
+    if (lo + threadIdx.x < hi) vn = c4[lo + threadIdx.x];
     for (long long base = lo; base < hi; base += kBlock) {
       const long long u = base + threadIdx.x;
-      uint4 v = make_uint4(0u, 0u, 0u, 0u);
-      if (u < hi) v = c4[u];
+      const uint4 v = vn;
+      vn = make_uint4(0u, 0u, 0u, 0u);
+      if (u + kBlock < hi) vn = c4[u + kBlock];
       const bool any = (v.x | v.y | v.z | v.w) != 0u;
       const unsigned sum = any ? bytesum(v.x) + bytesum(v.y) + bytesum(v.z) + bytesum(v.w) : 0u;
       unsigned long long tot;
@@ -709,22 +713,33 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
     }
   }
   if (blk == nblk - 1) {
-    // survivors of the wave and each k_resolve block's first child
-    unsigned long long sv = 0, crun = 0;
-    for (int base = 0; base < nres; base += kBlock) {
-      const int q = base + threadIdx.x;
-      unsigned long long c = 0;
-      if (q < nres) {
-        sv += (unsigned)B.lb_agg[kStatBase + q];
-        c = B.lb_pref[kStatBase + q];
+    // survivors of the wave and each k_resolve block's first child: thread t
+    // takes k_resolve blocks [t per, (t + 1) per) (nres <= kMaxBlocks), loads
+    // them all, and one scan of (survivors << 32 | children) numbers them
+    // (both totals are below 2^32: at most one per cell)
+    constexpr int kPer = kMaxBlocks / kBlock;
+    const int per = (nres + kBlock - 1) / kBlock;
+    const int q0 = threadIdx.x * per;
+    unsigned long long c[kPer];
+    unsigned long long both = 0;
+#pragma unroll
+    for (int x = 0; x < kPer; x++) {
+      c[x] = 0;
+      if (x < per && q0 + x < nres) {
+        c[x] = B.lb_pref[kStatBase + q0 + x];
+        both += ((unsigned long long)(unsigned)B.lb_agg[kStatBase + q0 + x] << 32) + c[x];
       }
-      unsigned long long ctot;
-      const unsigned long long off = crun + block_scan_excl(c, &ctot, s_sm);
-      if (q < nres) B.lb_pref[kStatBase + q] = off;
-      crun += ctot;
     }
-    unsigned long long svt;
-    block_scan_excl(sv, &svt, s_sm);
+    unsigned long long tot;
+    unsigned long long off = block_scan_excl(both, &tot, s_sm) & 0xffffffffull;
+#pragma unroll
+    for (int x = 0; x < kPer; x++) {
+      if (x < per && q0 + x < nres) {
+        B.lb_pref[kStatBase + q0 + x] = off;
+        off += c[x];
+      }
+    }
+    const unsigned long long svt = tot >> 32;
     if (threadIdx.x == 0) {
       const unsigned long long nc = start + mine;
       const int gbase = ctr->n_grains;
