@@ -75,7 +75,7 @@ class Batch:
                                     C.byref(rep)), "batch_run")
         return {"finished": int(fin.value), "ms": float(ms.value), "replays": int(rep.value)}
 
-    def autotune(self, trial_graphs=2, rounds=1):
+    def autotune(self, trial_graphs=1, rounds=2):
         s = Schedule()
         _check(lib().amca_batch_autotune(self.h, int(trial_graphs), int(rounds), C.byref(s)),
                "autotune")

@@ -1071,15 +1071,21 @@ amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finishe
 }
 
 // The autotuner runs the real batch: each candidate schedule takes
-// `trial_graphs` graph replays in turn, for `rounds` rounds, and the schedule
-// with the least mean time per replay wins.  Stage 1 tries one block budget
-// for all list kernels (4, 16, 64 blocks per CU) in both block orders; stage 2
-// sizes the claim grid and the resolve / emit grid apart, a factor 4 up and
-// down from the stage-1 choice.  Schedules do not change results, so the
-// trial steps are ordinary steps of the run.
+// `trial_graphs` replays of a short graph (kTrialSteps steps, or the
+// schedule's own length if shorter) in turn, for `rounds` rounds, and the
+// schedule with the least mean time per step wins.  The batch's work per step
+// drifts while the trials run, so odd rounds visit the candidates in reverse
+// order, which cancels a steady drift over each pair of rounds.  Stage 1 tries
+// one block budget for all list kernels (4, 16, 64 blocks per CU) in both
+// block orders; stage 2 sizes the claim grid and the resolve / emit grid
+// apart, a factor 4 up and down from the stage-1 choice.  Schedules do not
+// change results, so the trial steps are ordinary steps of the run.
 amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rounds,
                                 amca_batch_schedule* chosen) {
   if (!b || trial_graphs < 1 || rounds < 1) return fail(AMCA_ERR_INVALID, "bad arguments");
+  constexpr int kTrialSteps = 4;
+  const int run_steps = b->sched.steps_per_graph;
+  const int trial_steps = std::min(run_steps, kTrialSteps);
   const int emit_cap = (int)amca::kMaxBlocks * 8;
   auto make = [&](int list_budget, int emit_budget, int inter) {
     amca_batch_schedule s = b->sched;
@@ -1090,22 +1096,24 @@ amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rou
     return s;
   };
   bool all_done = false;
-  // mean ms per replay of each candidate over `rounds` passes; the best index
+  // mean ms per step of each candidate over `rounds` passes; the best index
   auto pick = [&](const std::vector<amca_batch_schedule>& cand, size_t* best) -> amca_status {
     std::vector<double> tot(cand.size(), 0.0);
     std::vector<int> runs(cand.size(), 0);
     for (int r = 0; r < rounds && !all_done; r++)
-      for (size_t c = 0; c < cand.size() && !all_done; c++) {
+      for (size_t u = 0; u < cand.size() && !all_done; u++) {
+        const size_t c = (r & 1) ? cand.size() - 1 - u : u;
         b->sched.list_blocks = cand[c].list_blocks;
         b->sched.emit_blocks = cand[c].emit_blocks;
         b->sched.min_blocks = cand[c].min_blocks;
         b->sched.interleave = cand[c].interleave;
+        b->sched.steps_per_graph = trial_steps;
         AMCA_TRY(batch_apply_schedule(b));
         int fin = 0;
         double ms = 0.0;
         long long rep = 0;
         AMCA_TRY(batch_replay(b, trial_graphs, &fin, &ms, &rep, false));
-        tot[c] += ms / std::max(1LL, rep);
+        tot[c] += ms / (double)(std::max(1LL, rep) * trial_steps);
         runs[c]++;
         if (fin == (int)b->doms.size()) all_done = true;
       }
@@ -1140,6 +1148,7 @@ amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rou
   b->sched.emit_blocks = win.emit_blocks;
   b->sched.min_blocks = win.min_blocks;
   b->sched.interleave = win.interleave;
+  b->sched.steps_per_graph = run_steps;
   AMCA_TRY(batch_apply_schedule(b));
   if (chosen) *chosen = b->sched;
   return AMCA_OK;

@@ -322,7 +322,7 @@ def batch_main(args):
     for s in sims:
         s.domain.counters()
     t0 = time.perf_counter()
-    tuned = b.autotune(trial_graphs=2, rounds=1) if not args.no_autotune else b.schedule()
+    tuned = b.autotune(trial_graphs=1, rounds=2) if not args.no_autotune else b.schedule()
     r = b.run()
     wall = time.perf_counter() - t0
     b.close()

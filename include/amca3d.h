@@ -196,11 +196,13 @@ amca_status amca_batch_get_schedule(amca_batch* b, amca_batch_schedule* s);
  * not change; finished domains get no blocks). */
 amca_status amca_batch_run(amca_batch* b, int64_t max_graphs, int32_t* n_finished, double* ms,
                            int64_t* replays);
-/* Tries candidate schedules on the running batch (trial_graphs replays each,
- * `rounds` rounds) and keeps the fastest: first one block budget for every
- * list kernel (4, 16 or 64 blocks per CU) in both block orders, then the claim
- * grid and the resolve / emit grid sized apart around that choice.  The trial
- * steps are ordinary steps of the run. */
+/* Tries candidate schedules on the running batch (trial_graphs replays each of
+ * a graph of min(4, steps_per_graph) steps, `rounds` rounds, odd rounds in
+ * reverse candidate order) and keeps the fastest per step: first one block
+ * budget for every list kernel (4, 16 or 64 blocks per CU) in both block
+ * orders, then the claim grid and the resolve / emit grid sized apart around
+ * that choice.  The trial steps are ordinary steps of the run; the schedule's
+ * steps_per_graph is kept. */
 amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rounds,
                                 amca_batch_schedule* chosen);
 
