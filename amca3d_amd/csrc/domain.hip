@@ -1086,6 +1086,19 @@ amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rou
   constexpr int kTrialSteps = 4;
   const int run_steps = b->sched.steps_per_graph;
   const int trial_steps = std::min(run_steps, kTrialSteps);
+  // the schedule as it was, restored if a trial fails
+  const amca_batch_schedule before = b->sched;
+  struct Restore {
+    amca_batch* b;
+    const amca_batch_schedule* s;
+    bool armed = true;
+    ~Restore() {
+      if (!armed) return;
+      (void)hipStreamSynchronize(b->stream);
+      b->sched = *s;
+      (void)batch_apply_schedule(b);
+    }
+  } restore{b, &before};
   const int emit_cap = (int)amca::kMaxBlocks * 8;
   auto make = [&](int list_budget, int emit_budget, int inter) {
     amca_batch_schedule s = b->sched;
@@ -1149,6 +1162,7 @@ amca_status amca_batch_autotune(amca_batch* b, int32_t trial_graphs, int32_t rou
   b->sched.min_blocks = win.min_blocks;
   b->sched.interleave = win.interleave;
   b->sched.steps_per_graph = run_steps;
+  restore.armed = false;
   AMCA_TRY(batch_apply_schedule(b));
   if (chosen) *chosen = b->sched;
   return AMCA_OK;

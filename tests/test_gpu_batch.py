@@ -116,17 +116,48 @@ def test_batch_member_matches_oracle(amca, oracle, singles):
     assert singles[0]["c"]["step"] == r["n_steps"]
 
 
-def test_batch_rebuilds_maps_as_domains_finish(amca, singles):
+EXTRA = [dict(edge=e, seed=s, Tdot=t, density=r) for e, s, t, r in [
+    (18, 13, -40.0, 5e15), (24, 17, -700.0, 3e15), (14, 19, -90.0, 7e15),
+    (30, 23, -2500.0, 1e15)]]
+
+
+@pytest.fixture(scope="module")
+def singles10(singles):
+    return singles + [_single(s) for s in EXTRA]
+
+
+def test_batch_rebuilds_maps_as_domains_finish(amca, singles10):
     """Ten domains finishing at different steps, one step per graph replay:
     the block maps are rebuilt over the running domains each time four more
     have finished, and every domain still ends bit-equal to its own run."""
-    extra = [dict(edge=e, seed=s, Tdot=t, density=r) for e, s, t, r in [
-        (18, 13, -40.0, 5e15), (24, 17, -700.0, 3e15), (14, 19, -90.0, 7e15),
-        (30, 23, -2500.0, 1e15)]]
-    mine = singles + [_single(s) for s in extra]
-    r, out = _batched(SPECS + extra, dict(list_blocks=64, emit_blocks=64, min_blocks=1,
+    r, out = _batched(SPECS + EXTRA, dict(list_blocks=64, emit_blocks=64, min_blocks=1,
                                           interleave=1, steps_per_graph=1))
-    assert r["finished"] == len(mine)
-    for s, b in zip(mine, out):
+    assert r["finished"] == len(singles10)
+    for s, b in zip(singles10, out):
         assert s["c"]["done"] == 1
         _same(s, b)
+
+
+def test_batch_run_in_chunks(amca, singles10):
+    """The same ten domains run one replay per call (maps rebuilt between and
+    within calls) end as their own runs do."""
+    from amca3d_amd.batch import Batch, simulations
+    sims = simulations(SPECS + EXTRA, termination_time=1.0)
+    for s in sims:
+        s.domain.enable_log(100000)
+    b = Batch([s.domain for s in sims])
+    b.set_schedule(list_blocks=32, emit_blocks=32, min_blocks=1, interleave=0, steps_per_graph=3)
+    calls, fin = 0, 0
+    while fin < len(sims):
+        r = b.run(max_graphs=1)
+        assert r["replays"] == 1
+        fin = r["finished"]
+        calls += 1
+        assert calls < 10000
+    b.close()
+    for s, single in zip(sims, singles10):
+        d = s.domain
+        o = d.download()
+        _same(single, dict(c=d.counters(), ori=o["ori"], gid=o["gid"], env=o["env"],
+                           log=d.read_log()))
+        s.close()
