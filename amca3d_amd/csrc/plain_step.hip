@@ -346,28 +346,6 @@ __device__ __forceinline__ void cmp_range(const DevParams& P, int blk, int nblk,
   hi = lo + per < nB ? lo + per : nB;
 }
 
-// Position of the r-th set bit (from 0) of w, r < popcount(w).
-__device__ __forceinline__ int nth_bit(unsigned long long w, int r) {
-  int pos = 0;
-  unsigned c = (unsigned)__popc((unsigned)w);
-  if ((unsigned)r >= c) {
-    r -= (int)c;
-    w >>= 32;
-    pos = 32;
-  }
-  unsigned x = (unsigned)w;
-#pragma unroll
-  for (int half = 16; half >= 1; half >>= 1) {
-    c = (unsigned)__popc(x & ((1u << half) - 1u));
-    if ((unsigned)r >= c) {
-      r -= (int)c;
-      x >>= half;
-      pos += half;
-    }
-  }
-  return pos;
-}
-
 __device__ __forceinline__ void count_body(const DevParams& P, const DevBuffers& B, int blk,
                                            int nblk) {
   if (B.ctr->skip) return;
@@ -392,27 +370,58 @@ __device__ __forceinline__ void compact_body(const DevParams& P, const DevBuffer
   unsigned long long run;
   block_scan_excl(pre, &run, s_sm);
   if (B.cmptot[blk] == 0u) return;
-  // Per tile of 256 bricks the slots are written one per thread at
-  // consecutive positions: position p of the tile belongs to the last brick
-  // whose tile offset is <= p, as its (p - offset)-th active lane.
-  __shared__ unsigned s_off[kBlock];
-  __shared__ unsigned long long s_w[kBlock];
-  for (long long base = lo; base < hi; base += kBlock) {
+  // Per tile of 256 bricks, one brick per thread: each thread puts its brick's
+  // active slots, lowest first, at the brick's position in its wave's stretch
+  // of the output (a wave scan of the active counts; the totals of the waves
+  // before come from LDS, double-buffered so that one barrier per tile
+  // suffices).  A wave stages its stretch in LDS and writes it out coalesced;
+  // a stretch too long for the stage is written directly.
+  constexpr int kWaveStage = 2048;
+  __shared__ int s_stage[kBlock / 64][kWaveStage];
+  __shared__ unsigned s_wt[2][kBlock / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  int par = 0;
+  for (long long base = lo; base < hi; base += kBlock, par ^= 1) {
     const long long b = base + threadIdx.x;
-    const unsigned long long w = b < hi ? B.amap[b] : 0ull;
-    unsigned long long tot;
-    const unsigned long long off = block_scan_excl(__popcll(w), &tot, s_sm);
-    s_off[threadIdx.x] = (unsigned)off;
-    s_w[threadIdx.x] = w;
-    __syncthreads();
-    for (unsigned p = threadIdx.x; p < (unsigned)tot; p += kBlock) {
-      int t = 0;
+    unsigned long long w = b < hi ? B.amap[b] : 0ull;
+    const unsigned pop = (unsigned)__popcll(w);
+    unsigned inc = pop;
 #pragma unroll
-      for (int step = kBlock / 2; step >= 1; step >>= 1)
-        if (s_off[t + step] <= p) t += step;
-      B.alist[run + p] = (int)(64 * (base + t) + nth_bit(s_w[t], (int)(p - s_off[t])));
+    for (int d = 1; d < 64; d <<= 1) {
+      const unsigned u = __shfl_up(inc, d, 64);
+      if (lane >= d) inc += u;
     }
+    if (lane == 63) s_wt[par][wv] = inc;
     __syncthreads();
+    unsigned wbase = 0, tot = 0;
+#pragma unroll
+    for (int q = 0; q < kBlock / 64; q++) {
+      const unsigned t = s_wt[par][q];
+      if (q < wv) wbase += t;
+      tot += t;
+    }
+    const unsigned wtot = s_wt[par][wv];
+    int* __restrict__ gout = B.alist + run + wbase;
+    const int q0 = (int)(64 * b);
+    if (wtot <= (unsigned)kWaveStage) {
+      int* st = s_stage[wv];
+      unsigned pos = inc - pop;
+      while (w) {
+        st[pos++] = q0 + __builtin_ctzll(w);
+        w &= w - 1;
+      }
+      // the wave's LDS operations complete in order: its reads below see
+      // every lane's writes above
+      __builtin_amdgcn_wave_barrier();
+      for (unsigned i = lane; i < wtot; i += 64) gout[i] = st[i];
+      __builtin_amdgcn_wave_barrier();
+    } else {
+      int* out = gout + (inc - pop);
+      while (w) {
+        *out++ = q0 + __builtin_ctzll(w);
+        w &= w - 1;
+      }
+    }
     run += tot;
   }
 }
