@@ -454,11 +454,10 @@ __device__ __forceinline__ IdRange id_range(const DevParams& P, const DevCounter
   return r;
 }
 
-// Envelope tests of one entry of a wave >= 1 (the grain in slot q): these are
-// the cells captured by the previous wave, whose claimed bits are cleared here
-// (not while that wave still reads the map).  The clear comes after the map
-// reads: issued first, it would hold them until it completed (the compiler
-// cannot tell the words apart).
+// Envelope tests of one entry of a wave >= 1 (the grain in slot q).  Its own
+// cell's claimed bit (set by the wave that captured it) stays set: the
+// claimed word is read only where the liquid word has the cell's bit, and a
+// captured cell is never liquid again.
 template <bool kCoherent>
 __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
                                                           int q) {
@@ -468,7 +467,6 @@ __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, co
   const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
   const unsigned im =
       claim_tests(P, B, r.id, r.o, ci, cj, ck, lm, make_double4(r.x, r.y, r.z, B.sL[q]));
-  brick_clear(P, B.bmap, 1, ci, cj, ck);
   return ((unsigned long long)im << 32) | lm;
 }
 
