@@ -70,6 +70,44 @@ __device__ __forceinline__ void brick_set(const DevParams& P, unsigned long long
   atomicOr(&bm[brick_word(P, i, j, k) + word], brick_bit(i, j, k));
 }
 
+// ---- per-work-group staging of map-word updates ----------------------------
+// Bits a list kernel sets in 64-bit map words are gathered per work-group in
+// an LDS table keyed by word index and applied with one memory atomic per
+// distinct word when the work-group is done (every device-scope atomic runs at
+// the memory side, so this saves most of them: neighbouring entries set bits
+// in the same words).  Only later kernels read those words.  A word that finds
+// no free entry within kStageProbe probes goes straight to memory.
+constexpr int kStageSlots = 512;  // a power of two
+constexpr int kStageProbe = 8;
+struct WordStage {
+  int key[kStageSlots];
+  unsigned long long bits[kStageSlots];
+};
+__device__ __forceinline__ void stage_init(WordStage& s) {
+  for (int i = threadIdx.x; i < kStageSlots; i += blockDim.x) {
+    s.key[i] = -1;
+    s.bits[i] = 0ull;
+  }
+}
+// false: no entry for `word` (the caller applies the bits itself)
+__device__ __forceinline__ bool stage_add(WordStage& s, int word, unsigned long long b) {
+  unsigned h = ((unsigned)word * 2654435761u) >> 23;  // 9 bits: kStageSlots
+  for (int p = 0; p < kStageProbe; p++) {
+    const int k = atomicCAS(&s.key[h], -1, word);
+    if (k == -1 || k == word) {
+      atomicOr(&s.bits[h], b);
+      return true;
+    }
+    h = (h + 1) & (kStageSlots - 1);
+  }
+  return false;
+}
+// after a barrier: one atomic OR per used entry
+__device__ __forceinline__ void stage_flush_or(const WordStage& s, unsigned long long* words) {
+  for (int i = threadIdx.x; i < kStageSlots; i += blockDim.x)
+    if (s.key[i] >= 0) atomicOr(&words[s.key[i]], s.bits[i]);
+}
+
 // The 26 neighbour bits of interior cell (i, j, k) from word `kWord` of the
 // brick map, in slot order (k, j, i, centre skipped).  The 3x3x3 block spans
 // bricks bx0..bx1 (bx1 = bx0 or bx0 + 1) per axis: the 8 corner words are

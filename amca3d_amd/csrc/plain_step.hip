@@ -273,7 +273,7 @@ __device__ __forceinline__ void list_wave(const DevBuffers& B, int wave, const i
 // (its leading 0.0 + p is p up to the sign of a zero, which fabs drops).
 __device__ __forceinline__ unsigned claim_tests(const DevParams& P, const DevBuffers& B, int g,
                                                 int o, int ci, int cj, int ck, unsigned lm,
-                                                const double4& e) {
+                                                const double4& e, WordStage* st) {
   if (lm == 0 || e.w == 0) return 0u;
   double r[9];
   const double* rp = B.l2g + 9 * (size_t)o;
@@ -328,7 +328,9 @@ __device__ __forceinline__ unsigned claim_tests(const DevParams& P, const DevBuf
     int di, dj, dk;
     slot_delta(s, di, dj, dk);
     atomicMin(&B.claim[claim_index(P, ci + di, cj + dj, ck + dk)], g);
-    brick_set(P, B.bmap, 1, ci + di, cj + dj, ck + dk);
+    const int wd = brick_word(P, ci + di, cj + dj, ck + dk) + 1;
+    const unsigned long long bt = brick_bit(ci + di, cj + dj, ck + dk);
+    if (!st || !stage_add(*st, wd, bt)) atomicOr(&B.bmap[wd], bt);
   }
   return im;
 }
@@ -460,13 +462,13 @@ __device__ __forceinline__ IdRange id_range(const DevParams& P, const DevCounter
 // captured cell is never liquid again.
 template <bool kCoherent>
 __device__ __forceinline__ unsigned long long claim_entry(const DevParams& P, const DevBuffers& B,
-                                                          int q) {
+                                                          int q, WordStage* st) {
   int ci, cj, ck;
   slot_ijk(P, q, ci, cj, ck);
   const SlotRec r = B.rec[rec_slot(q)];
   const unsigned lm = brick_bits26<0, kCoherent>(P, B.bmap, ci, cj, ck);
   const unsigned im =
-      claim_tests(P, B, r.id, r.o, ci, cj, ck, lm, make_double4(r.x, r.y, r.z, B.sL[q]));
+      claim_tests(P, B, r.id, r.o, ci, cj, ck, lm, make_double4(r.x, r.y, r.z, B.sL[q]), st);
   return ((unsigned long long)im << 32) | lm;
 }
 
@@ -486,6 +488,11 @@ __device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers&
   list_wave(B, wave, list, n);
   int lo, hi;
   chunk_of(n, blk, nblk, kBlock, lo, hi);
+  if (lo >= hi) return;
+  // the claimed bits, staged per work-group
+  __shared__ WordStage s_st;
+  stage_init(s_st);
+  __syncthreads();
   if (wave == 0) {
     const double inc = ctr->dt * ctr->v;
     for (int t = lo + threadIdx.x; t < hi; t += kBlock) {
@@ -496,12 +503,16 @@ __device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers&
       const double L = B.sL[q] + inc;
       B.sL[q] = L;
       const unsigned lm = brick_bits26<0, false>(P, B.bmap, i, j, k);
-      const unsigned im = claim_tests(P, B, r.id, r.o, i, j, k, lm, make_double4(r.x, r.y, r.z, L));
+      const unsigned im =
+          claim_tests(P, B, r.id, r.o, i, j, k, lm, make_double4(r.x, r.y, r.z, L), &s_st);
       B.masks[t] = ((unsigned long long)im << 32) | lm;
     }
-    return;
+  } else {
+    for (int t = lo + threadIdx.x; t < hi; t += kBlock)
+      B.masks[t] = claim_entry<false>(P, B, list[t], &s_st);
   }
-  for (int t = lo + threadIdx.x; t < hi; t += kBlock) B.masks[t] = claim_entry<false>(P, B, list[t]);
+  __syncthreads();
+  stage_flush_or(s_st, B.bmap);
 }
 
 // Captured slots and the keep bit of the grain in slot q at (ci, cj, ck) from
@@ -933,7 +944,7 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
     const int* list = B.listW[in];
     int* kids = B.listW[wave & 1];
     for (int i = threadIdx.x; i < n; i += kTailBlock)
-      B.masks[i] = claim_entry<true>(P, B, list[i]);
+      B.masks[i] = claim_entry<true>(P, B, list[i], nullptr);
     __threadfence();
     __syncthreads();
     unsigned long long run = 0;
