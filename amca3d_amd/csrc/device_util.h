@@ -77,7 +77,8 @@ __device__ __forceinline__ void brick_set(const DevParams& P, unsigned long long
 // the memory side, so this saves most of them: neighbouring entries set bits
 // in the same words).  Only later kernels read those words.  A word that finds
 // no free entry within kStageProbe probes goes straight to memory.
-constexpr int kStageSlots = 512;  // a power of two
+constexpr int kStageBits = 9;
+constexpr int kStageSlots = 1 << kStageBits;
 constexpr int kStageProbe = 8;
 struct WordStage {
   int key[kStageSlots];
@@ -91,7 +92,7 @@ __device__ __forceinline__ void stage_init(WordStage& s) {
 }
 // false: no entry for `word` (the caller applies the bits itself)
 __device__ __forceinline__ bool stage_add(WordStage& s, int word, unsigned long long b) {
-  unsigned h = ((unsigned)word * 2654435761u) >> 23;  // 9 bits: kStageSlots
+  unsigned h = ((unsigned)word * 2654435761u) >> (32 - kStageBits);  // Knuth's multiplicative hash
   for (int p = 0; p < kStageProbe; p++) {
     const int k = atomicCAS(&s.key[h], -1, word);
     if (k == -1 || k == word) {
