@@ -52,10 +52,13 @@ struct DevCounters {
   int max_wave;
   int step_wave;          // deepest capturing wave of the running step
   int wave_gbase;         // first new grain id of the running wave
-  int scan_lo[2], scan_n[2];  // ids of the entries of list waves 0 and 1
+  int w1_lo, w1_n;        // ids of wave 1's entries (wave 0's children)
   int err;                // 1: orientation table exhausted
   int log_cap;
   unsigned tickets[8];    // block tickets of the look-back kernels
+  // ids of wave 0's claiming grains (min, max), spread over 64 slots by k_claim
+  // block so that blocks rarely meet at one address
+  int claim_lo[64], claim_hi[64];
 };
 
 struct StepLogRow {

@@ -199,6 +199,11 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
     }
     run += tot;
   }
+  if (bid == nblk - 1)
+    for (int q = threadIdx.x; q < 64; q += kBlock) {
+      ctr->claim_lo[q] = INT_MAX;
+      ctr->claim_hi[q] = -1;
+    }
   if (threadIdx.x == 0 && bid == nblk - 1) {
     const int nn = (int)(run >> 32);
     const int nk = (int)(run & 0xffffffffull);
@@ -221,8 +226,7 @@ __device__ __forceinline__ void nucleate_body(const DevParams& P, const DevBuffe
     ctr->step_nA0 = nA;
     ctr->step_cap = 0;
     ctr->step_wave = 0;
-    ctr->scan_lo[0] = 0;
-    ctr->scan_n[0] = ng0 + nn;
+
     ctr->nA_out = 0;
     ctr->nW[0] = 0;
     ctr->nW[1] = 0;
@@ -439,17 +443,40 @@ __device__ __forceinline__ void slot_ijk(const DevParams& P, long long q, int& i
 }
 
 // The id scan of a list wave covers the 16-id groups [g0, g0 + n4) that hold
-// the ids of the wave's entries (scan_lo, scan_n: all grains for wave 0, wave
-// 0's children for wave 1); block b takes the groups g0 + [b per, (b + 1) per),
-// per a whole number of 256-thread tiles.  The counters it reads are written
-// by an earlier kernel, so every block sees the same range.
+// the ids of the grains that can capture in it: for wave 0 the range of the
+// grains that claimed a cell (a grain captures only cells it claimed), which
+// k_claim leaves in 64 (min, max) slots; for wave 1 wave 0's children.  Block b
+// takes the groups g0 + [b per, (b + 1) per), per a whole number of 256-thread
+// tiles.  The counters it reads are written by earlier kernels, so every block
+// sees the same range.  Called by every thread of the block.
 struct IdRange {
   long long g0, n4, per;
 };
 __device__ __forceinline__ IdRange id_range(const DevParams& P, const DevCounters* ctr, int wave) {
-  const long long lo = ctr->scan_lo[wave], hi = lo + ctr->scan_n[wave];
+  long long lo, hi;
+  if (wave == 0) {
+    __shared__ int s_lo, s_hi;
+    if (threadIdx.x < 64) {
+      int a = ctr->claim_lo[threadIdx.x], b = ctr->claim_hi[threadIdx.x];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) {
+        a = min(a, __shfl_xor(a, o, 64));
+        b = max(b, __shfl_xor(b, o, 64));
+      }
+      if (threadIdx.x == 0) {
+        s_lo = a;
+        s_hi = b;
+      }
+    }
+    __syncthreads();
+    lo = s_lo;
+    hi = (long long)s_hi + 1;
+  } else {
+    lo = ctr->w1_lo;
+    hi = lo + ctr->w1_n;
+  }
   IdRange r;
-  r.g0 = lo >> 4;
+  r.g0 = hi > lo ? lo >> 4 : 0;
   r.n4 = hi > lo ? ((hi + 15) >> 4) - r.g0 : 0;
   r.per = ((r.n4 + P.ids_blocks - 1) / P.ids_blocks + kBlock - 1) / kBlock * kBlock;
   if (r.per == 0) r.per = kBlock;
@@ -495,6 +522,7 @@ __device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers&
   __syncthreads();
   if (wave == 0) {
     const double inc = ctr->dt * ctr->v;
+    int id_lo = INT_MAX, id_hi = -1;  // this block's claiming grains
     for (int t = lo + threadIdx.x; t < hi; t += kBlock) {
       const long long q = list[t];
       int i, j, k;
@@ -506,6 +534,31 @@ __device__ __forceinline__ void claim_body(const DevParams& P, const DevBuffers&
       const unsigned im =
           claim_tests(P, B, r.id, r.o, i, j, k, lm, make_double4(r.x, r.y, r.z, L), &s_st);
       B.masks[t] = ((unsigned long long)im << 32) | lm;
+      if (im) {
+        id_lo = min(id_lo, r.id);
+        id_hi = max(id_hi, r.id);
+      }
+    }
+    // the block's claimer id range into one of the 64 slots
+    __shared__ int s_lo, s_hi;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      id_lo = min(id_lo, __shfl_xor(id_lo, o, 64));
+      id_hi = max(id_hi, __shfl_xor(id_hi, o, 64));
+    }
+    if (threadIdx.x == 0) {
+      s_lo = INT_MAX;
+      s_hi = -1;
+    }
+    __syncthreads();
+    if ((threadIdx.x & 63) == 0 && id_hi >= 0) {
+      atomicMin(&s_lo, id_lo);
+      atomicMax(&s_hi, id_hi);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && s_hi >= 0) {
+      atomicMin(&B.ctr->claim_lo[blk & 63], s_lo);
+      atomicMax(&B.ctr->claim_hi[blk & 63], s_hi);
     }
   } else {
     for (int t = lo + threadIdx.x; t < hi; t += kBlock)
@@ -767,8 +820,8 @@ __device__ __forceinline__ void idscan_body(const DevParams& P, const DevBuffers
       ctr->step_cap += (int)nc;
       ctr->nA_out += (int)svt;
       if (wave == 0) {
-        ctr->scan_lo[1] = gbase;
-        ctr->scan_n[1] = (int)nc;
+        ctr->w1_lo = gbase;
+        ctr->w1_n = (int)nc;
       }
       if (nc > 0 && ctr->max_wave < wave + 1) ctr->max_wave = wave + 1;
       if (nc > 0 && ctr->step_wave < wave + 1) ctr->step_wave = wave + 1;

@@ -40,8 +40,10 @@ def algorithmic_bytes(log, cells):
                orientation 4 read, grown L written 8); wave 1: 40 B per
                wave-1 grain (no growth write)
     k_resolve  1 bit per entry of waves 0 and 1 (its active bit)
-    k_idscan   1 B per grain id of the scanned ranges: every grain before
-               wave 0's children, then wave 0's children (the capture counts)
+    k_idscan   1 B per grain id of the scanned ranges (the capture counts):
+               for wave 0 the id range of the grains that claimed a cell, which
+               the step log does not hold, so every grain before wave 0's
+               children is counted (an upper bound); then wave 0's children
     k_emit     44 B per capture: child centre 24, L 8, grain id 4, orientation
                4, next-wave list entry 4 (captures of waves >= 2, made in
                k_tail, are a handful per step and are counted here too)
