@@ -15,6 +15,25 @@ __device__ __forceinline__ unsigned ld_acquire(const unsigned* p) {
 __device__ __forceinline__ void st_release(unsigned* p, unsigned v) {
   __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
+// A look-back flag, published after its payload.  The payload is written with
+// agent-scope atomic stores (through to the coherent level, sc1), so the
+// flag needs only those stores to have completed before it is issued:
+// s_waitcnt vmcnt(0) (gfx9 counts stores in vmcnt; 0x0F70 = vmcnt(0), expcnt
+// and lgkmcnt not waited).  A release store would also write back the whole
+// L2 of the XCD (buffer_wbl2), which the payload does not need.  The signal
+// fences keep the compiler from moving the stores across the wait.
+__device__ __forceinline__ void st_flag_after_payload(unsigned* p, unsigned v) {
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __builtin_amdgcn_s_waitcnt(0x0F70);
+  __atomic_signal_fence(__ATOMIC_SEQ_CST);
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// A look-back flag read: its payload is read only after the flag's value is
+// tested, with agent-scope atomic loads, so no acquire (which would also
+// invalidate the L2) is needed.
+__device__ __forceinline__ unsigned ld_flag(const unsigned* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
 __device__ __forceinline__ unsigned long long ld_relaxed(const unsigned long long* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
@@ -218,13 +237,13 @@ __device__ inline unsigned long long lookback(unsigned* flag, unsigned long long
   if (bid == 0) {
     if (lane == 0) {
       st_relaxed(&pref[0], aggregate);
-      st_release(&flag[0], tag | 2u);
+      st_flag_after_payload(&flag[0], tag | 2u);
     }
     return 0ull;
   }
   if (lane == 0) {
     st_relaxed(&agg[bid], aggregate);
-    st_release(&flag[bid], tag | 1u);
+    st_flag_after_payload(&flag[bid], tag | 1u);
   }
   unsigned long long excl = 0;
   int base = bid - 1;
@@ -232,7 +251,7 @@ __device__ inline unsigned long long lookback(unsigned* flag, unsigned long long
     const int p = base - lane;
     bool ready, is_pref;
     if (p >= 0) {
-      const unsigned f = ld_acquire(&flag[p]);
+      const unsigned f = ld_flag(&flag[p]);
       ready = (f & ~3u) == tag && (f & 3u) != 0u;
       is_pref = ready && (f & 3u) == 2u;
     } else {
@@ -259,7 +278,7 @@ __device__ inline unsigned long long lookback(unsigned* flag, unsigned long long
   }
   if (lane == 0) {
     st_relaxed(&pref[bid], excl + aggregate);
-    st_release(&flag[bid], tag | 2u);
+    st_flag_after_payload(&flag[bid], tag | 2u);
   }
   return excl;
 }
