@@ -998,7 +998,10 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
     int* kids = B.listW[wave & 1];
     for (int i = threadIdx.x; i < n; i += kTailBlock)
       B.masks[i] = claim_entry<true>(P, B, list[i], nullptr);
-    __threadfence();
+    // one work-group: a work-group-scope fence orders its phases (the claim
+    // and map words are read with agent-scope loads); an agent-scope fence
+    // would also write back and invalidate the L2
+    __threadfence_block();
     __syncthreads();
     unsigned long long run = 0;
     for (int base = 0; base < n; base += kTailBlock) {
@@ -1020,7 +1023,7 @@ __device__ __forceinline__ void tail_body(const DevParams& P, const DevBuffers& 
       }
       run += tot;
     }
-    __threadfence();
+    __threadfence_block();
     __syncthreads();
     if (threadIdx.x == 0) {
       const int nc = (int)(run >> 32);
