@@ -9,12 +9,6 @@
 
 namespace amca {
 
-__device__ __forceinline__ unsigned ld_acquire(const unsigned* p) {
-  return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void st_release(unsigned* p, unsigned v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
 // A look-back flag, published after its payload.  The payload is written with
 // agent-scope atomic stores (through to the coherent level, sc1), so the
 // flag needs only those stores to have completed before it is issued:
