@@ -339,12 +339,13 @@ __device__ __forceinline__ void block_box_merge(RmBox b, RmBox* dst) {
   }
 }
 
-// blocks of the grid-stride cell kernels: with the active box, 2048 blocks of
-// 256 (eight per CU) cost little when the box is small; over the whole grid,
-// up to 16384
+// blocks of the grid-stride cell kernels: with the active box, 1024 blocks of
+// 256 (four per CU; the box is a few hundred thousand cells); over the whole
+// grid, up to 16384.  k_rm_liquid writes one box slot per block, which
+// k_rm_rowscan reads, so both use this count.
 __host__ __device__ __forceinline__ int rm_cell_blocks(const RmParams& P) {
   const long long cb = ((long long)P.NG + kRmBlock - 1) / kRmBlock;
-  const int cap = P.window ? 2048 : 16384;
+  const int cap = P.window ? 1024 : 16384;
   return (int)(cb < cap ? cb : cap);
 }
 
@@ -1278,6 +1279,9 @@ const char* const kRmKernelNames[kRmKernels] = {
 hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int site_blocks,
                             hipStream_t s) {
   const unsigned cells = (unsigned)rm_cell_blocks(P);
+  // the kernels that work only on flagged cells (a small part of the box):
+  // with the box, two blocks per CU (their waves hold many registers)
+  const unsigned sparse = P.window && cells > 512u ? 512u : cells;
   long long nbk = (P.nodes + kRmBlock - 1) / kRmBlock;
   if (nbk > 8192) nbk = 8192;
   if (nbk < 1) nbk = 1;
@@ -1301,13 +1305,13 @@ hipError_t rm_launch_kernel(int q, const RmParams& P, const RmBuffers& B, int si
     case 3: hipLaunchKernelGGL(k_rm_phase, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 4: hipLaunchKernelGGL(k_rm_liquid, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 5:
-      hipLaunchKernelGGL(k_rm_nucleate_mint, dim3(cells + (unsigned)site_blocks), dim3(kRmBlock), 0,
+      hipLaunchKernelGGL(k_rm_nucleate_mint, dim3(sparse + (unsigned)site_blocks), dim3(kRmBlock), 0,
                          s, P, B, site_blocks);
       break;
-    case 6: hipLaunchKernelGGL(k_rm_claim, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 6: hipLaunchKernelGGL(k_rm_claim, dim3(sparse), dim3(kRmBlock), 0, s, P, B); break;
     case 7: hipLaunchKernelGGL(k_rm_resolve, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
     case 8: hipLaunchKernelGGL(k_rm_rowscan, dim3(1), dim3(kRmScan), 0, s, P, B); break;
-    case 9: hipLaunchKernelGGL(k_rm_emit, dim3(cells), dim3(kRmBlock), 0, s, P, B); break;
+    case 9: hipLaunchKernelGGL(k_rm_emit, dim3(sparse), dim3(kRmBlock), 0, s, P, B); break;
     default: return hipErrorInvalidValue;
   }
   return hipGetLastError();
