@@ -1044,8 +1044,9 @@ amca_status amca_batch_destroy(amca_batch* b) {
 
 amca_status amca_batch_set_schedule(amca_batch* b, const amca_batch_schedule* s) {
   if (!b || !s || s->list_blocks < 1 || s->emit_blocks < 1 || s->min_blocks < 1 ||
-      s->steps_per_graph < 1 || s->steps_per_graph > 4096)
-    return fail(AMCA_ERR_INVALID, "bad schedule");
+      s->min_blocks > amca::kMaxBlocks || s->steps_per_graph < 1 || s->steps_per_graph > 4096)
+    return fail(AMCA_ERR_INVALID, "bad schedule (budgets >= 1, min_blocks in [1, 4096], "
+                                  "steps_per_graph in [1, 4096])");
   HIP_TRY(hipStreamSynchronize(b->stream));
   b->sched = *s;
   return batch_apply_schedule(b);

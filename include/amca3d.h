@@ -178,7 +178,7 @@ typedef struct amca_batch amca_batch;
 typedef struct {
   int32_t list_blocks;      /* claim blocks, shared in proportion to domain cell counts */
   int32_t emit_blocks;      /* resolve / emit blocks, same sharing */
-  int32_t min_blocks;       /* at least this many per domain */
+  int32_t min_blocks;       /* at least this many per domain (1..4096) */
   int32_t interleave;       /* 0: blocks domain by domain (largest first), 1: round robin */
   int32_t steps_per_graph;  /* steps captured in one graph replay */
 } amca_batch_schedule;

@@ -88,6 +88,21 @@ def test_batch_without_graphs(amca, singles, monkeypatch):
         _same(s, b)
 
 
+def test_batch_schedule_is_checked(amca):
+    from amca3d_amd.batch import Batch, simulations
+    sims = simulations(SPECS[:1], termination_time=1.0)
+    b = Batch([s.domain for s in sims])
+    for bad in (dict(min_blocks=0), dict(min_blocks=4097), dict(list_blocks=0),
+                dict(steps_per_graph=0), dict(steps_per_graph=4097)):
+        with pytest.raises(Exception, match="bad schedule"):
+            b.set_schedule(**bad)
+    b.set_schedule(min_blocks=4096, steps_per_graph=2)
+    assert b.schedule()["min_blocks"] == 4096
+    b.close()
+    for s in sims:
+        s.close()
+
+
 def test_batch_graph_setting_is_checked(amca, monkeypatch):
     from amca3d_amd.batch import Batch, simulations
     sims = simulations(SPECS[:1], termination_time=1.0)
