@@ -282,7 +282,9 @@ def pbf_main(args):
                         bytes_per_launch=bytes_total / max(kt[dom_k][1], 1))
         else:
             roof.update(achieved=None, frac=None)
-        roof["traffic"] = measured_traffic(dom_k, "pmc_traffic_pbf_track.json")
+        roof["traffic"] = measured_traffic(
+            dom_k, "pmc_traffic_pbf_track_whole_grid.json" if args.whole_grid
+            else "pmc_traffic_pbf_track.json")
         result["roofline"] = roof
         result["kernel_ms"] = {k: round(v, 3) for k, v in total.items()}
     if not args.no_cpu_baseline:
