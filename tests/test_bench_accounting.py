@@ -95,3 +95,22 @@ def test_window_takes_the_last_steps_by_wave():
     bad = rows[:-1] + [(d, "k_emit", 0.0)]
     with pytest.raises(SystemExit):
         ps.window_rows(bad, 2)  # a dispatch out of the step's order
+
+
+def test_batch_workload_specs():
+    """The 64-domain batch workload the bench and its GPU test run: 336.8 M cells in cubes
+    of 64..256 cells, cooling rates and bulk densities in their stated log-uniform ranges,
+    seeds 0..63, the same list on every call."""
+    from amca3d_amd.batch import batch_specs, spec_deck
+    a, b = batch_specs(), batch_specs()
+    assert a == b and len(a) == 64
+    assert [s["seed"] for s in a] == list(range(64))
+    assert all(64 <= s["edge"] <= 256 for s in a)
+    assert all(-2000.0 <= s["Tdot"] <= -20.0 for s in a)
+    assert all(1e12 <= s["density"] <= 1e14 for s in a)
+    assert sum(s["edge"] ** 3 for s in a) == 336796673
+    s5 = a[5]
+    deck = spec_deck(s5)
+    assert f"random_seed: {s5['seed']}\n" in deck
+    assert f"number_cells: [{s5['edge']}, {s5['edge']}, {s5['edge']}]" in deck
+    assert f"t_dot: {s5['Tdot']!r}" in deck
